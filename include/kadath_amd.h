@@ -1,0 +1,112 @@
+/*
+ * kadath_amd.h -- C-ABI of the MI355X Newton-Raphson path for Kadath/FUKA.
+ *
+ * Plain pointers and sizes only.  Every buffer argument is DEVICE memory unless
+ * stated.  Every function returns an int status: 0 = ok, < 0 = error
+ * (KAD_ERR_*), and for the factorisation i > 0 = LAPACK-style info (exactly
+ * zero pivot U(i-1,i-1)).
+ *
+ * The reference has no C ABI: its Newton step is the C++ member
+ *   bool System_of_eqs::do_newton(double precision, double& error, SOLVER)
+ *   (reference include/system_of_eqs.hpp:871, src_par/do_newton.cpp:65-430)
+ * swapped at link time (build_release/CMakeLists.txt:24-31).  The C++ host
+ * library of this package keeps that member; the functions below are the thin
+ * layer it calls, one per reference subsystem that moves to the GPU.
+ */
+#ifndef KADATH_AMD_H
+#define KADATH_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KAD_OK 0
+#define KAD_ERR_ARG (-1)
+#define KAD_ERR_HIP (-2)
+#define KAD_ERR_ALLOC (-3)
+#define KAD_ERR_NOCONV (-4)
+
+/* Factor precisions. */
+#define KAD_PREC_F64 0  /* fp64 storage, fp64 MFMA                                  */
+#define KAD_PREC_F32 1  /* fp32 storage, fp32 MFMA                                  */
+#define KAD_PREC_BF16 2 /* fp32 storage, bf16 MFMA operands, fp32 accumulation      */
+
+typedef struct kad_ctx kad_ctx;
+
+/* ---- context ------------------------------------------------------------ */
+/* One context per GPU: owns a HIP stream and all scratch buffers.
+ * Replaces the process-global state of the reference (FFTW plan maps
+ * src/Coef/coef_1d.cpp:34-44, transform scratch include/memory.hpp:55-69). */
+int kad_ctx_create(kad_ctx** ctx, int device);
+int kad_ctx_destroy(kad_ctx* ctx);
+void* kad_ctx_stream(kad_ctx* ctx);
+int kad_sync(kad_ctx* ctx);
+
+/* ---- dense linear algebra (replaces pdgesv_, src_par/do_newton.cpp:317-320,
+ *      and Matrice::set_lu/solve, src/Matrice/matrice.cpp:192-250) ---------- */
+
+/* C -= A*B (op 0) or C = A*B (op 1); column-major; prec selects the MFMA type. */
+int kad_gemm(kad_ctx* ctx, int prec, int op, int m, int n, int k, const void* A, int64_t lda, const void* B,
+             int64_t ldb, void* C, int64_t ldc);
+
+/* In-place LU with partial pivoting of the n x n column-major A (fp64 for
+ * KAD_PREC_F64, fp32 otherwise).  ipiv: n ints, 0-based interchanges (LAPACK
+ * dgetrf semantics shifted by one); perm (may be NULL): n ints, composed row
+ * order used by kad_lu_solve.  nb: outer panel width (0 = default). */
+int kad_lu_factor(kad_ctx* ctx, int prec, int n, void* A, int64_t lda, int* ipiv, int* perm, int nb);
+
+/* Solve op(A) x = b in place with the factors (b fp64).  trans = 1: A^T x = b
+ * (adjoint solve; the reference's Matrice::solve uses trans 'T' on row-major
+ * storage, matrice.cpp:221-250). */
+int kad_lu_solve(kad_ctx* ctx, int prec, int n, const void* LU, int64_t lda, const int* perm, double* b, int trans);
+
+/* y = b - op(A) x (b != NULL) or y = op(A) x, fp64, column-major A. */
+int kad_gemv64(kad_ctx* ctx, int n, const double* A, int64_t lda, const double* x, const double* b, double* y,
+               int trans);
+
+/* B = (float)A, n x n column-major. */
+int kad_cast_f64_to_f32(kad_ctx* ctx, int n, const double* A, int64_t lda, float* B, int64_t ldb);
+
+/* Mixed-precision solve: fp64 residuals with A64, corrections from the
+ * low-precision factors (prec_lo, LUlo, perm).  mode 0 = classical IR,
+ * mode 1 = GMRES-IR.  Stops at ||b - op(A) x||_inf <= tol ||b||_inf.
+ * iters[0] = outer steps, iters[1] = inner GMRES steps.  Returns KAD_ERR_NOCONV
+ * if max_outer steps do not reach tol (x holds the last iterate). */
+int kad_ir_solve(kad_ctx* ctx, int n, const double* A64, int64_t lda, int prec_lo, const void* LUlo, int64_t ldlu,
+                 const int* perm, const double* b, double* x, double tol, int max_outer, int mode, int trans,
+                 int* iters, double* resid);
+
+/* ---- spectral transforms (replace Base_spectral::coef / coef_i,
+ *      src/Coef/coef.cpp:96-104, src/Coef/coef_i.cpp:96-104, and the 1-D
+ *      FFTW-backed dispatch coef_1d / coef_i_1d, src/Coef/coef_1d.cpp:607-636,
+ *      src/Coef/coef_i_1d.cpp:527-556) ----------------------------------- */
+
+/* Batched 3-D transform of nbatch domain arrays of one spherical domain.
+ * conf: nbatch x (nr*nt*np) doubles, r fastest; coef: nbatch x (nr*nt*(np+2)).
+ * base_r: HOST, nt*(np+2) basis ids of the r lines, index j + nt*k;
+ * base_t: HOST, np+2 basis ids of the theta lines (per phi coefficient k);
+ * base_p: basis of the phi lines (COSSIN = 4).  Ids as include/base_spectral.hpp. */
+int kad_coef_3d(kad_ctx* ctx, int nr, int nt, int np, const int* base_r, const int* base_t, int base_p, int nbatch,
+                const double* conf, double* coef);
+int kad_coef_i_3d(kad_ctx* ctx, int nr, int nt, int np, const int* base_r, const int* base_t, int base_p,
+                  int nbatch, const double* coef, double* conf);
+
+/* Batched 1-D transforms of nlines contiguous lines (one basis for all lines):
+ * conf lines of nconf values <-> coef lines of nconf (+2 for COSSIN) values. */
+int kad_coef_1d(kad_ctx* ctx, int base, int nconf, int64_t nlines, const double* conf, double* coef);
+int kad_coef_i_1d(kad_ctx* ctx, int base, int nconf, int64_t nlines, const double* coef, double* conf);
+
+/* Host-side tables (no device work): analysis (kind 0) / synthesis (kind 1)
+ * matrix of one 1-D basis, row-major; returns the element count or < 0. */
+int kad_host_transform_matrix(int kind, int base, int nconf, double* out, int cap);
+/* Matrix of a coefficient-space 1-D operator (op: 0 der_1d, 1 mult_sin_1d,
+ * 2 mult_cos_1d, 3 div_sin_1d, 4 mult_x_1d, 5 mult_xm1_1d, 6 div_xm1_1d;
+ * src/Ope_1d/*.cpp); *out_base receives the output basis. */
+int kad_host_op1d_matrix(int op, int base, int n, double* out, int cap, int* out_base);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

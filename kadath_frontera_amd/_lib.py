@@ -1,0 +1,68 @@
+"""ctypes binding of libkadath_amd.so (the C-ABI declared in include/kadath_amd.h).
+
+There is no fallback: if the shared library is missing, importing this module
+raises, and every GPU entry point goes through the HIP kernels.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libkadath_amd.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'` "
+        "or `python kadath_frontera_amd/build.py`")
+
+lib = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_GLOBAL)
+
+c_int = ctypes.c_int
+c_i64 = ctypes.c_int64
+c_dbl = ctypes.c_double
+c_vp = ctypes.c_void_p
+c_ip = ctypes.POINTER(ctypes.c_int)
+c_dp = ctypes.POINTER(ctypes.c_double)
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "kad_ctx_create": (c_int, [ctypes.POINTER(c_vp), c_int]),
+    "kad_ctx_destroy": (c_int, [c_vp]),
+    "kad_ctx_stream": (c_vp, [c_vp]),
+    "kad_sync": (c_int, [c_vp]),
+    "kad_gemm": (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64]),
+    "kad_lu_factor": (c_int, [c_vp, c_int, c_int, c_vp, c_i64, c_vp, c_vp, c_int]),
+    "kad_lu_solve": (c_int, [c_vp, c_int, c_int, c_vp, c_i64, c_vp, c_vp, c_int]),
+    "kad_gemv64": (c_int, [c_vp, c_int, c_vp, c_i64, c_vp, c_vp, c_vp, c_int]),
+    "kad_cast_f64_to_f32": (c_int, [c_vp, c_int, c_vp, c_i64, c_vp, c_i64]),
+    "kad_ir_solve": (c_int, [c_vp, c_int, c_vp, c_i64, c_int, c_vp, c_i64, c_vp, c_vp, c_vp, c_dbl, c_int, c_int,
+                             c_int, c_ip, c_dp]),
+    "kad_coef_3d": (c_int, [c_vp, c_int, c_int, c_int, c_ip, c_ip, c_int, c_int, c_vp, c_vp]),
+    "kad_coef_i_3d": (c_int, [c_vp, c_int, c_int, c_int, c_ip, c_ip, c_int, c_int, c_vp, c_vp]),
+    "kad_coef_1d": (c_int, [c_vp, c_int, c_int, c_i64, c_vp, c_vp]),
+    "kad_coef_i_1d": (c_int, [c_vp, c_int, c_int, c_i64, c_vp, c_vp]),
+    "kad_host_transform_matrix": (c_int, [c_int, c_int, c_int, c_dp, c_int]),
+    "kad_host_op1d_matrix": (c_int, [c_int, c_int, c_int, c_dp, c_int, c_ip]),
+}
+
+for _name, (_res, _args) in SIGNATURES.items():
+    _f = getattr(lib, _name)
+    _f.restype = _res
+    _f.argtypes = _args
+
+
+def declared_symbols(header=os.path.join(_HERE, "..", "include", "kadath_amd.h")):
+    """Function names declared in the public header."""
+    import re
+    txt = open(header).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:int|void\s*\*|double)\s+(kad_\w+)\s*\(", txt, flags=re.M)))
+
+
+class KadError(RuntimeError):
+    pass
+
+
+def check(rc, what):
+    if rc < 0:
+        raise KadError(f"{what} failed with status {rc}")
+    return rc

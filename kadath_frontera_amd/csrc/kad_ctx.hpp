@@ -1,0 +1,63 @@
+// Device context: one per (process, GPU).  Owns the stream and every scratch
+// buffer of the Newton path, so the C-ABI entry points carry no hidden globals
+// (the reference keeps FFT plans and transform scratch in statics:
+// src/Coef/coef_1d.cpp:34-44, include/memory.hpp:55-69).
+#pragma once
+#include "kad_common.hpp"
+#include "kad_spectral.hpp"
+#include <map>
+#include <string>
+#include <vector>
+
+namespace kad {
+
+// Scratch of the LU factorisation.
+struct LuWork {
+  hipStream_t stream = nullptr;
+  double* red_val = nullptr;    // per-block partial maxima of the pivot search
+  int* red_idx = nullptr;
+  unsigned* counter = nullptr;  // last-block counter (reset by the last block)
+  int* info = nullptr;          // first exactly-zero pivot (1-based), 0 if none
+  int red_cap = 0;
+  void* tinv = nullptr;         // unit-lower inverse, nb x nb
+  void* tmp = nullptr;          // GEMM scratch for L11^{-1} A12
+  size_t tmp_bytes = 0;
+  void* tmp2 = nullptr;         // scratch of the recursive triangular inverse
+  size_t tmp2_bytes = 0;
+};
+
+// Device copy of a chain of 1-D passes (matrix pool + per-line tables).
+struct DevPlan {
+  Passes P;
+  double* pool = nullptr;
+  int* lines = nullptr;
+  int d_in[3] = {0, 0, 0};
+  int d_out[3] = {0, 0, 0};
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  LuWork lu;
+  size_t cap_redv = 0, cap_redi = 0, cap_tinv = 0;
+  double* gemv_part = nullptr;
+  size_t cap_part = 0;
+  double* vwork = nullptr;
+  size_t cap_vwork = 0;
+  double* krylov = nullptr;
+  size_t cap_krylov = 0;
+
+  std::map<std::string, DevPlan*> plans;  // transform plans by key
+  std::vector<void*> owned;                // device buffers freed with the context
+
+  int ensure(void** p, size_t* cap, size_t bytes);
+  int upload(const void* host, size_t bytes, void** dev);
+  int lu_work(int n, int nb);
+  int vec_work(int n);
+  double absmax(const double* x, int n);
+  double dot(const double* x, const double* y, int n);
+  ~Ctx();
+};
+
+}  // namespace kad

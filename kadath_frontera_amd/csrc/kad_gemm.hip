@@ -1,0 +1,307 @@
+// MFMA GEMM for the trailing update of the blocked LU (the ScaLAPACK pdgesv
+// replacement, reference call site src_par/do_newton.cpp:317-320).
+//
+//   OP_SUB : C -= A * B      (trailing update A22 -= L21 * U12)
+//   OP_SET : C  = A * B      (triangular solve through an explicit inverse)
+//
+// All operands column-major with leading dimensions, as the dense Jacobian is
+// stored (the reference packs J column-major for pdgesv, do_newton.cpp:178-185).
+//
+// Tile: 128x128 per workgroup, 4 waves in a 2x2 grid, each wave 64x64 = 4x4
+// MFMA 16x16 tiles.  K is staged through LDS with register double buffering.
+// The MFMA is issued with the roles of A and B swapped (D' = B^T A^T) so that
+// each lane's accumulator registers hold 4 consecutive COLUMNS of one ROW ...
+// i.e. 16 lanes cover 16 consecutive rows of one column, which makes the
+// column-major epilogue coalesced.
+#include "kad_common.hpp"
+
+namespace kad {
+
+constexpr int GBM = 128, GBN = 128, GNT = 256;
+constexpr int OP_SUB = 0, OP_SET = 1;
+
+template <int P> struct GemmCfg;
+template <> struct GemmCfg<F64> { using T = double; static constexpr int BK = 16; };
+template <> struct GemmCfg<F32> { using T = float;  static constexpr int BK = 32; };
+template <> struct GemmCfg<BF16> { using T = float; static constexpr int BK = 64; };
+
+// Bijective block-id remap: consecutive tiles of the grouped order land on the
+// same XCD (hardware dispatches workgroups round-robin over the 8 XCDs), so the
+// A row-panels and B column-panels they share stay in that XCD's L2.
+__device__ inline void tile_coords(int bid, int nbm, int nbn, int& tm, int& tn) {
+  const int nblocks = nbm * nbn;
+  const int full = (nblocks / 8) * 8;
+  int id = bid;
+  if (bid < full) id = (bid % 8) * (full / 8) + bid / 8;
+  const int GROUP_M = 8;
+  const int in_group = GROUP_M * nbn;
+  const int group = id / in_group;
+  const int first_m = group * GROUP_M;
+  const int gsize = min(nbm - first_m, GROUP_M);
+  const int r = id % in_group;
+  tm = first_m + r % gsize;
+  tn = r / gsize;
+}
+
+// ---------------------------------------------------------------------------
+// f64 / f32 path: 16x16x4 MFMA, LDS tiles stored k-major: As[k][m], Bs[k][n].
+// ---------------------------------------------------------------------------
+template <int P, int OP>
+__global__ __launch_bounds__(GNT) void gemm_k4_kernel(int m, int n, int k,
+                                                      const typename GemmCfg<P>::T* __restrict__ A, int64_t lda,
+                                                      const typename GemmCfg<P>::T* __restrict__ B, int64_t ldb,
+                                                      typename GemmCfg<P>::T* __restrict__ C, int64_t ldc) {
+  using T = typename GemmCfg<P>::T;
+  constexpr int BK = GemmCfg<P>::BK;
+  constexpr int PADA = (sizeof(T) == 8) ? 4 : 8;
+  constexpr int LDA_S = GBM + PADA, LDB_S = GBN + PADA;
+  constexpr int NA = GBM * BK / GNT;  // elements of A per thread per stage
+  constexpr int NB = GBN * BK / GNT;
+  using acc_t = typename std::conditional<sizeof(T) == 8, d4v, f4v>::type;
+
+  __shared__ T As[2][BK * LDA_S];
+  __shared__ T Bs[2][BK * LDB_S];
+
+  const int nbm = (int)cdiv(m, GBM), nbn = (int)cdiv(n, GBN);
+  int tm, tn;
+  tile_coords(blockIdx.x, nbm, nbn, tm, tn);
+  const int row0 = tm * GBM, col0 = tn * GBN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+
+  acc_t acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+
+  T ra[NA], rb[NB];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int e = 0; e < NA; ++e) {
+      const int idx = tid + e * GNT;
+      const int r = idx % GBM, kk = idx / GBM;
+      const int gr = row0 + r, gk = k0 + kk;
+      ra[e] = (gr < m && gk < k) ? A[gr + (int64_t)gk * lda] : T(0);
+    }
+#pragma unroll
+    for (int e = 0; e < NB; ++e) {
+      const int idx = tid + e * GNT;
+      const int kk = idx % BK, c = idx / BK;
+      const int gk = k0 + kk, gc = col0 + c;
+      rb[e] = (gk < k && gc < n) ? B[gk + (int64_t)gc * ldb] : T(0);
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int e = 0; e < NA; ++e) {
+      const int idx = tid + e * GNT;
+      As[buf][(idx / GBM) * LDA_S + (idx % GBM)] = ra[e];
+    }
+#pragma unroll
+    for (int e = 0; e < NB; ++e) {
+      const int idx = tid + e * GNT;
+      Bs[buf][(idx % BK) * LDB_S + (idx / BK)] = rb[e];
+    }
+  };
+
+  const int nk = (int)cdiv(k, BK);
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) gload((kt + 1) * BK);
+    const T* as = As[cur];
+    const T* bs = Bs[cur];
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      const int kr = kk + (lane >> 4);
+      T af[4], bf[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        af[t] = as[kr * LDA_S + wm + t * 16 + (lane & 15)];
+        bf[t] = bs[kr * LDB_S + wn + t * 16 + (lane & 15)];
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          if constexpr (sizeof(T) == 8)
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j], af[i], acc[i][j], 0, 0, 0);
+          else
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(bf[j], af[i], acc[i][j], 0, 0, 0);
+        }
+    }
+    if (kt + 1 < nk) {
+      sstore(cur ^ 1);
+    }
+    __syncthreads();
+  }
+
+  // Epilogue: lane holds rows (lane&15) of tile i, columns 4*(lane>>4)+r of tile j.
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int gr = row0 + wm + i * 16 + (lane & 15);
+    if (gr >= m) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gc = col0 + wn + j * 16 + 4 * (lane >> 4) + r;
+        if (gc < n) {
+          T* p = C + gr + (int64_t)gc * ldc;
+          if (OP == OP_SUB) *p -= acc[i][j][r];
+          else *p = acc[i][j][r];
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// bf16 path: fp32 storage, operands rounded to bf16 in the LDS store,
+// 16x16x32 MFMA with fp32 accumulation.  LDS tiles are k-contiguous:
+// As[m][k], Bs[n][k], so each lane's 8 consecutive k values are one 16-byte read.
+// ---------------------------------------------------------------------------
+template <int OP>
+__global__ __launch_bounds__(GNT) void gemm_bf16_kernel(int m, int n, int k, const float* __restrict__ A, int64_t lda,
+                                                        const float* __restrict__ B, int64_t ldb, float* __restrict__ C,
+                                                        int64_t ldc) {
+  constexpr int BK = GemmCfg<BF16>::BK;
+  constexpr int LDK = BK + 8;  // bf16 elements per LDS row (pad 16 bytes)
+  constexpr int NA = GBM * BK / GNT, NB = GBN * BK / GNT;
+  __shared__ __bf16 As[2][GBM * LDK];
+  __shared__ __bf16 Bs[2][GBN * LDK];
+
+  const int nbm = (int)cdiv(m, GBM), nbn = (int)cdiv(n, GBN);
+  int tm, tn;
+  tile_coords(blockIdx.x, nbm, nbn, tm, tn);
+  const int row0 = tm * GBM, col0 = tn * GBN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+
+  f4v acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f4v{0, 0, 0, 0};
+
+  float ra[NA], rb[NB];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int e = 0; e < NA; ++e) {
+      const int idx = tid + e * GNT;
+      const int r = idx % GBM, kk = idx / GBM;
+      const int gr = row0 + r, gk = k0 + kk;
+      ra[e] = (gr < m && gk < k) ? A[gr + (int64_t)gk * lda] : 0.f;
+    }
+#pragma unroll
+    for (int e = 0; e < NB; ++e) {
+      const int idx = tid + e * GNT;
+      const int kk = idx % BK, c = idx / BK;
+      const int gk = k0 + kk, gc = col0 + c;
+      rb[e] = (gk < k && gc < n) ? B[gk + (int64_t)gc * ldb] : 0.f;
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int e = 0; e < NA; ++e) {
+      const int idx = tid + e * GNT;
+      As[buf][(idx % GBM) * LDK + idx / GBM] = (__bf16)ra[e];
+    }
+#pragma unroll
+    for (int e = 0; e < NB; ++e) {
+      const int idx = tid + e * GNT;
+      Bs[buf][(idx / BK) * LDK + idx % BK] = (__bf16)rb[e];
+    }
+  };
+
+  const int nk = (int)cdiv(k, BK);
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) gload((kt + 1) * BK);
+    const __bf16* as = As[cur];
+    const __bf16* bs = Bs[cur];
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 32) {
+      const int kof = kk + 8 * (lane >> 4);
+      bf8v af[4], bf[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        af[t] = *reinterpret_cast<const bf8v*>(as + (wm + t * 16 + (lane & 15)) * LDK + kof);
+        bf[t] = *reinterpret_cast<const bf8v*>(bs + (wn + t * 16 + (lane & 15)) * LDK + kof);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[j], af[i], acc[i][j], 0, 0, 0);
+    }
+    if (kt + 1 < nk) sstore(cur ^ 1);
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int gr = row0 + wm + i * 16 + (lane & 15);
+    if (gr >= m) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gc = col0 + wn + j * 16 + 4 * (lane >> 4) + r;
+        if (gc < n) {
+          float* p = C + gr + (int64_t)gc * ldc;
+          if (OP == OP_SUB) *p -= acc[i][j][r];
+          else *p = acc[i][j][r];
+        }
+      }
+  }
+}
+
+// Host launcher: C (-)= A*B for the given precision.  Returns KAD_OK or an error.
+int gemm_launch(int prec, int op, int m, int n, int k, const void* A, int64_t lda, const void* B, int64_t ldb, void* C,
+                int64_t ldc, hipStream_t s) {
+  if (m <= 0 || n <= 0) return KAD_OK;
+  if (k <= 0) {
+    if (op == OP_SET) {
+      const size_t es = (prec == F64) ? 8 : 4;
+      KAD_HIP_CHECK(hipMemset2DAsync(C, ldc * es, 0, (size_t)m * es, n, s));
+    }
+    return KAD_OK;
+  }
+  const int nblocks = (int)(cdiv(m, GBM) * cdiv(n, GBN));
+  dim3 grid(nblocks), block(GNT);
+  if (prec == F64) {
+    if (op == OP_SUB)
+      gemm_k4_kernel<F64, OP_SUB><<<grid, block, 0, s>>>(m, n, k, (const double*)A, lda, (const double*)B, ldb,
+                                                          (double*)C, ldc);
+    else
+      gemm_k4_kernel<F64, OP_SET><<<grid, block, 0, s>>>(m, n, k, (const double*)A, lda, (const double*)B, ldb,
+                                                          (double*)C, ldc);
+  } else if (prec == F32) {
+    if (op == OP_SUB)
+      gemm_k4_kernel<F32, OP_SUB><<<grid, block, 0, s>>>(m, n, k, (const float*)A, lda, (const float*)B, ldb,
+                                                          (float*)C, ldc);
+    else
+      gemm_k4_kernel<F32, OP_SET><<<grid, block, 0, s>>>(m, n, k, (const float*)A, lda, (const float*)B, ldb,
+                                                          (float*)C, ldc);
+  } else if (prec == BF16) {
+    if (op == OP_SUB)
+      gemm_bf16_kernel<OP_SUB><<<grid, block, 0, s>>>(m, n, k, (const float*)A, lda, (const float*)B, ldb, (float*)C,
+                                                      ldc);
+    else
+      gemm_bf16_kernel<OP_SET><<<grid, block, 0, s>>>(m, n, k, (const float*)A, lda, (const float*)B, ldb, (float*)C,
+                                                      ldc);
+  } else {
+    return KAD_ERR_ARG;
+  }
+  KAD_HIP_CHECK(hipGetLastError());
+  return KAD_OK;
+}
+
+}  // namespace kad

@@ -1,0 +1,599 @@
+// Dense LU with partial pivoting + solves + mixed-precision refinement, single MI355X.
+//
+// Replaces the ScaLAPACK solve of System_of_eqs::do_newton
+// (reference src_par/do_newton.cpp:317-320, pdgesv_) and the single-process
+// LAPACK path (src/Matrice/matrice.cpp:192-250, dgetrf_/dgetrs_).
+//
+// Layout: the n x n Jacobian is column-major with leading dimension lda (the
+// reference packs J_in column-major before pdgemr2d/pdgesv, do_newton.cpp:178-185).
+// Pivots are 0-based global row indices, LAPACK semantics: at step i row i was
+// interchanged with row ipiv[i] (ipiv[i] >= i).
+//
+// Factorisation: right-looking, outer panel width NB; each panel is factored by
+// a recursive split (LAPACK getrf2 style) whose base is a per-column kernel that
+// performs exact partial pivoting (first index of max |a|, as idamax) and whose
+// internal updates are MFMA GEMMs.  The trailing update A22 -= L21 U12 is the
+// MFMA GEMM of kad_gemm.hip in the factor precision (f64, f32, or bf16 operands
+// with f32 accumulation).  U12 = L11^{-1} A12 goes through an explicit inverse of
+// the unit-lower L11 and one GEMM.
+#include "kad_common.hpp"
+#include "kad_ctx.hpp"
+#include <vector>
+#include <cmath>
+#include <algorithm>
+
+namespace kad {
+
+int gemm_launch(int prec, int op, int m, int n, int k, const void* A, int64_t lda, const void* B, int64_t ldb, void* C,
+                int64_t ldc, hipStream_t s);
+constexpr int GOP_SUB = 0, GOP_SET = 1;
+
+constexpr int PANEL_BASE = 32;  // columns factored by the per-column kernel
+constexpr int COL_THREADS = 256;
+
+// ---------------------------------------------------------------------------
+// Pivot search of column j over rows [j, n): first-index max |a|.
+// Last block reduces, records ipiv[j] and swaps rows j <-> p across [c0, c1).
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ void block_argmax(double v, int idx, double* sv, int* si, double& bv, int& bi) {
+  const int tid = threadIdx.x;
+  sv[tid] = v;
+  si[tid] = idx;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (tid < s) {
+      const double ov = sv[tid + s];
+      const int oi = si[tid + s];
+      if (ov > sv[tid] || (ov == sv[tid] && oi < si[tid] && oi >= 0) || si[tid] < 0) {
+        if (oi >= 0) {
+          sv[tid] = ov;
+          si[tid] = oi;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  bv = sv[0];
+  bi = si[0];
+}
+
+template <typename T>
+__device__ void finish_pivot(T* A, int64_t lda, int n, int j, int c0, int c1, int* ipiv, double* red_val,
+                             int* red_idx, unsigned* counter, int* info, double* sv, int* si) {
+  // Called by every block after writing its partial (red_val/red_idx[blockIdx.x]).
+  __shared__ bool last;
+  __threadfence();
+  if (threadIdx.x == 0) {
+    unsigned t = atomicAdd(counter, 1u);
+    last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  double v = -1.0;
+  int id = -1;
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
+    const double ov = red_val[b];
+    const int oi = red_idx[b];
+    if (oi >= 0 && (id < 0 || ov > v || (ov == v && oi < id))) {
+      v = ov;
+      id = oi;
+    }
+  }
+  double bv;
+  int bi;
+  block_argmax<T>(v, id, sv, si, bv, bi);
+  if (bi < 0) bi = j;
+  if (threadIdx.x == 0) {
+    ipiv[j] = bi;
+    *counter = 0u;
+    if (bv == 0.0 && *info == 0) *info = j + 1;
+  }
+  if (bi != j) {
+    for (int c = c0 + (int)threadIdx.x; c < c1; c += blockDim.x) {
+      T* pa = A + j + (int64_t)c * lda;
+      T* pb = A + bi + (int64_t)c * lda;
+      const T t = *pa;
+      *pa = *pb;
+      *pb = t;
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(COL_THREADS) void pivot_first_kernel(T* A, int64_t lda, int n, int j, int c0, int c1,
+                                                                  int* ipiv, double* red_val, int* red_idx,
+                                                                  unsigned* counter, int* info) {
+  __shared__ double sv[COL_THREADS];
+  __shared__ int si[COL_THREADS];
+  const int i = j + blockIdx.x * blockDim.x + threadIdx.x;
+  double v = -1.0;
+  int id = -1;
+  if (i < n) {
+    v = fabs((double)A[i + (int64_t)j * lda]);
+    id = i;
+  }
+  double bv;
+  int bi;
+  block_argmax<T>(v, id, sv, si, bv, bi);
+  if (threadIdx.x == 0) {
+    red_val[blockIdx.x] = bv;
+    red_idx[blockIdx.x] = bi;
+  }
+  finish_pivot<T>(A, lda, n, j, c0, c1, ipiv, red_val, red_idx, counter, info, sv, si);
+}
+
+// One elimination step of column j inside the base panel [c0, c1):
+// rows i in (j, n): l = a_ij / a_jj ; a_i,c -= l a_j,c for c in (j, c1);
+// then the pivot of column j+1 is searched and applied (if j+1 < c1).
+template <typename T>
+__global__ __launch_bounds__(COL_THREADS) void panel_step_kernel(T* A, int64_t lda, int n, int j, int c0, int c1,
+                                                                 int* ipiv, double* red_val, int* red_idx,
+                                                                 unsigned* counter, int* info) {
+  __shared__ double sv[COL_THREADS];
+  __shared__ int si[COL_THREADS];
+  __shared__ T urow[PANEL_BASE];
+  const int w = c1 - j;  // columns j..c1-1
+  for (int c = threadIdx.x; c < w; c += blockDim.x) urow[c] = A[j + (int64_t)(j + c) * lda];
+  __syncthreads();
+  const T piv = urow[0];
+  const int i = j + 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  double v = -1.0;
+  int id = -1;
+  if (i < n) {
+    T* ai = A + i;
+    T l = ai[(int64_t)j * lda];
+    if (piv != T(0)) l = l / piv;
+    ai[(int64_t)j * lda] = l;
+    for (int c = 1; c < w; ++c) {
+      T x = ai[(int64_t)(j + c) * lda];
+      x -= l * urow[c];
+      ai[(int64_t)(j + c) * lda] = x;
+      if (c == 1) {
+        v = fabs((double)x);
+        id = i;
+      }
+    }
+  }
+  if (j + 1 >= c1) return;  // uniform over the grid: no pivot search after the last column
+  double bv;
+  int bi;
+  block_argmax<T>(v, id, sv, si, bv, bi);
+  if (threadIdx.x == 0) {
+    red_val[blockIdx.x] = bv;
+    red_idx[blockIdx.x] = bi;
+  }
+  finish_pivot<T>(A, lda, n, j + 1, c0, c1, ipiv, red_val, red_idx, counter, info, sv, si);
+}
+
+// Row interchanges of pivots [k1, k2) applied to columns [c0, c1): one thread per column.
+template <typename T>
+__global__ void laswp_kernel(T* A, int64_t lda, int c0, int c1, const int* __restrict__ ipiv, int k1, int k2) {
+  const int c = c0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= c1) return;
+  T* col = A + (int64_t)c * lda;
+  for (int i = k1; i < k2; ++i) {
+    const int p = ipiv[i];
+    if (p != i) {
+      const T t = col[i];
+      col[i] = col[p];
+      col[p] = t;
+    }
+  }
+}
+
+// Inverse of a unit-lower nb x nb block (nb <= 64), written as a full square
+// (zeros above, ones on the diagonal) into Linv (ld ldo).  One workgroup.
+template <typename T>
+__global__ __launch_bounds__(64) void trtri_small_kernel(const T* L, int64_t lda, int nb, T* Linv, int64_t ldo) {
+  __shared__ double Ls[64][65];
+  const int t = threadIdx.x;
+  for (int c = 0; c < nb; ++c)
+    if (t < nb) Ls[t][c] = (double)L[t + (int64_t)c * lda];
+  __syncthreads();
+  if (t < nb) {
+    // column t of the inverse: solve L x = e_t
+    double x[64];
+    for (int i = 0; i < nb; ++i) x[i] = 0.0;
+    x[t] = 1.0;
+    for (int jj = t; jj < nb; ++jj) {
+      const double xj = x[jj];
+      for (int i = jj + 1; i < nb; ++i) x[i] -= Ls[i][jj] * xj;
+    }
+    for (int i = 0; i < nb; ++i) Linv[i + (int64_t)t * ldo] = (T)x[i];
+  }
+}
+
+template <typename T>
+__global__ void negate_copy_kernel(const T* src, int64_t lds, T* dst, int64_t ldd, int m, int n, T scale) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (i < m && c < n) dst[i + (int64_t)c * ldd] = scale * src[i + (int64_t)c * lds];
+}
+
+template <typename T>
+__global__ void zero_upper_kernel(T* Lv, int64_t ld, int nb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (i < nb && c < nb && i < c) Lv[i + (int64_t)c * ld] = T(0);
+}
+
+template <typename T> constexpr int prec_of() { return sizeof(T) == 8 ? F64 : F32; }
+
+// Recursive inverse of unit-lower L (nb x nb) into Linv (ld ldo), full square.
+template <typename T>
+int trtri_rec(const T* L, int64_t lda, int nb, T* Linv, int64_t ldo, LuWork& w) {
+  if (nb <= 64) {
+    trtri_small_kernel<T><<<1, 64, 0, w.stream>>>(L, lda, nb, Linv, ldo);
+    KAD_HIP_CHECK(hipGetLastError());
+    return KAD_OK;
+  }
+  const int h = nb / 2;
+  int rc = trtri_rec<T>(L, lda, h, Linv, ldo, w);
+  if (rc) return rc;
+  rc = trtri_rec<T>(L + h + (int64_t)h * lda, lda, nb - h, Linv + h + (int64_t)h * ldo, ldo, w);
+  if (rc) return rc;
+  // T1 = L21 * Linv11  ((nb-h) x h) into tmp2
+  T* t1 = (T*)w.tmp2;
+  rc = gemm_launch(prec_of<T>(), GOP_SET, nb - h, h, h, L + h, lda, Linv, ldo, t1, nb - h, w.stream);
+  if (rc) return rc;
+  // Linv21 = -Linv22 * T1
+  T* out21 = Linv + h;
+  rc = gemm_launch(prec_of<T>(), GOP_SET, nb - h, h, nb - h, Linv + h + (int64_t)h * ldo, ldo, t1, nb - h, out21, ldo,
+                   w.stream);
+  if (rc) return rc;
+  dim3 g((unsigned)cdiv(nb - h, 256), h);
+  negate_copy_kernel<T><<<g, 256, 0, w.stream>>>(out21, ldo, out21, ldo, nb - h, h, T(-1));
+  dim3 gz((unsigned)cdiv(nb, 256), nb);
+  zero_upper_kernel<T><<<gz, 256, 0, w.stream>>>(Linv, ldo, nb);
+  KAD_HIP_CHECK(hipGetLastError());
+  return KAD_OK;
+}
+
+// A12 (r x ncols, ld lda) := L11^{-1} A12 where L11 = unit lower r x r at L (ld lda).
+template <typename T>
+int trsm_left_unit_lower(const T* L, int64_t lda, int r, T* A12, int64_t ld12, int ncols, LuWork& w, int gemm_prec) {
+  if (ncols <= 0 || r <= 0) return KAD_OK;
+  T* linv = (T*)w.tinv;
+  int rc = trtri_rec<T>(L, lda, r, linv, r, w);
+  if (rc) return rc;
+  // Column chunks so that the scratch holds r x chunk.
+  const int64_t cap = (int64_t)(w.tmp_bytes / sizeof(T)) / r;
+  const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(cap, ncols));
+  for (int c = 0; c < ncols; c += chunk) {
+    const int nc = std::min(chunk, ncols - c);
+    T* tmp = (T*)w.tmp;
+    rc = gemm_launch(gemm_prec == BF16 ? F32 : gemm_prec, GOP_SET, r, nc, r, linv, r, A12 + (int64_t)c * ld12, ld12,
+                     tmp, r, w.stream);
+    if (rc) return rc;
+    KAD_HIP_CHECK(hipMemcpy2DAsync(A12 + (int64_t)c * ld12, ld12 * sizeof(T), tmp, r * sizeof(T), r * sizeof(T), nc,
+                                   hipMemcpyDeviceToDevice, w.stream));
+  }
+  return KAD_OK;
+}
+
+template <typename T>
+int laswp(T* A, int64_t lda, int c0, int c1, const int* ipiv, int k1, int k2, LuWork& w) {
+  if (c1 <= c0 || k2 <= k1) return KAD_OK;
+  laswp_kernel<T><<<(unsigned)cdiv(c1 - c0, 256), 256, 0, w.stream>>>(A, lda, c0, c1, ipiv, k1, k2);
+  KAD_HIP_CHECK(hipGetLastError());
+  return KAD_OK;
+}
+
+// Base panel: columns [c0, c1) (c1 - c0 <= PANEL_BASE), rows [c0, n).
+template <typename T>
+int panel_base(T* A, int64_t lda, int n, int c0, int c1, int* ipiv, LuWork& w) {
+  const int nb0 = (int)cdiv(n - c0, COL_THREADS);
+  if (nb0 > w.red_cap) return KAD_ERR_ARG;
+  pivot_first_kernel<T><<<nb0, COL_THREADS, 0, w.stream>>>(A, lda, n, c0, c0, c1, ipiv, w.red_val, w.red_idx,
+                                                            w.counter, w.info);
+  KAD_HIP_CHECK(hipGetLastError());
+  for (int j = c0; j < c1; ++j) {
+    const int rows = n - j - 1;
+    if (rows <= 0) break;
+    const int nbk = (int)cdiv(rows, COL_THREADS);
+    panel_step_kernel<T><<<nbk, COL_THREADS, 0, w.stream>>>(A, lda, n, j, c0, c1, ipiv, w.red_val, w.red_idx,
+                                                             w.counter, w.info);
+    KAD_HIP_CHECK(hipGetLastError());
+  }
+  return KAD_OK;
+}
+
+// Recursive panel factorisation of columns [c0, c1), rows [c0, n).
+template <typename T>
+int panel_rec(T* A, int64_t lda, int n, int c0, int c1, int* ipiv, LuWork& w) {
+  const int wdt = c1 - c0;
+  if (wdt <= PANEL_BASE) return panel_base<T>(A, lda, n, c0, c1, ipiv, w);
+  int h = wdt / 2;
+  h = (h + 15) / 16 * 16;
+  const int cm = c0 + h;
+  int rc = panel_rec<T>(A, lda, n, c0, cm, ipiv, w);
+  if (rc) return rc;
+  rc = laswp<T>(A, lda, cm, c1, ipiv, c0, cm, w);
+  if (rc) return rc;
+  rc = trsm_left_unit_lower<T>(A + c0 + (int64_t)c0 * lda, lda, h, A + c0 + (int64_t)cm * lda, lda, c1 - cm, w,
+                               prec_of<T>());
+  if (rc) return rc;
+  rc = gemm_launch(prec_of<T>(), GOP_SUB, n - cm, c1 - cm, h, A + cm + (int64_t)c0 * lda, lda,
+                   A + c0 + (int64_t)cm * lda, lda, A + cm + (int64_t)cm * lda, lda, w.stream);
+  if (rc) return rc;
+  rc = panel_rec<T>(A, lda, n, cm, c1, ipiv, w);
+  if (rc) return rc;
+  return laswp<T>(A, lda, c0, cm, ipiv, cm, c1, w);
+}
+
+// Full factorisation.  gemm_prec selects the trailing-update MFMA precision
+// (F64 for double storage; F32 or BF16 for float storage).
+template <typename T>
+int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w) {
+  for (int k0 = 0; k0 < n; k0 += nb) {
+    const int k1 = std::min(k0 + nb, n);
+    int rc = panel_rec<T>(A, lda, n, k0, k1, ipiv, w);
+    if (rc) return rc;
+    rc = laswp<T>(A, lda, 0, k0, ipiv, k0, k1, w);
+    if (rc) return rc;
+    if (k1 < n) {
+      rc = laswp<T>(A, lda, k1, n, ipiv, k0, k1, w);
+      if (rc) return rc;
+      rc = trsm_left_unit_lower<T>(A + k0 + (int64_t)k0 * lda, lda, k1 - k0, A + k0 + (int64_t)k1 * lda, lda, n - k1,
+                                   w, gemm_prec);
+      if (rc) return rc;
+      rc = gemm_launch(gemm_prec, GOP_SUB, n - k1, n - k1, k1 - k0, A + k1 + (int64_t)k0 * lda, lda,
+                       A + k0 + (int64_t)k1 * lda, lda, A + k1 + (int64_t)k1 * lda, lda, w.stream);
+      if (rc) return rc;
+    }
+  }
+  return KAD_OK;
+}
+
+template int getrf<double>(double*, int64_t, int, int*, int, int, LuWork&);
+template int getrf<float>(float*, int64_t, int, int*, int, int, LuWork&);
+
+// ---------------------------------------------------------------------------
+// Triangular solves (vector in fp64; matrix entries of type T read and widened).
+// ---------------------------------------------------------------------------
+constexpr int TB = 128;  // diagonal block of the blocked triangular solves
+
+// Solve a TB diagonal block in place: lower unit (mode 0), upper non-unit (1),
+// upper-transposed = lower non-unit U^T (2), lower-transposed = upper unit L^T (3).
+template <typename T>
+__global__ __launch_bounds__(TB) void trsv_diag_kernel(const T* A, int64_t lda, int k0, int nbk, double* x, int mode) {
+  __shared__ double xs[TB];
+  __shared__ double M[TB][TB + 1];
+  const int t = threadIdx.x;
+  for (int c = 0; c < nbk; ++c)
+    if (t < nbk) M[t][c] = (double)A[(k0 + t) + (int64_t)(k0 + c) * lda];
+  if (t < nbk) xs[t] = x[k0 + t];
+  __syncthreads();
+  if (mode == 0) {  // L y = b, unit lower
+    for (int j = 0; j < nbk; ++j) {
+      if (t > j && t < nbk) xs[t] -= M[t][j] * xs[j];
+      __syncthreads();
+    }
+  } else if (mode == 1) {  // U x = y, upper
+    for (int j = nbk - 1; j >= 0; --j) {
+      if (t == j) xs[j] = xs[j] / M[j][j];
+      __syncthreads();
+      if (t < j) xs[t] -= M[t][j] * xs[j];
+      __syncthreads();
+    }
+  } else if (mode == 2) {  // U^T z = b : lower with diag, (U^T)[t][j] = U[j][t]
+    for (int j = 0; j < nbk; ++j) {
+      if (t == j) xs[j] = xs[j] / M[j][j];
+      __syncthreads();
+      if (t > j && t < nbk) xs[t] -= M[j][t] * xs[j];
+      __syncthreads();
+    }
+  } else {  // L^T w = z : upper unit, (L^T)[t][j] = L[j][t]
+    for (int j = nbk - 1; j >= 0; --j) {
+      if (t < j) xs[t] -= M[j][t] * xs[j];
+      __syncthreads();
+    }
+  }
+  if (t < nbk) x[k0 + t] = xs[t];
+}
+
+// x[r0:r1] -= A[r0:r1, c0:c1] * x[c0:c1]   (column-major panel, threads over rows)
+template <typename T>
+__global__ __launch_bounds__(256) void gemv_panel_n_kernel(const T* A, int64_t lda, int r0, int r1, int c0, int c1,
+                                                           double* x) {
+  __shared__ double xs[TB];
+  const int nc = c1 - c0;
+  for (int c = threadIdx.x; c < nc; c += blockDim.x) xs[c] = x[c0 + c];
+  __syncthreads();
+  const int i = r0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= r1) return;
+  double s = 0.0;
+  const T* a = A + i + (int64_t)c0 * lda;
+  for (int c = 0; c < nc; ++c) s += (double)a[(int64_t)c * lda] * xs[c];
+  x[i] -= s;
+}
+
+// x[c0:c1] -= A[r0:r1, c0:c1]^T * x[r0:r1]   (one workgroup per column, reduction over rows)
+template <typename T>
+__global__ __launch_bounds__(256) void gemv_panel_t_kernel(const T* A, int64_t lda, int r0, int r1, int c0,
+                                                           double* x) {
+  __shared__ double red[256];
+  const int c = c0 + blockIdx.x;
+  const T* a = A + (int64_t)c * lda;
+  double s = 0.0;
+  for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) s += (double)a[i] * x[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) x[c] -= red[0];
+}
+
+__global__ void gather_kernel(const double* __restrict__ src, const int* __restrict__ perm, double* dst, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[perm[i]];
+}
+__global__ void scatter_kernel(const double* __restrict__ src, const int* __restrict__ perm, double* dst, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[perm[i]] = src[i];
+}
+
+// Solve op(A) x = b with the factors; b is overwritten by x.  perm[i] = original
+// row that ends at position i (P b gathers b[perm]).  work: n doubles.
+template <typename T>
+int getrs(const T* LU, int64_t lda, int n, const int* perm, double* b, int trans, double* work, hipStream_t s) {
+  const unsigned g = (unsigned)cdiv(n, 256);
+  if (!trans) {
+    gather_kernel<<<g, 256, 0, s>>>(b, perm, work, n);
+    for (int k0 = 0; k0 < n; k0 += TB) {
+      const int nbk = std::min(TB, n - k0);
+      trsv_diag_kernel<T><<<1, TB, 0, s>>>(LU, lda, k0, nbk, work, 0);
+      if (k0 + nbk < n)
+        gemv_panel_n_kernel<T><<<(unsigned)cdiv(n - k0 - nbk, 256), 256, 0, s>>>(LU, lda, k0 + nbk, n, k0, k0 + nbk,
+                                                                                  work);
+    }
+    const int last = ((n - 1) / TB) * TB;
+    for (int k0 = last; k0 >= 0; k0 -= TB) {
+      const int nbk = std::min(TB, n - k0);
+      trsv_diag_kernel<T><<<1, TB, 0, s>>>(LU, lda, k0, nbk, work, 1);
+      if (k0 > 0) gemv_panel_n_kernel<T><<<(unsigned)cdiv(k0, 256), 256, 0, s>>>(LU, lda, 0, k0, k0, k0 + nbk, work);
+    }
+    KAD_HIP_CHECK(hipMemcpyAsync(b, work, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  } else {
+    // A^T = U^T L^T P : solve U^T z = b, L^T w = z, x = P^T w.
+    KAD_HIP_CHECK(hipMemcpyAsync(work, b, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    for (int k0 = 0; k0 < n; k0 += TB) {
+      const int nbk = std::min(TB, n - k0);
+      if (k0 > 0) gemv_panel_t_kernel<T><<<nbk, 256, 0, s>>>(LU, lda, 0, k0, k0, work);
+      trsv_diag_kernel<T><<<1, TB, 0, s>>>(LU, lda, k0, nbk, work, 2);
+    }
+    const int last = ((n - 1) / TB) * TB;
+    for (int k0 = last; k0 >= 0; k0 -= TB) {
+      const int nbk = std::min(TB, n - k0);
+      if (k0 + nbk < n) gemv_panel_t_kernel<T><<<nbk, 256, 0, s>>>(LU, lda, k0 + nbk, n, k0, work);
+      trsv_diag_kernel<T><<<1, TB, 0, s>>>(LU, lda, k0, nbk, work, 3);
+    }
+    scatter_kernel<<<g, 256, 0, s>>>(work, perm, b, n);
+  }
+  KAD_HIP_CHECK(hipGetLastError());
+  return KAD_OK;
+}
+template int getrs<double>(const double*, int64_t, int, const int*, double*, int, double*, hipStream_t);
+template int getrs<float>(const float*, int64_t, int, const int*, double*, int, double*, hipStream_t);
+
+// perm from LAPACK-style ipiv (host side, O(n)).
+void ipiv_to_perm(const int* ipiv, int n, int* perm) {
+  for (int i = 0; i < n; ++i) perm[i] = i;
+  for (int i = 0; i < n; ++i) {
+    const int p = ipiv[i];
+    if (p != i) std::swap(perm[i], perm[p]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// fp64 GEMV  y = b - A x  (or y = A x when b == nullptr), A column-major n x n.
+// Two passes: partial sums per (row block, column chunk), then a fixed-order
+// reduction, so results are bitwise reproducible.
+// ---------------------------------------------------------------------------
+constexpr int GEMV_CHUNK = 1024;
+
+__global__ __launch_bounds__(256) void gemv_partial_kernel(const double* __restrict__ A, int64_t lda, int n,
+                                                           const double* __restrict__ x, double* __restrict__ part,
+                                                           int trans) {
+  const int chunk = blockIdx.y;
+  const int c0 = chunk * GEMV_CHUNK, c1 = min(n, c0 + GEMV_CHUNK);
+  if (!trans) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    const double* a = A + i;
+    for (int c = c0; c < c1; ++c) s += a[(int64_t)c * lda] * x[c];
+    part[(int64_t)chunk * n + i] = s;
+  }
+}
+
+__global__ void gemv_reduce_kernel(const double* __restrict__ part, int nchunk, int n, const double* __restrict__ b,
+                                   double* __restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int c = 0; c < nchunk; ++c) s += part[(int64_t)c * n + i];
+  y[i] = b ? b[i] - s : s;
+}
+
+// y = A^T x  (one workgroup per column)
+__global__ __launch_bounds__(256) void gemv_t_kernel(const double* __restrict__ A, int64_t lda, int n,
+                                                     const double* __restrict__ x, const double* __restrict__ b,
+                                                     double* __restrict__ y) {
+  __shared__ double red[256];
+  const int c = blockIdx.x;
+  const double* a = A + (int64_t)c * lda;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) s += a[i] * x[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) y[c] = b ? b[c] - red[0] : red[0];
+}
+
+int gemv64(const double* A, int64_t lda, int n, const double* x, const double* b, double* y, int trans, double* part,
+           hipStream_t s) {
+  if (!trans) {
+    const int nchunk = (int)cdiv(n, GEMV_CHUNK);
+    dim3 g((unsigned)cdiv(n, 256), nchunk);
+    gemv_partial_kernel<<<g, 256, 0, s>>>(A, lda, n, x, part, 0);
+    gemv_reduce_kernel<<<(unsigned)cdiv(n, 256), 256, 0, s>>>(part, nchunk, n, b, y);
+  } else {
+    gemv_t_kernel<<<n, 256, 0, s>>>(A, lda, n, x, b, y);
+  }
+  KAD_HIP_CHECK(hipGetLastError());
+  return KAD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Small vector kernels.
+// ---------------------------------------------------------------------------
+__global__ void axpy_kernel(int n, double a, const double* x, double* y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] += a * x[i];
+}
+__global__ void cast_d2f_kernel(const double* __restrict__ a, float* __restrict__ b, int64_t nel) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nel) b[i] = (float)a[i];
+}
+// Column-major n x n copy with leading dimensions, fp64 -> fp32.
+__global__ void cast_mat_d2f_kernel(const double* __restrict__ a, int64_t lda, float* __restrict__ b, int64_t ldb,
+                                    int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (i < n) b[i + (int64_t)c * ldb] = (float)a[i + (int64_t)c * lda];
+}
+// Block reduction for max |x| and for dot products (fixed order, two passes).
+__global__ __launch_bounds__(256) void absmax_partial_kernel(const double* x, int n, double* part) {
+  __shared__ double red[256];
+  double m = 0.0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) m = fmax(m, fabs(x[i]));
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+__global__ __launch_bounds__(256) void dot_partial_kernel(const double* x, const double* y, int n, double* part) {
+  __shared__ double red[256];
+  double m = 0.0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) m += x[i] * y[i];
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+}  // namespace kad

@@ -1,0 +1,329 @@
+// Device context + C-ABI for the dense part of the Newton step: LU factorisation,
+// triangular solves, and fp64 iterative refinement (classical IR and GMRES-IR)
+// on low-precision (fp32 / bf16-updated) factors.
+//
+// Reference behaviour replaced: pdgesv_ in System_of_eqs::do_newton
+// (src_par/do_newton.cpp:317-330; info != 0 -> "Jacobian is singular!") and
+// Matrice::set_lu / Matrice::solve (src/Matrice/matrice.cpp:192-250).
+#include "kad_common.hpp"
+#include "kad_ctx.hpp"
+#include "../../include/kadath_amd.h"
+#include <vector>
+#include <cmath>
+#include <algorithm>
+#include <cstring>
+
+namespace kad {
+int gemm_launch(int prec, int op, int m, int n, int k, const void* A, int64_t lda, const void* B, int64_t ldb, void* C,
+                int64_t ldc, hipStream_t s);
+template <typename T> int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w);
+template <typename T>
+int getrs(const T* LU, int64_t lda, int n, const int* perm, double* b, int trans, double* work, hipStream_t s);
+void ipiv_to_perm(const int* ipiv, int n, int* perm);
+int gemv64(const double* A, int64_t lda, int n, const double* x, const double* b, double* y, int trans, double* part,
+           hipStream_t s);
+__global__ void axpy_kernel(int n, double a, const double* x, double* y);
+__global__ void absmax_partial_kernel(const double* x, int n, double* part);
+__global__ void dot_partial_kernel(const double* x, const double* y, int n, double* part);
+__global__ void cast_mat_d2f_kernel(const double* __restrict__ a, int64_t lda, float* __restrict__ b, int64_t ldb,
+                                    int n);
+
+static constexpr int RED_BLOCKS = 256;
+
+int Ctx::ensure(void** p, size_t* cap, size_t bytes) {
+  if (*cap >= bytes) return KAD_OK;
+  if (*p) KAD_HIP_CHECK(hipFree(*p));
+  *p = nullptr;
+  *cap = 0;
+  KAD_HIP_CHECK(hipMalloc(p, bytes));
+  *cap = bytes;
+  return KAD_OK;
+}
+
+int Ctx::lu_work(int n, int nb) {
+  const int red = (int)cdiv(n, 256) + 1;
+  int rc;
+  if ((rc = ensure((void**)&lu.red_val, &cap_redv, sizeof(double) * red))) return rc;
+  if ((rc = ensure((void**)&lu.red_idx, &cap_redi, sizeof(int) * red))) return rc;
+  lu.red_cap = red;
+  if (!lu.counter) {
+    KAD_HIP_CHECK(hipMalloc((void**)&lu.counter, 64));
+    KAD_HIP_CHECK(hipMemsetAsync(lu.counter, 0, 64, stream));
+    lu.info = (int*)((char*)lu.counter + 16);
+  }
+  if ((rc = ensure(&lu.tinv, &cap_tinv, (size_t)nb * nb * 8))) return rc;
+  // GEMM scratch for L11^{-1} A12: nb x chunk (bounded at 64 MiB).
+  const size_t want = std::min<size_t>((size_t)nb * (size_t)n * 8, (size_t)64 << 20);
+  if ((rc = ensure(&lu.tmp, &lu.tmp_bytes, std::max<size_t>(want, (size_t)nb * nb * 8)))) return rc;
+  if ((rc = ensure(&lu.tmp2, &lu.tmp2_bytes, (size_t)nb * nb * 8))) return rc;
+  lu.stream = stream;
+  return KAD_OK;
+}
+
+int Ctx::vec_work(int n) {
+  int rc;
+  const size_t part = (size_t)cdiv(n, 1024) * n * sizeof(double);
+  if ((rc = ensure((void**)&gemv_part, &cap_part, std::max<size_t>(part, RED_BLOCKS * sizeof(double))))) return rc;
+  if ((rc = ensure((void**)&vwork, &cap_vwork, (size_t)6 * n * sizeof(double)))) return rc;
+  return KAD_OK;
+}
+
+double Ctx::absmax(const double* x, int n) {
+  absmax_partial_kernel<<<RED_BLOCKS, 256, 0, stream>>>(x, n, gemv_part);
+  double h[RED_BLOCKS];
+  (void)hipMemcpyAsync(h, gemv_part, sizeof(h), hipMemcpyDeviceToHost, stream);
+  (void)hipStreamSynchronize(stream);
+  double m = 0.0;
+  for (int i = 0; i < RED_BLOCKS; ++i) m = std::max(m, h[i]);
+  return m;
+}
+
+double Ctx::dot(const double* x, const double* y, int n) {
+  dot_partial_kernel<<<RED_BLOCKS, 256, 0, stream>>>(x, y, n, gemv_part);
+  double h[RED_BLOCKS];
+  (void)hipMemcpyAsync(h, gemv_part, sizeof(h), hipMemcpyDeviceToHost, stream);
+  (void)hipStreamSynchronize(stream);
+  double s = 0.0;
+  for (int i = 0; i < RED_BLOCKS; ++i) s += h[i];
+  return s;
+}
+
+int Ctx::upload(const void* host, size_t bytes, void** dev) {
+  void* p = nullptr;
+  KAD_HIP_CHECK(hipMalloc(&p, bytes < 16 ? 16 : bytes));
+  owned.push_back(p);
+  KAD_HIP_CHECK(hipMemcpy(p, host, bytes, hipMemcpyHostToDevice));
+  *dev = p;
+  return KAD_OK;
+}
+
+Ctx::~Ctx() {
+  for (auto& kv : plans) delete kv.second;
+  for (void* p : owned) (void)hipFree(p);
+  void* bufs[] = {lu.red_val, lu.red_idx, lu.counter, lu.tinv, lu.tmp, lu.tmp2, gemv_part, vwork, krylov};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  if (own_stream && stream) (void)hipStreamDestroy(stream);
+}
+
+}  // namespace kad
+
+using kad::Ctx;
+
+extern "C" {
+
+int kad_ctx_create(kad_ctx** out, int device) {
+  if (!out) return KAD_ERR_ARG;
+  KAD_HIP_CHECK(hipSetDevice(device));
+  Ctx* c = new Ctx();
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return KAD_ERR_HIP;
+  }
+  c->own_stream = true;
+  *out = reinterpret_cast<kad_ctx*>(c);
+  return KAD_OK;
+}
+
+int kad_ctx_destroy(kad_ctx* ctx) {
+  if (!ctx) return KAD_ERR_ARG;
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  (void)hipStreamSynchronize(c->stream);
+  delete c;
+  return KAD_OK;
+}
+
+void* kad_ctx_stream(kad_ctx* ctx) { return ctx ? (void*)reinterpret_cast<Ctx*>(ctx)->stream : nullptr; }
+
+int kad_sync(kad_ctx* ctx) {
+  if (!ctx) return KAD_ERR_ARG;
+  KAD_HIP_CHECK(hipStreamSynchronize(reinterpret_cast<Ctx*>(ctx)->stream));
+  return KAD_OK;
+}
+
+int kad_gemm(kad_ctx* ctx, int prec, int op, int m, int n, int k, const void* A, int64_t lda, const void* B,
+             int64_t ldb, void* C, int64_t ldc) {
+  if (!ctx || prec < 0 || prec > 2 || (op != 0 && op != 1)) return KAD_ERR_ARG;
+  if (m < 0 || n < 0 || k < 0 || lda < std::max(1, m) || ldb < std::max(1, k) || ldc < std::max(1, m))
+    return KAD_ERR_ARG;
+  return kad::gemm_launch(prec, op, m, n, k, A, lda, B, ldb, C, ldc, reinterpret_cast<Ctx*>(ctx)->stream);
+}
+
+// In-place LU.  prec: 0 = fp64 storage/fp64 MFMA, 1 = fp32/fp32 MFMA, 2 = fp32 storage,
+// bf16 MFMA trailing update with fp32 accumulation.  ipiv (device, n ints) receives
+// 0-based LAPACK-style interchanges; perm (device, n ints) the composed row order.
+// Returns 0, a negative error, or i > 0 when U(i-1,i-1) is exactly zero (LAPACK info).
+int kad_lu_factor(kad_ctx* ctx, int prec, int n, void* A, int64_t lda, int* ipiv, int* perm, int nb) {
+  if (!ctx || !A || !ipiv || n <= 0 || lda < n || prec < 0 || prec > 2) return KAD_ERR_ARG;
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (nb <= 0) nb = (prec == kad::BF16) ? 512 : 256;
+  if (nb % 16 != 0) return KAD_ERR_ARG;
+  int rc = c->lu_work(n, nb);
+  if (rc) return rc;
+  KAD_HIP_CHECK(hipMemsetAsync(c->lu.info, 0, sizeof(int), c->stream));
+  if (prec == kad::F64)
+    rc = kad::getrf<double>((double*)A, lda, n, ipiv, nb, kad::F64, c->lu);
+  else
+    rc = kad::getrf<float>((float*)A, lda, n, ipiv, nb, prec, c->lu);
+  if (rc) return rc;
+  int info = 0;
+  KAD_HIP_CHECK(hipMemcpyAsync(&info, c->lu.info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  if (perm) {
+    std::vector<int> hp(n), hperm(n);
+    KAD_HIP_CHECK(hipMemcpyAsync(hp.data(), ipiv, n * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    KAD_HIP_CHECK(hipStreamSynchronize(c->stream));
+    kad::ipiv_to_perm(hp.data(), n, hperm.data());
+    KAD_HIP_CHECK(hipMemcpyAsync(perm, hperm.data(), n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  }
+  KAD_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return info;
+}
+
+// Solve op(A) x = b in place (b: device fp64, n).  trans = 1 solves A^T x = b
+// (the adjoint solve J^T lambda = dQ/du reuses the same factors).
+int kad_lu_solve(kad_ctx* ctx, int prec, int n, const void* LU, int64_t lda, const int* perm, double* b, int trans) {
+  if (!ctx || !LU || !perm || !b || n <= 0 || lda < n) return KAD_ERR_ARG;
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  int rc = c->vec_work(n);
+  if (rc) return rc;
+  if (prec == kad::F64)
+    return kad::getrs<double>((const double*)LU, lda, n, perm, b, trans, c->vwork, c->stream);
+  return kad::getrs<float>((const float*)LU, lda, n, perm, b, trans, c->vwork, c->stream);
+}
+
+int kad_gemv64(kad_ctx* ctx, int n, const double* A, int64_t lda, const double* x, const double* b, double* y,
+               int trans) {
+  if (!ctx || n <= 0 || lda < n) return KAD_ERR_ARG;
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  int rc = c->vec_work(n);
+  if (rc) return rc;
+  return kad::gemv64(A, lda, n, x, b, y, trans, c->gemv_part, c->stream);
+}
+
+// fp64 n x n column-major -> fp32 copy (device).
+int kad_cast_f64_to_f32(kad_ctx* ctx, int n, const double* A, int64_t lda, float* B, int64_t ldb) {
+  if (!ctx || n <= 0 || lda < n || ldb < n) return KAD_ERR_ARG;
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  dim3 g((unsigned)kad::cdiv(n, 256), n);
+  kad::cast_mat_d2f_kernel<<<g, 256, 0, c->stream>>>(A, lda, B, ldb, n);
+  KAD_HIP_CHECK(hipGetLastError());
+  return KAD_OK;
+}
+
+// Mixed-precision refinement: x = A^{-1} b with A given in fp64 (A64, for the
+// residual) and factored in prec_lo (LUlo, perm).  mode 0: classical IR;
+// mode 1: GMRES-IR (left preconditioned by the low-precision factors, CGS2,
+// restart m).  Stops when ||b - A x||_inf <= tol * ||b||_inf.  trans = 1 solves A^T x = b.
+// iters[0] = outer steps, iters[1] = total inner GMRES steps; resid = final relative residual.
+int kad_ir_solve(kad_ctx* ctx, int n, const double* A64, int64_t lda, int prec_lo, const void* LUlo, int64_t ldlu,
+                 const int* perm, const double* b, double* x, double tol, int max_outer, int mode, int trans,
+                 int* iters, double* resid) {
+  if (!ctx || n <= 0 || !A64 || !LUlo || !perm || !b || !x) return KAD_ERR_ARG;
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  int rc = c->vec_work(n);
+  if (rc) return rc;
+  hipStream_t s = c->stream;
+  const unsigned g = (unsigned)kad::cdiv(n, 256);
+  double* r = c->vwork + (size_t)2 * n;  // vwork[0:n] is getrs scratch
+  double* d = c->vwork + (size_t)3 * n;
+  auto solve_lo = [&](double* v) {
+    if (prec_lo == kad::F64) return kad::getrs<double>((const double*)LUlo, ldlu, n, perm, v, trans, c->vwork, s);
+    return kad::getrs<float>((const float*)LUlo, ldlu, n, perm, v, trans, c->vwork, s);
+  };
+  const double bnorm = c->absmax(b, n);
+  KAD_HIP_CHECK(hipMemsetAsync(x, 0, n * sizeof(double), s));
+  KAD_HIP_CHECK(hipMemcpyAsync(r, b, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  int outer = 0, inner_total = 0;
+  double rel = (bnorm == 0.0) ? 0.0 : 1.0;
+  const int m = 30;
+  std::vector<double> H, cs, sn, gvec, y;
+  double* V = nullptr;
+  double* hbuf = nullptr;
+  if (mode == 1) {
+    rc = c->ensure((void**)&c->krylov, &c->cap_krylov, (size_t)(m + 1) * n * sizeof(double) + 64 * sizeof(double));
+    if (rc) return rc;
+    V = c->krylov;
+    hbuf = c->krylov + (size_t)(m + 1) * n;
+  }
+  while (rel > tol && outer < max_outer) {
+    if (mode == 0) {
+      KAD_HIP_CHECK(hipMemcpyAsync(d, r, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+      if ((rc = solve_lo(d))) return rc;
+      kad::axpy_kernel<<<g, 256, 0, s>>>(n, 1.0, d, x);
+    } else {
+      // GMRES on M^{-1} A d = M^{-1} r, M = low-precision LU.
+      KAD_HIP_CHECK(hipMemcpyAsync(V, r, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+      if ((rc = solve_lo(V))) return rc;
+      const double beta = std::sqrt(c->dot(V, V, n));
+      if (beta == 0.0) break;
+      kad::axpy_kernel<<<g, 256, 0, s>>>(n, 1.0 / beta - 1.0, V, V);
+      H.assign((size_t)(m + 1) * m, 0.0);
+      cs.assign(m, 0.0);
+      sn.assign(m, 0.0);
+      gvec.assign(m + 1, 0.0);
+      gvec[0] = beta;
+      int kdim = 0;
+      const double inner_tol = 1e-10;
+      for (int j = 0; j < m; ++j) {
+        double* w = V + (size_t)(j + 1) * n;
+        if (trans) {
+          if ((rc = kad::gemv64(A64, lda, n, V + (size_t)j * n, nullptr, w, 1, c->gemv_part, s))) return rc;
+        } else {
+          if ((rc = kad::gemv64(A64, lda, n, V + (size_t)j * n, nullptr, w, 0, c->gemv_part, s))) return rc;
+        }
+        if ((rc = solve_lo(w))) return rc;
+        // CGS2 against V[0..j]
+        for (int pass = 0; pass < 2; ++pass) {
+          for (int i = 0; i <= j; ++i) {
+            const double h = c->dot(V + (size_t)i * n, w, n);
+            H[(size_t)i * m + j] += h;
+            kad::axpy_kernel<<<g, 256, 0, s>>>(n, -h, V + (size_t)i * n, w);
+          }
+        }
+        const double hn = std::sqrt(c->dot(w, w, n));
+        H[(size_t)(j + 1) * m + j] = hn;
+        if (hn > 0.0) kad::axpy_kernel<<<g, 256, 0, s>>>(n, 1.0 / hn - 1.0, w, w);
+        // Givens
+        for (int i = 0; i < j; ++i) {
+          const double a = H[(size_t)i * m + j], bb = H[(size_t)(i + 1) * m + j];
+          H[(size_t)i * m + j] = cs[i] * a + sn[i] * bb;
+          H[(size_t)(i + 1) * m + j] = -sn[i] * a + cs[i] * bb;
+        }
+        const double a = H[(size_t)j * m + j], bb = H[(size_t)(j + 1) * m + j];
+        const double rr = std::hypot(a, bb);
+        cs[j] = (rr == 0.0) ? 1.0 : a / rr;
+        sn[j] = (rr == 0.0) ? 0.0 : bb / rr;
+        H[(size_t)j * m + j] = rr;
+        H[(size_t)(j + 1) * m + j] = 0.0;
+        gvec[j + 1] = -sn[j] * gvec[j];
+        gvec[j] = cs[j] * gvec[j];
+        kdim = j + 1;
+        ++inner_total;
+        if (std::fabs(gvec[j + 1]) <= inner_tol * beta || hn == 0.0) break;
+      }
+      y.assign(kdim, 0.0);
+      for (int i = kdim - 1; i >= 0; --i) {
+        double sacc = gvec[i];
+        for (int l = i + 1; l < kdim; ++l) sacc -= H[(size_t)i * m + l] * y[l];
+        y[i] = sacc / H[(size_t)i * m + i];
+      }
+      for (int i = 0; i < kdim; ++i) kad::axpy_kernel<<<g, 256, 0, s>>>(n, y[i], V + (size_t)i * n, x);
+      (void)hbuf;
+    }
+    // r = b - op(A) x in fp64
+    if ((rc = kad::gemv64(A64, lda, n, x, b, r, trans, c->gemv_part, s))) return rc;
+    const double rn = c->absmax(r, n);
+    rel = (bnorm == 0.0) ? 0.0 : rn / bnorm;
+    ++outer;
+  }
+  if (iters) {
+    iters[0] = outer;
+    iters[1] = inner_total;
+  }
+  if (resid) *resid = rel;
+  KAD_HIP_CHECK(hipStreamSynchronize(s));
+  return (rel <= tol) ? KAD_OK : KAD_ERR_NOCONV;
+}
+
+}  // extern "C"

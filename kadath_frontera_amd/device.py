@@ -1,0 +1,162 @@
+"""Python handle on a device context of libkadath_amd (torch tensors in, HIP kernels out).
+
+torch supplies device memory; all work runs on the context's own HIP stream,
+exposed to torch as an ExternalStream so callers can time it with torch.cuda.Event
+or order torch work after it.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from ._lib import lib, check, c_vp
+
+PREC = {"f64": 0, "f32": 1, "bf16": 2}
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _iarr(a):
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.int32))
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+class Context:
+    def __init__(self, device=0):
+        if not torch.cuda.is_available():
+            raise RuntimeError("kadath_amd.Context needs a GPU (torch.cuda.is_available() is False)")
+        self.device = device
+        h = c_vp()
+        check(lib.kad_ctx_create(ctypes.byref(h), device), "kad_ctx_create")
+        self.h = h
+        self.stream = torch.cuda.ExternalStream(lib.kad_ctx_stream(h), device=torch.device("cuda", device))
+
+    def close(self):
+        if self.h:
+            lib.kad_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ordering helpers: torch work on the current stream must finish before ours
+    def _enter(self):
+        torch.cuda.current_stream(self.device).synchronize()
+
+    def sync(self):
+        check(lib.kad_sync(self.h), "kad_sync")
+
+    # ---- dense --------------------------------------------------------------
+    def gemm(self, prec, op, A, B, C):
+        """C -= A@B (op 0) or C = A@B (op 1); column-major views: pass tensors whose
+        transposes are contiguous, i.e. A.T.is_contiguous()."""
+        m, k = A.shape
+        k2, n = B.shape
+        assert k == k2 and C.shape == (m, n)
+        self._enter()
+        check(lib.kad_gemm(self.h, PREC[prec], op, m, n, k, _ptr(A), A.stride(1), _ptr(B), B.stride(1), _ptr(C),
+                           C.stride(1)), "kad_gemm")
+
+    def lu_factor(self, prec, A, nb=0):
+        """In-place LU of the column-major n x n A (A.T contiguous, dtype f64 or f32)."""
+        n = A.shape[0]
+        ipiv = torch.empty(n, dtype=torch.int32, device=A.device)
+        perm = torch.empty(n, dtype=torch.int32, device=A.device)
+        self._enter()
+        info = check(lib.kad_lu_factor(self.h, PREC[prec], n, _ptr(A), A.stride(1), _ptr(ipiv), _ptr(perm), nb),
+                     "kad_lu_factor")
+        return ipiv, perm, info
+
+    def lu_solve(self, prec, LU, perm, b, trans=0):
+        n = LU.shape[0]
+        self._enter()
+        check(lib.kad_lu_solve(self.h, PREC[prec], n, _ptr(LU), LU.stride(1), _ptr(perm), _ptr(b), trans),
+              "kad_lu_solve")
+        self.sync()
+        return b
+
+    def gemv64(self, A, x, b=None, trans=0):
+        n = A.shape[0]
+        y = torch.empty(n, dtype=torch.float64, device=A.device)
+        self._enter()
+        check(lib.kad_gemv64(self.h, n, _ptr(A), A.stride(1), _ptr(x), _ptr(b), _ptr(y), trans), "kad_gemv64")
+        self.sync()
+        return y
+
+    def cast_f32(self, A, out):
+        n = A.shape[0]
+        self._enter()
+        check(lib.kad_cast_f64_to_f32(self.h, n, _ptr(A), A.stride(1), _ptr(out), out.stride(1)), "kad_cast")
+
+    def ir_solve(self, A64, prec_lo, LUlo, perm, b, tol=1e-13, max_outer=20, mode=0, trans=0):
+        n = A64.shape[0]
+        x = torch.empty(n, dtype=torch.float64, device=A64.device)
+        iters = (ctypes.c_int * 2)()
+        resid = ctypes.c_double()
+        self._enter()
+        rc = lib.kad_ir_solve(self.h, n, _ptr(A64), A64.stride(1), PREC[prec_lo], _ptr(LUlo), LUlo.stride(1),
+                              _ptr(perm), _ptr(b), _ptr(x), tol, max_outer, mode, trans, iters, ctypes.byref(resid))
+        if rc < 0 and rc != -4:
+            check(rc, "kad_ir_solve")
+        return x, (iters[0], iters[1]), resid.value, rc == 0
+
+    # ---- spectral transforms ------------------------------------------------
+    def coef_3d(self, conf, base_r, base_t, base_p):
+        """conf: (nbatch, np, nt, nr) contiguous (r fastest) -> coef (nbatch, np+2, nt, nr)."""
+        nb, np_, nt, nr = conf.shape
+        out = torch.empty((nb, np_ + 2, nt, nr), dtype=torch.float64, device=conf.device)
+        br, pbr = _iarr(base_r)
+        bt, pbt = _iarr(base_t)
+        self._enter()
+        check(lib.kad_coef_3d(self.h, nr, nt, np_, pbr, pbt, int(base_p), nb, _ptr(conf), _ptr(out)), "kad_coef_3d")
+        self.sync()
+        return out
+
+    def coef_i_3d(self, coef, base_r, base_t, base_p):
+        nb, npc, nt, nr = coef.shape
+        out = torch.empty((nb, npc - 2, nt, nr), dtype=torch.float64, device=coef.device)
+        br, pbr = _iarr(base_r)
+        bt, pbt = _iarr(base_t)
+        self._enter()
+        check(lib.kad_coef_i_3d(self.h, nr, nt, npc - 2, pbr, pbt, int(base_p), nb, _ptr(coef), _ptr(out)),
+              "kad_coef_i_3d")
+        self.sync()
+        return out
+
+    def coef_1d(self, base, conf):
+        L, n = conf.shape
+        nout = n + 2 if base == 4 else n
+        out = torch.empty((L, nout), dtype=torch.float64, device=conf.device)
+        self._enter()
+        check(lib.kad_coef_1d(self.h, base, n, L, _ptr(conf), _ptr(out)), "kad_coef_1d")
+        self.sync()
+        return out
+
+    def coef_i_1d(self, base, coef, nconf):
+        L, _ = coef.shape
+        out = torch.empty((L, nconf), dtype=torch.float64, device=coef.device)
+        self._enter()
+        check(lib.kad_coef_i_1d(self.h, base, nconf, L, _ptr(coef), _ptr(out)), "kad_coef_i_1d")
+        self.sync()
+        return out
+
+
+def host_transform_matrix(kind, base, nconf):
+    """Analysis (kind 0) or synthesis (kind 1) matrix of a 1-D basis, from the C++ host tables."""
+    nc = nconf + 2 if base == 4 else nconf
+    buf = (ctypes.c_double * (nc * nconf))()
+    n = check(lib.kad_host_transform_matrix(kind, base, nconf, buf, nc * nconf), "kad_host_transform_matrix")
+    a = np.frombuffer(buf, dtype=np.float64, count=n).copy()
+    return a.reshape((nc, nconf) if kind == 0 else (nconf, nc))
+
+
+def host_op1d_matrix(op, base, n):
+    buf = (ctypes.c_double * (n * n))()
+    ob = ctypes.c_int()
+    check(lib.kad_host_op1d_matrix(op, base, n, buf, n * n, ctypes.byref(ob)), "kad_host_op1d_matrix")
+    return np.frombuffer(buf, dtype=np.float64).copy().reshape(n, n), ob.value

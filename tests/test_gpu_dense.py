@@ -1,0 +1,124 @@
+"""GPU: MFMA GEMM, LU with partial pivoting, triangular solves and mixed-precision
+refinement through the C-ABI, against LAPACK (scipy) and float64 references.
+
+Tolerances: f64 paths 1e-12 relative (backward-stable LU: ||PA - LU|| / ||A|| ~ n eps);
+f32 GEMM 1e-5 relative; bf16 GEMM 1e-2 relative (8-bit mantissa operands, fp32
+accumulation); refined solves reach ||b - Ax||_inf <= 1e-13 ||b||_inf.
+"""
+import numpy as np
+import pytest
+import scipy.linalg
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def colmajor(a_np, dtype, dev="cuda"):
+    """Device tensor holding a_np in column-major order; returns the (m, n) view."""
+    t = torch.tensor(np.ascontiguousarray(a_np.T), dtype=dtype, device=dev)
+    return t.T
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-13), ("f32", 1e-5), ("bf16", 2e-2)])
+@pytest.mark.parametrize("m,n,k", [(128, 128, 64), (300, 257, 129), (1, 5, 3), (517, 200, 512)])
+def test_gemm(ctx, prec, tol, m, n, k):
+    print(f"gemm {prec} {m}x{n}x{k}", flush=True)
+    rng = np.random.default_rng(m * 7 + n + k)
+    A = rng.standard_normal((m, k))
+    B = rng.standard_normal((k, n))
+    C = rng.standard_normal((m, n))
+    dt = torch.float64 if prec == "f64" else torch.float32
+    if prec == "bf16":
+        # reference with operands rounded to bf16, fp64 accumulation
+        Ar = torch.tensor(A).to(torch.bfloat16).double().numpy()
+        Br = torch.tensor(B).to(torch.bfloat16).double().numpy()
+    elif prec == "f32":
+        Ar, Br = A.astype(np.float32).astype(np.float64), B.astype(np.float32).astype(np.float64)
+    else:
+        Ar, Br = A, B
+    for op in (0, 1):
+        dA, dB, dC = colmajor(A, dt), colmajor(B, dt), colmajor(C, dt)
+        ctx.gemm(prec, op, dA, dB, dC)
+        ctx.sync()
+        got = dC.double().cpu().numpy()
+        ref = (C - Ar @ Br) if op == 0 else Ar @ Br
+        scale = np.abs(Ar) @ np.abs(Br) + (np.abs(C) if op == 0 else 0)
+        err = np.max(np.abs(got - ref) / scale)
+        assert err < tol, (prec, op, err)
+
+
+def _spd_like(n, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal((n, n)) + 0.5 * np.sqrt(n) * np.eye(n)
+
+
+@pytest.mark.parametrize("n,nb", [(64, 32), (257, 64), (1000, 256), (2100, 256)])
+def test_lu_f64_matches_lapack(ctx, n, nb):
+    print(f"lu f64 n={n} nb={nb}", flush=True)
+    A = np.random.default_rng(n).standard_normal((n, n))
+    dA = colmajor(A, torch.float64)
+    ipiv, perm, info = ctx.lu_factor("f64", dA, nb=nb)
+    assert info == 0
+    lu_ref, piv_ref = scipy.linalg.lu_factor(A)
+    got = dA.cpu().numpy()
+    # same pivot sequence as LAPACK dgetrf (partial pivoting, first max)
+    assert np.array_equal(ipiv.cpu().numpy(), piv_ref)
+    np.testing.assert_allclose(got, lu_ref, atol=1e-10 * np.abs(lu_ref).max(), rtol=1e-9)
+    # backward error ||P A - L U|| / ||A||
+    L = np.tril(got, -1) + np.eye(n)
+    U = np.triu(got)
+    p = perm.cpu().numpy()
+    be = np.abs(A[p] - L @ U).max() / np.abs(A).max()
+    assert be < 1e-13 * n
+
+
+@pytest.mark.parametrize("trans", [0, 1])
+def test_lu_solve_f64(ctx, trans):
+    n = 777
+    A = _spd_like(n, 3)
+    b = np.random.default_rng(4).standard_normal(n)
+    dA = colmajor(A, torch.float64)
+    ipiv, perm, info = ctx.lu_factor("f64", dA)
+    x = ctx.lu_solve("f64", dA, perm, torch.tensor(b, device="cuda"), trans=trans).cpu().numpy()
+    ref = np.linalg.solve(A.T if trans else A, b)
+    np.testing.assert_allclose(x, ref, rtol=1e-10, atol=1e-12)
+
+
+def test_singular_reports_info(ctx):
+    n = 96
+    A = np.random.default_rng(5).standard_normal((n, n))
+    A[:, 40] = 0.0  # exactly singular: zero column
+    dA = colmajor(A, torch.float64)
+    ipiv, perm, info = ctx.lu_factor("f64", dA, nb=32)
+    assert info == 41
+
+
+def test_gemv64(ctx):
+    n = 1500
+    rng = np.random.default_rng(6)
+    A = rng.standard_normal((n, n))
+    x = rng.standard_normal(n)
+    b = rng.standard_normal(n)
+    dA = colmajor(A, torch.float64)
+    y = ctx.gemv64(dA, torch.tensor(x, device="cuda"), torch.tensor(b, device="cuda")).cpu().numpy()
+    np.testing.assert_allclose(y, b - A @ x, rtol=1e-12, atol=1e-11)
+    yt = ctx.gemv64(dA, torch.tensor(x, device="cuda"), None, trans=1).cpu().numpy()
+    np.testing.assert_allclose(yt, A.T @ x, rtol=1e-12, atol=1e-11)
+
+
+@pytest.mark.parametrize("prec,mode", [("f32", 0), ("f32", 1), ("bf16", 1)])
+@pytest.mark.parametrize("trans", [0, 1])
+def test_mixed_precision_refinement(ctx, prec, mode, trans):
+    n = 1200
+    print(f"ir {prec} mode={mode} trans={trans}", flush=True)
+    A = _spd_like(n, 8)
+    b = np.random.default_rng(9).standard_normal(n)
+    dA64 = colmajor(A, torch.float64)
+    dlo = colmajor(A, torch.float32)
+    ipiv, perm, info = ctx.lu_factor(prec, dlo)
+    assert info == 0
+    x, iters, resid, ok = ctx.ir_solve(dA64, prec, dlo, perm, torch.tensor(b, device="cuda"), tol=1e-13,
+                                       max_outer=30, mode=mode, trans=trans)
+    assert ok, (iters, resid)
+    ref = np.linalg.solve(A.T if trans else A, b)
+    np.testing.assert_allclose(x.cpu().numpy(), ref, rtol=1e-9, atol=1e-11)
