@@ -9,10 +9,9 @@
 //
 // Tile: 128x128 per workgroup, 4 waves in a 2x2 grid, each wave 64x64 = 4x4
 // MFMA 16x16 tiles.  K is staged through LDS with register double buffering.
-// The MFMA is issued with the roles of A and B swapped (D' = B^T A^T) so that
-// each lane's accumulator registers hold 4 consecutive COLUMNS of one ROW ...
-// i.e. 16 lanes cover 16 consecutive rows of one column, which makes the
-// column-major epilogue coalesced.
+// The MFMA is issued with the roles of A and B swapped (D' = B^T A^T = C^T), so
+// lanes 0..15 of an accumulator register hold 16 consecutive ROWS of one column
+// of C: the column-major epilogue is coalesced.
 #include "kad_common.hpp"
 
 namespace kad {
@@ -139,7 +138,9 @@ __global__ __launch_bounds__(GNT) void gemm_k4_kernel(int m, int n, int k,
     __syncthreads();
   }
 
-  // Epilogue: lane holds rows (lane&15) of tile i, columns 4*(lane>>4)+r of tile j.
+  // Epilogue: lane holds row (lane&15) of tile i and 4 columns of tile j:
+  // f32 16x16x4 accumulators are D'[4*(lane>>4)+r][lane&15], f64 16x16x4 ones
+  // D'[4*r+(lane>>4)][lane&15] (measured lane layouts of the two instructions).
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int gr = row0 + wm + i * 16 + (lane & 15);
@@ -148,7 +149,7 @@ __global__ __launch_bounds__(GNT) void gemm_k4_kernel(int m, int n, int k,
     for (int j = 0; j < 4; ++j) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int gc = col0 + wn + j * 16 + 4 * (lane >> 4) + r;
+        const int gc = col0 + wn + j * 16 + (sizeof(T) == 8 ? 4 * r + (lane >> 4) : 4 * (lane >> 4) + r);
         if (gc < n) {
           T* p = C + gr + (int64_t)gc * ldc;
           if (OP == OP_SUB) *p -= acc[i][j][r];
