@@ -127,6 +127,9 @@ __global__ __launch_bounds__(COL_THREADS) void pivot_first_kernel(T* A, int64_t 
 // One elimination step of column j inside the base panel [c0, c1):
 // rows i in (j, n): l = a_ij / a_jj ; a_i,c -= l a_j,c for c in (j, c1);
 // then the pivot of column j+1 is searched and applied (if j+1 < c1).
+// Block = 64 rows x 4 column groups (256 threads); the multiplier l_i is read
+// into LDS before any thread writes it back scaled.
+constexpr int STEP_ROWS = 64;
 template <typename T>
 __global__ __launch_bounds__(COL_THREADS) void panel_step_kernel(T* A, int64_t lda, int n, int j, int c0, int c1,
                                                                  int* ipiv, double* red_val, int* red_idx,
@@ -134,19 +137,26 @@ __global__ __launch_bounds__(COL_THREADS) void panel_step_kernel(T* A, int64_t l
   __shared__ double sv[COL_THREADS];
   __shared__ int si[COL_THREADS];
   __shared__ T urow[PANEL_BASE];
+  __shared__ T lcol[STEP_ROWS];
   const int w = c1 - j;  // columns j..c1-1
+  const int tx = threadIdx.x % STEP_ROWS, ty = threadIdx.x / STEP_ROWS;
   for (int c = threadIdx.x; c < w; c += blockDim.x) urow[c] = A[j + (int64_t)(j + c) * lda];
+  const int i = j + 1 + blockIdx.x * STEP_ROWS + tx;
   __syncthreads();
   const T piv = urow[0];
-  const int i = j + 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (ty == 0) {
+    T l = (i < n) ? A[i + (int64_t)j * lda] : T(0);
+    if (piv != T(0)) l = l / piv;
+    lcol[tx] = l;
+    if (i < n) A[i + (int64_t)j * lda] = l;
+  }
+  __syncthreads();
   double v = -1.0;
   int id = -1;
   if (i < n) {
+    const T l = lcol[tx];
     T* ai = A + i;
-    T l = ai[(int64_t)j * lda];
-    if (piv != T(0)) l = l / piv;
-    ai[(int64_t)j * lda] = l;
-    for (int c = 1; c < w; ++c) {
+    for (int c = 1 + ty; c < w; c += 4) {
       T x = ai[(int64_t)(j + c) * lda];
       x -= l * urow[c];
       ai[(int64_t)(j + c) * lda] = x;
@@ -184,25 +194,40 @@ __global__ void laswp_kernel(T* A, int64_t lda, int c0, int c1, const int* __res
 }
 
 // Inverse of a unit-lower nb x nb block (nb <= 64), written as a full square
-// (zeros above, ones on the diagonal) into Linv (ld ldo).  One workgroup.
+// (zeros above, ones on the diagonal) into Linv (ld ldo).  One workgroup of 256:
+// X = I, then for j = 0..nb-1 every (i > j, c <= j) entry does
+// X[i][c] -= L[i][j] X[j][c]  (row-oriented forward substitution, one barrier per j).
 template <typename T>
-__global__ __launch_bounds__(64) void trtri_small_kernel(const T* L, int64_t lda, int nb, T* Linv, int64_t ldo) {
+__global__ __launch_bounds__(256) void trtri_small_kernel(const T* L, int64_t lda, int nb, T* Linv, int64_t ldo) {
   __shared__ double Ls[64][65];
+  __shared__ double X[64][65];
   const int t = threadIdx.x;
-  for (int c = 0; c < nb; ++c)
-    if (t < nb) Ls[t][c] = (double)L[t + (int64_t)c * lda];
-  __syncthreads();
-  if (t < nb) {
-    // column t of the inverse: solve L x = e_t
-    double x[64];
-    for (int i = 0; i < nb; ++i) x[i] = 0.0;
-    x[t] = 1.0;
-    for (int jj = t; jj < nb; ++jj) {
-      const double xj = x[jj];
-      for (int i = jj + 1; i < nb; ++i) x[i] -= Ls[i][jj] * xj;
-    }
-    for (int i = 0; i < nb; ++i) Linv[i + (int64_t)t * ldo] = (T)x[i];
+  for (int e = t; e < nb * nb; e += 256) {
+    const int i = e % nb, c = e / nb;
+    Ls[i][c] = (double)L[i + (int64_t)c * lda];
+    X[i][c] = (i == c) ? 1.0 : 0.0;
   }
+  __syncthreads();
+  for (int j = 0; j < nb - 1; ++j) {
+    // rows i in (j, nb), columns c in [0, j]
+    const int nrow = nb - 1 - j, ncol = j + 1;
+    for (int e = t; e < nrow * ncol; e += 256) {
+      const int i = j + 1 + e % nrow, c = e / nrow;
+      X[i][c] -= Ls[i][j] * X[j][c];
+    }
+    __syncthreads();
+  }
+  for (int e = t; e < nb * nb; e += 256) {
+    const int i = e % nb, c = e / nb;
+    Linv[i + (int64_t)c * ldo] = (T)X[i][c];
+  }
+}
+
+template <typename T>
+__global__ void copy2d_kernel(const T* src, int64_t lds, T* dst, int64_t ldd, int m, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (i < m && c < n) dst[i + (int64_t)c * ldd] = src[i + (int64_t)c * lds];
 }
 
 template <typename T>
@@ -225,7 +250,7 @@ template <typename T> constexpr int prec_of() { return sizeof(T) == 8 ? F64 : F3
 template <typename T>
 int trtri_rec(const T* L, int64_t lda, int nb, T* Linv, int64_t ldo, LuWork& w) {
   if (nb <= 64) {
-    trtri_small_kernel<T><<<1, 64, 0, w.stream>>>(L, lda, nb, Linv, ldo);
+    trtri_small_kernel<T><<<1, 256, 0, w.stream>>>(L, lda, nb, Linv, ldo);
     KAD_HIP_CHECK(hipGetLastError());
     return KAD_OK;
   }
@@ -267,8 +292,9 @@ int trsm_left_unit_lower(const T* L, int64_t lda, int r, T* A12, int64_t ld12, i
     rc = gemm_launch(gemm_prec == BF16 ? F32 : gemm_prec, GOP_SET, r, nc, r, linv, r, A12 + (int64_t)c * ld12, ld12,
                      tmp, r, w.stream);
     if (rc) return rc;
-    KAD_HIP_CHECK(hipMemcpy2DAsync(A12 + (int64_t)c * ld12, ld12 * sizeof(T), tmp, r * sizeof(T), r * sizeof(T), nc,
-                                   hipMemcpyDeviceToDevice, w.stream));
+    dim3 g((unsigned)cdiv(r, 256), nc);
+    copy2d_kernel<T><<<g, 256, 0, w.stream>>>(tmp, r, A12 + (int64_t)c * ld12, ld12, r, nc);
+    KAD_HIP_CHECK(hipGetLastError());
   }
   return KAD_OK;
 }
@@ -292,7 +318,8 @@ int panel_base(T* A, int64_t lda, int n, int c0, int c1, int* ipiv, LuWork& w) {
   for (int j = c0; j < c1; ++j) {
     const int rows = n - j - 1;
     if (rows <= 0) break;
-    const int nbk = (int)cdiv(rows, COL_THREADS);
+    const int nbk = (int)cdiv(rows, STEP_ROWS);
+    if (nbk > w.red_cap) return KAD_ERR_ARG;
     panel_step_kernel<T><<<nbk, COL_THREADS, 0, w.stream>>>(A, lda, n, j, c0, c1, ipiv, w.red_val, w.red_idx,
                                                              w.counter, w.info);
     KAD_HIP_CHECK(hipGetLastError());

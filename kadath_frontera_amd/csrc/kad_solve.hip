@@ -41,7 +41,7 @@ int Ctx::ensure(void** p, size_t* cap, size_t bytes) {
 }
 
 int Ctx::lu_work(int n, int nb) {
-  const int red = (int)cdiv(n, 256) + 1;
+  const int red = (int)cdiv(n, 64) + 1;
   int rc;
   if ((rc = ensure((void**)&lu.red_val, &cap_redv, sizeof(double) * red))) return rc;
   if ((rc = ensure((void**)&lu.red_idx, &cap_redi, sizeof(int) * red))) return rc;
