@@ -27,6 +27,7 @@ extern "C" {
 #define KAD_ERR_HIP (-2)
 #define KAD_ERR_ALLOC (-3)
 #define KAD_ERR_NOCONV (-4)
+#define KAD_ERR_SINGULAR (-5)
 
 /* Factor precisions. */
 #define KAD_PREC_F64 0  /* fp64 storage, fp64 MFMA                                  */
@@ -105,6 +106,45 @@ int kad_host_transform_matrix(int kind, int base, int nconf, double* out, int ca
  * 2 mult_cos_1d, 3 div_sin_1d, 4 mult_x_1d, 5 mult_xm1_1d, 6 div_xm1_1d;
  * src/Ope_1d/*.cpp); *out_base receives the output basis. */
 int kad_host_op1d_matrix(int op, int base, int n, double* out, int cap, int* out_base);
+
+/* ---- the Newton system (replaces System_of_eqs::do_newton and what it calls:
+ *      sec_member / check_equations src/System_of_eqs/solver.cpp:24-85,
+ *      do_col_J solver.cpp:120-214, xx_to_vars system_of_eqs.cpp:597-612) -- */
+typedef struct kad_system kad_system;
+
+/* XCTS black-hole system (codes/Par_version/Kerr/kerr.cpp:69-99 in the Cartesian
+ * basis) on Space_spheric(CHEB_TYPE, 0, (nr,nt,np), bounds[0..nbounds)): solved
+ * from shell dmin to the compactified domain dmax = nbounds.  n0: lapse on the
+ * excision sphere r = a; ome: horizon rotation.  batch: Jacobian columns per
+ * tangent launch chain. */
+int kad_xcts_create(kad_ctx* ctx, int nr, int nt, int np, const double* bounds, int nbounds, int dmin, int dmax,
+                    double n0, double a, double ome, int batch, kad_system** out);
+int kad_xcts_destroy(kad_system* s);
+/* stats (may be NULL): value instructions, tangent instructions, value slots, tangent slots */
+int kad_xcts_sizes(kad_system* s, int* n_unknowns, int* n_rows, int* stats);
+/* 4 ints per (field, domain): field (0 P, 1 N, 2..4 bet^x,y,z), domain, first unknown, count */
+int kad_xcts_layout(kad_system* s, int* out, int cap);
+/* fields := flat guess + affecte_tau(X0); X0 HOST (n_unknowns) or NULL */
+int kad_xcts_set_guess(kad_system* s, const double* X0);
+/* F(u) (device, n_rows): System_of_eqs::sec_member */
+int kad_xcts_residual(kad_system* s, double* F);
+/* J (device, column-major, ld >= n_rows): every do_col_J column */
+int kad_xcts_jacobian(kad_system* s, double* J, int64_t ldj);
+/* fields -= affecte_tau(X), X device */
+int kad_xcts_update(kad_system* s, const double* X);
+/* coefficients of field f in domain d, HOST out (nr*nt*(np+2)) */
+int kad_xcts_get_coef(kad_system* s, int f, int d, double* out);
+/* One Newton iteration.  solver: 0 fp64 LU, 1 fp32 LU + fp64 IR, 2 fp32 LU +
+ * GMRES-IR, 3 bf16-update LU + GMRES-IR.  Returns 1 if ||F||_inf < precision at
+ * entry (no step), 0 after a step, KAD_ERR_SINGULAR on an exactly singular J.
+ * times[4]: residual, Jacobian fill, factor + solve, update (seconds). */
+int kad_xcts_do_newton(kad_system* s, double precision, double* error, int solver, double* times);
+/* refinement statistics of the last low-precision solve */
+int kad_xcts_last_solve(kad_system* s, int* iters, double* resid);
+/* host-only tracing of the same system (no device): info[6] = n_unknowns, n_rows,
+ * value instr, tangent instr, value slots, tangent programs; returns #layout entries */
+int kad_xcts_host_info(int nr, int nt, int np, const double* bounds, int nbounds, int dmin, int dmax, double n0,
+                       double a, double ome, int* info, int* layout, int cap);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,103 @@
+// The XCTS black-hole system on a spherical space, traced into device programs.
+//
+// Equations: the Kerr prototype of codes/Par_version/Kerr/kerr.cpp:69-99 in the
+// Cartesian tensor basis of the FUKA BH solver (bh_3d_xcts_solver_imp.cpp:119-159):
+//   A^ij = (D^i bet^j + D^j bet^i - 2/3 D_k bet^k f^ij) / 2 / N
+//   inner: N = n0 ; bet^i = n0 / P^2 s^i + Ome a m^i ;
+//          dn(P) + 0.5 P / a + P^3 A_ij s^i s^j / 4 = 0
+//   D_i D^i N + 2 D_i P D^i N / P - N P^4 A_ij A^ij = 0   (matching N, dn(N))
+//   D_i D^i P + P^5 A_ij A^ij / 8 = 0                    (matching P, dn(P))
+//   D_j A^ij + 6 A^ij D_j P / P = 0                       (matching bet^i, dn(bet^i))
+//   outer: N = 1, P = 1, bet^i = 0
+// Equation, unknown and row order as System_of_eqs (src/System_of_eqs/solver.cpp,
+// system_of_eqs.cpp) with Space_spheric::add_eq / add_inner_bc / add_outer_bc
+// (src/Space/Spheric/spheric_add_eq.cpp:26-52).
+#pragma once
+#include <vector>
+#include <string>
+#include <array>
+#include "tape.hpp"
+
+namespace kadhost {
+
+struct XctsParams {
+  int nr = 9, nt = 7, np = 4;
+  std::vector<double> bounds;  // Space_spheric bounds: nucleus [0, b0], shells, compact [b_last, inf)
+  int dmin = 1, dmax = 3;
+  double n0 = 0.5, a = 1.0, ome = 0.0;
+};
+
+struct EqRec {
+  int kind;  // 0 inside, 1 match, 2 outer bc, 3 inner bc
+  int d;
+  std::string which;
+  int ncomp;
+  int row0;
+  std::vector<int> nrows;  // per component
+};
+
+struct SeedTab {
+  std::vector<int> pos;     // 3 per unknown
+  std::vector<double> val;  // 3 per unknown
+  std::vector<int> cnt;
+};
+
+class XctsSystem {
+ public:
+  static constexpr int NF = 5;  // P, N, bet^x, bet^y, bet^z
+  XctsParams prm;
+  std::vector<Domain> doms;
+  Tracer tr;
+  std::vector<EqRec> eqs;
+  int n_unknowns = 0, n_rows = 0;
+  // unknown layout: (field, d) -> offset, count
+  std::vector<std::array<int, 4>> layout;  // field, d, off, cnt
+  // value-space input slots of the fields' coefficients, [field][d]
+  std::vector<std::vector<int>> field_slot;
+  // tangent programs, one per (field, d), and their seed tables
+  std::vector<Program> tprog;
+  std::vector<std::array<int, 2>> tprog_fd;
+  std::vector<SeedTab> seeds;
+  // affecte_tau maps per (field, d): CSR coefficient <- unknowns
+  struct Csr {
+    std::vector<int> rowptr, col;
+    std::vector<double> w;
+  };
+  std::vector<Csr> tau_maps;  // indexed like layout
+
+  explicit XctsSystem(const XctsParams& p);
+  Base field_base(int f, int d) const { return f == 4 ? doms[d].anti_base() : doms[d].std_base(); }
+  void build();
+
+ private:
+  struct Defs {
+    int d;
+    TE P, N, bet[3];
+    TE A[3][3];
+    TE AA;
+    TE ome;
+  };
+  std::vector<std::array<VDp, NF>> field_nodes;  // value nodes per domain
+  std::map<std::string, TE> consts;              // constant fields, created once per domain
+  TE cached_const(int d, const std::string& name, const std::vector<double>& v, const Base& b);
+
+  TE t_add(const TE& a, const TE& b);
+  TE t_sub(const TE& a, const TE& b);
+  TE t_mul(const TE& a, const TE& b);
+  TE t_div(const TE& a, const TE& b);
+  TE t_scal(const TE& a, double x);
+  TE t_div_scal(const TE& a, double x);
+  TE t_pow(const TE& a, int n, int d);
+  void t_grad(const TE& a, TE out[3]);
+  TE t_dn(const TE& a, int bound);
+  TE t_const(int d, const std::vector<double>& v, const Base& b);
+  TE lap(const TE& a);
+
+  Defs defs(int d, const std::array<VDp, NF>& ders);
+  std::vector<TE> expr(const EqRec& e, int part, Defs& D);
+  void trace_value();
+  void trace_tangent(int f, int d);
+  std::vector<std::array<VDp, NF>> zero_ders();
+};
+
+}  // namespace kadhost

@@ -1,0 +1,717 @@
+// Device-resident Newton step of the XCTS system: executes the traced programs
+// (value program -> residual F; tangent programs -> Jacobian columns, batched
+// over unit seeds), factors and solves, and updates the fields in place.
+//
+// Replaces System_of_eqs::do_newton (src_par/do_newton.cpp:65-430):
+//   sec_member / check_equations   -> value program + |F|_inf
+//   do_col_J over all columns      -> tangent programs, B columns per launch chain
+//   pdgesv_                        -> kad LU (fp64) or low-precision LU + fp64 refinement
+//   xx_to_vars + old - var         -> CSR update kernel on the coefficient arrays
+#include "kad_common.hpp"
+#include "kad_ctx.hpp"
+#include "kad_spectral.hpp"
+#include "host/xcts.hpp"
+#include "../../include/kadath_amd.h"
+#include <vector>
+#include <chrono>
+#include <algorithm>
+#include <stdexcept>
+#include <cstring>
+
+namespace kad {
+template <typename T> int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w);
+template <typename T>
+int getrs(const T* LU, int64_t lda, int n, const int* perm, double* b, int trans, double* work, hipStream_t s);
+void ipiv_to_perm(const int* ipiv, int n, int* perm);
+}  // namespace kad
+
+using namespace kadhost;
+
+namespace {
+
+struct DevRow {
+  int pos, gpos;
+  double gw;
+};
+
+// dst[b][e] = op(a[b or 0][e], b[b or 0][e], s)
+__global__ void ew_kernel(int kind, double* __restrict__ dst, int64_t dstride, const double* __restrict__ a,
+                          int64_t astride, const double* __restrict__ bsrc, int64_t bstride, double s, int size,
+                          int nbatch) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)size * nbatch) return;
+  const int64_t bb = t / size;
+  const int e = (int)(t % size);
+  const double x = a[bb * astride + e];
+  double r;
+  switch (kind) {
+    case EW_ADD: r = x + bsrc[bb * bstride + e]; break;
+    case EW_SUB: r = x - bsrc[bb * bstride + e]; break;
+    case EW_MUL: r = x * bsrc[bb * bstride + e]; break;
+    case EW_DIV: r = x / bsrc[bb * bstride + e]; break;
+    case EW_SCAL: r = x * s; break;
+    case EW_SDIV: r = x / s; break;
+    case EW_ADDS: r = x + s; break;
+    default: r = x; break;
+  }
+  dst[bb * dstride + e] = r;
+}
+
+// Inside rows: out[(row0 + r) + (col0 + b) ldo] (+)= sign (src[pos] - gw src[gpos])
+__global__ void export_in_kernel(const DevRow* __restrict__ rows, int nrows, const double* __restrict__ src,
+                                 int64_t sstride, double* __restrict__ out, int row0, int64_t col0, int64_t ldo,
+                                 double sign, int acc, int nbatch) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)nrows * nbatch) return;
+  const int64_t bb = t / nrows;
+  const int r = (int)(t % nrows);
+  const DevRow R = rows[r];
+  const double* x = src + bb * sstride;
+  double v = x[R.pos];
+  if (R.gpos >= 0) v -= R.gw * x[R.gpos];
+  double* o = out + (row0 + r) + (col0 + bb) * ldo;
+  *o = acc ? (*o + sign * v) : sign * v;
+}
+
+// Boundary rows: X(line) = sum_i bsign^i cf(i, line)  (val_boundary, domain_shell_systems.cpp:43-74)
+__global__ void export_bd_kernel(const DevRow* __restrict__ rows, int nrows, const double* __restrict__ src,
+                                 int64_t sstride, int nr, int bsign, double* __restrict__ out, int row0,
+                                 int64_t col0, int64_t ldo, double sign, int acc, int nbatch) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)nrows * nbatch) return;
+  const int64_t bb = t / nrows;
+  const int r = (int)(t % nrows);
+  const DevRow R = rows[r];
+  const double* x = src + bb * sstride;
+  double v = 0.0, g = 0.0, sg = 1.0;
+  for (int i = 0; i < nr; ++i) {
+    v += sg * x[i + (int64_t)nr * R.pos];
+    if (R.gpos >= 0) g += sg * x[i + (int64_t)nr * R.gpos];
+    sg *= bsign;
+  }
+  if (R.gpos >= 0) v -= R.gw * g;
+  double* o = out + (row0 + r) + (col0 + bb) * ldo;
+  *o = acc ? (*o + sign * v) : sign * v;
+}
+
+// Unit seeds of unknowns u0 .. u0 + nbatch - 1 (affecte_tau_one_coef), zero elsewhere.
+__global__ void seed_kernel(double* __restrict__ dst, int64_t dstride, int size, const int* __restrict__ spos,
+                            const double* __restrict__ sval, const int* __restrict__ scnt, int u0, int nu,
+                            int nbatch) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)size * nbatch) return;
+  const int bb = (int)(t / size);
+  const int e = (int)(t % size);
+  const int u = u0 + bb;
+  double v = 0.0;
+  if (u < nu)
+    for (int q = 0; q < scnt[u]; ++q)
+      if (spos[3 * u + q] == e) v = sval[3 * u + q];
+  dst[(int64_t)bb * dstride + e] = v;
+}
+
+// cf[q] += scale * sum_t w[t] X[off + col[t]]   (affecte_tau as a sparse map)
+__global__ void csr_update_kernel(double* __restrict__ cf, const int* __restrict__ rowptr, const int* __restrict__ col,
+                                  const double* __restrict__ w, const double* __restrict__ X, int off, int ncf,
+                                  double scale) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ncf) return;
+  double s = 0.0;
+  for (int t = rowptr[q]; t < rowptr[q + 1]; ++t) s += w[t] * X[off + col[t]];
+  cf[q] += scale * s;
+}
+
+__global__ void absmax_kernel(const double* x, int n, double* out) {
+  __shared__ double red[256];
+  double m = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) m = fmax(m, fabs(x[i]));
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+struct DevPlanX {
+  kad::Passes P;
+  double* pool = nullptr;
+};
+
+struct DevProgram {
+  std::vector<Instr> code;       // slots of tangent operands remapped to physical slots
+  int nphys = 0;
+  int seed_tab = -1;
+  int field = 0, dom = 0, off = 0, cnt = 0;
+};
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+struct kad_system_impl {
+  kad::Ctx* ctx = nullptr;
+  XctsSystem* sys = nullptr;
+  int B = 256;
+  int64_t stride = 0;          // doubles per slot per batch element
+  double* val_arena = nullptr;  // nval_slots * stride
+  double* tan_arena = nullptr;  // nphys_max * B * stride
+  double* F = nullptr;          // n_rows
+  double* scratch = nullptr;    // small
+  std::vector<DevPlanX> plans;
+  std::vector<DevRow*> row_tabs;
+  std::vector<int*> seed_pos;
+  std::vector<double*> seed_val;
+  std::vector<int*> seed_cnt;
+  std::vector<int*> csr_rowptr, csr_col;
+  std::vector<double*> csr_w;
+  std::vector<DevProgram> tprogs;
+  int nphys_max = 0;
+  // dense linear algebra buffers (lazily allocated)
+  double* J = nullptr;
+  float* Jlo = nullptr;
+  int* ipiv = nullptr;
+  int* perm = nullptr;
+  double* X = nullptr;
+  double* Fsave = nullptr;
+  std::vector<void*> allocs;
+  double last_times[4] = {0, 0, 0, 0};
+  int last_iters[2] = {0, 0};
+  double last_resid = 0;
+
+  ~kad_system_impl() {
+    for (void* p : allocs) (void)hipFree(p);
+    delete sys;
+  }
+  int dmalloc(void** p, size_t bytes) {
+    KAD_HIP_CHECK(hipMalloc(p, bytes < 16 ? 16 : bytes));
+    allocs.push_back(*p);
+    return KAD_OK;
+  }
+  template <typename T>
+  int upload(const std::vector<T>& h, T** d) {
+    int rc = dmalloc((void**)d, h.size() * sizeof(T));
+    if (rc) return rc;
+    if (!h.empty()) KAD_HIP_CHECK(hipMemcpy(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return KAD_OK;
+  }
+  double* slot_ptr(const Opnd& o) const {
+    return o.space == SP_VAL ? val_arena + (int64_t)o.slot * stride : tan_arena + (int64_t)o.slot * B * stride;
+  }
+  int64_t slot_stride(const Opnd& o) const { return o.space == SP_VAL ? 0 : stride; }
+  int setup();
+  int run(const std::vector<Instr>& code, int nb, int u0, int seed_tab, double* out, int64_t col0, int64_t ldo);
+  int value_pass();
+  int jacobian(double* Jd, int64_t ldj);
+  int update(const double* Xd, double scale);
+};
+
+// Liveness-based reuse of tangent slots within one program.
+static int allocate_slots(std::vector<Instr>& code, int nvirt) {
+  std::vector<int> last(nvirt, -1);
+  auto use = [&](const Opnd& o, int t) {
+    if (o.slot >= 0 && o.space == SP_TAN) last[o.slot] = std::max(last[o.slot], t);
+  };
+  for (int t = 0; t < (int)code.size(); ++t) {
+    const Instr& in = code[t];
+    use(in.a, t);
+    use(in.b, t);
+    use(in.dst, t);
+  }
+  std::vector<int> phys(nvirt, -1), freelist;
+  int nphys = 0;
+  for (int t = 0; t < (int)code.size(); ++t) {
+    Instr& in = code[t];
+    auto map_read = [&](Opnd& o) {
+      if (o.slot >= 0 && o.space == SP_TAN) o.slot = phys[o.slot];
+    };
+    const int dv = (in.dst.space == SP_TAN) ? in.dst.slot : -1;
+    const int av = (in.a.space == SP_TAN) ? in.a.slot : -1;
+    const int bv = (in.b.space == SP_TAN) ? in.b.slot : -1;
+    map_read(in.a);
+    map_read(in.b);
+    if (dv >= 0) {
+      if (phys[dv] < 0) {
+        if (!freelist.empty()) {
+          phys[dv] = freelist.back();
+          freelist.pop_back();
+        } else {
+          phys[dv] = nphys++;
+        }
+      }
+      in.dst.slot = phys[dv];
+    }
+    // release operands whose last use is this instruction (after the destination is taken,
+    // so an instruction never writes the slot it reads)
+    for (int v : {av, bv}) {
+      if (v >= 0 && last[v] == t && v != dv) {
+        freelist.push_back(phys[v]);
+        last[v] = -2;
+      }
+    }
+    if (dv >= 0 && last[dv] == t) {
+      freelist.push_back(phys[dv]);
+      last[dv] = -2;
+    }
+  }
+  return nphys;
+}
+
+int kad_system_impl::setup() {
+  Tracer& tr = sys->tr;
+  // slot stride: the largest domain array (coef layout), rounded to 8 doubles
+  int maxsz = 0;
+  for (int d = sys->prm.dmin; d <= sys->prm.dmax; ++d) maxsz = std::max(maxsz, sys->doms[d].coef_size());
+  stride = (maxsz + 7) / 8 * 8;
+  int rc = dmalloc((void**)&val_arena, (size_t)std::max(1, tr.nval_slots) * stride * sizeof(double));
+  if (rc) return rc;
+  KAD_HIP_CHECK(hipMemset(val_arena, 0, (size_t)std::max(1, tr.nval_slots) * stride * sizeof(double)));
+  for (auto& cs : tr.const_slots)
+    KAD_HIP_CHECK(hipMemcpy(val_arena + (int64_t)cs.first * stride, cs.second.data(),
+                            cs.second.size() * sizeof(double), hipMemcpyHostToDevice));
+  // plans
+  for (auto& hp : tr.plans) {
+    DevPlanX dp;
+    if ((rc = upload(hp.pool, &dp.pool))) return rc;
+    dp.P.npass = (int)hp.passes.size();
+    dp.P.maxsize = hp.maxsize;
+    if (dp.P.npass > kad::MAX_PASSES) return KAD_ERR_ARG;
+    for (int p = 0; p < dp.P.npass; ++p) {
+      int* lm;
+      if ((rc = upload(hp.passes[p].line_mat, &lm))) return rc;
+      dp.P.axis[p] = hp.passes[p].axis;
+      dp.P.nout[p] = hp.passes[p].nout;
+      dp.P.line_mat[p] = lm;
+    }
+    plans.push_back(dp);
+  }
+  for (auto& rt : tr.row_tabs) {
+    std::vector<DevRow> h(rt.size());
+    for (size_t i = 0; i < rt.size(); ++i) h[i] = {rt[i].pos, rt[i].gpos, rt[i].gw};
+    DevRow* d;
+    if ((rc = upload(h, &d))) return rc;
+    row_tabs.push_back(d);
+  }
+  for (auto& st : sys->seeds) {
+    int* p;
+    double* v;
+    int* c;
+    if ((rc = upload(st.pos, &p)) || (rc = upload(st.val, &v)) || (rc = upload(st.cnt, &c))) return rc;
+    seed_pos.push_back(p);
+    seed_val.push_back(v);
+    seed_cnt.push_back(c);
+  }
+  for (auto& m : sys->tau_maps) {
+    int *rp, *cl;
+    double* w;
+    if ((rc = upload(m.rowptr, &rp)) || (rc = upload(m.col, &cl)) || (rc = upload(m.w, &w))) return rc;
+    csr_rowptr.push_back(rp);
+    csr_col.push_back(cl);
+    csr_w.push_back(w);
+  }
+  // tangent programs: physical slots
+  for (size_t p = 0; p < sys->tprog.size(); ++p) {
+    DevProgram dp;
+    dp.code = sys->tprog[p].code;
+    dp.nphys = allocate_slots(dp.code, sys->tprog[p].nslots);
+    dp.field = sys->layout[p][0];
+    dp.dom = sys->layout[p][1];
+    dp.off = sys->layout[p][2];
+    dp.cnt = sys->layout[p][3];
+    for (auto& in : dp.code)
+      if (in.kind == I_SEED) dp.seed_tab = in.seed_tab;
+    nphys_max = std::max(nphys_max, dp.nphys);
+    tprogs.push_back(std::move(dp));
+  }
+  if ((rc = dmalloc((void**)&tan_arena, (size_t)std::max(1, nphys_max) * B * stride * sizeof(double)))) return rc;
+  if ((rc = dmalloc((void**)&F, (size_t)sys->n_rows * sizeof(double)))) return rc;
+  if ((rc = dmalloc((void**)&scratch, 64 * sizeof(double)))) return rc;
+  return KAD_OK;
+}
+
+int kad_system_impl::run(const std::vector<Instr>& code, int nb, int u0, int seed_tab, double* out, int64_t col0,
+                         int64_t ldo) {
+  hipStream_t s = ctx->stream;
+  for (const Instr& in : code) {
+    const Domain& dm = sys->doms[in.dom];
+    switch (in.kind) {
+      case I_LIN: {
+        const DevPlanX& p = plans[in.plan];
+        const int batch = (in.dst.space == SP_VAL) ? 1 : nb;
+        int rc = kad::apply3d_launch(slot_ptr(in.a), stride, slot_ptr(in.dst), stride, batch, in.din[0], in.din[1],
+                                     in.din[2], p.P, p.pool, 0, 1.0, s);
+        if (rc) return rc;
+        break;
+      }
+      case I_EW: {
+        const int batch = (in.dst.space == SP_VAL) ? 1 : nb;
+        const int64_t tot = (int64_t)in.size * batch;
+        ew_kernel<<<(unsigned)kad::cdiv(tot, 256), 256, 0, s>>>(
+            in.ew, slot_ptr(in.dst), slot_stride(in.dst), slot_ptr(in.a), slot_stride(in.a),
+            in.b.slot >= 0 ? slot_ptr(in.b) : nullptr, in.b.slot >= 0 ? slot_stride(in.b) : 0, in.s, in.size, batch);
+        break;
+      }
+      case I_EXPORT_IN:
+      case I_EXPORT_BD: {
+        const int batch = (in.a.space == SP_VAL) ? 1 : nb;
+        const int64_t tot = (int64_t)in.nrows * batch;
+        if (in.kind == I_EXPORT_IN)
+          export_in_kernel<<<(unsigned)kad::cdiv(tot, 256), 256, 0, s>>>(row_tabs[in.rows], in.nrows, slot_ptr(in.a),
+                                                                          slot_stride(in.a), out, in.row0, col0, ldo,
+                                                                          in.sign, in.accumulate, batch);
+        else
+          export_bd_kernel<<<(unsigned)kad::cdiv(tot, 256), 256, 0, s>>>(
+              row_tabs[in.rows], in.nrows, slot_ptr(in.a), slot_stride(in.a), dm.nr, in.bsign, out, in.row0, col0,
+              ldo, in.sign, in.accumulate, batch);
+        break;
+      }
+      case I_SEED: {
+        const int64_t tot = (int64_t)in.size * nb;
+        const int nu = (int)sys->seeds[seed_tab].cnt.size();
+        seed_kernel<<<(unsigned)kad::cdiv(tot, 256), 256, 0, s>>>(slot_ptr(in.dst), stride, in.size,
+                                                                    seed_pos[seed_tab], seed_val[seed_tab],
+                                                                    seed_cnt[seed_tab], u0, nu, nb);
+        break;
+      }
+      default:
+        return KAD_ERR_ARG;
+    }
+  }
+  KAD_HIP_CHECK(hipGetLastError());
+  return KAD_OK;
+}
+
+int kad_system_impl::value_pass() { return run(sys->tr.val.code, 1, 0, -1, F, 0, 0); }
+
+int kad_system_impl::jacobian(double* Jd, int64_t ldj) {
+  int rc = value_pass();
+  if (rc) return rc;
+  KAD_HIP_CHECK(hipMemsetAsync(Jd, 0, (size_t)ldj * sys->n_unknowns * sizeof(double), ctx->stream));
+  for (auto& p : tprogs) {
+    for (int u0 = 0; u0 < p.cnt; u0 += B) {
+      const int nb = std::min(B, p.cnt - u0);
+      // exports write columns off+u0 .. off+u0+nb-1
+      if ((rc = run(p.code, nb, u0, p.seed_tab, Jd, (int64_t)p.off + u0, ldj))) return rc;
+    }
+  }
+  return KAD_OK;
+}
+
+int kad_system_impl::update(const double* Xd, double scale) {
+  for (size_t q = 0; q < sys->layout.size(); ++q) {
+    const int f = sys->layout[q][0], d = sys->layout[q][1], off = sys->layout[q][2];
+    const int ncf = sys->doms[d].coef_size();
+    double* cf = val_arena + (int64_t)sys->field_slot[f][d] * stride;
+    csr_update_kernel<<<(unsigned)kad::cdiv(ncf, 256), 256, 0, ctx->stream>>>(cf, csr_rowptr[q], csr_col[q], csr_w[q],
+                                                                              Xd, off, ncf, scale);
+  }
+  KAD_HIP_CHECK(hipGetLastError());
+  return KAD_OK;
+}
+
+extern "C" {
+
+int kad_xcts_create(kad_ctx* ctx, int nr, int nt, int np, const double* bounds, int nbounds, int dmin, int dmax,
+                    double n0, double a, double ome, int batch, kad_system** out) {
+  if (!ctx || !bounds || nbounds < 2 || !out || batch <= 0) return KAD_ERR_ARG;
+  auto* S = new kad_system_impl();
+  S->ctx = reinterpret_cast<kad::Ctx*>(ctx);
+  S->B = batch;
+  XctsParams p;
+  p.nr = nr;
+  p.nt = nt;
+  p.np = np;
+  p.bounds.assign(bounds, bounds + nbounds);
+  p.dmin = dmin;
+  p.dmax = dmax;
+  p.n0 = n0;
+  p.a = a;
+  p.ome = ome;
+  try {
+    S->sys = new XctsSystem(p);
+    S->sys->build();
+  } catch (const std::exception& e) {
+    fprintf(stderr, "[kadath_amd] kad_xcts_create: %s\n", e.what());
+    delete S;
+    return KAD_ERR_ARG;
+  }
+  int rc = S->setup();
+  if (rc) {
+    delete S;
+    return rc;
+  }
+  *out = reinterpret_cast<kad_system*>(S);
+  return KAD_OK;
+}
+
+int kad_xcts_destroy(kad_system* s) {
+  if (!s) return KAD_ERR_ARG;
+  auto* S = reinterpret_cast<kad_system_impl*>(s);
+  (void)hipStreamSynchronize(S->ctx->stream);
+  delete S;
+  return KAD_OK;
+}
+
+int kad_xcts_sizes(kad_system* s, int* n_unknowns, int* n_rows, int* stats) {
+  if (!s) return KAD_ERR_ARG;
+  auto* S = reinterpret_cast<kad_system_impl*>(s);
+  if (n_unknowns) *n_unknowns = S->sys->n_unknowns;
+  if (n_rows) *n_rows = S->sys->n_rows;
+  if (stats) {
+    size_t tcode = 0;
+    for (auto& p : S->tprogs) tcode += p.code.size();
+    stats[0] = (int)S->sys->tr.val.code.size();
+    stats[1] = (int)tcode;
+    stats[2] = S->sys->tr.nval_slots;
+    stats[3] = S->nphys_max;
+  }
+  return KAD_OK;
+}
+
+// layout: 4 ints per (field, domain): field (0 P, 1 N, 2-4 bet x,y,z), domain, offset, count
+int kad_xcts_layout(kad_system* s, int* out, int cap) {
+  if (!s || !out) return KAD_ERR_ARG;
+  auto* S = reinterpret_cast<kad_system_impl*>(s);
+  const int n = (int)S->sys->layout.size();
+  if (cap < 4 * n) return KAD_ERR_ARG;
+  for (int q = 0; q < n; ++q)
+    for (int c = 0; c < 4; ++c) out[4 * q + c] = S->sys->layout[q][c];
+  return n;
+}
+
+// Fields := flat guess (P = 1 + a/r, N = 1, bet = 0; codes/Par_version/Kerr/kerr.cpp:36-52)
+// + affecte_tau(X0).  X0: HOST, n_unknowns doubles (may be NULL for the flat guess).
+int kad_xcts_set_guess(kad_system* s, const double* X0) {
+  if (!s) return KAD_ERR_ARG;
+  auto* S = reinterpret_cast<kad_system_impl*>(s);
+  XctsSystem& sy = *S->sys;
+  hipStream_t st = S->ctx->stream;
+  for (int d = sy.prm.dmin; d <= sy.prm.dmax; ++d) {
+    const Domain& dm = sy.doms[d];
+    for (int f = 0; f < XctsSystem::NF; ++f) {
+      std::vector<double> conf(dm.conf_size(), 0.0);
+      for (size_t q = 0; q < conf.size(); ++q) {
+        if (f == 0) {
+          const double v = 1.0 + sy.prm.a / dm.R[q];
+          conf[q] = std::isfinite(v) ? v : 1.0;
+        } else if (f == 1) {
+          conf[q] = 1.0;
+        }
+      }
+      // forward transform on the host with the same tables as the device
+      const Base b = sy.field_base(f, d);
+      Mat Mp = coef_matrix(b.bp, dm.np);
+      std::vector<double> t1((size_t)dm.nr * dm.nt * dm.npc, 0.0), t2(t1.size(), 0.0), t3(t1.size(), 0.0);
+      for (int kk = 0; kk < dm.npc; ++kk)
+        for (int k = 0; k < dm.np; ++k)
+          for (int j = 0; j < dm.nt; ++j)
+            for (int i = 0; i < dm.nr; ++i)
+              t1[i + dm.nr * (j + (size_t)dm.nt * kk)] += Mp(kk, k) * conf[i + dm.nr * (j + (size_t)dm.nt * k)];
+      for (int k = 0; k < dm.npc; ++k) {
+        Mat Mt = coef_matrix(b.bt[k], dm.nt);
+        for (int jj = 0; jj < dm.nt; ++jj)
+          for (int j = 0; j < dm.nt; ++j)
+            for (int i = 0; i < dm.nr; ++i)
+              t2[i + dm.nr * (jj + (size_t)dm.nt * k)] += Mt(jj, j) * t1[i + dm.nr * (j + (size_t)dm.nt * k)];
+      }
+      Mat Mr = coef_matrix(CHEB, dm.nr);
+      for (int k = 0; k < dm.npc; ++k)
+        for (int j = 0; j < dm.nt; ++j)
+          for (int ii = 0; ii < dm.nr; ++ii)
+            for (int i = 0; i < dm.nr; ++i)
+              t3[ii + dm.nr * (j + (size_t)dm.nt * k)] += Mr(ii, i) * t2[i + dm.nr * (j + (size_t)dm.nt * k)];
+      KAD_HIP_CHECK(hipMemcpyAsync(S->val_arena + (int64_t)sy.field_slot[f][d] * S->stride, t3.data(),
+                                   t3.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    }
+  }
+  if (X0) {
+    double* dX;
+    KAD_HIP_CHECK(hipMalloc(&dX, (size_t)sy.n_unknowns * sizeof(double)));
+    KAD_HIP_CHECK(hipMemcpyAsync(dX, X0, (size_t)sy.n_unknowns * sizeof(double), hipMemcpyHostToDevice, st));
+    int rc = S->update(dX, 1.0);
+    KAD_HIP_CHECK(hipStreamSynchronize(st));
+    (void)hipFree(dX);
+    if (rc) return rc;
+  }
+  KAD_HIP_CHECK(hipStreamSynchronize(st));
+  return KAD_OK;
+}
+
+// sec_member: F (device, n_rows) for the current fields.
+int kad_xcts_residual(kad_system* s, double* Fout) {
+  if (!s || !Fout) return KAD_ERR_ARG;
+  auto* S = reinterpret_cast<kad_system_impl*>(s);
+  int rc = S->value_pass();
+  if (rc) return rc;
+  KAD_HIP_CHECK(hipMemcpyAsync(Fout, S->F, (size_t)S->sys->n_rows * sizeof(double), hipMemcpyDeviceToDevice,
+                               S->ctx->stream));
+  KAD_HIP_CHECK(hipStreamSynchronize(S->ctx->stream));
+  return KAD_OK;
+}
+
+// Full Jacobian (device, column-major, ld ldj >= n_rows), all do_col_J columns.
+int kad_xcts_jacobian(kad_system* s, double* Jd, int64_t ldj) {
+  if (!s || !Jd) return KAD_ERR_ARG;
+  auto* S = reinterpret_cast<kad_system_impl*>(s);
+  if (ldj < S->sys->n_rows) return KAD_ERR_ARG;
+  int rc = S->jacobian(Jd, ldj);
+  if (rc) return rc;
+  KAD_HIP_CHECK(hipStreamSynchronize(S->ctx->stream));
+  return KAD_OK;
+}
+
+// fields -= affecte_tau(X)  (X device, n_unknowns)
+int kad_xcts_update(kad_system* s, const double* Xd) {
+  if (!s || !Xd) return KAD_ERR_ARG;
+  auto* S = reinterpret_cast<kad_system_impl*>(s);
+  int rc = S->update(Xd, -1.0);
+  if (rc) return rc;
+  KAD_HIP_CHECK(hipStreamSynchronize(S->ctx->stream));
+  return KAD_OK;
+}
+
+// Coefficients of field f (0 P, 1 N, 2-4 bet) in domain d: HOST out, nr*nt*(np+2) doubles.
+int kad_xcts_get_coef(kad_system* s, int f, int d, double* out) {
+  if (!s || !out) return KAD_ERR_ARG;
+  auto* S = reinterpret_cast<kad_system_impl*>(s);
+  XctsSystem& sy = *S->sys;
+  if (f < 0 || f >= XctsSystem::NF || d < sy.prm.dmin || d > sy.prm.dmax) return KAD_ERR_ARG;
+  KAD_HIP_CHECK(hipMemcpyAsync(out, S->val_arena + (int64_t)sy.field_slot[f][d] * S->stride,
+                               (size_t)sy.doms[d].coef_size() * sizeof(double), hipMemcpyDeviceToHost,
+                               S->ctx->stream));
+  KAD_HIP_CHECK(hipStreamSynchronize(S->ctx->stream));
+  return KAD_OK;
+}
+
+// One Newton-Raphson iteration (System_of_eqs::do_newton, src_par/do_newton.cpp:65-430).
+// solver: 0 fp64 LU; 1 fp32 LU + fp64 IR; 2 fp32 LU + fp64 GMRES-IR; 3 bf16-update LU + fp64 GMRES-IR.
+// Returns 1 when ||F||_inf < precision at entry (converged; no step taken), 0 after a step,
+// < 0 on error, KAD_ERR_SINGULAR (-5) when the factorisation meets an exact zero pivot.
+// times[4] (may be NULL): residual, Jacobian fill, factor+solve, update seconds.
+int kad_xcts_do_newton(kad_system* s, double precision, double* error, int solver, double* times) {
+  if (!s || !error || solver < 0 || solver > 3) return KAD_ERR_ARG;
+  auto* S = reinterpret_cast<kad_system_impl*>(s);
+  XctsSystem& sy = *S->sys;
+  hipStream_t st = S->ctx->stream;
+  const int n = sy.n_unknowns;
+  int rc;
+  double t0 = now_s();
+  if ((rc = S->value_pass())) return rc;
+  absmax_kernel<<<1, 256, 0, st>>>(S->F, sy.n_rows, S->scratch);
+  double err = 0;
+  KAD_HIP_CHECK(hipMemcpyAsync(&err, S->scratch, sizeof(double), hipMemcpyDeviceToHost, st));
+  KAD_HIP_CHECK(hipStreamSynchronize(st));
+  *error = err;
+  double t1 = now_s();
+  if (times) times[0] = t1 - t0;
+  if (err < precision) return 1;
+  if (!S->J) {
+    if ((rc = S->dmalloc((void**)&S->J, (size_t)n * n * sizeof(double)))) return rc;
+    if ((rc = S->dmalloc((void**)&S->ipiv, (size_t)n * sizeof(int)))) return rc;
+    if ((rc = S->dmalloc((void**)&S->perm, (size_t)n * sizeof(int)))) return rc;
+    if ((rc = S->dmalloc((void**)&S->X, (size_t)n * sizeof(double)))) return rc;
+  }
+  if (solver > 0 && !S->Jlo)
+    if ((rc = S->dmalloc((void**)&S->Jlo, (size_t)n * n * sizeof(float)))) return rc;
+  // Jacobian (the value pass above already holds the intermediates)
+  KAD_HIP_CHECK(hipMemsetAsync(S->J, 0, (size_t)n * n * sizeof(double), st));
+  for (auto& p : S->tprogs)
+    for (int u0 = 0; u0 < p.cnt; u0 += S->B) {
+      const int nb = std::min(S->B, p.cnt - u0);
+      if ((rc = S->run(p.code, nb, u0, p.seed_tab, S->J, (int64_t)p.off + u0, n))) return rc;
+    }
+  KAD_HIP_CHECK(hipStreamSynchronize(st));
+  double t2 = now_s();
+  if (times) times[1] = t2 - t1;
+  // solve J X = F
+  int info = 0;
+  if (solver == 0) {
+    if ((rc = kad_lu_factor(reinterpret_cast<kad_ctx*>(S->ctx), KAD_PREC_F64, n, S->J, n, S->ipiv, S->perm, 0)) < 0)
+      return rc;
+    info = rc;
+    if (info == 0) {
+      KAD_HIP_CHECK(hipMemcpyAsync(S->X, S->F, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, st));
+      if ((rc = kad_lu_solve(reinterpret_cast<kad_ctx*>(S->ctx), KAD_PREC_F64, n, S->J, n, S->perm, S->X, 0)))
+        return rc;
+    }
+  } else {
+    if ((rc = kad_cast_f64_to_f32(reinterpret_cast<kad_ctx*>(S->ctx), n, S->J, n, S->Jlo, n))) return rc;
+    const int prec = (solver == 3) ? KAD_PREC_BF16 : KAD_PREC_F32;
+    if ((rc = kad_lu_factor(reinterpret_cast<kad_ctx*>(S->ctx), prec, n, S->Jlo, n, S->ipiv, S->perm, 0)) < 0)
+      return rc;
+    info = rc;
+    if (info == 0) {
+      rc = kad_ir_solve(reinterpret_cast<kad_ctx*>(S->ctx), n, S->J, n, prec, S->Jlo, n, S->perm, S->F, S->X, 1e-14,
+                        solver == 1 ? 30 : 10, solver == 1 ? 0 : 1, 0, S->last_iters, &S->last_resid);
+      if (rc < 0 && rc != KAD_ERR_NOCONV) return rc;
+    }
+  }
+  if (info != 0) {
+    fprintf(stderr, "Jacobian is singular! Stopping.\n");
+    return -5;
+  }
+  KAD_HIP_CHECK(hipStreamSynchronize(st));
+  double t3 = now_s();
+  if (times) times[2] = t3 - t2;
+  if ((rc = S->update(S->X, -1.0))) return rc;
+  KAD_HIP_CHECK(hipStreamSynchronize(st));
+  if (times) times[3] = now_s() - t3;
+  return 0;
+}
+
+int kad_xcts_last_solve(kad_system* s, int* iters, double* resid) {
+  if (!s) return KAD_ERR_ARG;
+  auto* S = reinterpret_cast<kad_system_impl*>(s);
+  if (iters) {
+    iters[0] = S->last_iters[0];
+    iters[1] = S->last_iters[1];
+  }
+  if (resid) *resid = S->last_resid;
+  return KAD_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+// Host-only tracing (no device work): sizes of the system and of its programs.
+// info[0] n_unknowns, [1] n_rows, [2] value instructions, [3] tangent instructions,
+// [4] value slots, [5] tangent programs.  layout: 4 ints per (field, domain) entry.
+int kad_xcts_host_info(int nr, int nt, int np, const double* bounds, int nbounds, int dmin, int dmax, double n0,
+                       double a, double ome, int* info, int* layout, int cap) {
+  if (!bounds || !info) return KAD_ERR_ARG;
+  XctsParams p;
+  p.nr = nr;
+  p.nt = nt;
+  p.np = np;
+  p.bounds.assign(bounds, bounds + nbounds);
+  p.dmin = dmin;
+  p.dmax = dmax;
+  p.n0 = n0;
+  p.a = a;
+  p.ome = ome;
+  try {
+    XctsSystem sy(p);
+    sy.build();
+    size_t tcode = 0;
+    for (auto& pr : sy.tprog) tcode += pr.code.size();
+    info[0] = sy.n_unknowns;
+    info[1] = sy.n_rows;
+    info[2] = (int)sy.tr.val.code.size();
+    info[3] = (int)tcode;
+    info[4] = sy.tr.nval_slots;
+    info[5] = (int)sy.tprog.size();
+    if (layout) {
+      if (cap < 4 * (int)sy.layout.size()) return KAD_ERR_ARG;
+      for (size_t q = 0; q < sy.layout.size(); ++q)
+        for (int c = 0; c < 4; ++c) layout[4 * q + c] = sy.layout[q][c];
+    }
+    return (int)sy.layout.size();
+  } catch (const std::exception& e) {
+    fprintf(stderr, "[kadath_amd] kad_xcts_host_info: %s\n", e.what());
+    return KAD_ERR_ARG;
+  }
+}
+}

@@ -1,0 +1,104 @@
+"""Python handle on the device XCTS Newton system (C-ABI kad_xcts_*).
+
+Mirrors the driver loop of the reference (codes/Par_version/Kerr/kerr.cpp:102-112):
+    while not converged: converged = system.do_newton(prec)
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from ._lib import lib, check, c_vp
+
+FIELDS = ("P", "N", "bx", "by", "bz")
+SOLVERS = {"f64": 0, "f32-ir": 1, "f32-gmres": 2, "bf16-gmres": 3}
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def host_info(res, bounds, n0=0.5, a=1.0, ome=0.0):
+    """Host-only trace of the system: sizes and (field, domain, offset, count) layout."""
+    b = np.asarray(bounds, dtype=np.float64)
+    info = (ctypes.c_int * 6)()
+    cap = 4 * 5 * 64
+    lay = (ctypes.c_int * cap)()
+    nl = check(lib.kad_xcts_host_info(res[0], res[1], res[2], _dp(b), len(b), 1, len(b), n0, a, ome, info, lay, cap),
+               "kad_xcts_host_info")
+    layout = [tuple(lay[4 * q:4 * q + 4]) for q in range(nl)]
+    keys = ("n_unknowns", "n_rows", "value_instr", "tangent_instr", "value_slots", "tangent_programs")
+    return dict(zip(keys, list(info))), layout
+
+
+class XctsSystem:
+    def __init__(self, ctx, res, bounds, n0=0.5, a=1.0, ome=0.0, batch=256):
+        self.ctx = ctx
+        self.res = tuple(res)
+        self.bounds = np.asarray(bounds, dtype=np.float64)
+        h = c_vp()
+        check(lib.kad_xcts_create(ctx.h, res[0], res[1], res[2], _dp(self.bounds), len(self.bounds), 1,
+                                  len(self.bounds), n0, a, ome, batch, ctypes.byref(h)), "kad_xcts_create")
+        self.h = h
+        n, m = ctypes.c_int(), ctypes.c_int()
+        st = (ctypes.c_int * 4)()
+        check(lib.kad_xcts_sizes(h, ctypes.byref(n), ctypes.byref(m), st), "kad_xcts_sizes")
+        self.n, self.m = n.value, m.value
+        self.stats = dict(value_instr=st[0], tangent_instr=st[1], value_slots=st[2], tangent_slots=st[3])
+        cap = 4 * 5 * 64
+        lay = (ctypes.c_int * cap)()
+        nl = check(lib.kad_xcts_layout(h, lay, cap), "kad_xcts_layout")
+        self.layout = [tuple(lay[4 * q:4 * q + 4]) for q in range(nl)]
+
+    def close(self):
+        if self.h:
+            lib.kad_xcts_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_guess(self, X0=None):
+        if X0 is None:
+            check(lib.kad_xcts_set_guess(self.h, None), "kad_xcts_set_guess")
+        else:
+            X0 = np.ascontiguousarray(X0, dtype=np.float64)
+            check(lib.kad_xcts_set_guess(self.h, _dp(X0)), "kad_xcts_set_guess")
+
+    def residual(self):
+        F = torch.empty(self.m, dtype=torch.float64, device="cuda")
+        self.ctx._enter()
+        check(lib.kad_xcts_residual(self.h, ctypes.c_void_p(F.data_ptr())), "kad_xcts_residual")
+        return F
+
+    def jacobian(self):
+        J = torch.empty((self.n, self.m), dtype=torch.float64, device="cuda")  # column-major view J.T
+        self.ctx._enter()
+        check(lib.kad_xcts_jacobian(self.h, ctypes.c_void_p(J.data_ptr()), self.m), "kad_xcts_jacobian")
+        return J.T
+
+    def update(self, X):
+        self.ctx._enter()
+        check(lib.kad_xcts_update(self.h, ctypes.c_void_p(X.data_ptr())), "kad_xcts_update")
+
+    def coef(self, f, d):
+        nr, nt, npt = self.res
+        out = np.zeros((npt + 2) * nt * nr)
+        check(lib.kad_xcts_get_coef(self.h, f, d, _dp(out)), "kad_xcts_get_coef")
+        return out.reshape(npt + 2, nt, nr)
+
+    def do_newton(self, prec, solver="f64"):
+        err = ctypes.c_double()
+        times = np.zeros(4)
+        self.ctx._enter()
+        rc = lib.kad_xcts_do_newton(self.h, prec, ctypes.byref(err), SOLVERS[solver], _dp(times))
+        if rc == -5:
+            raise RuntimeError("Jacobian is singular! Stopping.")
+        check(rc, "kad_xcts_do_newton")
+        it = (ctypes.c_int * 2)()
+        res = ctypes.c_double()
+        lib.kad_xcts_last_solve(self.h, it, ctypes.byref(res))
+        return rc == 1, err.value, times, (it[0], it[1], res.value)

@@ -157,14 +157,19 @@ class XCTSSystem:
 
     # ---- unknowns (System_of_eqs::add_var order: var_double, then var x domain x component)
     def unknown_layout(self):
+        """Variables in add_var order (P, N, bet); each variable domain by domain,
+        a vector's Cartesian components together inside each domain
+        (System_of_eqs::xx_to_vars, system_of_eqs.cpp:597-612; Domain_shell::affecte_tau
+        on a CARTESIAN vector, domain_shell_affecte_tau.cpp:140-150)."""
         lay = []
         off = 1 if self.spin is not None else 0
-        for name in self.FIELDS:
+        for var in (("P",), ("N",), ("bx", "by", "bz")):
             for d in range(self.dmin, self.dmax + 1):
-                dom = self.sp.doms[d]
-                n = dom.nbr_unknowns(self.comp_base_t(name, d))
-                lay.append((name, d, off, n))
-                off += n
+                for name in var:
+                    dom = self.sp.doms[d]
+                    n = dom.nbr_unknowns(self.comp_base_t(name, d))
+                    lay.append((name, d, off, n))
+                    off += n
         return lay, off
 
     def xx_to_fields(self, X):
@@ -463,10 +468,26 @@ class XCTSSystem:
 # ---------------------------------------------------------------------------
 # Initial data and Newton
 # ---------------------------------------------------------------------------
-def initial_fields(space, dmin, dmax, rexc, seed=20261202, amp=1e-3, sys=None):
-    """Flat guess P = 1 + rexc/r, N = 1, bet = 0 (codes/Par_version/Kerr/kerr.cpp:36-52),
-    plus amp * 2^-(i+j+k) U(-1,1) on the admissible coefficients (through affecte_tau)."""
+def initial_perturbation(sys, seed=20261202, amp=1e-3):
+    """amp * 2^-(i+j+k) U(-1,1) on every admissible coefficient, in unknown order
+    (numpy.random.default_rng(seed))."""
     rng = np.random.default_rng(seed)
+    lay, n = sys.unknown_layout()
+    X = np.zeros(n)
+    for (name, d, off, cnt) in lay:
+        dom = sys.sp.doms[d]
+        u = off
+        for (k, j) in dom.unknown_slots(sys.comp_base_t(name, d)):
+            for i in range(dom.nr):
+                X[u] = amp * 2.0 ** (-(i + j + k)) * rng.uniform(-1, 1)
+                u += 1
+    return X
+
+
+def initial_fields(sys, rexc, X0=None):
+    """Flat guess P = 1 + rexc/r, N = 1, bet = 0 (codes/Par_version/Kerr/kerr.cpp:36-52),
+    plus affecte_tau(X0) (X0 in unknown order, e.g. initial_perturbation)."""
+    space, dmin, dmax = sys.sp, sys.dmin, sys.dmax
     fields = {n: {} for n in XCTSSystem.FIELDS}
     for d in range(dmin, dmax + 1):
         dom = space.doms[d]
@@ -478,22 +499,11 @@ def initial_fields(space, dmin, dmax, rexc, seed=20261202, amp=1e-3, sys=None):
         fields["bx"][d] = dom.conf_field(0.0)
         fields["by"][d] = dom.conf_field(0.0)
         fields["bz"][d] = dom.conf_field(0.0, anti=True)
-    if sys is None:
-        sys = XCTSSystem(space, dmin, dmax)
-    lay, n = sys.unknown_layout()
-    X = np.zeros(n)
-    for (name, d, off, cnt) in lay:
-        dom = space.doms[d]
-        slots = dom.unknown_slots(sys.comp_base_t(name, d))
-        u = off
-        for (k, j) in slots:
-            for i in range(dom.nr):
-                X[u] = amp * 2.0 ** (-(i + j + k)) * rng.uniform(-1, 1)
-                u += 1
-    corr, _ = sys.xx_to_fields(X)
-    for name in XCTSSystem.FIELDS:
-        for d in range(dmin, dmax + 1):
-            fields[name][d] = vadd(fields[name][d], corr[name][d])
+    if X0 is not None:
+        corr, _ = sys.xx_to_fields(X0)
+        for name in XCTSSystem.FIELDS:
+            for d in range(dmin, dmax + 1):
+                fields[name][d] = vadd(fields[name][d], corr[name][d])
     return fields
 
 

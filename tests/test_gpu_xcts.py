@@ -1,0 +1,93 @@
+"""GPU: the device Newton path (residual, batched-JVP Jacobian, LU, update) against
+the oracle's restatement of System_of_eqs on the 9x7x4 three-domain space.
+
+Tolerances: F and J rows 1e-10 relative to the row scale (FP64 roundoff of O(10^3)
+operations and different summation orders); converged fields 1e-10 relative
+(both Newton iterations stop at ||F||_inf < 1e-10, the acceptance bound).
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import xcts as OX
+
+pytestmark = pytest.mark.gpu
+
+RES, BOUNDS = (9, 7, 4), [1.0, 2.0, 4.0]
+FIELD_IDX = {"P": 0, "N": 1, "bx": 2, "by": 3, "bz": 4}
+
+
+@pytest.fixture(scope="module")
+def oracle_sys():
+    sp = OX.Space(RES, BOUNDS)
+    sy = OX.XCTSSystem(sp, 1, len(BOUNDS), n0=0.5, a=1.0)
+    X0 = OX.initial_perturbation(sy)
+    sy.set_fields(OX.initial_fields(sy, 1.0, X0))
+    return sy, X0
+
+
+@pytest.fixture(scope="module")
+def gpu_sys(ctx):
+    from kadath_frontera_amd.xcts import XctsSystem
+    s = XctsSystem(ctx, RES, BOUNDS, n0=0.5, a=1.0, batch=128)
+    yield s
+    s.close()
+
+
+def test_sizes_and_layout(oracle_sys, gpu_sys):
+    sy, _ = oracle_sys
+    lay, n = sy.unknown_layout()
+    assert gpu_sys.n == n == gpu_sys.m
+    assert [(FIELD_IDX[a], b, c, d) for (a, b, c, d) in lay] == gpu_sys.layout
+
+
+def test_residual_matches_oracle(oracle_sys, gpu_sys):
+    sy, X0 = oracle_sys
+    gpu_sys.set_guess(X0)
+    F = gpu_sys.residual().cpu().numpy()
+    Fo = sy.residual()
+    scale = max(1.0, np.abs(Fo).max())
+    assert np.abs(F - Fo).max() < 1e-11 * scale, np.abs(F - Fo).max()
+
+
+def test_jacobian_matches_oracle(oracle_sys, gpu_sys):
+    sy, X0 = oracle_sys
+    gpu_sys.set_guess(X0)
+    J = gpu_sys.jacobian().cpu().numpy()
+    Jo = sy.jacobian()
+    colscale = np.maximum(1.0, np.abs(Jo).max(axis=0))
+    err = (np.abs(J - Jo) / colscale).max()
+    assert err < 1e-11, err
+    # structure: entries the oracle has at exactly zero are zero up to roundoff
+    assert (np.abs(J)[Jo == 0] / np.broadcast_to(colscale, J.shape)[Jo == 0]).max() < 1e-12
+
+
+@pytest.mark.parametrize("solver", ["f64", "f32-gmres"])
+def test_newton_converges_to_oracle_solution(oracle_sys, gpu_sys, solver):
+    sy, X0 = oracle_sys
+    # oracle solution (cached on the fixture)
+    if not hasattr(sy, "_solved"):
+        sy.set_fields(OX.initial_fields(sy, 1.0, X0))
+        for _ in range(12):
+            err, *_ = OX.newton_step(sy)
+            if err < 1e-10:
+                break
+        sy._solved = {}
+        for k in sy.fields:
+            sy._solved[k] = {}
+            for d, v in sy.fields[k].items():
+                v.coef()
+                sy._solved[k][d] = v.cf
+        sy._err = sy.check_equations()
+    gpu_sys.set_guess(X0)
+    conv, it = False, 0
+    while not conv and it < 15:
+        conv, err, times, ir = gpu_sys.do_newton(1e-10, solver)
+        print(f"{solver} it {it} err {err:.3e} times {times} ir {ir}", flush=True)
+        it += 1
+    assert conv and err < 1e-10
+    for name, f in FIELD_IDX.items():
+        for d in range(1, len(BOUNDS) + 1):
+            ref = sy._solved[name][d]
+            got = gpu_sys.coef(f, d)
+            assert np.abs(got - ref).max() <= 1e-10 * max(1.0, np.abs(ref).max()), (name, d)
