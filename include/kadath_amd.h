@@ -45,6 +45,11 @@ int kad_ctx_destroy(kad_ctx* ctx);
 void* kad_ctx_stream(kad_ctx* ctx);
 int kad_sync(kad_ctx* ctx);
 
+/* Live timing of the MFMA GEMM launches with HIP events (on = 1 starts a new
+ * measurement); read returns total flops, kernel milliseconds and launches. */
+int kad_gemm_timing(kad_ctx* ctx, int on);
+int kad_gemm_timing_read(kad_ctx* ctx, double* flops, double* ms, int* launches);
+
 /* ---- dense linear algebra (replaces pdgesv_, src_par/do_newton.cpp:317-320,
  *      and Matrice::set_lu/solve, src/Matrice/matrice.cpp:192-250) ---------- */
 
@@ -137,8 +142,11 @@ int kad_xcts_get_coef(kad_system* s, int f, int d, double* out);
 /* One Newton iteration.  solver: 0 fp64 LU, 1 fp32 LU + fp64 IR, 2 fp32 LU +
  * GMRES-IR, 3 bf16-update LU + GMRES-IR.  Returns 1 if ||F||_inf < precision at
  * entry (no step), 0 after a step, KAD_ERR_SINGULAR on an exactly singular J.
- * times[4]: residual, Jacobian fill, factor + solve, update (seconds). */
+ * times[5]: residual, Jacobian fill, factor + solve (incl. refinement), update,
+ * factorisation alone (seconds). */
 int kad_xcts_do_newton(kad_system* s, double precision, double* error, int solver, double* times);
+/* seeded perturbation amp 2^-(i+j+k) U(-1,1) of every unknown (splitmix64), HOST out */
+int kad_xcts_perturbation(kad_system* s, uint64_t seed, double amp, double* out);
 /* refinement statistics of the last low-precision solve */
 int kad_xcts_last_solve(kad_system* s, int* iters, double* resid);
 /* host-only tracing of the same system (no device): info[6] = n_unknowns, n_rows,

@@ -54,6 +54,9 @@ SIGNATURES = {
     "kad_xcts_get_coef": (c_int, [c_vp, c_int, c_int, c_dp]),
     "kad_xcts_do_newton": (c_int, [c_vp, c_dbl, c_dp, c_int, c_dp]),
     "kad_xcts_last_solve": (c_int, [c_vp, c_ip, c_dp]),
+    "kad_xcts_perturbation": (c_int, [c_vp, ctypes.c_uint64, c_dbl, c_dp]),
+    "kad_gemm_timing": (c_int, [c_vp, c_int]),
+    "kad_gemm_timing_read": (c_int, [c_vp, c_dp, c_dp, c_ip]),
     "kad_xcts_host_info": (c_int, [c_int, c_int, c_int, c_dp, c_int, c_int, c_int, c_dbl, c_dbl, c_dbl, c_ip, c_ip,
                                    c_int]),
 }

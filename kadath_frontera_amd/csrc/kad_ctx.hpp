@@ -26,6 +26,15 @@ struct LuWork {
   size_t tmp2_bytes = 0;
 };
 
+// Optional live timing of the MFMA GEMM launches (HIP events on the launch
+// stream): the measured kernel of the LU roofline.
+struct GemmTimer {
+  std::vector<hipEvent_t> ev;   // pairs
+  std::vector<double> flops;    // per pair
+  size_t used = 0;
+};
+extern GemmTimer* g_gemm_timer;
+
 // Device copy of a chain of 1-D passes (matrix pool + per-line tables).
 struct DevPlan {
   Passes P;

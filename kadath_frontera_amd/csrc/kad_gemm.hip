@@ -13,6 +13,7 @@
 // lanes 0..15 of an accumulator register hold 16 consecutive ROWS of one column
 // of C: the column-major epilogue is coalesced.
 #include "kad_common.hpp"
+#include "kad_ctx.hpp"
 
 namespace kad {
 
@@ -277,6 +278,16 @@ int gemm_launch(int prec, int op, int m, int n, int k, const void* A, int64_t ld
   }
   const int nblocks = (int)(cdiv(m, GBM) * cdiv(n, GBN));
   dim3 grid(nblocks), block(GNT);
+  GemmTimer* tm = g_gemm_timer;
+  if (tm) {
+    if (2 * tm->used + 2 > tm->ev.size()) {
+      const size_t old = tm->ev.size();
+      tm->ev.resize(old + 2048);
+      for (size_t q = old; q < tm->ev.size(); ++q) KAD_HIP_CHECK(hipEventCreate(&tm->ev[q]));
+    }
+    tm->flops.push_back(2.0 * m * n * (double)k);
+    KAD_HIP_CHECK(hipEventRecord(tm->ev[2 * tm->used], s));
+  }
   if (prec == F64) {
     if (op == OP_SUB)
       gemm_k4_kernel<F64, OP_SUB><<<grid, block, 0, s>>>(m, n, k, (const double*)A, lda, (const double*)B, ldb,
@@ -302,6 +313,10 @@ int gemm_launch(int prec, int op, int m, int n, int k, const void* A, int64_t ld
     return KAD_ERR_ARG;
   }
   KAD_HIP_CHECK(hipGetLastError());
+  if (tm) {
+    KAD_HIP_CHECK(hipEventRecord(tm->ev[2 * tm->used + 1], s));
+    tm->used++;
+  }
   return KAD_OK;
 }
 

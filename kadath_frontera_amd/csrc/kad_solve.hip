@@ -29,6 +29,7 @@ __global__ void cast_mat_d2f_kernel(const double* __restrict__ a, int64_t lda, f
                                     int n);
 
 static constexpr int RED_BLOCKS = 256;
+GemmTimer* g_gemm_timer = nullptr;
 
 int Ctx::ensure(void** p, size_t* cap, size_t bytes) {
   if (*cap >= bytes) return KAD_OK;
@@ -131,6 +132,45 @@ int kad_ctx_destroy(kad_ctx* ctx) {
   Ctx* c = reinterpret_cast<Ctx*>(ctx);
   (void)hipStreamSynchronize(c->stream);
   delete c;
+  return KAD_OK;
+}
+
+// Live timing of the MFMA GEMM launches (on: start a new measurement; off: stop).
+int kad_gemm_timing(kad_ctx* ctx, int on) {
+  if (!ctx) return KAD_ERR_ARG;
+  static kad::GemmTimer timer;
+  if (on) {
+    timer.used = 0;
+    timer.flops.clear();
+    kad::g_gemm_timer = &timer;
+  } else {
+    kad::g_gemm_timer = nullptr;
+  }
+  return KAD_OK;
+}
+
+// Sum of flops and kernel milliseconds of the GEMM launches since kad_gemm_timing(ctx, 1).
+int kad_gemm_timing_read(kad_ctx* ctx, double* flops, double* ms, int* launches) {
+  if (!ctx) return KAD_ERR_ARG;
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  KAD_HIP_CHECK(hipStreamSynchronize(c->stream));
+  static kad::GemmTimer* t = nullptr;
+  (void)t;
+  kad::GemmTimer* tm = kad::g_gemm_timer;
+  double f = 0, tot = 0;
+  int n = 0;
+  if (tm) {
+    for (size_t q = 0; q < tm->used; ++q) {
+      float e = 0;
+      KAD_HIP_CHECK(hipEventElapsedTime(&e, tm->ev[2 * q], tm->ev[2 * q + 1]));
+      tot += e;
+      f += tm->flops[q];
+      ++n;
+    }
+  }
+  if (flops) *flops = f;
+  if (ms) *ms = tot;
+  if (launches) *launches = n;
   return KAD_OK;
 }
 

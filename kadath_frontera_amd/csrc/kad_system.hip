@@ -590,7 +590,7 @@ int kad_xcts_get_coef(kad_system* s, int f, int d, double* out) {
 // solver: 0 fp64 LU; 1 fp32 LU + fp64 IR; 2 fp32 LU + fp64 GMRES-IR; 3 bf16-update LU + fp64 GMRES-IR.
 // Returns 1 when ||F||_inf < precision at entry (converged; no step taken), 0 after a step,
 // < 0 on error, KAD_ERR_SINGULAR (-5) when the factorisation meets an exact zero pivot.
-// times[4] (may be NULL): residual, Jacobian fill, factor+solve, update seconds.
+// times[5] (may be NULL): residual, Jacobian fill, factor+solve, update, factor alone (seconds).
 int kad_xcts_do_newton(kad_system* s, double precision, double* error, int solver, double* times) {
   if (!s || !error || solver < 0 || solver > 3) return KAD_ERR_ARG;
   auto* S = reinterpret_cast<kad_system_impl*>(s);
@@ -628,9 +628,12 @@ int kad_xcts_do_newton(kad_system* s, double precision, double* error, int solve
   if (times) times[1] = t2 - t1;
   // solve J X = F
   int info = 0;
+  double tf0 = 0, tf1 = 0;
   if (solver == 0) {
+    tf0 = now_s();
     if ((rc = kad_lu_factor(reinterpret_cast<kad_ctx*>(S->ctx), KAD_PREC_F64, n, S->J, n, S->ipiv, S->perm, 0)) < 0)
       return rc;
+    tf1 = now_s();
     info = rc;
     if (info == 0) {
       KAD_HIP_CHECK(hipMemcpyAsync(S->X, S->F, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, st));
@@ -640,8 +643,11 @@ int kad_xcts_do_newton(kad_system* s, double precision, double* error, int solve
   } else {
     if ((rc = kad_cast_f64_to_f32(reinterpret_cast<kad_ctx*>(S->ctx), n, S->J, n, S->Jlo, n))) return rc;
     const int prec = (solver == 3) ? KAD_PREC_BF16 : KAD_PREC_F32;
+    KAD_HIP_CHECK(hipStreamSynchronize(st));
+    tf0 = now_s();
     if ((rc = kad_lu_factor(reinterpret_cast<kad_ctx*>(S->ctx), prec, n, S->Jlo, n, S->ipiv, S->perm, 0)) < 0)
       return rc;
+    tf1 = now_s();
     info = rc;
     if (info == 0) {
       rc = kad_ir_solve(reinterpret_cast<kad_ctx*>(S->ctx), n, S->J, n, prec, S->Jlo, n, S->perm, S->F, S->X, 1e-14,
@@ -649,6 +655,7 @@ int kad_xcts_do_newton(kad_system* s, double precision, double* error, int solve
       if (rc < 0 && rc != KAD_ERR_NOCONV) return rc;
     }
   }
+  if (times) times[4] = tf1 - tf0;
   if (info != 0) {
     fprintf(stderr, "Jacobian is singular! Stopping.\n");
     return -5;
@@ -660,6 +667,29 @@ int kad_xcts_do_newton(kad_system* s, double precision, double* error, int solve
   KAD_HIP_CHECK(hipStreamSynchronize(st));
   if (times) times[3] = now_s() - t3;
   return 0;
+}
+
+// Seeded initial perturbation amp * 2^-(i+j+k) * U(-1,1) on every admissible
+// coefficient, in unknown order (HOST out, n_unknowns).  U from splitmix64.
+int kad_xcts_perturbation(kad_system* s, uint64_t seed, double amp, double* out) {
+  if (!s || !out) return KAD_ERR_ARG;
+  auto* S = reinterpret_cast<kad_system_impl*>(s);
+  XctsSystem& sy = *S->sys;
+  uint64_t x = seed;
+  auto next = [&]() {
+    uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= (z >> 31);
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+  };
+  for (auto& L : sy.layout) {
+    const Domain& dm = sy.doms[L[1]];
+    int u = L[2];
+    for (auto& kj : dm.unknown_slots(sy.field_base(L[0], L[1]).bt))
+      for (int i = 0; i < dm.nr; ++i) out[u++] = amp * std::ldexp(1.0, -(i + kj.second + kj.first)) * (2 * next() - 1);
+  }
+  return KAD_OK;
 }
 
 int kad_xcts_last_solve(kad_system* s, int* iters, double* resid) {

@@ -68,6 +68,11 @@ class XctsSystem:
             X0 = np.ascontiguousarray(X0, dtype=np.float64)
             check(lib.kad_xcts_set_guess(self.h, _dp(X0)), "kad_xcts_set_guess")
 
+    def perturbation(self, seed=20261202, amp=1e-3):
+        out = np.zeros(self.n)
+        check(lib.kad_xcts_perturbation(self.h, seed, amp, _dp(out)), "kad_xcts_perturbation")
+        return out
+
     def residual(self):
         F = torch.empty(self.m, dtype=torch.float64, device="cuda")
         self.ctx._enter()
@@ -92,7 +97,7 @@ class XctsSystem:
 
     def do_newton(self, prec, solver="f64"):
         err = ctypes.c_double()
-        times = np.zeros(4)
+        times = np.zeros(5)
         self.ctx._enter()
         rc = lib.kad_xcts_do_newton(self.h, prec, ctypes.byref(err), SOLVERS[solver], _dp(times))
         if rc == -5:
