@@ -72,6 +72,16 @@ int kad_lu_solve(kad_ctx* ctx, int prec, int n, const void* LU, int64_t lda, con
 int kad_gemv64(kad_ctx* ctx, int n, const double* A, int64_t lda, const double* x, const double* b, double* y,
                int trans);
 
+/* A <- diag(R) A diag(C) in place; R, C (device, n) are powers of two chosen
+ * as LAPACK dgeequb does (row max, then column max of the row-scaled matrix),
+ * so the scaling is exact in fp64. */
+int kad_equilibrate(kad_ctx* ctx, int n, double* A, int64_t lda, double* R, double* C);
+/* B = (float)(diag(R) A diag(C)) with R, C computed as by kad_equilibrate; A unchanged. */
+int kad_cast_equilibrated(kad_ctx* ctx, int n, const double* A, int64_t lda, float* B, int64_t ldb, double* R,
+                          double* C);
+/* v <- s .* v (device) */
+int kad_vec_scale(kad_ctx* ctx, int n, const double* s, double* v);
+
 /* B = (float)A, n x n column-major. */
 int kad_cast_f64_to_f32(kad_ctx* ctx, int n, const double* A, int64_t lda, float* B, int64_t ldb);
 
@@ -153,6 +163,13 @@ int kad_xcts_last_solve(kad_system* s, int* iters, double* resid);
  * value instr, tangent instr, value slots, tangent programs; returns #layout entries */
 int kad_xcts_host_info(int nr, int nt, int np, const double* bounds, int nbounds, int dmin, int dmax, double n0,
                        double a, double ome, int* info, int* layout, int cap);
+
+/* As kad_ir_solve when LUlo holds the factors of diag(Rs) A diag(Cs) (see
+ * kad_cast_equilibrated): the preconditioner is v -> Cs .* LU^{-1} (Rs .* v);
+ * residuals and the stopping test use the unscaled A. */
+int kad_ir_solve_scaled(kad_ctx* ctx, int n, const double* A64, int64_t lda, int prec_lo, const void* LUlo,
+                        int64_t ldlu, const int* perm, const double* Rs, const double* Cs, const double* b, double* x,
+                        double tol, int max_outer, int mode, int trans, int* iters, double* resid);
 
 #ifdef __cplusplus
 }

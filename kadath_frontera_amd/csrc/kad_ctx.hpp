@@ -22,8 +22,9 @@ struct LuWork {
   void* tinv = nullptr;         // unit-lower inverse, nb x nb
   void* tmp = nullptr;          // GEMM scratch for L11^{-1} A12
   size_t tmp_bytes = 0;
-  void* tmp2 = nullptr;         // scratch of the recursive triangular inverse
+  void* tmp2 = nullptr;         // scratch of the recursive triangular inverse / panel row gather
   size_t tmp2_bytes = 0;
+  int* map = nullptr;           // 2 x n logical -> physical row maps of the base panel
 };
 
 // Optional live timing of the MFMA GEMM launches (HIP events on the launch
@@ -49,7 +50,7 @@ struct Ctx {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   LuWork lu;
-  size_t cap_redv = 0, cap_redi = 0, cap_tinv = 0;
+  size_t cap_redv = 0, cap_redi = 0, cap_tinv = 0, cap_map = 0;
   double* gemv_part = nullptr;
   size_t cap_part = 0;
   double* vwork = nullptr;

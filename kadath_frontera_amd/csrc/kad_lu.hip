@@ -29,12 +29,8 @@ int gemm_launch(int prec, int op, int m, int n, int k, const void* A, int64_t ld
 constexpr int GOP_SUB = 0, GOP_SET = 1;
 
 constexpr int PANEL_BASE = 32;  // columns factored by the per-column kernel
-constexpr int COL_THREADS = 256;
 
-// ---------------------------------------------------------------------------
-// Pivot search of column j over rows [j, n): first-index max |a|.
-// Last block reduces, records ipiv[j] and swaps rows j <-> p across [c0, c1).
-// ---------------------------------------------------------------------------
+// Block-wide (max |a|, first index) reduction.
 template <typename T>
 __device__ void block_argmax(double v, int idx, double* sv, int* si, double& bv, int& bi) {
   const int tid = threadIdx.x;
@@ -56,125 +52,6 @@ __device__ void block_argmax(double v, int idx, double* sv, int* si, double& bv,
   }
   bv = sv[0];
   bi = si[0];
-}
-
-template <typename T>
-__device__ void finish_pivot(T* A, int64_t lda, int n, int j, int c0, int c1, int* ipiv, double* red_val,
-                             int* red_idx, unsigned* counter, int* info, double* sv, int* si) {
-  // Called by every block after writing its partial (red_val/red_idx[blockIdx.x]).
-  __shared__ bool last;
-  __threadfence();
-  if (threadIdx.x == 0) {
-    unsigned t = atomicAdd(counter, 1u);
-    last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  double v = -1.0;
-  int id = -1;
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
-    const double ov = red_val[b];
-    const int oi = red_idx[b];
-    if (oi >= 0 && (id < 0 || ov > v || (ov == v && oi < id))) {
-      v = ov;
-      id = oi;
-    }
-  }
-  double bv;
-  int bi;
-  block_argmax<T>(v, id, sv, si, bv, bi);
-  if (bi < 0) bi = j;
-  if (threadIdx.x == 0) {
-    ipiv[j] = bi;
-    *counter = 0u;
-    if (bv == 0.0 && *info == 0) *info = j + 1;
-  }
-  if (bi != j) {
-    for (int c = c0 + (int)threadIdx.x; c < c1; c += blockDim.x) {
-      T* pa = A + j + (int64_t)c * lda;
-      T* pb = A + bi + (int64_t)c * lda;
-      const T t = *pa;
-      *pa = *pb;
-      *pb = t;
-    }
-  }
-}
-
-template <typename T>
-__global__ __launch_bounds__(COL_THREADS) void pivot_first_kernel(T* A, int64_t lda, int n, int j, int c0, int c1,
-                                                                  int* ipiv, double* red_val, int* red_idx,
-                                                                  unsigned* counter, int* info) {
-  __shared__ double sv[COL_THREADS];
-  __shared__ int si[COL_THREADS];
-  const int i = j + blockIdx.x * blockDim.x + threadIdx.x;
-  double v = -1.0;
-  int id = -1;
-  if (i < n) {
-    v = fabs((double)A[i + (int64_t)j * lda]);
-    id = i;
-  }
-  double bv;
-  int bi;
-  block_argmax<T>(v, id, sv, si, bv, bi);
-  if (threadIdx.x == 0) {
-    red_val[blockIdx.x] = bv;
-    red_idx[blockIdx.x] = bi;
-  }
-  finish_pivot<T>(A, lda, n, j, c0, c1, ipiv, red_val, red_idx, counter, info, sv, si);
-}
-
-// One elimination step of column j inside the base panel [c0, c1):
-// rows i in (j, n): l = a_ij / a_jj ; a_i,c -= l a_j,c for c in (j, c1);
-// then the pivot of column j+1 is searched and applied (if j+1 < c1).
-// Block = 64 rows x 4 column groups (256 threads); the multiplier l_i is read
-// into LDS before any thread writes it back scaled.
-constexpr int STEP_ROWS = 64;
-template <typename T>
-__global__ __launch_bounds__(COL_THREADS) void panel_step_kernel(T* A, int64_t lda, int n, int j, int c0, int c1,
-                                                                 int* ipiv, double* red_val, int* red_idx,
-                                                                 unsigned* counter, int* info) {
-  __shared__ double sv[COL_THREADS];
-  __shared__ int si[COL_THREADS];
-  __shared__ T urow[PANEL_BASE];
-  __shared__ T lcol[STEP_ROWS];
-  const int w = c1 - j;  // columns j..c1-1
-  const int tx = threadIdx.x % STEP_ROWS, ty = threadIdx.x / STEP_ROWS;
-  for (int c = threadIdx.x; c < w; c += blockDim.x) urow[c] = A[j + (int64_t)(j + c) * lda];
-  const int i = j + 1 + blockIdx.x * STEP_ROWS + tx;
-  __syncthreads();
-  const T piv = urow[0];
-  if (ty == 0) {
-    T l = (i < n) ? A[i + (int64_t)j * lda] : T(0);
-    if (piv != T(0)) l = l / piv;
-    lcol[tx] = l;
-    if (i < n) A[i + (int64_t)j * lda] = l;
-  }
-  __syncthreads();
-  double v = -1.0;
-  int id = -1;
-  if (i < n) {
-    const T l = lcol[tx];
-    T* ai = A + i;
-    for (int c = 1 + ty; c < w; c += 4) {
-      T x = ai[(int64_t)(j + c) * lda];
-      x -= l * urow[c];
-      ai[(int64_t)(j + c) * lda] = x;
-      if (c == 1) {
-        v = fabs((double)x);
-        id = i;
-      }
-    }
-  }
-  if (j + 1 >= c1) return;  // uniform over the grid: no pivot search after the last column
-  double bv;
-  int bi;
-  block_argmax<T>(v, id, sv, si, bv, bi);
-  if (threadIdx.x == 0) {
-    red_val[blockIdx.x] = bv;
-    red_idx[blockIdx.x] = bi;
-  }
-  finish_pivot<T>(A, lda, n, j + 1, c0, c1, ipiv, red_val, red_idx, counter, info, sv, si);
 }
 
 // Row interchanges of pivots [k1, k2) applied to columns [c0, c1): one thread per column.
@@ -307,23 +184,168 @@ int laswp(T* A, int64_t lda, int c0, int c1, const int* ipiv, int k1, int k2, Lu
   return KAD_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Base panel without atomics or device-scope fences.  Inside the panel rows are
+// never moved: a logical -> physical row map (double-buffered per step) tracks
+// the interchanges, and every block re-reduces the previous step's per-block
+// maxima, so all communication between blocks goes through kernel boundaries.
+// The interchange sequence is LAPACK's (first index of max |a| among the
+// logical rows >= j); the panel rows are permuted physically once at the end.
+// ---------------------------------------------------------------------------
+constexpr int FF_ROWS = 64;  // logical rows per block (x 4 column groups = 256 threads)
+
+template <typename T>
+__device__ void ff_reduce_partials(const double* __restrict__ pv, const int* __restrict__ pi, int np, double* sv,
+                                   int* si, double& bv, int& bi) {
+  double v = -1.0;
+  int id = -1;
+  for (int b = threadIdx.x; b < np; b += blockDim.x) {
+    const double ov = pv[b];
+    const int oi = pi[b];
+    if (oi >= 0 && (id < 0 || ov > v || (ov == v && oi < id))) {
+      v = ov;
+      id = oi;
+    }
+  }
+  block_argmax<T>(v, id, sv, si, bv, bi);
+}
+
+// Partial maxima of column c0 over logical rows [c0, n) (identity map).
+template <typename T>
+__global__ __launch_bounds__(256) void ff_first_kernel(const T* __restrict__ A, int64_t lda, int n, int c0,
+                                                       int* __restrict__ map_out, double* __restrict__ pv,
+                                                       int* __restrict__ pi) {
+  __shared__ double sv[256];
+  __shared__ int si[256];
+  const int i = c0 + blockIdx.x * 256 + threadIdx.x;
+  double v = -1.0;
+  int id = -1;
+  if (i < n) {
+    map_out[i] = i;
+    v = fabs((double)A[i + (int64_t)c0 * lda]);
+    id = i;
+  }
+  double bv;
+  int bi;
+  block_argmax<T>(v, id, sv, si, bv, bi);
+  if (threadIdx.x == 0) {
+    pv[blockIdx.x] = bv;
+    pi[blockIdx.x] = bi;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void ff_step_kernel(T* __restrict__ A, int64_t lda, int n, int c0, int j, int c1,
+                                                      const int* __restrict__ map_in, int* __restrict__ map_out,
+                                                      const double* __restrict__ pv_in, const int* __restrict__ pi_in,
+                                                      int np_in, double* __restrict__ pv_out, int* __restrict__ pi_out,
+                                                      int* __restrict__ ipiv, int* __restrict__ info) {
+  __shared__ double sv[256];
+  __shared__ int si[256];
+  __shared__ T urow[PANEL_BASE];
+  __shared__ T lcol[FF_ROWS];
+  __shared__ int pshared;
+  double bv;
+  int p;
+  ff_reduce_partials<T>(pv_in, pi_in, np_in, sv, si, bv, p);
+  if (p < 0) p = j;
+  const int phys_piv = map_in[p];  // physical row that becomes logical row j
+  const int phys_j = map_in[j];    // physical row that moves to logical row p
+  const int w = c1 - j;
+  for (int c = threadIdx.x; c < w; c += blockDim.x) urow[c] = A[phys_piv + (int64_t)(j + c) * lda];
+  if (blockIdx.x == 0) {
+    // logical rows [c0, j) are final: carry them into the other map buffer
+    for (int k = threadIdx.x; k < j - c0; k += blockDim.x) map_out[c0 + k] = map_in[c0 + k];
+    if (threadIdx.x == 0) {
+      ipiv[j] = p;
+      map_out[j] = phys_piv;
+      if (bv == 0.0 && *info == 0) *info = j + 1;
+    }
+  }
+  const int tx = threadIdx.x % FF_ROWS, ty = threadIdx.x / FF_ROWS;
+  const int i = j + 1 + blockIdx.x * FF_ROWS + tx;  // logical row
+  int phys = -1;
+  if (i < n) phys = (i == p) ? phys_j : map_in[i];
+  __syncthreads();
+  const T piv = urow[0];
+  if (ty == 0) {
+    T l = T(0);
+    if (i < n) {
+      l = A[phys + (int64_t)j * lda];
+      if (piv != T(0)) l = l / piv;
+      A[phys + (int64_t)j * lda] = l;
+      map_out[i] = phys;
+    }
+    lcol[tx] = l;
+  }
+  __syncthreads();
+  double v = -1.0;
+  int id = -1;
+  if (i < n) {
+    const T l = lcol[tx];
+    T* ai = A + phys;
+    for (int c = 1 + ty; c < w; c += 4) {
+      T x = ai[(int64_t)(j + c) * lda];
+      x -= l * urow[c];
+      ai[(int64_t)(j + c) * lda] = x;
+      if (c == 1) {
+        v = fabs((double)x);
+        id = i;
+      }
+    }
+  }
+  if (j + 1 >= c1) return;  // uniform: no partials after the last column
+  int bi;
+  block_argmax<T>(v, id, sv, si, bv, bi);
+  if (threadIdx.x == 0) {
+    pv_out[blockIdx.x] = bv;
+    pi_out[blockIdx.x] = bi;
+  }
+  (void)pshared;
+}
+
+// Physical permutation of the panel rows: tmp[i - c0][c] = A[map[i]][c], then back.
+template <typename T>
+__global__ void ff_gather_kernel(const T* __restrict__ A, int64_t lda, int n, int c0, int c1,
+                                 const int* __restrict__ map, T* __restrict__ tmp) {
+  const int i = c0 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = c0 + blockIdx.y;
+  if (i < n && c < c1) tmp[(i - c0) + (int64_t)(c - c0) * (n - c0)] = A[map[i] + (int64_t)c * lda];
+}
+template <typename T>
+__global__ void ff_scatter_kernel(T* __restrict__ A, int64_t lda, int n, int c0, int c1, const T* __restrict__ tmp) {
+  const int i = c0 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = c0 + blockIdx.y;
+  if (i < n && c < c1) A[i + (int64_t)c * lda] = tmp[(i - c0) + (int64_t)(c - c0) * (n - c0)];
+}
+
 // Base panel: columns [c0, c1) (c1 - c0 <= PANEL_BASE), rows [c0, n).
 template <typename T>
 int panel_base(T* A, int64_t lda, int n, int c0, int c1, int* ipiv, LuWork& w) {
-  const int nb0 = (int)cdiv(n - c0, COL_THREADS);
-  if (nb0 > w.red_cap) return KAD_ERR_ARG;
-  pivot_first_kernel<T><<<nb0, COL_THREADS, 0, w.stream>>>(A, lda, n, c0, c0, c1, ipiv, w.red_val, w.red_idx,
-                                                            w.counter, w.info);
-  KAD_HIP_CHECK(hipGetLastError());
+  const int nb0 = (int)cdiv(n - c0, 256);
+  const int nbs = (int)cdiv(n - c0, FF_ROWS);
+  if (nb0 > w.red_cap || nbs > w.red_cap) return KAD_ERR_ARG;
+  int* map[2] = {w.map, w.map + n};
+  double* pv[2] = {w.red_val, w.red_val + w.red_cap};
+  int* pi[2] = {w.red_idx, w.red_idx + w.red_cap};
+  ff_first_kernel<T><<<nb0, 256, 0, w.stream>>>(A, lda, n, c0, map[0], pv[0], pi[0]);
+  int np = nb0, cur = 0;
   for (int j = c0; j < c1; ++j) {
     const int rows = n - j - 1;
-    if (rows <= 0) break;
-    const int nbk = (int)cdiv(rows, STEP_ROWS);
-    if (nbk > w.red_cap) return KAD_ERR_ARG;
-    panel_step_kernel<T><<<nbk, COL_THREADS, 0, w.stream>>>(A, lda, n, j, c0, c1, ipiv, w.red_val, w.red_idx,
-                                                             w.counter, w.info);
-    KAD_HIP_CHECK(hipGetLastError());
+    const int nbk = rows > 0 ? (int)cdiv(rows, FF_ROWS) : 1;
+    ff_step_kernel<T><<<nbk, 256, 0, w.stream>>>(A, lda, n, c0, j, c1, map[cur], map[cur ^ 1], pv[cur], pi[cur], np,
+                                                 pv[cur ^ 1], pi[cur ^ 1], ipiv, w.info);
+    np = nbk;
+    cur ^= 1;
   }
+  KAD_HIP_CHECK(hipGetLastError());
+  // map[cur] now holds the final logical -> physical order of rows [c0, n)
+  const size_t need = (size_t)(n - c0) * (c1 - c0) * sizeof(T);
+  if (need > w.tmp2_bytes) return KAD_ERR_ARG;
+  dim3 g((unsigned)cdiv(n - c0, 256), c1 - c0);
+  ff_gather_kernel<T><<<g, 256, 0, w.stream>>>(A, lda, n, c0, c1, map[cur], (T*)w.tmp2);
+  ff_scatter_kernel<T><<<g, 256, 0, w.stream>>>(A, lda, n, c0, c1, (const T*)w.tmp2);
+  KAD_HIP_CHECK(hipGetLastError());
   return KAD_OK;
 }
 
@@ -597,18 +619,110 @@ __global__ void cast_mat_d2f_kernel(const double* __restrict__ a, int64_t lda, f
   const int c = blockIdx.y;
   if (i < n) b[i + (int64_t)c * ldb] = (float)a[i + (int64_t)c * lda];
 }
-// Block reduction for max |x| and for dot products (fixed order, two passes).
-__global__ __launch_bounds__(256) void absmax_partial_kernel(const double* x, int n, double* part) {
-  __shared__ double red[256];
+// Equilibration by powers of two (LAPACK dgeequb-style, exact in fp64):
+// R_i = 2^-round(log2 max_j |a_ij|), then C_j = 2^-round(log2 max_i |R_i a_ij|).
+__device__ inline double pow2_recip(double m) {
+  if (!(m > 0.0) || !isfinite(m)) return 1.0;
+  int e;
+  frexp(m, &e);  // m = f 2^e, f in [0.5, 1)
+  return ldexp(1.0, 1 - e);
+}
+__global__ __launch_bounds__(256) void row_absmax_kernel(const double* __restrict__ A, int64_t lda, int n,
+                                                         double* __restrict__ R) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
   double m = 0.0;
-  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) m = fmax(m, fabs(x[i]));
+  for (int c = 0; c < n; ++c) m = fmax(m, fabs(A[i + (int64_t)c * lda]));
+  R[i] = pow2_recip(m);
+}
+__global__ __launch_bounds__(256) void col_scale_kernel(double* __restrict__ A, int64_t lda, int n,
+                                                        const double* __restrict__ R, double* __restrict__ C) {
+  __shared__ double red[256];
+  const int c = blockIdx.x;
+  double* a = A + (int64_t)c * lda;
+  double m = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) m = fmax(m, fabs(R[i] * a[i]));
   red[threadIdx.x] = m;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
     if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
     __syncthreads();
   }
+  const double cj = pow2_recip(red[0]);
+  if (threadIdx.x == 0) C[c] = cj;
+  for (int i = threadIdx.x; i < n; i += 256) a[i] = R[i] * a[i] * cj;
+}
+// Column scales of diag(R) A without modifying A.
+__global__ __launch_bounds__(256) void col_scale_only_kernel(const double* __restrict__ A, int64_t lda, int n,
+                                                             const double* __restrict__ R, double* __restrict__ C) {
+  __shared__ double red[256];
+  const int c = blockIdx.x;
+  const double* a = A + (int64_t)c * lda;
+  double m = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) m = fmax(m, fabs(R[i] * a[i]));
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) C[c] = pow2_recip(red[0]);
+}
+// B = (float)(R_i A_ij C_j)
+__global__ void cast_scaled_kernel(const double* __restrict__ a, int64_t lda, float* __restrict__ b, int64_t ldb,
+                                   int n, const double* __restrict__ R, const double* __restrict__ C) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (i < n) b[i + (int64_t)c * ldb] = (float)(R[i] * a[i + (int64_t)c * lda] * C[c]);
+}
+__global__ void vec_scale_kernel(int n, const double* __restrict__ s, double* __restrict__ v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] *= s[i];
+}
+
+// Block reduction for max |x| and for dot products (fixed order, two passes).
+__global__ __launch_bounds__(256) void absmax_partial_kernel(const double* x, int n, double* part) {
+  __shared__ double red[256];
+  double m = 0.0;
+  // NaN propagates (fmax would drop it): a NaN residual must never read as converged
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    const double a = fabs(x[i]);
+    m = (a > m || a != a) ? a : m;
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      const double o = red[threadIdx.x + s];
+      if (o > red[threadIdx.x] || o != o) red[threadIdx.x] = o;
+    }
+    __syncthreads();
+  }
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+// h[i] = <V_i, w> for i < k  (V: k vectors of length n, contiguous), one workgroup per i
+__global__ __launch_bounds__(256) void multidot_kernel(const double* __restrict__ V, int n, int k,
+                                                       const double* __restrict__ w, double* __restrict__ h) {
+  __shared__ double red[256];
+  const double* v = V + (int64_t)blockIdx.x * n;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) s += v[i] * w[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) h[blockIdx.x] = red[0];
+}
+// w -= sum_i h[i] V_i   (or, with sub = 0, w = sum_i h[i] V_i)
+__global__ void multiaxpy_kernel(const double* __restrict__ V, int n, int k, const double* __restrict__ h,
+                                 double* __restrict__ w, int sub) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  double s = 0.0;
+  for (int i = 0; i < k; ++i) s += h[i] * V[(int64_t)i * n + e];
+  w[e] = sub ? w[e] - s : s;
 }
 __global__ __launch_bounds__(256) void dot_partial_kernel(const double* x, const double* y, int n, double* part) {
   __shared__ double red[256];

@@ -25,8 +25,20 @@ int gemv64(const double* A, int64_t lda, int n, const double* x, const double* b
 __global__ void axpy_kernel(int n, double a, const double* x, double* y);
 __global__ void absmax_partial_kernel(const double* x, int n, double* part);
 __global__ void dot_partial_kernel(const double* x, const double* y, int n, double* part);
+__global__ void multidot_kernel(const double* __restrict__ V, int n, int k, const double* __restrict__ w,
+                                double* __restrict__ h);
+__global__ void multiaxpy_kernel(const double* __restrict__ V, int n, int k, const double* __restrict__ h,
+                                 double* __restrict__ w, int sub);
 __global__ void cast_mat_d2f_kernel(const double* __restrict__ a, int64_t lda, float* __restrict__ b, int64_t ldb,
                                     int n);
+__global__ void row_absmax_kernel(const double* __restrict__ A, int64_t lda, int n, double* __restrict__ R);
+__global__ void col_scale_kernel(double* __restrict__ A, int64_t lda, int n, const double* __restrict__ R,
+                                 double* __restrict__ C);
+__global__ void vec_scale_kernel(int n, const double* __restrict__ s, double* __restrict__ v);
+__global__ void col_scale_only_kernel(const double* __restrict__ A, int64_t lda, int n, const double* __restrict__ R,
+                                      double* __restrict__ C);
+__global__ void cast_scaled_kernel(const double* __restrict__ a, int64_t lda, float* __restrict__ b, int64_t ldb,
+                                   int n, const double* __restrict__ R, const double* __restrict__ C);
 
 static constexpr int RED_BLOCKS = 256;
 GemmTimer* g_gemm_timer = nullptr;
@@ -44,8 +56,9 @@ int Ctx::ensure(void** p, size_t* cap, size_t bytes) {
 int Ctx::lu_work(int n, int nb) {
   const int red = (int)cdiv(n, 64) + 1;
   int rc;
-  if ((rc = ensure((void**)&lu.red_val, &cap_redv, sizeof(double) * red))) return rc;
-  if ((rc = ensure((void**)&lu.red_idx, &cap_redi, sizeof(int) * red))) return rc;
+  if ((rc = ensure((void**)&lu.red_val, &cap_redv, 2 * sizeof(double) * red))) return rc;
+  if ((rc = ensure((void**)&lu.red_idx, &cap_redi, 2 * sizeof(int) * red))) return rc;
+  if ((rc = ensure((void**)&lu.map, &cap_map, 2 * sizeof(int) * (size_t)n))) return rc;
   lu.red_cap = red;
   if (!lu.counter) {
     KAD_HIP_CHECK(hipMalloc((void**)&lu.counter, 64));
@@ -56,7 +69,7 @@ int Ctx::lu_work(int n, int nb) {
   // GEMM scratch for L11^{-1} A12: nb x chunk (bounded at 64 MiB).
   const size_t want = std::min<size_t>((size_t)nb * (size_t)n * 8, (size_t)64 << 20);
   if ((rc = ensure(&lu.tmp, &lu.tmp_bytes, std::max<size_t>(want, (size_t)nb * nb * 8)))) return rc;
-  if ((rc = ensure(&lu.tmp2, &lu.tmp2_bytes, (size_t)nb * nb * 8))) return rc;
+  if ((rc = ensure(&lu.tmp2, &lu.tmp2_bytes, std::max<size_t>((size_t)nb * nb * 8, (size_t)n * 32 * 8)))) return rc;
   lu.stream = stream;
   return KAD_OK;
 }
@@ -75,7 +88,8 @@ double Ctx::absmax(const double* x, int n) {
   (void)hipMemcpyAsync(h, gemv_part, sizeof(h), hipMemcpyDeviceToHost, stream);
   (void)hipStreamSynchronize(stream);
   double m = 0.0;
-  for (int i = 0; i < RED_BLOCKS; ++i) m = std::max(m, h[i]);
+  for (int i = 0; i < RED_BLOCKS; ++i)
+    if (h[i] > m || h[i] != h[i]) m = h[i];
   return m;
 }
 
@@ -101,7 +115,7 @@ int Ctx::upload(const void* host, size_t bytes, void** dev) {
 Ctx::~Ctx() {
   for (auto& kv : plans) delete kv.second;
   for (void* p : owned) (void)hipFree(p);
-  void* bufs[] = {lu.red_val, lu.red_idx, lu.counter, lu.tinv, lu.tmp, lu.tmp2, gemv_part, vwork, krylov};
+  void* bufs[] = {lu.red_val, lu.red_idx, lu.counter, lu.tinv, lu.tmp, lu.tmp2, lu.map, gemv_part, vwork, krylov};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (own_stream && stream) (void)hipStreamDestroy(stream);
@@ -241,6 +255,39 @@ int kad_gemv64(kad_ctx* ctx, int n, const double* A, int64_t lda, const double* 
   return kad::gemv64(A, lda, n, x, b, y, trans, c->gemv_part, c->stream);
 }
 
+// A <- diag(R) A diag(C) in place with power-of-two R, C (device vectors, n each).
+int kad_equilibrate(kad_ctx* ctx, int n, double* A, int64_t lda, double* R, double* C) {
+  if (!ctx || n <= 0 || lda < n || !A || !R || !C) return KAD_ERR_ARG;
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  kad::row_absmax_kernel<<<(unsigned)kad::cdiv(n, 256), 256, 0, c->stream>>>(A, lda, n, R);
+  kad::col_scale_kernel<<<n, 256, 0, c->stream>>>(A, lda, n, R, C);
+  KAD_HIP_CHECK(hipGetLastError());
+  return KAD_OK;
+}
+
+// Equilibrated low-precision copy: computes R, C (powers of two, as
+// kad_equilibrate) and B = (float)(diag(R) A diag(C)) without modifying A.
+int kad_cast_equilibrated(kad_ctx* ctx, int n, const double* A, int64_t lda, float* B, int64_t ldb, double* R,
+                          double* C) {
+  if (!ctx || n <= 0 || lda < n || ldb < n || !A || !B || !R || !C) return KAD_ERR_ARG;
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  kad::row_absmax_kernel<<<(unsigned)kad::cdiv(n, 256), 256, 0, c->stream>>>(A, lda, n, R);
+  kad::col_scale_only_kernel<<<n, 256, 0, c->stream>>>(A, lda, n, R, C);
+  dim3 g((unsigned)kad::cdiv(n, 256), n);
+  kad::cast_scaled_kernel<<<g, 256, 0, c->stream>>>(A, lda, B, ldb, n, R, C);
+  KAD_HIP_CHECK(hipGetLastError());
+  return KAD_OK;
+}
+
+// v <- s * v (elementwise, device)
+int kad_vec_scale(kad_ctx* ctx, int n, const double* s, double* v) {
+  if (!ctx || n <= 0 || !s || !v) return KAD_ERR_ARG;
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  kad::vec_scale_kernel<<<(unsigned)kad::cdiv(n, 256), 256, 0, c->stream>>>(n, s, v);
+  KAD_HIP_CHECK(hipGetLastError());
+  return KAD_OK;
+}
+
 // fp64 n x n column-major -> fp32 copy (device).
 int kad_cast_f64_to_f32(kad_ctx* ctx, int n, const double* A, int64_t lda, float* B, int64_t ldb) {
   if (!ctx || n <= 0 || lda < n || ldb < n) return KAD_ERR_ARG;
@@ -256,9 +303,23 @@ int kad_cast_f64_to_f32(kad_ctx* ctx, int n, const double* A, int64_t lda, float
 // mode 1: GMRES-IR (left preconditioned by the low-precision factors, CGS2,
 // restart m).  Stops when ||b - A x||_inf <= tol * ||b||_inf.  trans = 1 solves A^T x = b.
 // iters[0] = outer steps, iters[1] = total inner GMRES steps; resid = final relative residual.
+int kad_ir_solve_scaled(kad_ctx* ctx, int n, const double* A64, int64_t lda, int prec_lo, const void* LUlo,
+                        int64_t ldlu, const int* perm, const double* Rs, const double* Cs, const double* b, double* x,
+                        double tol, int max_outer, int mode, int trans, int* iters, double* resid);
+
 int kad_ir_solve(kad_ctx* ctx, int n, const double* A64, int64_t lda, int prec_lo, const void* LUlo, int64_t ldlu,
                  const int* perm, const double* b, double* x, double tol, int max_outer, int mode, int trans,
                  int* iters, double* resid) {
+  return kad_ir_solve_scaled(ctx, n, A64, lda, prec_lo, LUlo, ldlu, perm, nullptr, nullptr, b, x, tol, max_outer,
+                             mode, trans, iters, resid);
+}
+
+// As kad_ir_solve, with the low-precision factors being those of diag(Rs) A diag(Cs):
+// the preconditioner applies v -> Cs .* LU^{-1} (Rs .* v) (Rs <-> Cs for the transpose),
+// while residuals and the stopping test use the unscaled A.
+int kad_ir_solve_scaled(kad_ctx* ctx, int n, const double* A64, int64_t lda, int prec_lo, const void* LUlo,
+                        int64_t ldlu, const int* perm, const double* Rs, const double* Cs, const double* b, double* x,
+                        double tol, int max_outer, int mode, int trans, int* iters, double* resid) {
   if (!ctx || n <= 0 || !A64 || !LUlo || !perm || !b || !x) return KAD_ERR_ARG;
   Ctx* c = reinterpret_cast<Ctx*>(ctx);
   int rc = c->vec_work(n);
@@ -267,21 +328,28 @@ int kad_ir_solve(kad_ctx* ctx, int n, const double* A64, int64_t lda, int prec_l
   const unsigned g = (unsigned)kad::cdiv(n, 256);
   double* r = c->vwork + (size_t)2 * n;  // vwork[0:n] is getrs scratch
   double* d = c->vwork + (size_t)3 * n;
+  const double* pre = trans ? Cs : Rs;   // applied before the factor solve
+  const double* post = trans ? Rs : Cs;  // applied after
   auto solve_lo = [&](double* v) {
-    if (prec_lo == kad::F64) return kad::getrs<double>((const double*)LUlo, ldlu, n, perm, v, trans, c->vwork, s);
-    return kad::getrs<float>((const float*)LUlo, ldlu, n, perm, v, trans, c->vwork, s);
+    if (pre) kad::vec_scale_kernel<<<g, 256, 0, s>>>(n, pre, v);
+    int r;
+    if (prec_lo == kad::F64) r = kad::getrs<double>((const double*)LUlo, ldlu, n, perm, v, trans, c->vwork, s);
+    else r = kad::getrs<float>((const float*)LUlo, ldlu, n, perm, v, trans, c->vwork, s);
+    if (!r && post) kad::vec_scale_kernel<<<g, 256, 0, s>>>(n, post, v);
+    return r;
   };
   const double bnorm = c->absmax(b, n);
   KAD_HIP_CHECK(hipMemsetAsync(x, 0, n * sizeof(double), s));
   KAD_HIP_CHECK(hipMemcpyAsync(r, b, n * sizeof(double), hipMemcpyDeviceToDevice, s));
   int outer = 0, inner_total = 0;
   double rel = (bnorm == 0.0) ? 0.0 : 1.0;
-  const int m = 30;
+  const int m = 60;           // Krylov dimension per outer step
+  const double inner_tol = 1e-12;
   std::vector<double> H, cs, sn, gvec, y;
   double* V = nullptr;
   double* hbuf = nullptr;
   if (mode == 1) {
-    rc = c->ensure((void**)&c->krylov, &c->cap_krylov, (size_t)(m + 1) * n * sizeof(double) + 64 * sizeof(double));
+    rc = c->ensure((void**)&c->krylov, &c->cap_krylov, (size_t)(m + 1) * n * sizeof(double) + (m + 8) * sizeof(double));
     if (rc) return rc;
     V = c->krylov;
     hbuf = c->krylov + (size_t)(m + 1) * n;
@@ -292,11 +360,12 @@ int kad_ir_solve(kad_ctx* ctx, int n, const double* A64, int64_t lda, int prec_l
       if ((rc = solve_lo(d))) return rc;
       kad::axpy_kernel<<<g, 256, 0, s>>>(n, 1.0, d, x);
     } else {
-      // GMRES on M^{-1} A d = M^{-1} r, M = low-precision LU.
+      // Right-preconditioned GMRES on A M^{-1} u = r (M = low-precision LU), CGS2
+      // with batched projections; the Givens residual is the true inner residual.
+      double* w = V + (size_t)m * n;       // work vector (slot m of the basis buffer is spare)
+      const double beta = std::sqrt(c->dot(r, r, n));
+      if (!(beta > 0.0)) break;
       KAD_HIP_CHECK(hipMemcpyAsync(V, r, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-      if ((rc = solve_lo(V))) return rc;
-      const double beta = std::sqrt(c->dot(V, V, n));
-      if (beta == 0.0) break;
       kad::axpy_kernel<<<g, 256, 0, s>>>(n, 1.0 / beta - 1.0, V, V);
       H.assign((size_t)(m + 1) * m, 0.0);
       cs.assign(m, 0.0);
@@ -304,27 +373,22 @@ int kad_ir_solve(kad_ctx* ctx, int n, const double* A64, int64_t lda, int prec_l
       gvec.assign(m + 1, 0.0);
       gvec[0] = beta;
       int kdim = 0;
-      const double inner_tol = 1e-10;
-      for (int j = 0; j < m; ++j) {
-        double* w = V + (size_t)(j + 1) * n;
-        if (trans) {
-          if ((rc = kad::gemv64(A64, lda, n, V + (size_t)j * n, nullptr, w, 1, c->gemv_part, s))) return rc;
-        } else {
-          if ((rc = kad::gemv64(A64, lda, n, V + (size_t)j * n, nullptr, w, 0, c->gemv_part, s))) return rc;
-        }
-        if ((rc = solve_lo(w))) return rc;
-        // CGS2 against V[0..j]
+      std::vector<double> hh(m + 1);
+      for (int j = 0; j < m - 1; ++j) {
+        double* vj1 = V + (size_t)(j + 1) * n;
+        KAD_HIP_CHECK(hipMemcpyAsync(d, V + (size_t)j * n, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        if ((rc = solve_lo(d))) return rc;  // z = M^{-1} v_j
+        if ((rc = kad::gemv64(A64, lda, n, d, nullptr, vj1, trans, c->gemv_part, s))) return rc;
         for (int pass = 0; pass < 2; ++pass) {
-          for (int i = 0; i <= j; ++i) {
-            const double h = c->dot(V + (size_t)i * n, w, n);
-            H[(size_t)i * m + j] += h;
-            kad::axpy_kernel<<<g, 256, 0, s>>>(n, -h, V + (size_t)i * n, w);
-          }
+          kad::multidot_kernel<<<j + 1, 256, 0, s>>>(V, n, j + 1, vj1, hbuf);
+          kad::multiaxpy_kernel<<<g, 256, 0, s>>>(V, n, j + 1, hbuf, vj1, 1);
+          KAD_HIP_CHECK(hipMemcpyAsync(hh.data(), hbuf, (j + 1) * sizeof(double), hipMemcpyDeviceToHost, s));
+          KAD_HIP_CHECK(hipStreamSynchronize(s));
+          for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] += hh[i];
         }
-        const double hn = std::sqrt(c->dot(w, w, n));
+        const double hn = std::sqrt(c->dot(vj1, vj1, n));
         H[(size_t)(j + 1) * m + j] = hn;
-        if (hn > 0.0) kad::axpy_kernel<<<g, 256, 0, s>>>(n, 1.0 / hn - 1.0, w, w);
-        // Givens
+        if (hn > 0.0) kad::axpy_kernel<<<g, 256, 0, s>>>(n, 1.0 / hn - 1.0, vj1, vj1);
         for (int i = 0; i < j; ++i) {
           const double a = H[(size_t)i * m + j], bb = H[(size_t)(i + 1) * m + j];
           H[(size_t)i * m + j] = cs[i] * a + sn[i] * bb;
@@ -348,8 +412,10 @@ int kad_ir_solve(kad_ctx* ctx, int n, const double* A64, int64_t lda, int prec_l
         for (int l = i + 1; l < kdim; ++l) sacc -= H[(size_t)i * m + l] * y[l];
         y[i] = sacc / H[(size_t)i * m + i];
       }
-      for (int i = 0; i < kdim; ++i) kad::axpy_kernel<<<g, 256, 0, s>>>(n, y[i], V + (size_t)i * n, x);
-      (void)hbuf;
+      KAD_HIP_CHECK(hipMemcpyAsync(hbuf, y.data(), kdim * sizeof(double), hipMemcpyHostToDevice, s));
+      kad::multiaxpy_kernel<<<g, 256, 0, s>>>(V, n, kdim, hbuf, w, 0);  // u = V y
+      if ((rc = solve_lo(w))) return rc;                                // x += M^{-1} u
+      kad::axpy_kernel<<<g, 256, 0, s>>>(n, 1.0, w, x);
     }
     // r = b - op(A) x in fp64
     if ((rc = kad::gemv64(A64, lda, n, x, b, r, trans, c->gemv_part, s))) return rc;

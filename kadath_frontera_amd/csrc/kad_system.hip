@@ -177,6 +177,8 @@ struct kad_system_impl {
   int* perm = nullptr;
   double* X = nullptr;
   double* Fsave = nullptr;
+  double* Rs = nullptr;  // equilibration scales
+  double* Cs = nullptr;
   std::vector<void*> allocs;
   double last_times[4] = {0, 0, 0, 0};
   int last_iters[2] = {0, 0};
@@ -641,7 +643,14 @@ int kad_xcts_do_newton(kad_system* s, double precision, double* error, int solve
         return rc;
     }
   } else {
-    if ((rc = kad_cast_f64_to_f32(reinterpret_cast<kad_ctx*>(S->ctx), n, S->J, n, S->Jlo, n))) return rc;
+    // low-precision factors of the equilibrated R J C (exact powers of two); the
+    // refinement keeps the unscaled fp64 J, so its stopping test is on ||F - J X||.
+    if (!S->Rs) {
+      if ((rc = S->dmalloc((void**)&S->Rs, (size_t)n * sizeof(double)))) return rc;
+      if ((rc = S->dmalloc((void**)&S->Cs, (size_t)n * sizeof(double)))) return rc;
+    }
+    if ((rc = kad_cast_equilibrated(reinterpret_cast<kad_ctx*>(S->ctx), n, S->J, n, S->Jlo, n, S->Rs, S->Cs)))
+      return rc;
     const int prec = (solver == 3) ? KAD_PREC_BF16 : KAD_PREC_F32;
     KAD_HIP_CHECK(hipStreamSynchronize(st));
     tf0 = now_s();
@@ -650,8 +659,9 @@ int kad_xcts_do_newton(kad_system* s, double precision, double* error, int solve
     tf1 = now_s();
     info = rc;
     if (info == 0) {
-      rc = kad_ir_solve(reinterpret_cast<kad_ctx*>(S->ctx), n, S->J, n, prec, S->Jlo, n, S->perm, S->F, S->X, 1e-14,
-                        solver == 1 ? 30 : 10, solver == 1 ? 0 : 1, 0, S->last_iters, &S->last_resid);
+      rc = kad_ir_solve_scaled(reinterpret_cast<kad_ctx*>(S->ctx), n, S->J, n, prec, S->Jlo, n, S->perm, S->Rs, S->Cs,
+                               S->F, S->X, 1e-13, solver == 1 ? 30 : 8, solver == 1 ? 0 : 1, 0, S->last_iters,
+                               &S->last_resid);
       if (rc < 0 && rc != KAD_ERR_NOCONV) return rc;
     }
   }
