@@ -87,9 +87,10 @@ int kad_cast_f64_to_f32(kad_ctx* ctx, int n, const double* A, int64_t lda, float
 
 /* Mixed-precision solve: fp64 residuals with A64, corrections from the
  * low-precision factors (prec_lo, LUlo, perm).  mode 0 = classical IR,
- * mode 1 = GMRES-IR.  Stops at ||b - op(A) x||_inf <= tol ||b||_inf.
- * iters[0] = outer steps, iters[1] = inner GMRES steps.  Returns KAD_ERR_NOCONV
- * if max_outer steps do not reach tol (x holds the last iterate). */
+ * mode 1 = GMRES-IR.  Stops at ||b - op(A) x||_inf <= tol ||b||_inf, or when an
+ * outer step gains less than a factor 2 (a step that increases the residual is
+ * undone).  iters[0] = outer steps, iters[1] = inner GMRES steps.  Returns
+ * KAD_ERR_NOCONV if tol is not reached (x holds the best iterate). */
 int kad_ir_solve(kad_ctx* ctx, int n, const double* A64, int64_t lda, int prec_lo, const void* LUlo, int64_t ldlu,
                  const int* perm, const double* b, double* x, double tol, int max_outer, int mode, int trans,
                  int* iters, double* resid);

@@ -57,6 +57,8 @@ struct Ctx {
   size_t cap_vwork = 0;
   double* krylov = nullptr;
   size_t cap_krylov = 0;
+  double* trs_tab = nullptr;    // triangular-solve tables of the current factors
+  size_t cap_trs_tab = 0;
 
   std::map<std::string, DevPlan*> plans;  // transform plans by key
   std::vector<void*> owned;                // device buffers freed with the context
@@ -65,6 +67,8 @@ struct Ctx {
   int upload(const void* host, size_t bytes, void** dev);
   int lu_work(int n, int nb);
   int vec_work(int n);
+  int solve_prepare(int prec, int n, const void* LU, int64_t lda, int trans);
+  int solve(int prec, int n, const void* LU, int64_t lda, const int* perm, double* b, int trans);
   double absmax(const double* x, int n);
   double dot(const double* x, const double* y, int n);
   ~Ctx();

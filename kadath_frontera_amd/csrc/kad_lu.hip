@@ -399,136 +399,6 @@ int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w)
 template int getrf<double>(double*, int64_t, int, int*, int, int, LuWork&);
 template int getrf<float>(float*, int64_t, int, int*, int, int, LuWork&);
 
-// ---------------------------------------------------------------------------
-// Triangular solves (vector in fp64; matrix entries of type T read and widened).
-// ---------------------------------------------------------------------------
-constexpr int TB = 128;  // diagonal block of the blocked triangular solves
-
-// Solve a TB diagonal block in place: lower unit (mode 0), upper non-unit (1),
-// upper-transposed = lower non-unit U^T (2), lower-transposed = upper unit L^T (3).
-template <typename T>
-__global__ __launch_bounds__(TB) void trsv_diag_kernel(const T* A, int64_t lda, int k0, int nbk, double* x, int mode) {
-  __shared__ double xs[TB];
-  __shared__ double M[TB][TB + 1];
-  const int t = threadIdx.x;
-  for (int c = 0; c < nbk; ++c)
-    if (t < nbk) M[t][c] = (double)A[(k0 + t) + (int64_t)(k0 + c) * lda];
-  if (t < nbk) xs[t] = x[k0 + t];
-  __syncthreads();
-  if (mode == 0) {  // L y = b, unit lower
-    for (int j = 0; j < nbk; ++j) {
-      if (t > j && t < nbk) xs[t] -= M[t][j] * xs[j];
-      __syncthreads();
-    }
-  } else if (mode == 1) {  // U x = y, upper
-    for (int j = nbk - 1; j >= 0; --j) {
-      if (t == j) xs[j] = xs[j] / M[j][j];
-      __syncthreads();
-      if (t < j) xs[t] -= M[t][j] * xs[j];
-      __syncthreads();
-    }
-  } else if (mode == 2) {  // U^T z = b : lower with diag, (U^T)[t][j] = U[j][t]
-    for (int j = 0; j < nbk; ++j) {
-      if (t == j) xs[j] = xs[j] / M[j][j];
-      __syncthreads();
-      if (t > j && t < nbk) xs[t] -= M[j][t] * xs[j];
-      __syncthreads();
-    }
-  } else {  // L^T w = z : upper unit, (L^T)[t][j] = L[j][t]
-    for (int j = nbk - 1; j >= 0; --j) {
-      if (t < j) xs[t] -= M[j][t] * xs[j];
-      __syncthreads();
-    }
-  }
-  if (t < nbk) x[k0 + t] = xs[t];
-}
-
-// x[r0:r1] -= A[r0:r1, c0:c1] * x[c0:c1]   (column-major panel, threads over rows)
-template <typename T>
-__global__ __launch_bounds__(256) void gemv_panel_n_kernel(const T* A, int64_t lda, int r0, int r1, int c0, int c1,
-                                                           double* x) {
-  __shared__ double xs[TB];
-  const int nc = c1 - c0;
-  for (int c = threadIdx.x; c < nc; c += blockDim.x) xs[c] = x[c0 + c];
-  __syncthreads();
-  const int i = r0 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= r1) return;
-  double s = 0.0;
-  const T* a = A + i + (int64_t)c0 * lda;
-  for (int c = 0; c < nc; ++c) s += (double)a[(int64_t)c * lda] * xs[c];
-  x[i] -= s;
-}
-
-// x[c0:c1] -= A[r0:r1, c0:c1]^T * x[r0:r1]   (one workgroup per column, reduction over rows)
-template <typename T>
-__global__ __launch_bounds__(256) void gemv_panel_t_kernel(const T* A, int64_t lda, int r0, int r1, int c0,
-                                                           double* x) {
-  __shared__ double red[256];
-  const int c = c0 + blockIdx.x;
-  const T* a = A + (int64_t)c * lda;
-  double s = 0.0;
-  for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) s += (double)a[i] * x[i];
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
-    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) x[c] -= red[0];
-}
-
-__global__ void gather_kernel(const double* __restrict__ src, const int* __restrict__ perm, double* dst, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dst[i] = src[perm[i]];
-}
-__global__ void scatter_kernel(const double* __restrict__ src, const int* __restrict__ perm, double* dst, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dst[perm[i]] = src[i];
-}
-
-// Solve op(A) x = b with the factors; b is overwritten by x.  perm[i] = original
-// row that ends at position i (P b gathers b[perm]).  work: n doubles.
-template <typename T>
-int getrs(const T* LU, int64_t lda, int n, const int* perm, double* b, int trans, double* work, hipStream_t s) {
-  const unsigned g = (unsigned)cdiv(n, 256);
-  if (!trans) {
-    gather_kernel<<<g, 256, 0, s>>>(b, perm, work, n);
-    for (int k0 = 0; k0 < n; k0 += TB) {
-      const int nbk = std::min(TB, n - k0);
-      trsv_diag_kernel<T><<<1, TB, 0, s>>>(LU, lda, k0, nbk, work, 0);
-      if (k0 + nbk < n)
-        gemv_panel_n_kernel<T><<<(unsigned)cdiv(n - k0 - nbk, 256), 256, 0, s>>>(LU, lda, k0 + nbk, n, k0, k0 + nbk,
-                                                                                  work);
-    }
-    const int last = ((n - 1) / TB) * TB;
-    for (int k0 = last; k0 >= 0; k0 -= TB) {
-      const int nbk = std::min(TB, n - k0);
-      trsv_diag_kernel<T><<<1, TB, 0, s>>>(LU, lda, k0, nbk, work, 1);
-      if (k0 > 0) gemv_panel_n_kernel<T><<<(unsigned)cdiv(k0, 256), 256, 0, s>>>(LU, lda, 0, k0, k0, k0 + nbk, work);
-    }
-    KAD_HIP_CHECK(hipMemcpyAsync(b, work, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-  } else {
-    // A^T = U^T L^T P : solve U^T z = b, L^T w = z, x = P^T w.
-    KAD_HIP_CHECK(hipMemcpyAsync(work, b, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-    for (int k0 = 0; k0 < n; k0 += TB) {
-      const int nbk = std::min(TB, n - k0);
-      if (k0 > 0) gemv_panel_t_kernel<T><<<nbk, 256, 0, s>>>(LU, lda, 0, k0, k0, work);
-      trsv_diag_kernel<T><<<1, TB, 0, s>>>(LU, lda, k0, nbk, work, 2);
-    }
-    const int last = ((n - 1) / TB) * TB;
-    for (int k0 = last; k0 >= 0; k0 -= TB) {
-      const int nbk = std::min(TB, n - k0);
-      if (k0 + nbk < n) gemv_panel_t_kernel<T><<<nbk, 256, 0, s>>>(LU, lda, k0 + nbk, n, k0, work);
-      trsv_diag_kernel<T><<<1, TB, 0, s>>>(LU, lda, k0, nbk, work, 3);
-    }
-    scatter_kernel<<<g, 256, 0, s>>>(work, perm, b, n);
-  }
-  KAD_HIP_CHECK(hipGetLastError());
-  return KAD_OK;
-}
-template int getrs<double>(const double*, int64_t, int, const int*, double*, int, double*, hipStream_t);
-template int getrs<float>(const float*, int64_t, int, const int*, double*, int, double*, hipStream_t);
-
 // perm from LAPACK-style ipiv (host side, O(n)).
 void ipiv_to_perm(const int* ipiv, int n, int* perm) {
   for (int i = 0; i < n; ++i) perm[i] = i;

@@ -17,8 +17,11 @@ namespace kad {
 int gemm_launch(int prec, int op, int m, int n, int k, const void* A, int64_t lda, const void* B, int64_t ldb, void* C,
                 int64_t ldc, hipStream_t s);
 template <typename T> int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w);
+size_t trsv_table_doubles(int n);
+template <typename T> int trsv_prepare(const T* LU, int64_t lda, int n, int trans, double* W, hipStream_t s);
 template <typename T>
-int getrs(const T* LU, int64_t lda, int n, const int* perm, double* b, int trans, double* work, hipStream_t s);
+int getrs_fast(const T* LU, int64_t lda, int n, const int* perm, const double* W, double* b, int trans,
+               double* work, hipStream_t s);
 void ipiv_to_perm(const int* ipiv, int n, int* perm);
 int gemv64(const double* A, int64_t lda, int n, const double* x, const double* b, double* y, int trans, double* part,
            hipStream_t s);
@@ -78,8 +81,22 @@ int Ctx::vec_work(int n) {
   int rc;
   const size_t part = (size_t)cdiv(n, 1024) * n * sizeof(double);
   if ((rc = ensure((void**)&gemv_part, &cap_part, std::max<size_t>(part, RED_BLOCKS * sizeof(double))))) return rc;
-  if ((rc = ensure((void**)&vwork, &cap_vwork, (size_t)6 * n * sizeof(double)))) return rc;
+  if ((rc = ensure((void**)&vwork, &cap_vwork, (size_t)8 * n * sizeof(double)))) return rc;
+  if ((rc = ensure((void**)&trs_tab, &cap_trs_tab, trsv_table_doubles(n) * sizeof(double)))) return rc;
   return KAD_OK;
+}
+
+// Triangular-solve tables of one set of factors (for op(A), trans as in solve),
+// then b <- op(A)^{-1} b with them.
+int Ctx::solve_prepare(int prec, int n, const void* LU, int64_t lda, int trans) {
+  if (prec == F64) return trsv_prepare<double>((const double*)LU, lda, n, trans, trs_tab, stream);
+  return trsv_prepare<float>((const float*)LU, lda, n, trans, trs_tab, stream);
+}
+
+int Ctx::solve(int prec, int n, const void* LU, int64_t lda, const int* perm, double* b, int trans) {
+  double* w = vwork + (size_t)4 * n;  // 3 n doubles; vwork[0:4n] belongs to the refinement loop
+  if (prec == F64) return getrs_fast<double>((const double*)LU, lda, n, perm, trs_tab, b, trans, w, stream);
+  return getrs_fast<float>((const float*)LU, lda, n, perm, trs_tab, b, trans, w, stream);
 }
 
 double Ctx::absmax(const double* x, int n) {
@@ -115,7 +132,7 @@ int Ctx::upload(const void* host, size_t bytes, void** dev) {
 Ctx::~Ctx() {
   for (auto& kv : plans) delete kv.second;
   for (void* p : owned) (void)hipFree(p);
-  void* bufs[] = {lu.red_val, lu.red_idx, lu.counter, lu.tinv, lu.tmp, lu.tmp2, lu.map, gemv_part, vwork, krylov};
+  void* bufs[] = {lu.red_val, lu.red_idx, lu.counter, lu.tinv, lu.tmp, lu.tmp2, lu.map, gemv_part, vwork, krylov, trs_tab};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (own_stream && stream) (void)hipStreamDestroy(stream);
@@ -241,9 +258,8 @@ int kad_lu_solve(kad_ctx* ctx, int prec, int n, const void* LU, int64_t lda, con
   Ctx* c = reinterpret_cast<Ctx*>(ctx);
   int rc = c->vec_work(n);
   if (rc) return rc;
-  if (prec == kad::F64)
-    return kad::getrs<double>((const double*)LU, lda, n, perm, b, trans, c->vwork, c->stream);
-  return kad::getrs<float>((const float*)LU, lda, n, perm, b, trans, c->vwork, c->stream);
+  if ((rc = c->solve_prepare(prec, n, LU, lda, trans))) return rc;
+  return c->solve(prec, n, LU, lda, perm, b, trans);
 }
 
 int kad_gemv64(kad_ctx* ctx, int n, const double* A, int64_t lda, const double* x, const double* b, double* y,
@@ -300,8 +316,10 @@ int kad_cast_f64_to_f32(kad_ctx* ctx, int n, const double* A, int64_t lda, float
 
 // Mixed-precision refinement: x = A^{-1} b with A given in fp64 (A64, for the
 // residual) and factored in prec_lo (LUlo, perm).  mode 0: classical IR;
-// mode 1: GMRES-IR (left preconditioned by the low-precision factors, CGS2,
-// restart m).  Stops when ||b - A x||_inf <= tol * ||b||_inf.  trans = 1 solves A^T x = b.
+// mode 1: GMRES-IR (right preconditioned by the low-precision factors, CGS2,
+// restart m).  Stops when ||b - A x||_inf <= tol * ||b||_inf, or at the attainable
+// accuracy: an outer step that gains less than a factor 2 ends the loop, and one
+// that increases the residual is undone.  trans = 1 solves A^T x = b.
 // iters[0] = outer steps, iters[1] = total inner GMRES steps; resid = final relative residual.
 int kad_ir_solve_scaled(kad_ctx* ctx, int n, const double* A64, int64_t lda, int prec_lo, const void* LUlo,
                         int64_t ldlu, const int* perm, const double* Rs, const double* Cs, const double* b, double* x,
@@ -326,15 +344,14 @@ int kad_ir_solve_scaled(kad_ctx* ctx, int n, const double* A64, int64_t lda, int
   if (rc) return rc;
   hipStream_t s = c->stream;
   const unsigned g = (unsigned)kad::cdiv(n, 256);
-  double* r = c->vwork + (size_t)2 * n;  // vwork[0:n] is getrs scratch
+  double* r = c->vwork + (size_t)2 * n;
   double* d = c->vwork + (size_t)3 * n;
+  if ((rc = c->solve_prepare(prec_lo, n, LUlo, ldlu, trans))) return rc;
   const double* pre = trans ? Cs : Rs;   // applied before the factor solve
   const double* post = trans ? Rs : Cs;  // applied after
   auto solve_lo = [&](double* v) {
     if (pre) kad::vec_scale_kernel<<<g, 256, 0, s>>>(n, pre, v);
-    int r;
-    if (prec_lo == kad::F64) r = kad::getrs<double>((const double*)LUlo, ldlu, n, perm, v, trans, c->vwork, s);
-    else r = kad::getrs<float>((const float*)LUlo, ldlu, n, perm, v, trans, c->vwork, s);
+    const int r = c->solve(prec_lo, n, LUlo, ldlu, perm, v, trans);
     if (!r && post) kad::vec_scale_kernel<<<g, 256, 0, s>>>(n, post, v);
     return r;
   };
@@ -354,11 +371,13 @@ int kad_ir_solve_scaled(kad_ctx* ctx, int n, const double* A64, int64_t lda, int
     V = c->krylov;
     hbuf = c->krylov + (size_t)(m + 1) * n;
   }
+  double* corr = d;  // the correction added by the current outer step
   while (rel > tol && outer < max_outer) {
     if (mode == 0) {
       KAD_HIP_CHECK(hipMemcpyAsync(d, r, n * sizeof(double), hipMemcpyDeviceToDevice, s));
       if ((rc = solve_lo(d))) return rc;
       kad::axpy_kernel<<<g, 256, 0, s>>>(n, 1.0, d, x);
+      corr = d;
     } else {
       // Right-preconditioned GMRES on A M^{-1} u = r (M = low-precision LU), CGS2
       // with batched projections; the Givens residual is the true inner residual.
@@ -404,7 +423,8 @@ int kad_ir_solve_scaled(kad_ctx* ctx, int n, const double* A64, int64_t lda, int
         gvec[j] = cs[j] * gvec[j];
         kdim = j + 1;
         ++inner_total;
-        if (std::fabs(gvec[j + 1]) <= inner_tol * beta || hn == 0.0) break;
+        // |g_{j+1}| is ||r - A M^{-1} V y||_2 >= the inf-norm the outer test uses
+        if (std::fabs(gvec[j + 1]) <= std::max(inner_tol * beta, 0.5 * tol * bnorm) || hn == 0.0) break;
       }
       y.assign(kdim, 0.0);
       for (int i = kdim - 1; i >= 0; --i) {
@@ -416,12 +436,22 @@ int kad_ir_solve_scaled(kad_ctx* ctx, int n, const double* A64, int64_t lda, int
       kad::multiaxpy_kernel<<<g, 256, 0, s>>>(V, n, kdim, hbuf, w, 0);  // u = V y
       if ((rc = solve_lo(w))) return rc;                                // x += M^{-1} u
       kad::axpy_kernel<<<g, 256, 0, s>>>(n, 1.0, w, x);
+      corr = w;
     }
     // r = b - op(A) x in fp64
     if ((rc = kad::gemv64(A64, lda, n, x, b, r, trans, c->gemv_part, s))) return rc;
     const double rn = c->absmax(r, n);
-    rel = (bnorm == 0.0) ? 0.0 : rn / bnorm;
+    const double rel_new = (bnorm == 0.0) ? 0.0 : rn / bnorm;
     ++outer;
+    if (rel_new > rel) {
+      // the step made the residual worse (attainable accuracy reached): undo it
+      kad::axpy_kernel<<<g, 256, 0, s>>>(n, -1.0, corr, x);
+      if ((rc = kad::gemv64(A64, lda, n, x, b, r, trans, c->gemv_part, s))) return rc;
+      break;
+    }
+    const bool stalled = outer > 1 && rel_new > 0.5 * rel;
+    rel = rel_new;
+    if (stalled) break;  // less than a halving per step: stop at the attainable accuracy
   }
   if (iters) {
     iters[0] = outer;

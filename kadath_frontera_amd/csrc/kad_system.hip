@@ -20,8 +20,6 @@
 
 namespace kad {
 template <typename T> int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w);
-template <typename T>
-int getrs(const T* LU, int64_t lda, int n, const int* perm, double* b, int trans, double* work, hipStream_t s);
 void ipiv_to_perm(const int* ipiv, int n, int* perm);
 }  // namespace kad
 
@@ -659,8 +657,12 @@ int kad_xcts_do_newton(kad_system* s, double precision, double* error, int solve
     tf1 = now_s();
     info = rc;
     if (info == 0) {
+      // Inexact-Newton forcing term: ||F - J X|| <= eta ||F|| with eta ~ ||F|| (quadratic
+      // rate kept), capped at 1e-4, and small enough that the step that ends the
+      // iteration leaves a linear residual of at most precision / 10.
+      const double eta = std::max(1e-14, std::min(1e-4, std::max(0.1 * err, 0.1 * precision / err)));
       rc = kad_ir_solve_scaled(reinterpret_cast<kad_ctx*>(S->ctx), n, S->J, n, prec, S->Jlo, n, S->perm, S->Rs, S->Cs,
-                               S->F, S->X, 1e-13, solver == 1 ? 30 : 8, solver == 1 ? 0 : 1, 0, S->last_iters,
+                               S->F, S->X, eta, solver == 1 ? 30 : 8, solver == 1 ? 0 : 1, 0, S->last_iters,
                                &S->last_resid);
       if (rc < 0 && rc != KAD_ERR_NOCONV) return rc;
     }
