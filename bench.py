@@ -5,11 +5,13 @@ Space_spheric with 5 solved domains (4 shells + compactified) at 17x13x12
 collocation points, n = 58905 unknowns, dense Jacobian 27.8 GB in FP64; seeded
 1e-3 perturbation of the flat guess; horizon rotation Ome = 0.05.
 
-One "step" = one System_of_eqs::do_newton iteration from the perturbed guess:
-residual F(u) + batched-JVP Jacobian fill + factorisation + solve (+ fp64
-refinement) + field update, all device-resident.  The headline solver is
-FP32-LU + FP64 GMRES-IR; the plain FP64-LU step is timed beside it, and both are
-then iterated to ||F||_inf < 1e-10 and their converged fields compared.
+One "step" = one System_of_eqs::do_newton iteration: residual F(u) + batched-JVP
+Jacobian fill + factorisation + solve (+ fp64 refinement) + field update, all
+device-resident.  The K timed steps are the iterations of the Newton solve from
+the perturbed guess (restarting from the guess once ||F||_inf < 1e-10).  The
+headline solver is FP32-LU + FP64 GMRES-IR; the plain FP64-LU solve runs beside
+it, both to ||F||_inf < 1e-10, with time to solution and the converged fields
+compared.
 
 Prints ONE JSON line (rank 0).
 """
@@ -69,18 +71,22 @@ def cpu_baseline(budget_s=25.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=6, help="timed Newton iterations")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--solver", default="f32-gmres", choices=["f64", "f32-ir", "f32-gmres", "bf16-gmres"])
     ap.add_argument("--batch", type=int, default=512)
+    ap.add_argument("--order", type=int, default=1, choices=[0, 1],
+                    help="unknown/row order: 0 reference, 1 domain-major (block-tridiagonal J)")
+    ap.add_argument("--max-newton", type=int, default=12)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-converge", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 solve beside the headline solver")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only "
-                                   "(see DESIGN.md), --gpus must be 1"}), flush=True)
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only, "
+                                   "--gpus must be 1"}), flush=True)
         sys.exit(2)
 
+    import ctypes
     import numpy as np
     import torch
     from kadath_frontera_amd.device import Context
@@ -88,73 +94,88 @@ def main():
     from kadath_frontera_amd._lib import lib
 
     ctx = Context(0)
-    sysd = XctsSystem(ctx, RES, BOUNDS, n0=N0, a=A, ome=OME, batch=args.batch)
+    sysd = XctsSystem(ctx, RES, BOUNDS, n0=N0, a=A, ome=OME, batch=args.batch, order=args.order)
     n = sysd.n
     X0 = sysd.perturbation(SEED)
     prec_of = {"f64": "f64", "f32-ir": "f32", "f32-gmres": "f32", "bf16-gmres": "bf16"}
+    prec = 1e-10
 
-    def one_step(solver, gemm_timing=False):
-        sysd.set_guess(X0)
+    def log(msg):
+        print("[bench] " + msg, file=sys.stderr, flush=True)
+
+    def newton_iter(solver):
+        """One do_newton call; returns (converged_at_entry, |F| at entry, wall s, phases, ir stats)."""
         torch.cuda.synchronize()
-        if gemm_timing:
-            lib.kad_gemm_timing(ctx.h, 1)
         t0 = time.perf_counter()
-        conv, err, times, ir = sysd.do_newton(1e-10, solver)
+        conv, err, times, ir = sysd.do_newton(prec, solver)
         torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        print(f"[bench] {solver} step {dt:.3f} s phases {[round(x, 4) for x in times]} ir {ir} |F| {err:.3e}",
-              file=sys.stderr, flush=True)
-        g = None
-        if gemm_timing:
-            import ctypes
-            fl, ms, nl = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
-            lib.kad_gemm_timing_read(ctx.h, ctypes.byref(fl), ctypes.byref(ms), ctypes.byref(nl))
-            lib.kad_gemm_timing(ctx.h, 0)
-            g = (fl.value, ms.value, nl.value)
-        return dt, times, ir, g
+        return conv, err, time.perf_counter() - t0, times, ir
 
+    def solve(solver, timed_steps=None):
+        """Newton from the perturbed guess until ||F||_inf < 1e-10.  Returns the per-iteration
+        records, the time to solution and the converged coefficients."""
+        sysd.set_guess(X0)
+        recs, t_total, conv = [], 0.0, False
+        for it in range(args.max_newton + 1):
+            conv, err, dt, times, ir = newton_iter(solver)
+            t_total += dt
+            log(f"{solver} it {it} |F| {err:.3e} {dt:.3f} s phases {[round(float(x), 4) for x in times]} ir {ir}")
+            if conv:
+                break
+            recs.append((err, dt, list(times), ir))
+        fields = np.concatenate([sysd.coef(f, d).ravel() for f in range(5) for d in range(1, len(BOUNDS) + 1)])
+        return recs, t_total, conv, err, fields
+
+    # warmup: first iteration(s) from the guess (allocations, kernel loads)
     for _ in range(args.warmup):
-        one_step(args.solver)
-    steps = [one_step(args.solver) for _ in range(args.steps)]
-    ms_step = 1e3 * float(np.mean([s[0] for s in steps]))
-    t_fact = float(np.mean([s[1][4] for s in steps]))
-    lu_tflops = (2.0 / 3.0) * n ** 3 / t_fact / 1e12
-    phases = np.mean([s[1] for s in steps], axis=0)
-    # live GEMM roofline (HIP events on the launch stream) over one step
-    _, _, _, g = one_step(args.solver, gemm_timing=True)
-    gemm_flops, gemm_ms, gemm_n = g
-    achieved = gemm_flops / (gemm_ms / 1e3) / 1e12
-    peak = PEAK[prec_of[args.solver]]
-    # plain FP64 step beside it
-    f64 = None
-    if args.solver != "f64":
-        one_step("f64")
-        d64 = [one_step("f64") for _ in range(max(1, min(2, args.steps)))]
-        f64 = {"ms_per_step": 1e3 * float(np.mean([d[0] for d in d64])),
-               "lu_tflops": (2.0 / 3.0) * n ** 3 / float(np.mean([d[1][4] for d in d64])) / 1e12}
-    # convergence of both solvers to ||F||_inf < 1e-10 and field agreement
-    conv_info = None
-    if not args.no_converge:
-        fields = {}
-        iters = {}
-        for solver in ([args.solver, "f64"] if args.solver != "f64" else ["f64"]):
+        sysd.set_guess(X0)
+        newton_iter(args.solver)
+    # timed: K Newton iterations along the converging sequence (restarting from the
+    # guess when it has converged); every timed call is a full iteration
+    sysd.set_guess(X0)
+    step_times, step_phases, step_ir = [], [], []
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    while len(step_times) < args.steps:
+        conv, err, dt, times, ir = newton_iter(args.solver)
+        if conv:
             sysd.set_guess(X0)
-            it, conv, err = 0, False, None
-            while not conv and it < 12:
-                conv, err, _, _ = sysd.do_newton(1e-10, solver)
-                print(f"[bench] converge {solver} it {it} |F| {err:.3e}", file=sys.stderr, flush=True)
-                it += 1
-            iters[solver] = (it - 1, err, conv)
-            fields[solver] = np.concatenate([sysd.coef(f, d).ravel() for f in range(5)
-                                             for d in range(1, len(BOUNDS) + 1)])
-        diff = None
-        if len(fields) == 2:
-            a, b = fields[args.solver], fields["f64"]
-            diff = float(np.abs(a - b).max() / max(1.0, np.abs(b).max()))
-        conv_info = {"newton_iters": {k: v[0] for k, v in iters.items()},
-                     "final_Finf": {k: v[1] for k, v in iters.items()},
-                     "converged": all(v[2] for v in iters.values()),
-                     "fields_rel_diff_vs_fp64": diff}
+            continue
+        step_times.append(dt)
+        step_phases.append(list(times))
+        step_ir.append(ir)
+    torch.cuda.synchronize()
+    t_region = time.perf_counter() - t_start
+    ms_step = 1e3 * float(np.mean(step_times))
+    phases = np.mean(step_phases, axis=0)
+    t_fact = float(np.mean([p[4] for p in step_phases]))
+    lu_tflops = (2.0 / 3.0) * n ** 3 / t_fact / 1e12
+    log(f"timed {len(step_times)} iterations: {ms_step:.1f} ms/iteration (region {t_region:.2f} s)")
+
+    # live GEMM roofline (HIP events on the launch stream) over one iteration from the guess
+    sysd.set_guess(X0)
+    lib.kad_gemm_timing(ctx.h, 1)
+    newton_iter(args.solver)
+    fl, gms, nl = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+    lib.kad_gemm_timing_read(ctx.h, ctypes.byref(fl), ctypes.byref(gms), ctypes.byref(nl))
+    lib.kad_gemm_timing(ctx.h, 0)
+    achieved = fl.value / (gms.value / 1e3) / 1e12
+    peak = PEAK[prec_of[args.solver]]
+
+    # time to solution, headline solver and plain FP64 beside it
+    runs = {args.solver: solve(args.solver)}
+    if args.solver != "f64" and not args.no_fp64:
+        runs["f64"] = solve("f64")
+    conv_info = {k: {"newton_iters": len(v[0]), "time_to_solution_s": v[1], "converged": v[2], "final_Finf": v[3],
+                     "ms_per_iteration": 1e3 * float(np.mean([r[1] for r in v[0]])) if v[0] else None,
+                     "factor_s_per_iteration": float(np.mean([r[2][4] for r in v[0]])) if v[0] else None,
+                     "gmres_iters": [r[3][1] for r in v[0]] if k != "f64" else None}
+                 for k, v in runs.items()}
+    diff = None
+    if "f64" in runs and args.solver != "f64":
+        a, b = runs[args.solver][4], runs["f64"][4]
+        diff = float(np.abs(a - b).max() / max(1.0, np.abs(b).max()))
+    f64 = conv_info.get("f64") if args.solver != "f64" else None
     cpu = None if args.no_cpu_baseline else cpu_baseline()
     out = {
         "metric": "Newton-step s + LU TFLOP/s: FP32/BF16+IR vs FP64 at ||F||inf<1e-10, 1 GPU",
@@ -170,18 +191,21 @@ def main():
         "dtype": {"f64": "f64", "f32-ir": "f32-lu+f64-ir", "f32-gmres": "f32-lu+f64-gmres-ir",
                   "bf16-gmres": "bf16-lu+f64-gmres-ir"}[args.solver],
         "data": "synthetic (seeded 1e-3 perturbation of the flat guess)",
-        "config": {"workload": "spinning-BH XCTS Newton step, 5 spherical domains 17x13x12, n=%d" % n,
-                   "n_unknowns": n, "solver": args.solver, "jvp_batch": args.batch},
+        "config": {"workload": "spinning-BH XCTS Newton iteration, 5 spherical domains 17x13x12, n=%d" % n,
+                   "n_unknowns": n, "solver": args.solver, "jvp_batch": args.batch,
+                   "order": ["reference", "domain-major"][args.order]},
         "lu_tflops": lu_tflops,
         "phases_s": {"residual": phases[0], "jacobian_fill": phases[1], "factor_solve": phases[2],
                      "update": phases[3], "factor": phases[4]},
-        "fp64_step": f64,
-        "speedup_vs_fp64": (f64["ms_per_step"] / ms_step) if f64 else None,
-        "convergence": conv_info,
+        "fp64": f64,
+        "speedup_vs_fp64": {"per_iteration": f64["ms_per_iteration"] / ms_step,
+                            "time_to_solution": f64["time_to_solution_s"] / conv_info[args.solver][
+                                "time_to_solution_s"]} if f64 else None,
+        "convergence": {"solvers": conv_info, "fields_rel_diff_vs_fp64": diff},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": None,
-                     "kernel": "gemm trailing update (%d launches, %.3e flop, %.1f ms)" % (gemm_n, gemm_flops,
-                                                                                        gemm_ms)},
+                     "kernel": "LU trailing-update GEMM (%d launches, %.3e flop, %.1f ms in one iteration)" % (
+                         nl.value, fl.value, gms.value)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

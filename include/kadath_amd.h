@@ -95,6 +95,13 @@ int kad_ir_solve(kad_ctx* ctx, int n, const double* A64, int64_t lda, int prec_l
                  const int* perm, const double* b, double* x, double tol, int max_outer, int mode, int trans,
                  int* iters, double* resid);
 
+/* GMRES-IR restart length of the context (default 60, at most 4096). */
+int kad_ir_set_restart(kad_ctx* ctx, int m);
+/* Residual history of the last refinement (HOST out): kind 0 = inner GMRES
+ * residual 2-norms, kind 1 = ||b||_inf then ||b - op(A) x||_inf after each outer
+ * step.  Returns the entry count. */
+int kad_ir_history(kad_ctx* ctx, int kind, double* out, int cap);
+
 /* ---- spectral transforms (replace Base_spectral::coef / coef_i,
  *      src/Coef/coef.cpp:96-104, src/Coef/coef_i.cpp:96-104, and the 1-D
  *      FFTW-backed dispatch coef_1d / coef_i_1d, src/Coef/coef_1d.cpp:607-636,
@@ -131,15 +138,22 @@ typedef struct kad_system kad_system;
 /* XCTS black-hole system (codes/Par_version/Kerr/kerr.cpp:69-99 in the Cartesian
  * basis) on Space_spheric(CHEB_TYPE, 0, (nr,nt,np), bounds[0..nbounds)): solved
  * from shell dmin to the compactified domain dmax = nbounds.  n0: lapse on the
- * excision sphere r = a; ome: horizon rotation.  batch: Jacobian columns per
- * tangent launch chain. */
+ * excision sphere r = a; ome: horizon rotation.  order 0: the reference's unknown
+ * and row order (system_of_eqs.cpp:597-612, kerr.cpp:84-99); order 1: domain-major
+ * (unknowns by domain then field, rows by the equation's domain with matching
+ * d|d+1 rows in d+1), which makes J block-tridiagonal by domain -- same system,
+ * rows and columns permuted.  batch: Jacobian columns per tangent launch chain. */
 int kad_xcts_create(kad_ctx* ctx, int nr, int nt, int np, const double* bounds, int nbounds, int dmin, int dmax,
-                    double n0, double a, double ome, int batch, kad_system** out);
+                    double n0, double a, double ome, int order, int batch, kad_system** out);
 int kad_xcts_destroy(kad_system* s);
 /* stats (may be NULL): value instructions, tangent instructions, value slots, tangent slots */
 int kad_xcts_sizes(kad_system* s, int* n_unknowns, int* n_rows, int* stats);
 /* 4 ints per (field, domain): field (0 P, 1 N, 2..4 bet^x,y,z), domain, first unknown, count */
 int kad_xcts_layout(kad_system* s, int* out, int cap);
+/* 5 ints per (equation, component) in row order: kind (0 inside, 1 matching
+ * d|d+1, 2 outer bc, 3 inner bc), domain, component, first row, row count
+ * (the per-equation row blocks of sec_member, solver.cpp:49-85) */
+int kad_xcts_row_layout(kad_system* s, int* out, int cap);
 /* fields := flat guess + affecte_tau(X0); X0 HOST (n_unknowns) or NULL */
 int kad_xcts_set_guess(kad_system* s, const double* X0);
 /* F(u) (device, n_rows): System_of_eqs::sec_member */
@@ -156,14 +170,15 @@ int kad_xcts_get_coef(kad_system* s, int f, int d, double* out);
  * times[5]: residual, Jacobian fill, factor + solve (incl. refinement), update,
  * factorisation alone (seconds). */
 int kad_xcts_do_newton(kad_system* s, double precision, double* error, int solver, double* times);
-/* seeded perturbation amp 2^-(i+j+k) U(-1,1) of every unknown (splitmix64), HOST out */
+/* seeded perturbation amp 2^-(i+j+k) U(-1,1) of every unknown (splitmix64, drawn in
+ * the reference unknown order whatever the system's order), HOST out */
 int kad_xcts_perturbation(kad_system* s, uint64_t seed, double amp, double* out);
 /* refinement statistics of the last low-precision solve */
 int kad_xcts_last_solve(kad_system* s, int* iters, double* resid);
 /* host-only tracing of the same system (no device): info[6] = n_unknowns, n_rows,
  * value instr, tangent instr, value slots, tangent programs; returns #layout entries */
 int kad_xcts_host_info(int nr, int nt, int np, const double* bounds, int nbounds, int dmin, int dmax, double n0,
-                       double a, double ome, int* info, int* layout, int cap);
+                       double a, double ome, int order, int* info, int* layout, int cap);
 
 /* As kad_ir_solve when LUlo holds the factors of diag(Rs) A diag(Cs) (see
  * kad_cast_equilibrated): the preconditioner is v -> Cs .* LU^{-1} (Rs .* v);

@@ -43,10 +43,11 @@ SIGNATURES = {
     "kad_host_transform_matrix": (c_int, [c_int, c_int, c_int, c_dp, c_int]),
     "kad_host_op1d_matrix": (c_int, [c_int, c_int, c_int, c_dp, c_int, c_ip]),
     "kad_xcts_create": (c_int, [c_vp, c_int, c_int, c_int, c_dp, c_int, c_int, c_int, c_dbl, c_dbl, c_dbl, c_int,
-                                ctypes.POINTER(c_vp)]),
+                                c_int, ctypes.POINTER(c_vp)]),
     "kad_xcts_destroy": (c_int, [c_vp]),
     "kad_xcts_sizes": (c_int, [c_vp, c_ip, c_ip, c_ip]),
     "kad_xcts_layout": (c_int, [c_vp, c_ip, c_int]),
+    "kad_xcts_row_layout": (c_int, [c_vp, c_ip, c_int]),
     "kad_xcts_set_guess": (c_int, [c_vp, c_dp]),
     "kad_xcts_residual": (c_int, [c_vp, c_vp]),
     "kad_xcts_jacobian": (c_int, [c_vp, c_vp, c_i64]),
@@ -61,9 +62,11 @@ SIGNATURES = {
     "kad_ir_solve_scaled": (c_int, [c_vp, c_int, c_vp, c_i64, c_int, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl,
                                     c_int, c_int, c_int, c_ip, c_dp]),
     "kad_vec_scale": (c_int, [c_vp, c_int, c_vp, c_vp]),
+    "kad_ir_set_restart": (c_int, [c_vp, c_int]),
+    "kad_ir_history": (c_int, [c_vp, c_int, c_dp, c_int]),
     "kad_gemm_timing_read": (c_int, [c_vp, c_dp, c_dp, c_ip]),
-    "kad_xcts_host_info": (c_int, [c_int, c_int, c_int, c_dp, c_int, c_int, c_int, c_dbl, c_dbl, c_dbl, c_ip, c_ip,
-                                   c_int]),
+    "kad_xcts_host_info": (c_int, [c_int, c_int, c_int, c_dp, c_int, c_int, c_int, c_dbl, c_dbl, c_dbl, c_int, c_ip,
+                                   c_ip, c_int]),
 }
 
 for _name, (_res, _args) in SIGNATURES.items():

@@ -1,5 +1,6 @@
 // XCTS system traced into device programs.  See xcts.hpp.
 #include "xcts.hpp"
+#include <algorithm>
 #include <cmath>
 #include <stdexcept>
 #include <map>
@@ -294,26 +295,68 @@ void XctsSystem::trace_tangent(int f, int d) {
   tr.tan = nullptr;
 }
 
+// Domain-major rows: equation blocks move as a whole (their components stay
+// contiguous), ordered by the domain they live in and, inside a domain, as in the
+// reference order.  The value program's export rows are shifted accordingly;
+// tangent programs are traced afterwards from the updated row0.
+void XctsSystem::reorder_rows() {
+  struct Blk {
+    int old0, len, new0;
+  };
+  std::vector<Blk> blks;
+  int row = 0;
+  for (int dom = prm.dmin; dom <= prm.dmax; ++dom)
+    for (auto& e : eqs) {
+      const int rd = e.kind == 1 ? e.d + 1 : e.d;
+      if (rd != dom) continue;
+      int len = 0;
+      for (int c : e.nrows) len += c;
+      blks.push_back({e.row0, len, row});
+      e.row0 = row;
+      row += len;
+    }
+  if (row != n_rows) throw std::runtime_error("row reordering lost rows");
+  std::sort(blks.begin(), blks.end(), [](const Blk& x, const Blk& y) { return x.old0 < y.old0; });
+  for (auto& in : tr.val.code) {
+    if (in.kind != I_EXPORT_IN && in.kind != I_EXPORT_BD) continue;
+    auto it = std::upper_bound(blks.begin(), blks.end(), in.row0, [](int r, const Blk& b) { return r < b.old0; });
+    if (it == blks.begin()) throw std::runtime_error("export row outside every equation block");
+    --it;
+    if (in.row0 >= it->old0 + it->len) throw std::runtime_error("export row outside its equation block");
+    in.row0 += it->new0 - it->old0;
+  }
+}
+
 void XctsSystem::build() {
   trace_value();
-  // unknown layout (system_of_eqs.cpp:597-612): P, N by domain; bet by domain with x, y, z together
+  if (prm.order == 1) reorder_rows();
+  // unknown layout (system_of_eqs.cpp:597-612): P, N by domain; bet by domain with x, y, z
+  // together -- or, domain-major, all five fields of a domain together
   layout.clear();
   tau_maps.clear();
   int off = 0;
-  const std::vector<std::vector<int>> vars = {{0}, {1}, {2, 3, 4}};
-  for (auto& var : vars)
+  std::vector<std::pair<int, int>> order_fd;  // (field, domain) in column order
+  if (prm.order == 1) {
     for (int d = prm.dmin; d <= prm.dmax; ++d)
-      for (int f : var) {
-        const auto bt = field_base(f, d).bt;
-        const int cnt = doms[d].nbr_unknowns(bt);
-        layout.push_back({f, d, off, cnt});
-        Csr m;
-        int used = 0;
-        doms[d].affecte_tau_map(bt, m.rowptr, m.col, m.w, &used);
-        if (used != cnt) throw std::runtime_error("affecte_tau consumes a different count than nbr_unknowns");
-        tau_maps.push_back(m);
-        off += cnt;
-      }
+      for (int f = 0; f < NF; ++f) order_fd.push_back({f, d});
+  } else {
+    const std::vector<std::vector<int>> vars = {{0}, {1}, {2, 3, 4}};
+    for (auto& var : vars)
+      for (int d = prm.dmin; d <= prm.dmax; ++d)
+        for (int f : var) order_fd.push_back({f, d});
+  }
+  for (auto& fd : order_fd) {
+    const int f = fd.first, d = fd.second;
+    const auto bt = field_base(f, d).bt;
+    const int cnt = doms[d].nbr_unknowns(bt);
+    layout.push_back({f, d, off, cnt});
+    Csr m;
+    int used = 0;
+    doms[d].affecte_tau_map(bt, m.rowptr, m.col, m.w, &used);
+    if (used != cnt) throw std::runtime_error("affecte_tau consumes a different count than nbr_unknowns");
+    tau_maps.push_back(m);
+    off += cnt;
+  }
   n_unknowns = off;
   if (n_unknowns != n_rows) throw std::runtime_error("Number of unknowns is different from the number of equations");
   for (auto& L : layout) trace_tangent(L[0], L[1]);

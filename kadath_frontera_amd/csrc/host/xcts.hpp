@@ -25,6 +25,11 @@ struct XctsParams {
   std::vector<double> bounds;  // Space_spheric bounds: nucleus [0, b0], shells, compact [b_last, inf)
   int dmin = 1, dmax = 3;
   double n0 = 0.5, a = 1.0, ome = 0.0;
+  // 0: reference order (unknowns variable-major as system_of_eqs.cpp:597-612, rows in
+  //    equation order as kerr.cpp:84-99); 1: domain-major (unknowns by domain, then
+  //    field; rows by the domain an equation lives in, matching d|d+1 with d+1), which
+  //    makes J block-tridiagonal by domain
+  int order = 0;
 };
 
 struct EqRec {
@@ -96,6 +101,7 @@ class XctsSystem {
   Defs defs(int d, const std::array<VDp, NF>& ders);
   std::vector<TE> expr(const EqRec& e, int part, Defs& D);
   void trace_value();
+  void reorder_rows();
   void trace_tangent(int f, int d);
   std::vector<std::array<VDp, NF>> zero_ders();
 };

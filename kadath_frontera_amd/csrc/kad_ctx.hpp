@@ -25,6 +25,8 @@ struct LuWork {
   void* tmp2 = nullptr;         // scratch of the recursive triangular inverse / panel row gather
   size_t tmp2_bytes = 0;
   int* map = nullptr;           // 2 x n logical -> physical row maps of the base panel
+  int* mask_r = nullptr;        // per 128-row tile of the L panel: any nonzero
+  int* mask_c = nullptr;        // per 128-column tile of the U row panel: any nonzero
 };
 
 // Optional live timing of the MFMA GEMM launches (HIP events on the launch
@@ -50,7 +52,7 @@ struct Ctx {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   LuWork lu;
-  size_t cap_redv = 0, cap_redi = 0, cap_tinv = 0, cap_map = 0;
+  size_t cap_redv = 0, cap_redi = 0, cap_tinv = 0, cap_map = 0, cap_mask = 0;
   double* gemv_part = nullptr;
   size_t cap_part = 0;
   double* vwork = nullptr;
@@ -59,6 +61,9 @@ struct Ctx {
   size_t cap_krylov = 0;
   double* trs_tab = nullptr;    // triangular-solve tables of the current factors
   size_t cap_trs_tab = 0;
+  int gmres_m = 60;                 // GMRES-IR restart length
+  std::vector<double> hist_inner;   // last refinement: inner GMRES residual 2-norms
+  std::vector<double> hist_outer;   // last refinement: true residual inf-norms
 
   std::map<std::string, DevPlan*> plans;  // transform plans by key
   std::vector<void*> owned;                // device buffers freed with the context

@@ -43,6 +43,19 @@ __device__ inline void tile_coords(int bid, int nbm, int nbn, int& tm, int& tn) 
   tn = r / gsize;
 }
 
+// Structural zeros: a 128-row tile of A (rmask[tm] == 0) or a 128-column tile of B
+// (cmask[tn] == 0) that is exactly zero contributes nothing to its C tile.
+__device__ inline bool skip_tile(const int* rmask, const int* cmask, int tm, int tn) {
+  return (rmask && rmask[tm] == 0) || (cmask && cmask[tn] == 0);
+}
+template <typename T>
+__device__ inline void zero_tile(T* C, int64_t ldc, int m, int n, int row0, int col0, int tid) {
+  for (int e = tid; e < GBM * GBN; e += GNT) {
+    const int r = row0 + e % GBM, c = col0 + e / GBM;
+    if (r < m && c < n) C[r + (int64_t)c * ldc] = T(0);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // f64 / f32 path: 16x16x4 MFMA, LDS tiles stored k-major: As[k][m], Bs[k][n].
 // ---------------------------------------------------------------------------
@@ -50,7 +63,8 @@ template <int P, int OP>
 __global__ __launch_bounds__(GNT) void gemm_k4_kernel(int m, int n, int k,
                                                       const typename GemmCfg<P>::T* __restrict__ A, int64_t lda,
                                                       const typename GemmCfg<P>::T* __restrict__ B, int64_t ldb,
-                                                      typename GemmCfg<P>::T* __restrict__ C, int64_t ldc) {
+                                                      typename GemmCfg<P>::T* __restrict__ C, int64_t ldc,
+                                                      const int* __restrict__ rmask, const int* __restrict__ cmask) {
   using T = typename GemmCfg<P>::T;
   constexpr int BK = GemmCfg<P>::BK;
   constexpr int PADA = (sizeof(T) == 8) ? 4 : 8;
@@ -68,6 +82,10 @@ __global__ __launch_bounds__(GNT) void gemm_k4_kernel(int m, int n, int k,
   const int row0 = tm * GBM, col0 = tn * GBN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+  if (skip_tile(rmask, cmask, tm, tn)) {
+    if (OP == OP_SET) zero_tile(C, ldc, m, n, row0, col0, tid);
+    return;
+  }
 
   acc_t acc[4][4];
 #pragma unroll
@@ -169,7 +187,8 @@ __global__ __launch_bounds__(GNT) void gemm_k4_kernel(int m, int n, int k,
 template <int OP>
 __global__ __launch_bounds__(GNT) void gemm_bf16_kernel(int m, int n, int k, const float* __restrict__ A, int64_t lda,
                                                         const float* __restrict__ B, int64_t ldb, float* __restrict__ C,
-                                                        int64_t ldc) {
+                                                        int64_t ldc, const int* __restrict__ rmask,
+                                                        const int* __restrict__ cmask) {
   constexpr int BK = GemmCfg<BF16>::BK;
   constexpr int LDK = BK + 8;  // bf16 elements per LDS row (pad 16 bytes)
   constexpr int NA = GBM * BK / GNT, NB = GBN * BK / GNT;
@@ -182,6 +201,10 @@ __global__ __launch_bounds__(GNT) void gemm_bf16_kernel(int m, int n, int k, con
   const int row0 = tm * GBM, col0 = tn * GBN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+  if (skip_tile(rmask, cmask, tm, tn)) {
+    if (OP == OP_SET) zero_tile(C, ldc, m, n, row0, col0, tid);
+    return;
+  }
 
   f4v acc[4][4];
 #pragma unroll
@@ -266,8 +289,19 @@ __global__ __launch_bounds__(GNT) void gemm_bf16_kernel(int m, int n, int k, con
 }
 
 // Host launcher: C (-)= A*B for the given precision.  Returns KAD_OK or an error.
+int gemm_launch_masked(int prec, int op, int m, int n, int k, const void* A, int64_t lda, const void* B, int64_t ldb,
+                       void* C, int64_t ldc, const int* rmask, const int* cmask, hipStream_t s);
+
 int gemm_launch(int prec, int op, int m, int n, int k, const void* A, int64_t lda, const void* B, int64_t ldb, void* C,
                 int64_t ldc, hipStream_t s) {
+  return gemm_launch_masked(prec, op, m, n, k, A, lda, B, ldb, C, ldc, nullptr, nullptr, s);
+}
+
+// As gemm_launch; rmask (cdiv(m, 128) ints) / cmask (cdiv(n, 128) ints), device, may
+// be NULL: a zero entry marks an all-zero 128-row tile of A / 128-column tile of B,
+// whose C tiles are skipped (OP_SUB) or zeroed (OP_SET).
+int gemm_launch_masked(int prec, int op, int m, int n, int k, const void* A, int64_t lda, const void* B, int64_t ldb,
+                       void* C, int64_t ldc, const int* rmask, const int* cmask, hipStream_t s) {
   if (m <= 0 || n <= 0) return KAD_OK;
   if (k <= 0) {
     if (op == OP_SET) {
@@ -285,30 +319,44 @@ int gemm_launch(int prec, int op, int m, int n, int k, const void* A, int64_t ld
       tm->ev.resize(old + 2048);
       for (size_t q = old; q < tm->ev.size(); ++q) KAD_HIP_CHECK(hipEventCreate(&tm->ev[q]));
     }
-    tm->flops.push_back(2.0 * m * n * (double)k);
+    // executed work: the tile pairs that are not skipped (the masks are read back,
+    // which synchronises; timing runs only)
+    double frac = 1.0;
+    if (rmask || cmask) {
+      const int nbm = (int)cdiv(m, GBM), nbn = (int)cdiv(n, GBN);
+      std::vector<int> hr(nbm, 1), hc(nbn, 1);
+      if (rmask) KAD_HIP_CHECK(hipMemcpyAsync(hr.data(), rmask, nbm * sizeof(int), hipMemcpyDeviceToHost, s));
+      if (cmask) KAD_HIP_CHECK(hipMemcpyAsync(hc.data(), cmask, nbn * sizeof(int), hipMemcpyDeviceToHost, s));
+      KAD_HIP_CHECK(hipStreamSynchronize(s));
+      double ar = 0, ac = 0;
+      for (int v : hr) ar += (v != 0);
+      for (int v : hc) ac += (v != 0);
+      frac = (ar / nbm) * (ac / nbn);
+    }
+    tm->flops.push_back(2.0 * m * n * (double)k * frac);
     KAD_HIP_CHECK(hipEventRecord(tm->ev[2 * tm->used], s));
   }
   if (prec == F64) {
     if (op == OP_SUB)
       gemm_k4_kernel<F64, OP_SUB><<<grid, block, 0, s>>>(m, n, k, (const double*)A, lda, (const double*)B, ldb,
-                                                          (double*)C, ldc);
+                                                          (double*)C, ldc, rmask, cmask);
     else
       gemm_k4_kernel<F64, OP_SET><<<grid, block, 0, s>>>(m, n, k, (const double*)A, lda, (const double*)B, ldb,
-                                                          (double*)C, ldc);
+                                                          (double*)C, ldc, rmask, cmask);
   } else if (prec == F32) {
     if (op == OP_SUB)
       gemm_k4_kernel<F32, OP_SUB><<<grid, block, 0, s>>>(m, n, k, (const float*)A, lda, (const float*)B, ldb,
-                                                          (float*)C, ldc);
+                                                          (float*)C, ldc, rmask, cmask);
     else
       gemm_k4_kernel<F32, OP_SET><<<grid, block, 0, s>>>(m, n, k, (const float*)A, lda, (const float*)B, ldb,
-                                                          (float*)C, ldc);
+                                                          (float*)C, ldc, rmask, cmask);
   } else if (prec == BF16) {
     if (op == OP_SUB)
       gemm_bf16_kernel<OP_SUB><<<grid, block, 0, s>>>(m, n, k, (const float*)A, lda, (const float*)B, ldb, (float*)C,
-                                                      ldc);
+                                                      ldc, rmask, cmask);
     else
       gemm_bf16_kernel<OP_SET><<<grid, block, 0, s>>>(m, n, k, (const float*)A, lda, (const float*)B, ldb, (float*)C,
-                                                      ldc);
+                                                      ldc, rmask, cmask);
   } else {
     return KAD_ERR_ARG;
   }

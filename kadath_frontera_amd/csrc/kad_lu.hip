@@ -14,8 +14,10 @@
 // performs exact partial pivoting (first index of max |a|, as idamax) and whose
 // internal updates are MFMA GEMMs.  The trailing update A22 -= L21 U12 is the
 // MFMA GEMM of kad_gemm.hip in the factor precision (f64, f32, or bf16 operands
-// with f32 accumulation).  U12 = L11^{-1} A12 goes through an explicit inverse of
-// the unit-lower L11 and one GEMM.
+// with f32 accumulation).  U12 = L11^{-1} A12 is a blocked substitution whose
+// updates are GEMMs and whose 64-row leaves use explicit inverses.  Trailing-update
+// tile pairs whose L21 row tile or U12 column tile is exactly zero are skipped
+// (structural zeros of a block-sparse Jacobian); the result is unchanged.
 #include "kad_common.hpp"
 #include "kad_ctx.hpp"
 #include <vector>
@@ -26,6 +28,8 @@ namespace kad {
 
 int gemm_launch(int prec, int op, int m, int n, int k, const void* A, int64_t lda, const void* B, int64_t ldb, void* C,
                 int64_t ldc, hipStream_t s);
+int gemm_launch_masked(int prec, int op, int m, int n, int k, const void* A, int64_t lda, const void* B, int64_t ldb,
+                       void* C, int64_t ldc, const int* rmask, const int* cmask, hipStream_t s);
 constexpr int GOP_SUB = 0, GOP_SET = 1;
 
 constexpr int PANEL_BASE = 32;  // columns factored by the per-column kernel
@@ -153,21 +157,62 @@ int trtri_rec(const T* L, int64_t lda, int nb, T* Linv, int64_t ldo, LuWork& w) 
   return KAD_OK;
 }
 
-// A12 (r x ncols, ld lda) := L11^{-1} A12 where L11 = unit lower r x r at L (ld lda).
+// Structural-zero masks of the trailing update: per 128-row tile of the L panel
+// (rows x ncols) and per 128-column tile of the U row panel (nrows x cols): 1 if the
+// tile holds any entry != 0 (NaN counts as nonzero), else 0.
 template <typename T>
-int trsm_left_unit_lower(const T* L, int64_t lda, int r, T* A12, int64_t ld12, int ncols, LuWork& w, int gemm_prec) {
+__global__ __launch_bounds__(256) void row_tile_mask_kernel(const T* __restrict__ A, int64_t lda, int nrows,
+                                                            int ncols, int* __restrict__ mask) {
+  const int r = blockIdx.x * 128 + (threadIdx.x & 127);
+  int any = 0;
+  if (r < nrows)
+    for (int c = threadIdx.x >> 7; c < ncols; c += 2) any |= (A[r + (int64_t)c * lda] != T(0));
+  any = __syncthreads_or(any);
+  if (threadIdx.x == 0) mask[blockIdx.x] = any;
+}
+template <typename T>
+__global__ __launch_bounds__(256) void col_tile_mask_kernel(const T* __restrict__ A, int64_t lda, int nrows,
+                                                            int ncols, int* __restrict__ mask) {
+  const int c0 = blockIdx.x * 128;
+  const int ce = min(c0 + 128, ncols);
+  int any = 0;
+  for (int c = c0; c < ce; ++c)
+    for (int r = threadIdx.x; r < nrows; r += 256) any |= (A[r + (int64_t)c * lda] != T(0));
+  any = __syncthreads_or(any);
+  if (threadIdx.x == 0) mask[blockIdx.x] = any;
+}
+
+// A12 <- L^{-1} A12 with L unit lower (r x r).  Blocked substitution as LAPACK's
+// dtrsm: X1 = L11^{-1} B1, B2 -= L21 X1, X2 = L22^{-1} B2, recursively; only the
+// leaves (at most 64 rows) use an explicit inverse, whose error grows with the
+// leaf's condition number.
+constexpr int TRSM_LEAF = 64;
+
+template <typename T>
+int trsm_left_unit_lower(const T* L, int64_t lda, int r, T* A12, int64_t ld12, int ncols, LuWork& w, int gemm_prec,
+                         const int* cmask = nullptr) {
   if (ncols <= 0 || r <= 0) return KAD_OK;
+  if (r > TRSM_LEAF) {
+    const int h = std::max(32, (r / 2) / 32 * 32);
+    int rc = trsm_left_unit_lower<T>(L, lda, h, A12, ld12, ncols, w, gemm_prec, cmask);
+    if (rc) return rc;
+    rc = gemm_launch_masked(gemm_prec == BF16 ? F32 : gemm_prec, GOP_SUB, r - h, ncols, h, L + h, lda, A12, ld12,
+                            A12 + h, ld12, nullptr, cmask, w.stream);
+    if (rc) return rc;
+    return trsm_left_unit_lower<T>(L + h + (int64_t)h * lda, lda, r - h, A12 + h, ld12, ncols, w, gemm_prec, cmask);
+  }
   T* linv = (T*)w.tinv;
   int rc = trtri_rec<T>(L, lda, r, linv, r, w);
   if (rc) return rc;
-  // Column chunks so that the scratch holds r x chunk.
+  // Column chunks (multiples of 128, the mask granularity) so that the scratch holds r x chunk.
   const int64_t cap = (int64_t)(w.tmp_bytes / sizeof(T)) / r;
-  const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(cap, ncols));
+  int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(cap, ncols));
+  if (chunk < ncols) chunk = std::max(128, chunk / 128 * 128);
   for (int c = 0; c < ncols; c += chunk) {
     const int nc = std::min(chunk, ncols - c);
     T* tmp = (T*)w.tmp;
-    rc = gemm_launch(gemm_prec == BF16 ? F32 : gemm_prec, GOP_SET, r, nc, r, linv, r, A12 + (int64_t)c * ld12, ld12,
-                     tmp, r, w.stream);
+    rc = gemm_launch_masked(gemm_prec == BF16 ? F32 : gemm_prec, GOP_SET, r, nc, r, linv, r, A12 + (int64_t)c * ld12,
+                            ld12, tmp, r, nullptr, cmask ? cmask + c / 128 : nullptr, w.stream);
     if (rc) return rc;
     dim3 g((unsigned)cdiv(r, 256), nc);
     copy2d_kernel<T><<<g, 256, 0, w.stream>>>(tmp, r, A12 + (int64_t)c * ld12, ld12, r, nc);
@@ -385,11 +430,19 @@ int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w)
     if (k1 < n) {
       rc = laswp<T>(A, lda, k1, n, ipiv, k0, k1, w);
       if (rc) return rc;
+      // structural zeros of L21 (row tiles) and A12 (column tiles; U12 = L11^{-1} A12
+      // keeps them): the tile pairs they touch are skipped exactly
+      row_tile_mask_kernel<T><<<(unsigned)cdiv(n - k1, 128), 256, 0, w.stream>>>(A + k1 + (int64_t)k0 * lda, lda,
+                                                                                n - k1, k1 - k0, w.mask_r);
+      col_tile_mask_kernel<T><<<(unsigned)cdiv(n - k1, 128), 256, 0, w.stream>>>(A + k0 + (int64_t)k1 * lda, lda,
+                                                                                k1 - k0, n - k1, w.mask_c);
+      KAD_HIP_CHECK(hipGetLastError());
       rc = trsm_left_unit_lower<T>(A + k0 + (int64_t)k0 * lda, lda, k1 - k0, A + k0 + (int64_t)k1 * lda, lda, n - k1,
-                                   w, gemm_prec);
+                                   w, gemm_prec, w.mask_c);
       if (rc) return rc;
-      rc = gemm_launch(gemm_prec, GOP_SUB, n - k1, n - k1, k1 - k0, A + k1 + (int64_t)k0 * lda, lda,
-                       A + k0 + (int64_t)k1 * lda, lda, A + k1 + (int64_t)k1 * lda, lda, w.stream);
+      rc = gemm_launch_masked(gemm_prec, GOP_SUB, n - k1, n - k1, k1 - k0, A + k1 + (int64_t)k0 * lda, lda,
+                              A + k0 + (int64_t)k1 * lda, lda, A + k1 + (int64_t)k1 * lda, lda, w.mask_r, w.mask_c,
+                              w.stream);
       if (rc) return rc;
     }
   }

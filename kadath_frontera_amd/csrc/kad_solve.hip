@@ -62,6 +62,8 @@ int Ctx::lu_work(int n, int nb) {
   if ((rc = ensure((void**)&lu.red_val, &cap_redv, 2 * sizeof(double) * red))) return rc;
   if ((rc = ensure((void**)&lu.red_idx, &cap_redi, 2 * sizeof(int) * red))) return rc;
   if ((rc = ensure((void**)&lu.map, &cap_map, 2 * sizeof(int) * (size_t)n))) return rc;
+  if ((rc = ensure((void**)&lu.mask_r, &cap_mask, 2 * sizeof(int) * (size_t)(cdiv(n, 128) + 1)))) return rc;
+  lu.mask_c = lu.mask_r + cdiv(n, 128) + 1;
   lu.red_cap = red;
   if (!lu.counter) {
     KAD_HIP_CHECK(hipMalloc((void**)&lu.counter, 64));
@@ -132,7 +134,8 @@ int Ctx::upload(const void* host, size_t bytes, void** dev) {
 Ctx::~Ctx() {
   for (auto& kv : plans) delete kv.second;
   for (void* p : owned) (void)hipFree(p);
-  void* bufs[] = {lu.red_val, lu.red_idx, lu.counter, lu.tinv, lu.tmp, lu.tmp2, lu.map, gemv_part, vwork, krylov, trs_tab};
+  void* bufs[] = {lu.red_val, lu.red_idx, lu.counter, lu.tinv,   lu.tmp,  lu.tmp2,
+                  lu.map,     lu.mask_r,  gemv_part,  vwork,     krylov,  trs_tab};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (own_stream && stream) (void)hipStreamDestroy(stream);
@@ -356,11 +359,13 @@ int kad_ir_solve_scaled(kad_ctx* ctx, int n, const double* A64, int64_t lda, int
     return r;
   };
   const double bnorm = c->absmax(b, n);
+  c->hist_inner.clear();
+  c->hist_outer.assign(1, bnorm);
   KAD_HIP_CHECK(hipMemsetAsync(x, 0, n * sizeof(double), s));
   KAD_HIP_CHECK(hipMemcpyAsync(r, b, n * sizeof(double), hipMemcpyDeviceToDevice, s));
   int outer = 0, inner_total = 0;
   double rel = (bnorm == 0.0) ? 0.0 : 1.0;
-  const int m = 60;           // Krylov dimension per outer step
+  const int m = std::max(2, c->gmres_m);  // Krylov dimension per outer step
   const double inner_tol = 1e-12;
   std::vector<double> H, cs, sn, gvec, y;
   double* V = nullptr;
@@ -423,6 +428,7 @@ int kad_ir_solve_scaled(kad_ctx* ctx, int n, const double* A64, int64_t lda, int
         gvec[j] = cs[j] * gvec[j];
         kdim = j + 1;
         ++inner_total;
+        c->hist_inner.push_back(std::fabs(gvec[j + 1]));
         // |g_{j+1}| is ||r - A M^{-1} V y||_2 >= the inf-norm the outer test uses
         if (std::fabs(gvec[j + 1]) <= std::max(inner_tol * beta, 0.5 * tol * bnorm) || hn == 0.0) break;
       }
@@ -442,6 +448,7 @@ int kad_ir_solve_scaled(kad_ctx* ctx, int n, const double* A64, int64_t lda, int
     if ((rc = kad::gemv64(A64, lda, n, x, b, r, trans, c->gemv_part, s))) return rc;
     const double rn = c->absmax(r, n);
     const double rel_new = (bnorm == 0.0) ? 0.0 : rn / bnorm;
+    c->hist_outer.push_back(rn);
     ++outer;
     if (rel_new > rel) {
       // the step made the residual worse (attainable accuracy reached): undo it
@@ -460,6 +467,24 @@ int kad_ir_solve_scaled(kad_ctx* ctx, int n, const double* A64, int64_t lda, int
   if (resid) *resid = rel;
   KAD_HIP_CHECK(hipStreamSynchronize(s));
   return (rel <= tol) ? KAD_OK : KAD_ERR_NOCONV;
+}
+
+// GMRES-IR restart length of the context (default 60).
+int kad_ir_set_restart(kad_ctx* ctx, int m) {
+  if (!ctx || m < 2 || m > 4096) return KAD_ERR_ARG;
+  reinterpret_cast<Ctx*>(ctx)->gmres_m = m;
+  return KAD_OK;
+}
+
+// Residual history of the last refinement: kind 0 = inner GMRES residual 2-norms
+// (one per inner step), kind 1 = ||b||_inf then the true ||b - op(A) x||_inf after
+// each outer step.  Returns the number of entries (copies at most cap).
+int kad_ir_history(kad_ctx* ctx, int kind, double* out, int cap) {
+  if (!ctx || kind < 0 || kind > 1) return KAD_ERR_ARG;
+  const auto& h = kind == 0 ? reinterpret_cast<Ctx*>(ctx)->hist_inner : reinterpret_cast<Ctx*>(ctx)->hist_outer;
+  if (out)
+    for (int i = 0; i < (int)h.size() && i < cap; ++i) out[i] = h[i];
+  return (int)h.size();
 }
 
 }  // extern "C"

@@ -18,13 +18,14 @@ def _dp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
-def host_info(res, bounds, n0=0.5, a=1.0, ome=0.0):
+def host_info(res, bounds, n0=0.5, a=1.0, ome=0.0, order=0):
     """Host-only trace of the system: sizes and (field, domain, offset, count) layout."""
     b = np.asarray(bounds, dtype=np.float64)
     info = (ctypes.c_int * 6)()
     cap = 4 * 5 * 64
     lay = (ctypes.c_int * cap)()
-    nl = check(lib.kad_xcts_host_info(res[0], res[1], res[2], _dp(b), len(b), 1, len(b), n0, a, ome, info, lay, cap),
+    nl = check(lib.kad_xcts_host_info(res[0], res[1], res[2], _dp(b), len(b), 1, len(b), n0, a, ome, order, info, lay,
+                                      cap),
                "kad_xcts_host_info")
     layout = [tuple(lay[4 * q:4 * q + 4]) for q in range(nl)]
     keys = ("n_unknowns", "n_rows", "value_instr", "tangent_instr", "value_slots", "tangent_programs")
@@ -32,14 +33,16 @@ def host_info(res, bounds, n0=0.5, a=1.0, ome=0.0):
 
 
 class XctsSystem:
-    def __init__(self, ctx, res, bounds, n0=0.5, a=1.0, ome=0.0, batch=256):
+    def __init__(self, ctx, res, bounds, n0=0.5, a=1.0, ome=0.0, batch=256, order=0):
+        """order 0: the reference's unknown/row order; 1: domain-major (block-tridiagonal J)."""
         self.ctx = ctx
         self.res = tuple(res)
         self.bounds = np.asarray(bounds, dtype=np.float64)
         h = c_vp()
         check(lib.kad_xcts_create(ctx.h, res[0], res[1], res[2], _dp(self.bounds), len(self.bounds), 1,
-                                  len(self.bounds), n0, a, ome, batch, ctypes.byref(h)), "kad_xcts_create")
+                                  len(self.bounds), n0, a, ome, order, batch, ctypes.byref(h)), "kad_xcts_create")
         self.h = h
+        self.order = order
         n, m = ctypes.c_int(), ctypes.c_int()
         st = (ctypes.c_int * 4)()
         check(lib.kad_xcts_sizes(h, ctypes.byref(n), ctypes.byref(m), st), "kad_xcts_sizes")
@@ -49,6 +52,11 @@ class XctsSystem:
         lay = (ctypes.c_int * cap)()
         nl = check(lib.kad_xcts_layout(h, lay, cap), "kad_xcts_layout")
         self.layout = [tuple(lay[4 * q:4 * q + 4]) for q in range(nl)]
+        rcap = 5 * 1024
+        rl = (ctypes.c_int * rcap)()
+        nr = check(lib.kad_xcts_row_layout(h, rl, rcap), "kad_xcts_row_layout")
+        # (kind, domain, component, first row, row count) per equation component
+        self.row_layout = [tuple(rl[5 * q:5 * q + 5]) for q in range(nr)]
 
     def close(self):
         if self.h:

@@ -52,6 +52,51 @@ def _spd_like(n, seed):
     return rng.standard_normal((n, n)) + 0.5 * np.sqrt(n) * np.eye(n)
 
 
+def _block_tridiagonal(nblk, m, seed, coupling_rows=24):
+    """Block-tridiagonal matrix shaped like a domain-major Kadath Jacobian: dense
+    diagonal blocks, off-diagonal blocks nonzero only in a few (matching) rows."""
+    rng = np.random.default_rng(seed)
+    n = nblk * m
+    A = np.zeros((n, n))
+    for b in range(nblk):
+        s = slice(b * m, (b + 1) * m)
+        A[s, s] = rng.standard_normal((m, m)) + 0.5 * np.sqrt(m) * np.eye(m)
+        for nb_ in (b - 1, b + 1):
+            if 0 <= nb_ < nblk:
+                rows = rng.choice(m, coupling_rows, replace=False) + b * m
+                A[rows, nb_ * m:(nb_ + 1) * m] = rng.standard_normal((coupling_rows, m))
+    return A
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_lu_block_sparse_matches_lapack(ctx, prec):
+    """Tile skipping (structural-zero masks) must not change the factorisation:
+    same pivots as LAPACK and the same factors."""
+    print(f"lu block-sparse {prec}", flush=True)
+    A = _block_tridiagonal(5, 700, 7)
+    n = A.shape[0]
+    dt = torch.float64 if prec == "f64" else torch.float32
+    dA = colmajor(A, dt)
+    ipiv, perm, info = ctx.lu_factor(prec, dA, nb=256)
+    assert info == 0
+    if prec == "f64":
+        lu_ref, piv_ref = scipy.linalg.lu_factor(A)
+        assert np.array_equal(ipiv.cpu().numpy(), piv_ref)
+        np.testing.assert_allclose(dA.cpu().numpy(), lu_ref, atol=1e-10 * np.abs(lu_ref).max(), rtol=1e-9)
+    else:
+        # backward error within a small factor of LAPACK sgetrf's on the same matrix
+        def bwd(lu, p):
+            L = np.tril(lu, -1) + np.eye(n)
+            return np.abs(A[p] - L @ np.triu(lu)).max() / np.abs(A).max()
+        got = dA.cpu().numpy().astype(np.float64)
+        lu32, piv32 = scipy.linalg.lu_factor(A.astype(np.float32))
+        p32 = np.arange(n)
+        for i, q in enumerate(piv32):
+            p32[i], p32[q] = p32[q], p32[i]
+        ref = bwd(lu32.astype(np.float64), p32)
+        assert bwd(got, perm.cpu().numpy()) < 4 * ref + 1e-7, (bwd(got, perm.cpu().numpy()), ref)
+
+
 @pytest.mark.parametrize("n,nb", [(64, 32), (257, 64), (1000, 256), (2100, 256)])
 def test_lu_f64_matches_lapack(ctx, n, nb):
     print(f"lu f64 n={n} nb={nb}", flush=True)

@@ -91,3 +91,74 @@ def test_newton_converges_to_oracle_solution(oracle_sys, gpu_sys, solver):
             ref = sy._solved[name][d]
             got = gpu_sys.coef(f, d)
             assert np.abs(got - ref).max() <= 1e-10 * max(1.0, np.abs(ref).max()), (name, d)
+
+
+@pytest.fixture(scope="module")
+def gpu_sys_dm(ctx):
+    from kadath_frontera_amd.xcts import XctsSystem
+    s = XctsSystem(ctx, RES, BOUNDS, n0=0.5, a=1.0, batch=128, order=1)
+    yield s
+    s.close()
+
+
+def _perms(ref, dm):
+    """Row and column permutations taking the reference-order system to the
+    domain-major one: dm[i, j] = ref[rows[i], cols[j]]."""
+    cols = np.zeros(ref.n, dtype=np.int64)
+    for (f, d, off, cnt) in dm.layout:
+        (o0,) = [o for (ff, dd, o, c) in ref.layout if (ff, dd) == (f, d)]
+        cols[off:off + cnt] = np.arange(o0, o0 + cnt)
+    rows = np.zeros(ref.n, dtype=np.int64)
+    ref_rows = {}
+    for q, (k, d, c, r0, cnt) in enumerate(ref.row_layout):
+        ref_rows.setdefault((k, d, c), []).append(r0)
+    seen = {}
+    for (k, d, c, r0, cnt) in dm.row_layout:
+        j = seen.get((k, d, c), 0)
+        seen[(k, d, c)] = j + 1
+        rows[r0:r0 + cnt] = np.arange(ref_rows[(k, d, c)][j], ref_rows[(k, d, c)][j] + cnt)
+    return rows, cols
+
+
+def test_domain_major_is_a_permutation(oracle_sys, gpu_sys, gpu_sys_dm):
+    """The domain-major system is the reference-order one with rows and columns
+    permuted: same guess, F and J equal entry for entry (same kernels, same data)."""
+    _, X0 = oracle_sys
+    rows, cols = _perms(gpu_sys, gpu_sys_dm)
+    assert sorted(rows.tolist()) == list(range(gpu_sys.n)) and sorted(cols.tolist()) == list(range(gpu_sys.n))
+    gpu_sys.set_guess(X0)
+    Xdm = np.empty_like(X0)
+    Xdm[:] = X0[cols]
+    gpu_sys_dm.set_guess(Xdm)
+    F0 = gpu_sys.residual().cpu().numpy()
+    F1 = gpu_sys_dm.residual().cpu().numpy()
+    assert np.array_equal(F1, F0[rows])
+    J0 = gpu_sys.jacobian().cpu().numpy()
+    J1 = gpu_sys_dm.jacobian().cpu().numpy()
+    assert np.array_equal(J1, J0[np.ix_(rows, cols)])
+    # the seeded perturbation is drawn in the reference order whatever the layout
+    assert np.array_equal(gpu_sys_dm.perturbation(1234)[:], gpu_sys.perturbation(1234)[cols])
+
+
+@pytest.mark.parametrize("solver", ["f64", "f32-gmres"])
+def test_domain_major_newton_matches_oracle(oracle_sys, gpu_sys_dm, solver):
+    sy, X0 = oracle_sys
+    if not hasattr(sy, "_solved"):
+        pytest.skip("needs the oracle solution of test_newton_converges_to_oracle_solution")
+    lay, _ = sy.unknown_layout()
+    ref_off = {(FIELD_IDX[a], b): c for (a, b, c, d) in lay}
+    Xdm = np.empty_like(X0)
+    for (f, d, off, cnt) in gpu_sys_dm.layout:
+        Xdm[off:off + cnt] = X0[ref_off[(f, d)]:ref_off[(f, d)] + cnt]
+    gpu_sys_dm.set_guess(Xdm)
+    conv, it = False, 0
+    while not conv and it < 15:
+        conv, err, times, ir = gpu_sys_dm.do_newton(1e-10, solver)
+        print(f"domain-major {solver} it {it} err {err:.3e} ir {ir}", flush=True)
+        it += 1
+    assert conv and err < 1e-10
+    for name, f in FIELD_IDX.items():
+        for d in range(1, len(BOUNDS) + 1):
+            ref = sy._solved[name][d]
+            got = gpu_sys_dm.coef(f, d)
+            assert np.abs(got - ref).max() <= 1e-10 * max(1.0, np.abs(ref).max()), (name, d)
