@@ -11,6 +11,20 @@
 
 namespace kad {
 
+// Selection of the C tiles a GEMM computes (all NULL: every tile).
+//  mask mode: rmask[tm] == 0 / cmask[tn] == 0 marks an exactly-zero 128-row tile of A
+//    / 128-column tile of B; those C tiles are skipped (C -= AB) or zeroed (C = AB).
+//  list mode (cnt != NULL): only the pairs rlist[0..cnt[0]) x clist[0..cnt[1]) are
+//    computed, compacted over the grid so the active tiles spread over every XCD
+//    (C -= AB only).
+struct TileSel {
+  const int* rmask = nullptr;
+  const int* cmask = nullptr;
+  const int* rlist = nullptr;
+  const int* clist = nullptr;
+  const int* cnt = nullptr;
+};
+
 // Scratch of the LU factorisation.
 struct LuWork {
   hipStream_t stream = nullptr;
@@ -27,6 +41,7 @@ struct LuWork {
   int* map = nullptr;           // 2 x n logical -> physical row maps of the base panel
   int* mask_r = nullptr;        // per 128-row tile of the L panel: any nonzero
   int* mask_c = nullptr;        // per 128-column tile of the U row panel: any nonzero
+  int* tile_list = nullptr;     // compacted active row tiles, column tiles, and the 2 counts
 };
 
 // Optional live timing of the MFMA GEMM launches (HIP events on the launch

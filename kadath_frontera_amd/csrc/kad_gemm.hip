@@ -43,10 +43,25 @@ __device__ inline void tile_coords(int bid, int nbm, int nbn, int& tm, int& tn) 
   tn = r / gsize;
 }
 
-// Structural zeros: a 128-row tile of A (rmask[tm] == 0) or a 128-column tile of B
-// (cmask[tn] == 0) that is exactly zero contributes nothing to its C tile.
-__device__ inline bool skip_tile(const int* rmask, const int* cmask, int tm, int tn) {
-  return (rmask && rmask[tm] == 0) || (cmask && cmask[tn] == 0);
+// Tile of this workgroup under a TileSel; false: nothing to compute (the caller
+// zeroes the tile for C = AB in mask mode, returns otherwise).
+__device__ inline bool select_tile(const TileSel& sel, int nbm, int nbn, int& tm, int& tn, bool& zero_fill) {
+  zero_fill = false;
+  if (sel.cnt) {
+    const int nr = sel.cnt[0], nc = sel.cnt[1];
+    if ((int)blockIdx.x >= nr * nc) return false;
+    int i, j;
+    tile_coords(blockIdx.x, nr, nc, i, j);
+    tm = sel.rlist[i];
+    tn = sel.clist[j];
+    return true;
+  }
+  tile_coords(blockIdx.x, nbm, nbn, tm, tn);
+  if ((sel.rmask && sel.rmask[tm] == 0) || (sel.cmask && sel.cmask[tn] == 0)) {
+    zero_fill = true;
+    return false;
+  }
+  return true;
 }
 template <typename T>
 __device__ inline void zero_tile(T* C, int64_t ldc, int m, int n, int row0, int col0, int tid) {
@@ -64,7 +79,7 @@ __global__ __launch_bounds__(GNT) void gemm_k4_kernel(int m, int n, int k,
                                                       const typename GemmCfg<P>::T* __restrict__ A, int64_t lda,
                                                       const typename GemmCfg<P>::T* __restrict__ B, int64_t ldb,
                                                       typename GemmCfg<P>::T* __restrict__ C, int64_t ldc,
-                                                      const int* __restrict__ rmask, const int* __restrict__ cmask) {
+                                                      TileSel sel) {
   using T = typename GemmCfg<P>::T;
   constexpr int BK = GemmCfg<P>::BK;
   constexpr int PADA = (sizeof(T) == 8) ? 4 : 8;
@@ -77,13 +92,14 @@ __global__ __launch_bounds__(GNT) void gemm_k4_kernel(int m, int n, int k,
   __shared__ T Bs[2][BK * LDB_S];
 
   const int nbm = (int)cdiv(m, GBM), nbn = (int)cdiv(n, GBN);
-  int tm, tn;
-  tile_coords(blockIdx.x, nbm, nbn, tm, tn);
+  int tm = 0, tn = 0;
+  bool zero_fill;
+  const bool active = select_tile(sel, nbm, nbn, tm, tn, zero_fill);
   const int row0 = tm * GBM, col0 = tn * GBN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
-  if (skip_tile(rmask, cmask, tm, tn)) {
-    if (OP == OP_SET) zero_tile(C, ldc, m, n, row0, col0, tid);
+  if (!active) {
+    if (OP == OP_SET && zero_fill) zero_tile(C, ldc, m, n, row0, col0, tid);
     return;
   }
 
@@ -187,8 +203,7 @@ __global__ __launch_bounds__(GNT) void gemm_k4_kernel(int m, int n, int k,
 template <int OP>
 __global__ __launch_bounds__(GNT) void gemm_bf16_kernel(int m, int n, int k, const float* __restrict__ A, int64_t lda,
                                                         const float* __restrict__ B, int64_t ldb, float* __restrict__ C,
-                                                        int64_t ldc, const int* __restrict__ rmask,
-                                                        const int* __restrict__ cmask) {
+                                                        int64_t ldc, TileSel sel) {
   constexpr int BK = GemmCfg<BF16>::BK;
   constexpr int LDK = BK + 8;  // bf16 elements per LDS row (pad 16 bytes)
   constexpr int NA = GBM * BK / GNT, NB = GBN * BK / GNT;
@@ -196,13 +211,14 @@ __global__ __launch_bounds__(GNT) void gemm_bf16_kernel(int m, int n, int k, con
   __shared__ __bf16 Bs[2][GBN * LDK];
 
   const int nbm = (int)cdiv(m, GBM), nbn = (int)cdiv(n, GBN);
-  int tm, tn;
-  tile_coords(blockIdx.x, nbm, nbn, tm, tn);
+  int tm = 0, tn = 0;
+  bool zero_fill;
+  const bool active = select_tile(sel, nbm, nbn, tm, tn, zero_fill);
   const int row0 = tm * GBM, col0 = tn * GBN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
-  if (skip_tile(rmask, cmask, tm, tn)) {
-    if (OP == OP_SET) zero_tile(C, ldc, m, n, row0, col0, tid);
+  if (!active) {
+    if (OP == OP_SET && zero_fill) zero_tile(C, ldc, m, n, row0, col0, tid);
     return;
   }
 
@@ -289,19 +305,27 @@ __global__ __launch_bounds__(GNT) void gemm_bf16_kernel(int m, int n, int k, con
 }
 
 // Host launcher: C (-)= A*B for the given precision.  Returns KAD_OK or an error.
-int gemm_launch_masked(int prec, int op, int m, int n, int k, const void* A, int64_t lda, const void* B, int64_t ldb,
-                       void* C, int64_t ldc, const int* rmask, const int* cmask, hipStream_t s);
+int gemm_launch_sel(int prec, int op, int m, int n, int k, const void* A, int64_t lda, const void* B, int64_t ldb,
+                    void* C, int64_t ldc, const TileSel& sel, hipStream_t s);
 
 int gemm_launch(int prec, int op, int m, int n, int k, const void* A, int64_t lda, const void* B, int64_t ldb, void* C,
                 int64_t ldc, hipStream_t s) {
-  return gemm_launch_masked(prec, op, m, n, k, A, lda, B, ldb, C, ldc, nullptr, nullptr, s);
+  return gemm_launch_sel(prec, op, m, n, k, A, lda, B, ldb, C, ldc, TileSel(), s);
 }
 
-// As gemm_launch; rmask (cdiv(m, 128) ints) / cmask (cdiv(n, 128) ints), device, may
-// be NULL: a zero entry marks an all-zero 128-row tile of A / 128-column tile of B,
-// whose C tiles are skipped (OP_SUB) or zeroed (OP_SET).
 int gemm_launch_masked(int prec, int op, int m, int n, int k, const void* A, int64_t lda, const void* B, int64_t ldb,
                        void* C, int64_t ldc, const int* rmask, const int* cmask, hipStream_t s) {
+  TileSel sel;
+  sel.rmask = rmask;
+  sel.cmask = cmask;
+  return gemm_launch_sel(prec, op, m, n, k, A, lda, B, ldb, C, ldc, sel, s);
+}
+
+// As gemm_launch, computing only the C tiles that sel selects (list mode: OP_SUB only;
+// masks and lists are device arrays over 128-row / 128-column tiles of C).
+int gemm_launch_sel(int prec, int op, int m, int n, int k, const void* A, int64_t lda, const void* B, int64_t ldb,
+                    void* C, int64_t ldc, const TileSel& sel, hipStream_t s) {
+  if (sel.cnt && op != OP_SUB) return KAD_ERR_ARG;
   if (m <= 0 || n <= 0) return KAD_OK;
   if (k <= 0) {
     if (op == OP_SET) {
@@ -322,11 +346,16 @@ int gemm_launch_masked(int prec, int op, int m, int n, int k, const void* A, int
     // executed work: the tile pairs that are not skipped (the masks are read back,
     // which synchronises; timing runs only)
     double frac = 1.0;
-    if (rmask || cmask) {
-      const int nbm = (int)cdiv(m, GBM), nbn = (int)cdiv(n, GBN);
+    const int nbm = (int)cdiv(m, GBM), nbn = (int)cdiv(n, GBN);
+    if (sel.cnt) {
+      int hc[2];
+      KAD_HIP_CHECK(hipMemcpyAsync(hc, sel.cnt, sizeof(hc), hipMemcpyDeviceToHost, s));
+      KAD_HIP_CHECK(hipStreamSynchronize(s));
+      frac = ((double)hc[0] / nbm) * ((double)hc[1] / nbn);
+    } else if (sel.rmask || sel.cmask) {
       std::vector<int> hr(nbm, 1), hc(nbn, 1);
-      if (rmask) KAD_HIP_CHECK(hipMemcpyAsync(hr.data(), rmask, nbm * sizeof(int), hipMemcpyDeviceToHost, s));
-      if (cmask) KAD_HIP_CHECK(hipMemcpyAsync(hc.data(), cmask, nbn * sizeof(int), hipMemcpyDeviceToHost, s));
+      if (sel.rmask) KAD_HIP_CHECK(hipMemcpyAsync(hr.data(), sel.rmask, nbm * sizeof(int), hipMemcpyDeviceToHost, s));
+      if (sel.cmask) KAD_HIP_CHECK(hipMemcpyAsync(hc.data(), sel.cmask, nbn * sizeof(int), hipMemcpyDeviceToHost, s));
       KAD_HIP_CHECK(hipStreamSynchronize(s));
       double ar = 0, ac = 0;
       for (int v : hr) ar += (v != 0);
@@ -339,24 +368,24 @@ int gemm_launch_masked(int prec, int op, int m, int n, int k, const void* A, int
   if (prec == F64) {
     if (op == OP_SUB)
       gemm_k4_kernel<F64, OP_SUB><<<grid, block, 0, s>>>(m, n, k, (const double*)A, lda, (const double*)B, ldb,
-                                                          (double*)C, ldc, rmask, cmask);
+                                                          (double*)C, ldc, sel);
     else
       gemm_k4_kernel<F64, OP_SET><<<grid, block, 0, s>>>(m, n, k, (const double*)A, lda, (const double*)B, ldb,
-                                                          (double*)C, ldc, rmask, cmask);
+                                                          (double*)C, ldc, sel);
   } else if (prec == F32) {
     if (op == OP_SUB)
       gemm_k4_kernel<F32, OP_SUB><<<grid, block, 0, s>>>(m, n, k, (const float*)A, lda, (const float*)B, ldb,
-                                                          (float*)C, ldc, rmask, cmask);
+                                                          (float*)C, ldc, sel);
     else
       gemm_k4_kernel<F32, OP_SET><<<grid, block, 0, s>>>(m, n, k, (const float*)A, lda, (const float*)B, ldb,
-                                                          (float*)C, ldc, rmask, cmask);
+                                                          (float*)C, ldc, sel);
   } else if (prec == BF16) {
     if (op == OP_SUB)
       gemm_bf16_kernel<OP_SUB><<<grid, block, 0, s>>>(m, n, k, (const float*)A, lda, (const float*)B, ldb, (float*)C,
-                                                      ldc, rmask, cmask);
+                                                      ldc, sel);
     else
       gemm_bf16_kernel<OP_SET><<<grid, block, 0, s>>>(m, n, k, (const float*)A, lda, (const float*)B, ldb, (float*)C,
-                                                      ldc, rmask, cmask);
+                                                      ldc, sel);
   } else {
     return KAD_ERR_ARG;
   }

@@ -30,6 +30,8 @@ int gemm_launch(int prec, int op, int m, int n, int k, const void* A, int64_t ld
                 int64_t ldc, hipStream_t s);
 int gemm_launch_masked(int prec, int op, int m, int n, int k, const void* A, int64_t lda, const void* B, int64_t ldb,
                        void* C, int64_t ldc, const int* rmask, const int* cmask, hipStream_t s);
+int gemm_launch_sel(int prec, int op, int m, int n, int k, const void* A, int64_t lda, const void* B, int64_t ldb,
+                    void* C, int64_t ldc, const TileSel& sel, hipStream_t s);
 constexpr int GOP_SUB = 0, GOP_SET = 1;
 
 constexpr int PANEL_BASE = 32;  // columns factored by the per-column kernel
@@ -170,6 +172,21 @@ __global__ __launch_bounds__(256) void row_tile_mask_kernel(const T* __restrict_
   any = __syncthreads_or(any);
   if (threadIdx.x == 0) mask[blockIdx.x] = any;
 }
+// Compacted lists of the active row / column tiles: list[0..) rows, list[stride..)
+// columns, list[2 stride] and list[2 stride + 1] their counts.  One thread (a few
+// hundred tiles).
+__global__ void tile_lists_kernel(const int* __restrict__ rmask, int nr, const int* __restrict__ cmask, int nc,
+                                  int* __restrict__ list, int stride) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  int a = 0, b = 0;
+  for (int i = 0; i < nr; ++i)
+    if (rmask[i]) list[a++] = i;
+  for (int j = 0; j < nc; ++j)
+    if (cmask[j]) list[stride + b++] = j;
+  list[2 * stride] = a;
+  list[2 * stride + 1] = b;
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void col_tile_mask_kernel(const T* __restrict__ A, int64_t lda, int nrows,
                                                             int ncols, int* __restrict__ mask) {
@@ -440,9 +457,17 @@ int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w)
       rc = trsm_left_unit_lower<T>(A + k0 + (int64_t)k0 * lda, lda, k1 - k0, A + k0 + (int64_t)k1 * lda, lda, n - k1,
                                    w, gemm_prec, w.mask_c);
       if (rc) return rc;
-      rc = gemm_launch_masked(gemm_prec, GOP_SUB, n - k1, n - k1, k1 - k0, A + k1 + (int64_t)k0 * lda, lda,
-                              A + k0 + (int64_t)k1 * lda, lda, A + k1 + (int64_t)k1 * lda, lda, w.mask_r, w.mask_c,
-                              w.stream);
+      // active tile pairs compacted over the grid, so they spread over every XCD
+      const int stride = (int)cdiv(n, 128) + 1;
+      tile_lists_kernel<<<1, 64, 0, w.stream>>>(w.mask_r, (int)cdiv(n - k1, 128), w.mask_c, (int)cdiv(n - k1, 128),
+                                                w.tile_list, stride);
+      KAD_HIP_CHECK(hipGetLastError());
+      TileSel sel;
+      sel.rlist = w.tile_list;
+      sel.clist = w.tile_list + stride;
+      sel.cnt = w.tile_list + 2 * stride;
+      rc = gemm_launch_sel(gemm_prec, GOP_SUB, n - k1, n - k1, k1 - k0, A + k1 + (int64_t)k0 * lda, lda,
+                           A + k0 + (int64_t)k1 * lda, lda, A + k1 + (int64_t)k1 * lda, lda, sel, w.stream);
       if (rc) return rc;
     }
   }

@@ -62,8 +62,10 @@ int Ctx::lu_work(int n, int nb) {
   if ((rc = ensure((void**)&lu.red_val, &cap_redv, 2 * sizeof(double) * red))) return rc;
   if ((rc = ensure((void**)&lu.red_idx, &cap_redi, 2 * sizeof(int) * red))) return rc;
   if ((rc = ensure((void**)&lu.map, &cap_map, 2 * sizeof(int) * (size_t)n))) return rc;
-  if ((rc = ensure((void**)&lu.mask_r, &cap_mask, 2 * sizeof(int) * (size_t)(cdiv(n, 128) + 1)))) return rc;
-  lu.mask_c = lu.mask_r + cdiv(n, 128) + 1;
+  const size_t nt = (size_t)cdiv(n, 128) + 1;
+  if ((rc = ensure((void**)&lu.mask_r, &cap_mask, sizeof(int) * (4 * nt + 2)))) return rc;
+  lu.mask_c = lu.mask_r + nt;
+  lu.tile_list = lu.mask_r + 2 * nt;
   lu.red_cap = red;
   if (!lu.counter) {
     KAD_HIP_CHECK(hipMalloc((void**)&lu.counter, 64));
