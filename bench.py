@@ -9,9 +9,10 @@ One "step" = one System_of_eqs::do_newton iteration: residual F(u) + batched-JVP
 Jacobian fill + factorisation + solve (+ fp64 refinement) + field update, all
 device-resident.  The K timed steps are the iterations of the Newton solve from
 the perturbed guess (restarting from the guess once ||F||_inf < 1e-10).  The
-headline solver is FP32-LU + FP64 GMRES-IR; the plain FP64-LU solve runs beside
-it, both to ||F||_inf < 1e-10, with time to solution and the converged fields
-compared.
+system is assembled in domain-major order (block-tridiagonal J) and the FP64 LU
+skips the structurally zero tile pairs.  The headline solver is FP64 (the one
+that reaches 1e-10 here); FP32-LU + FP64 GMRES-IR runs beside it to the same
+test, with time to solution and the converged fields compared.
 
 Prints ONE JSON line (rank 0).
 """
@@ -73,13 +74,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=6, help="timed Newton iterations")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--solver", default="f32-gmres", choices=["f64", "f32-ir", "f32-gmres", "bf16-gmres"])
+    ap.add_argument("--solver", default="f64", choices=["f64", "f32-ir", "f32-gmres", "bf16-gmres"])
+    ap.add_argument("--compare", default="f32-gmres", choices=["none", "f64", "f32-ir", "f32-gmres", "bf16-gmres"],
+                    help="solver run to ||F|| < 1e-10 beside the headline one")
     ap.add_argument("--batch", type=int, default=512)
     ap.add_argument("--order", type=int, default=1, choices=[0, 1],
                     help="unknown/row order: 0 reference, 1 domain-major (block-tridiagonal J)")
     ap.add_argument("--max-newton", type=int, default=12)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first copy a 256 MiB device buffer once (reference kernel for HBM counters)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 solve beside the headline solver")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only, "
@@ -93,6 +97,12 @@ def main():
     from kadath_frontera_amd.xcts import XctsSystem
     from kadath_frontera_amd._lib import lib
 
+    if args.copy_calibration:
+        src = torch.empty(1 << 25, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     ctx = Context(0)
     sysd = XctsSystem(ctx, RES, BOUNDS, n0=N0, a=A, ome=OME, batch=args.batch, order=args.order)
     n = sysd.n
@@ -152,30 +162,50 @@ def main():
     lu_tflops = (2.0 / 3.0) * n ** 3 / t_fact / 1e12
     log(f"timed {len(step_times)} iterations: {ms_step:.1f} ms/iteration (region {t_region:.2f} s)")
 
-    # live GEMM roofline (HIP events on the launch stream) over one iteration from the guess
+    # live roofline of the two kernel classes (HIP events on the launch stream) over one
+    # iteration from the guess: kind 0 MFMA GEMMs (flops), kind 1 fused 3-D spectral
+    # passes of the residual / Jacobian programs (algorithmic HBM bytes)
     sysd.set_guess(X0)
-    lib.kad_gemm_timing(ctx.h, 1)
+    lib.kad_kernel_timing(ctx.h, 0, 1)
+    lib.kad_kernel_timing(ctx.h, 1, 1)
     newton_iter(args.solver)
-    fl, gms, nl = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
-    lib.kad_gemm_timing_read(ctx.h, ctypes.byref(fl), ctypes.byref(gms), ctypes.byref(nl))
-    lib.kad_gemm_timing(ctx.h, 0)
-    achieved = fl.value / (gms.value / 1e3) / 1e12
-    peak = PEAK[prec_of[args.solver]]
+    meas = {}
+    for kind in (0, 1):
+        w, kms, nl = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        lib.kad_kernel_timing_read(ctx.h, kind, ctypes.byref(w), ctypes.byref(kms), ctypes.byref(nl))
+        lib.kad_kernel_timing(ctx.h, kind, 0)
+        meas[kind] = (w.value, kms.value, nl.value)
+    gw, gms, gn = meas[0]
+    aw, ams, an = meas[1]
+    roof_gemm = {"bound": "mfma", "achieved": gw / (gms / 1e3) / 1e12, "peak": PEAK[prec_of[args.solver]],
+                 "unit": "TFLOP/s", "traffic": None,
+                 "kernel": "gemm_k4_kernel: LU trailing update + blocked-TRSM updates, executed tile pairs "
+                           "(%d launches, %.3e flop, %.1f ms per iteration)" % (gn, gw, gms)}
+    roof_gemm["frac"] = roof_gemm["achieved"] / roof_gemm["peak"]
+    roof_apply = {"bound": "hbm", "achieved": aw / (ams / 1e3) / 1e9, "peak": 8000.0, "unit": "GB/s",
+                  "traffic": None,
+                  "kernel": "apply3d_kernel: fused 3-D spectral passes of the residual and Jacobian programs "
+                            "(%d launches, %.3e B, %.1f ms per iteration)" % (an, aw, ams)}
+    roof_apply["frac"] = roof_apply["achieved"] / roof_apply["peak"]
+    dominant, other = (roof_apply, roof_gemm) if ams > gms else (roof_gemm, roof_apply)
 
-    # time to solution, headline solver and plain FP64 beside it
+    # time to solution: headline solver and the comparison solver beside it
     runs = {args.solver: solve(args.solver)}
-    if args.solver != "f64" and not args.no_fp64:
-        runs["f64"] = solve("f64")
+    if args.compare not in ("none", args.solver):
+        runs[args.compare] = solve(args.compare)
     conv_info = {k: {"newton_iters": len(v[0]), "time_to_solution_s": v[1], "converged": v[2], "final_Finf": v[3],
                      "ms_per_iteration": 1e3 * float(np.mean([r[1] for r in v[0]])) if v[0] else None,
                      "factor_s_per_iteration": float(np.mean([r[2][4] for r in v[0]])) if v[0] else None,
                      "gmres_iters": [r[3][1] for r in v[0]] if k != "f64" else None}
                  for k, v in runs.items()}
     diff = None
-    if "f64" in runs and args.solver != "f64":
-        a, b = runs[args.solver][4], runs["f64"][4]
-        diff = float(np.abs(a - b).max() / max(1.0, np.abs(b).max()))
-    f64 = conv_info.get("f64") if args.solver != "f64" else None
+    if len(runs) == 2:
+        ref_key = "f64" if "f64" in runs else args.solver
+        oth_key = [k for k in runs if k != ref_key][0]
+        a, b = runs[oth_key][4], runs[ref_key][4]
+        diff = {"solver": oth_key, "vs": ref_key,
+                "max_rel": float(np.abs(a - b).max() / max(1.0, np.abs(b).max()))}
+    cmp_info = conv_info.get(args.compare) if args.compare in conv_info and args.compare != args.solver else None
     cpu = None if args.no_cpu_baseline else cpu_baseline()
     out = {
         "metric": "Newton-step s + LU TFLOP/s: FP32/BF16+IR vs FP64 at ||F||inf<1e-10, 1 GPU",
@@ -194,18 +224,13 @@ def main():
         "config": {"workload": "spinning-BH XCTS Newton iteration, 5 spherical domains 17x13x12, n=%d" % n,
                    "n_unknowns": n, "solver": args.solver, "jvp_batch": args.batch,
                    "order": ["reference", "domain-major"][args.order]},
-        "lu_tflops": lu_tflops,
+        "lu_tflops_dense_equivalent": lu_tflops,
         "phases_s": {"residual": phases[0], "jacobian_fill": phases[1], "factor_solve": phases[2],
                      "update": phases[3], "factor": phases[4]},
-        "fp64": f64,
-        "speedup_vs_fp64": {"per_iteration": f64["ms_per_iteration"] / ms_step,
-                            "time_to_solution": f64["time_to_solution_s"] / conv_info[args.solver][
-                                "time_to_solution_s"]} if f64 else None,
-        "convergence": {"solvers": conv_info, "fields_rel_diff_vs_fp64": diff},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
-                     "kernel": "LU trailing-update GEMM (%d launches, %.3e flop, %.1f ms in one iteration)" % (
-                         nl.value, fl.value, gms.value)},
+        "compare": cmp_info,
+        "convergence": {"solvers": conv_info, "fields_rel_diff": diff},
+        "roofline": dominant,
+        "roofline_other": other,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -45,10 +45,13 @@ int kad_ctx_destroy(kad_ctx* ctx);
 void* kad_ctx_stream(kad_ctx* ctx);
 int kad_sync(kad_ctx* ctx);
 
-/* Live timing of the MFMA GEMM launches with HIP events (on = 1 starts a new
- * measurement); read returns total flops, kernel milliseconds and launches. */
-int kad_gemm_timing(kad_ctx* ctx, int on);
-int kad_gemm_timing_read(kad_ctx* ctx, double* flops, double* ms, int* launches);
+/* Live timing of one kernel class with HIP events on the context stream: kind 0 =
+ * the MFMA GEMM launches (work in flops), kind 1 = the fused 3-D spectral passes
+ * of the residual / Jacobian programs (work in algorithmic HBM bytes).  on = 1
+ * starts a new measurement; read returns total work, kernel milliseconds and
+ * launches. */
+int kad_kernel_timing(kad_ctx* ctx, int kind, int on);
+int kad_kernel_timing_read(kad_ctx* ctx, int kind, double* work, double* ms, int* launches);
 
 /* ---- dense linear algebra (replaces pdgesv_, src_par/do_newton.cpp:317-320,
  *      and Matrice::set_lu/solve, src/Matrice/matrice.cpp:192-250) ---------- */

@@ -56,7 +56,7 @@ SIGNATURES = {
     "kad_xcts_do_newton": (c_int, [c_vp, c_dbl, c_dp, c_int, c_dp]),
     "kad_xcts_last_solve": (c_int, [c_vp, c_ip, c_dp]),
     "kad_xcts_perturbation": (c_int, [c_vp, ctypes.c_uint64, c_dbl, c_dp]),
-    "kad_gemm_timing": (c_int, [c_vp, c_int]),
+    "kad_kernel_timing": (c_int, [c_vp, c_int, c_int]),
     "kad_equilibrate": (c_int, [c_vp, c_int, c_vp, c_i64, c_vp, c_vp]),
     "kad_cast_equilibrated": (c_int, [c_vp, c_int, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp]),
     "kad_ir_solve_scaled": (c_int, [c_vp, c_int, c_vp, c_i64, c_int, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl,
@@ -64,7 +64,7 @@ SIGNATURES = {
     "kad_vec_scale": (c_int, [c_vp, c_int, c_vp, c_vp]),
     "kad_ir_set_restart": (c_int, [c_vp, c_int]),
     "kad_ir_history": (c_int, [c_vp, c_int, c_dp, c_int]),
-    "kad_gemm_timing_read": (c_int, [c_vp, c_dp, c_dp, c_ip]),
+    "kad_kernel_timing_read": (c_int, [c_vp, c_int, c_dp, c_dp, c_ip]),
     "kad_xcts_host_info": (c_int, [c_int, c_int, c_int, c_dp, c_int, c_int, c_int, c_dbl, c_dbl, c_dbl, c_int, c_ip,
                                    c_ip, c_int]),
 }

@@ -44,14 +44,18 @@ struct LuWork {
   int* tile_list = nullptr;     // compacted active row tiles, column tiles, and the 2 counts
 };
 
-// Optional live timing of the MFMA GEMM launches (HIP events on the launch
-// stream): the measured kernel of the LU roofline.
+// Optional live timing of one kernel class (HIP events on the launch stream) and
+// its algorithmic work per launch: flops of the MFMA GEMMs, bytes of the fused
+// 3-D spectral passes.
 struct GemmTimer {
   std::vector<hipEvent_t> ev;   // pairs
-  std::vector<double> flops;    // per pair
+  std::vector<double> flops;    // work per pair (flops or bytes)
   size_t used = 0;
+  int begin(hipStream_t s, double work);
+  int end(hipStream_t s);
 };
-extern GemmTimer* g_gemm_timer;
+extern GemmTimer* g_gemm_timer;   // GEMM launches (work = flops)
+extern GemmTimer* g_apply_timer;  // apply3d launches (work = HBM bytes: arrays in + out)
 
 // Device copy of a chain of 1-D passes (matrix pool + per-line tables).
 struct DevPlan {

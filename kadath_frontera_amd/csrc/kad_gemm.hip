@@ -338,11 +338,6 @@ int gemm_launch_sel(int prec, int op, int m, int n, int k, const void* A, int64_
   dim3 grid(nblocks), block(GNT);
   GemmTimer* tm = g_gemm_timer;
   if (tm) {
-    if (2 * tm->used + 2 > tm->ev.size()) {
-      const size_t old = tm->ev.size();
-      tm->ev.resize(old + 2048);
-      for (size_t q = old; q < tm->ev.size(); ++q) KAD_HIP_CHECK(hipEventCreate(&tm->ev[q]));
-    }
     // executed work: the tile pairs that are not skipped (the masks are read back,
     // which synchronises; timing runs only)
     double frac = 1.0;
@@ -362,8 +357,7 @@ int gemm_launch_sel(int prec, int op, int m, int n, int k, const void* A, int64_
       for (int v : hc) ac += (v != 0);
       frac = (ar / nbm) * (ac / nbn);
     }
-    tm->flops.push_back(2.0 * m * n * (double)k * frac);
-    KAD_HIP_CHECK(hipEventRecord(tm->ev[2 * tm->used], s));
+    if (tm->begin(s, 2.0 * m * n * (double)k * frac)) return KAD_ERR_HIP;
   }
   if (prec == F64) {
     if (op == OP_SUB)
@@ -390,10 +384,7 @@ int gemm_launch_sel(int prec, int op, int m, int n, int k, const void* A, int64_
     return KAD_ERR_ARG;
   }
   KAD_HIP_CHECK(hipGetLastError());
-  if (tm) {
-    KAD_HIP_CHECK(hipEventRecord(tm->ev[2 * tm->used + 1], s));
-    tm->used++;
-  }
+  if (tm && tm->end(s)) return KAD_ERR_HIP;
   return KAD_OK;
 }
 

@@ -45,6 +45,24 @@ __global__ void cast_scaled_kernel(const double* __restrict__ a, int64_t lda, fl
 
 static constexpr int RED_BLOCKS = 256;
 GemmTimer* g_gemm_timer = nullptr;
+GemmTimer* g_apply_timer = nullptr;
+
+int GemmTimer::begin(hipStream_t s, double work) {
+  if (2 * used + 2 > ev.size()) {
+    const size_t old = ev.size();
+    ev.resize(old + 2048);
+    for (size_t q = old; q < ev.size(); ++q)
+      if (hipEventCreate(&ev[q]) != hipSuccess) return KAD_ERR_HIP;
+  }
+  flops.push_back(work);
+  return hipEventRecord(ev[2 * used], s) == hipSuccess ? KAD_OK : KAD_ERR_HIP;
+}
+
+int GemmTimer::end(hipStream_t s) {
+  if (hipEventRecord(ev[2 * used + 1], s) != hipSuccess) return KAD_ERR_HIP;
+  ++used;
+  return KAD_OK;
+}
 
 int Ctx::ensure(void** p, size_t* cap, size_t bytes) {
   if (*cap >= bytes) return KAD_OK;
@@ -171,28 +189,30 @@ int kad_ctx_destroy(kad_ctx* ctx) {
   return KAD_OK;
 }
 
-// Live timing of the MFMA GEMM launches (on: start a new measurement; off: stop).
-int kad_gemm_timing(kad_ctx* ctx, int on) {
-  if (!ctx) return KAD_ERR_ARG;
-  static kad::GemmTimer timer;
+// Live timing of one kernel class: kind 0 = MFMA GEMM launches (work in flops),
+// kind 1 = fused 3-D spectral passes (work in HBM bytes).  on: start a new
+// measurement; off: stop.
+int kad_kernel_timing(kad_ctx* ctx, int kind, int on) {
+  if (!ctx || kind < 0 || kind > 1) return KAD_ERR_ARG;
+  static kad::GemmTimer timers[2];
+  kad::GemmTimer*& slot = kind == 0 ? kad::g_gemm_timer : kad::g_apply_timer;
   if (on) {
-    timer.used = 0;
-    timer.flops.clear();
-    kad::g_gemm_timer = &timer;
+    timers[kind].used = 0;
+    timers[kind].flops.clear();
+    slot = &timers[kind];
   } else {
-    kad::g_gemm_timer = nullptr;
+    slot = nullptr;
   }
   return KAD_OK;
 }
 
-// Sum of flops and kernel milliseconds of the GEMM launches since kad_gemm_timing(ctx, 1).
-int kad_gemm_timing_read(kad_ctx* ctx, double* flops, double* ms, int* launches) {
-  if (!ctx) return KAD_ERR_ARG;
+// Total work and kernel milliseconds of the launches of one class since
+// kad_kernel_timing(ctx, kind, 1).
+int kad_kernel_timing_read(kad_ctx* ctx, int kind, double* work, double* ms, int* launches) {
+  if (!ctx || kind < 0 || kind > 1) return KAD_ERR_ARG;
   Ctx* c = reinterpret_cast<Ctx*>(ctx);
   KAD_HIP_CHECK(hipStreamSynchronize(c->stream));
-  static kad::GemmTimer* t = nullptr;
-  (void)t;
-  kad::GemmTimer* tm = kad::g_gemm_timer;
+  kad::GemmTimer* tm = kind == 0 ? kad::g_gemm_timer : kad::g_apply_timer;
   double f = 0, tot = 0;
   int n = 0;
   if (tm) {
@@ -204,7 +224,7 @@ int kad_gemm_timing_read(kad_ctx* ctx, double* flops, double* ms, int* launches)
       ++n;
     }
   }
-  if (flops) *flops = f;
+  if (work) *work = f;
   if (ms) *ms = tot;
   if (launches) *launches = n;
   return KAD_OK;

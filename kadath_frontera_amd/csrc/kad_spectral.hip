@@ -14,23 +14,37 @@
 // Array/Index order (include/implementation/index.cpp:84-96).
 #include "kad_common.hpp"
 #include "kad_spectral.hpp"
+#include "kad_ctx.hpp"
 
 namespace kad {
 
-// One workgroup per batch element.  Dynamic LDS: two ping-pong buffers.
+// One workgroup per batch element.  Dynamic LDS: two ping-pong buffers, then the
+// plan's matrix pool and per-line matrix tables (a few KB, staged once), so the
+// contraction of every output element reads only LDS.
 __global__ __launch_bounds__(256) void apply3d_kernel(const double* __restrict__ in, int64_t in_stride,
                                                       double* __restrict__ out, int64_t out_stride, int d0, int d1,
                                                       int d2, Passes P, const double* __restrict__ pool,
                                                       int accumulate, double scale) {
   extern __shared__ double lds[];
   double* buf[2] = {lds, lds + P.maxsize};
+  double* mats = lds + 2 * P.maxsize;
+  int* ltab = reinterpret_cast<int*>(mats + P.pool_size);
   const int64_t b = blockIdx.x;
   const int nin = d0 * d1 * d2;
   const double* src = in + b * in_stride;
   for (int e = threadIdx.x; e < nin; e += blockDim.x) buf[0][e] = src[e];
+  for (int e = threadIdx.x; e < P.pool_size; e += blockDim.x) mats[e] = pool[e];
+  {
+    int o = 0;
+    for (int p = 0; p < P.npass; ++p) {
+      for (int e = threadIdx.x; e < P.nlines[p]; e += blockDim.x) ltab[o + e] = P.line_mat[p][e];
+      o += P.nlines[p];
+    }
+  }
   __syncthreads();
   int cur = 0;
   int dims[3] = {d0, d1, d2};
+  int loff = 0;
   for (int p = 0; p < P.npass; ++p) {
     const int ax = P.axis[p];
     const int nout_ax = P.nout[p];
@@ -40,37 +54,44 @@ __global__ __launch_bounds__(256) void apply3d_kernel(const double* __restrict__
     const int nout = od[0] * od[1] * od[2];
     const double* x = buf[cur];
     double* y = buf[cur ^ 1];
-    const int* lm = P.line_mat[p];
+    const int* lm = ltab + loff;
+    const int stride = ax == 0 ? 1 : (ax == 1 ? dims[0] : dims[0] * dims[1]);
     for (int e = threadIdx.x; e < nout; e += blockDim.x) {
       const int i = e % od[0];
       const int t = e / od[0];
       const int j = t % od[1];
       const int k = t / od[1];
-      int line, o, base_idx, stride;
+      int line, o, base_idx;
       if (ax == 0) {
         line = j + od[1] * k;
         o = i;
         base_idx = dims[0] * (j + dims[1] * k);
-        stride = 1;
       } else if (ax == 1) {
         line = k;
         o = j;
         base_idx = i + dims[0] * dims[1] * k;
-        stride = dims[0];
       } else {
         line = 0;
         o = k;
         base_idx = i + dims[0] * j;
-        stride = dims[0] * dims[1];
       }
       const int off = lm[line];
-      double s = 0.0;
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
       if (off >= 0) {
-        const double* M = pool + off + (int64_t)o * nin_ax;
-        for (int q = 0; q < nin_ax; ++q) s += M[q] * x[base_idx + q * stride];
+        const double* M = mats + off + o * nin_ax;
+        const double* xv = x + base_idx;
+        int q = 0;
+        for (; q + 4 <= nin_ax; q += 4) {
+          s0 += M[q] * xv[q * stride];
+          s1 += M[q + 1] * xv[(q + 1) * stride];
+          s2 += M[q + 2] * xv[(q + 2) * stride];
+          s3 += M[q + 3] * xv[(q + 3) * stride];
+        }
+        for (; q < nin_ax; ++q) s0 += M[q] * xv[q * stride];
       }
-      y[e] = s;
+      y[e] = (s0 + s1) + (s2 + s3);
     }
+    loff += P.nlines[p];
     dims[0] = od[0];
     dims[1] = od[1];
     dims[2] = od[2];
@@ -113,10 +134,22 @@ __global__ __launch_bounds__(256) void apply_lines_kernel(const double* __restri
 int apply3d_launch(const double* in, int64_t in_stride, double* out, int64_t out_stride, int nbatch, int d0, int d1,
                    int d2, const Passes& P, const double* pool, int accumulate, double scale, hipStream_t s) {
   if (nbatch <= 0) return KAD_OK;
-  const size_t lds = 2 * (size_t)P.maxsize * sizeof(double);
+  size_t nl = 0;
+  for (int p = 0; p < P.npass; ++p) nl += (size_t)P.nlines[p];
+  const size_t lds = (2 * (size_t)P.maxsize + (size_t)P.pool_size) * sizeof(double) + nl * sizeof(int);
   if (lds > 160 * 1024) return KAD_ERR_ARG;
+  GemmTimer* tm = g_apply_timer;
+  if (tm) {
+    // algorithmic HBM bytes: the input arrays read once, the output arrays written
+    // once (accumulate also reads them); the per-line matrices are cache-resident
+    int od[3] = {d0, d1, d2};
+    for (int p = 0; p < P.npass; ++p) od[P.axis[p]] = P.nout[p];
+    const double nin = (double)d0 * d1 * d2, nout = (double)od[0] * od[1] * od[2];
+    if (tm->begin(s, 8.0 * nbatch * (nin + nout * (accumulate ? 2 : 1)))) return KAD_ERR_HIP;
+  }
   apply3d_kernel<<<nbatch, 256, lds, s>>>(in, in_stride, out, out_stride, d0, d1, d2, P, pool, accumulate, scale);
   KAD_HIP_CHECK(hipGetLastError());
+  if (tm && tm->end(s)) return KAD_ERR_HIP;
   return KAD_OK;
 }
 

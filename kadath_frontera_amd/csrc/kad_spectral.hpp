@@ -14,9 +14,11 @@ constexpr int MAX_PASSES = 6;
 // axis 2 by 0.
 struct Passes {
   int npass = 0;
-  int maxsize = 0;  // largest intermediate array (doubles)
+  int maxsize = 0;    // largest intermediate array (doubles)
+  int pool_size = 0;  // doubles of the plan's matrix pool (staged in LDS)
   int axis[MAX_PASSES];
   int nout[MAX_PASSES];
+  int nlines[MAX_PASSES];  // entries of line_mat[p] (staged in LDS)
   const int* line_mat[MAX_PASSES];
 };
 

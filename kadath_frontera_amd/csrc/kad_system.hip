@@ -278,6 +278,7 @@ int kad_system_impl::setup() {
     if ((rc = upload(hp.pool, &dp.pool))) return rc;
     dp.P.npass = (int)hp.passes.size();
     dp.P.maxsize = hp.maxsize;
+    dp.P.pool_size = (int)hp.pool.size();
     if (dp.P.npass > kad::MAX_PASSES) return KAD_ERR_ARG;
     for (int p = 0; p < dp.P.npass; ++p) {
       int* lm;
@@ -285,6 +286,7 @@ int kad_system_impl::setup() {
       dp.P.axis[p] = hp.passes[p].axis;
       dp.P.nout[p] = hp.passes[p].nout;
       dp.P.line_mat[p] = lm;
+      dp.P.nlines[p] = (int)hp.passes[p].line_mat.size();
     }
     plans.push_back(dp);
   }

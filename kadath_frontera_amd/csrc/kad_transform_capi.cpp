@@ -70,16 +70,17 @@ int get_plan(Ctx* c, int dir, int nr, int nt, int np, const int* base_r, const i
   const int* dlr = p->lines + 1 + npc;
   p->P.npass = 3;
   p->P.maxsize = nr * nt * npc;
+  p->P.pool_size = (int)pool.size();
   if (dir == 0) {  // phi, theta, r  (coef.cpp:100-101: d = ndim-1 .. 0)
-    p->P.axis[0] = 2; p->P.nout[0] = npc; p->P.line_mat[0] = dlp;
-    p->P.axis[1] = 1; p->P.nout[1] = nt;  p->P.line_mat[1] = dlt;
-    p->P.axis[2] = 0; p->P.nout[2] = nr;  p->P.line_mat[2] = dlr;
+    p->P.axis[0] = 2; p->P.nout[0] = npc; p->P.line_mat[0] = dlp; p->P.nlines[0] = 1;
+    p->P.axis[1] = 1; p->P.nout[1] = nt;  p->P.line_mat[1] = dlt; p->P.nlines[1] = npc;
+    p->P.axis[2] = 0; p->P.nout[2] = nr;  p->P.line_mat[2] = dlr; p->P.nlines[2] = nt * npc;
     p->d_in[0] = nr; p->d_in[1] = nt; p->d_in[2] = np;
     p->d_out[0] = nr; p->d_out[1] = nt; p->d_out[2] = npc;
   } else {  // r, theta, phi  (coef_i.cpp:100-101: d = 0 .. ndim-1)
-    p->P.axis[0] = 0; p->P.nout[0] = nr; p->P.line_mat[0] = dlr;
-    p->P.axis[1] = 1; p->P.nout[1] = nt; p->P.line_mat[1] = dlt;
-    p->P.axis[2] = 2; p->P.nout[2] = np; p->P.line_mat[2] = dlp;
+    p->P.axis[0] = 0; p->P.nout[0] = nr; p->P.line_mat[0] = dlr; p->P.nlines[0] = nt * npc;
+    p->P.axis[1] = 1; p->P.nout[1] = nt; p->P.line_mat[1] = dlt; p->P.nlines[1] = npc;
+    p->P.axis[2] = 2; p->P.nout[2] = np; p->P.line_mat[2] = dlp; p->P.nlines[2] = 1;
     p->d_in[0] = nr; p->d_in[1] = nt; p->d_in[2] = npc;
     p->d_out[0] = nr; p->d_out[1] = nt; p->d_out[2] = np;
   }
