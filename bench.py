@@ -77,7 +77,7 @@ def main():
     ap.add_argument("--solver", default="f64", choices=["f64", "f32-ir", "f32-gmres", "bf16-gmres"])
     ap.add_argument("--compare", default="f32-gmres", choices=["none", "f64", "f32-ir", "f32-gmres", "bf16-gmres"],
                     help="solver run to ||F|| < 1e-10 beside the headline one")
-    ap.add_argument("--batch", type=int, default=512)
+    ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--order", type=int, default=1, choices=[0, 1],
                     help="unknown/row order: 0 reference, 1 domain-major (block-tridiagonal J)")
     ap.add_argument("--max-newton", type=int, default=12)

@@ -42,6 +42,8 @@ struct LuWork {
   int* mask_r = nullptr;        // per 128-row tile of the L panel: any nonzero
   int* mask_c = nullptr;        // per 128-column tile of the U row panel: any nonzero
   int* tile_list = nullptr;     // compacted active row tiles, column tiles, and the 2 counts
+  int* swap_map = nullptr;      // row gathers of the deferred left interchanges
+  size_t swap_cap = 0;          // ints
 };
 
 // Optional live timing of one kernel class (HIP events on the launch stream) and

@@ -434,15 +434,89 @@ int panel_rec(T* A, int64_t lda, int n, int c0, int c1, int* ipiv, LuWork& w) {
   return laswp<T>(A, lda, c0, cm, ipiv, cm, c1, w);
 }
 
+// dst[i + c ldt] = A[map[i] + c lda] for i < nrows, c < ncols (row gather of a column block).
+template <typename T>
+__global__ void gather_rows_kernel(const T* __restrict__ A, int64_t lda, const int* __restrict__ map, int nrows,
+                                   T* __restrict__ dst, int64_t ldt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (i < nrows) dst[i + (int64_t)c * ldt] = A[map[i] + (int64_t)c * lda];
+}
+
+// The interchanges of later panels applied to the L columns of earlier panels, once,
+// after the factorisation (those columns are never read again while factoring, so
+// deferring them is exact).  For panel p the interchanges of steps k1_p .. n-1,
+// composed on the host, are one row gather per column of rows [k1_p, n).
+template <typename T>
+int apply_left_swaps(T* A, int64_t lda, int n, const int* ipiv, int nb, LuWork& w) {
+  const int np = (int)cdiv(n, nb);
+  if (np < 2) return KAD_OK;
+  std::vector<int> hp(n);
+  KAD_HIP_CHECK(hipMemcpyAsync(hp.data(), ipiv, n * sizeof(int), hipMemcpyDeviceToHost, w.stream));
+  KAD_HIP_CHECK(hipStreamSynchronize(w.stream));
+  size_t tot = 0;
+  for (int p = 0; p + 1 < np; ++p) tot += (size_t)(n - (p + 1) * nb);
+  if (tot > w.swap_cap) {
+    if (w.swap_map) KAD_HIP_CHECK(hipFree(w.swap_map));
+    w.swap_map = nullptr;
+    w.swap_cap = 0;
+    KAD_HIP_CHECK(hipMalloc((void**)&w.swap_map, tot * sizeof(int)));
+    w.swap_cap = tot;
+  }
+  std::vector<int> maps(tot);
+  std::vector<size_t> moff(np, 0);
+  size_t o = 0;
+  for (int p = 0; p + 1 < np; ++p) {
+    const int k1 = (p + 1) * nb;
+    moff[p] = o;
+    int* idx = maps.data() + o;
+    for (int i = 0; i < n - k1; ++i) idx[i] = k1 + i;
+    for (int i = k1; i < n; ++i) {
+      const int q = hp[i];
+      if (q != i) std::swap(idx[i - k1], idx[q - k1]);
+    }
+    o += (size_t)(n - k1);
+  }
+  KAD_HIP_CHECK(hipMemcpyAsync(w.swap_map, maps.data(), tot * sizeof(int), hipMemcpyHostToDevice, w.stream));
+  for (int p = 0; p + 1 < np; ++p) {
+    const int k0 = p * nb, k1 = k0 + nb;
+    const int nrows = n - k1;
+    const int chunk = (int)std::max<size_t>(1, std::min<size_t>(k1 - k0, w.tmp_bytes / sizeof(T) / nrows));
+    for (int c = k0; c < k1; c += chunk) {
+      const int nc = std::min(chunk, k1 - c);
+      dim3 g((unsigned)cdiv(nrows, 256), nc);
+      gather_rows_kernel<T><<<g, 256, 0, w.stream>>>(A + (int64_t)c * lda, lda, w.swap_map + moff[p], nrows,
+                                                     (T*)w.tmp, nrows);
+      copy2d_kernel<T><<<g, 256, 0, w.stream>>>((const T*)w.tmp, nrows, A + k1 + (int64_t)c * lda, lda, nrows, nc);
+      KAD_HIP_CHECK(hipGetLastError());
+    }
+  }
+  // the host map buffer must outlive its asynchronous upload
+  KAD_HIP_CHECK(hipStreamSynchronize(w.stream));
+  return KAD_OK;
+}
+
 // Full factorisation.  gemm_prec selects the trailing-update MFMA precision
 // (F64 for double storage; F32 or BF16 for float storage).
 template <typename T>
 int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w) {
+  std::vector<int> hmask;
   for (int k0 = 0; k0 < n; k0 += nb) {
     const int k1 = std::min(k0 + nb, n);
-    int rc = panel_rec<T>(A, lda, n, k0, k1, ipiv, w);
-    if (rc) return rc;
-    rc = laswp<T>(A, lda, 0, k0, ipiv, k0, k1, w);
+    // rows below which the panel's columns are exactly zero: the panel (pivot
+    // search, interchanges, its internal updates) runs on rows [k0, R) only
+    const int ntile = (int)cdiv(n - k0, 128);
+    row_tile_mask_kernel<T><<<(unsigned)ntile, 256, 0, w.stream>>>(A + k0 + (int64_t)k0 * lda, lda, n - k0, k1 - k0,
+                                                                   w.mask_r);
+    KAD_HIP_CHECK(hipGetLastError());
+    hmask.resize(ntile);
+    KAD_HIP_CHECK(hipMemcpyAsync(hmask.data(), w.mask_r, ntile * sizeof(int), hipMemcpyDeviceToHost, w.stream));
+    KAD_HIP_CHECK(hipStreamSynchronize(w.stream));
+    int last = -1;
+    for (int t = 0; t < ntile; ++t)
+      if (hmask[t]) last = t;
+    const int R = std::max(k1, std::min(n, k0 + (last + 1) * 128));
+    int rc = panel_rec<T>(A, lda, R, k0, k1, ipiv, w);
     if (rc) return rc;
     if (k1 < n) {
       rc = laswp<T>(A, lda, k1, n, ipiv, k0, k1, w);
@@ -471,7 +545,7 @@ int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w)
       if (rc) return rc;
     }
   }
-  return KAD_OK;
+  return apply_left_swaps<T>(A, lda, n, ipiv, nb, w);
 }
 
 template int getrf<double>(double*, int64_t, int, int*, int, int, LuWork&);

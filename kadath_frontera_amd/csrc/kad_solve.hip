@@ -155,7 +155,7 @@ Ctx::~Ctx() {
   for (auto& kv : plans) delete kv.second;
   for (void* p : owned) (void)hipFree(p);
   void* bufs[] = {lu.red_val, lu.red_idx, lu.counter, lu.tinv,   lu.tmp,  lu.tmp2,
-                  lu.map,     lu.mask_r,  gemv_part,  vwork,     krylov,  trs_tab};
+                  lu.map,     lu.mask_r,  lu.swap_map, gemv_part, vwork,  krylov, trs_tab};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (own_stream && stream) (void)hipStreamDestroy(stream);
