@@ -167,6 +167,12 @@ int kad_xcts_jacobian(kad_system* s, double* J, int64_t ldj);
 int kad_xcts_update(kad_system* s, const double* X);
 /* coefficients of field f in domain d, HOST out (nr*nt*(np+2)) */
 int kad_xcts_get_coef(kad_system* s, int f, int d, double* out);
+/* overwrite them, HOST in (Scalar::set_domain of the reference's field API) */
+int kad_xcts_set_coef(kad_system* s, int f, int d, const double* in);
+/* ADM mass, Komar mass and ADM angular momentum about z of the current fields,
+ * surface integrals at infinity (codes/Par_version/Kerr/reader.cpp:114-125 with
+ * Domain_compact::integ, domain_compact.cpp:89-121), HOST out[3] */
+int kad_xcts_observables(kad_system* s, double* out);
 /* One Newton iteration.  solver: 0 fp64 LU, 1 fp32 LU + fp64 IR, 2 fp32 LU +
  * GMRES-IR, 3 bf16-update LU + GMRES-IR.  Returns 1 if ||F||_inf < precision at
  * entry (no step), 0 after a step, KAD_ERR_SINGULAR on an exactly singular J.

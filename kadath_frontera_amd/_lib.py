@@ -53,6 +53,8 @@ SIGNATURES = {
     "kad_xcts_jacobian": (c_int, [c_vp, c_vp, c_i64]),
     "kad_xcts_update": (c_int, [c_vp, c_vp]),
     "kad_xcts_get_coef": (c_int, [c_vp, c_int, c_int, c_dp]),
+    "kad_xcts_set_coef": (c_int, [c_vp, c_int, c_int, c_dp]),
+    "kad_xcts_observables": (c_int, [c_vp, c_dp]),
     "kad_xcts_do_newton": (c_int, [c_vp, c_dbl, c_dp, c_int, c_dp]),
     "kad_xcts_last_solve": (c_int, [c_vp, c_ip, c_dp]),
     "kad_xcts_perturbation": (c_int, [c_vp, ctypes.c_uint64, c_dbl, c_dp]),

@@ -33,6 +33,34 @@ TE XctsSystem::t_div(const TE& a, const TE& b) {
 }
 TE XctsSystem::t_scal(const TE& a, double x) { return {tr.scal(a.val, x), tr.scal(a.der, x)}; }
 TE XctsSystem::t_div_scal(const TE& a, double x) { return {tr.div_scal(a.val, x), tr.div_scal(a.der, x)}; }
+// s^i radial unit vector and m^i = (-y, x, 0) (the rotation generator) of domain d as
+// constant fields; x/y components standard base, z anti-symmetric, m^z = 0.  With
+// unit_m, m / r = sin(theta) (-sin(phi), cos(phi), 0) instead (bounded at infinity).
+void XctsSystem::unit_vectors(int d, TE sm[3], TE mg[3], bool unit_m) {
+  const Domain& dm = doms[d];
+  std::vector<double> sx(dm.conf_size()), sy(dm.conf_size()), sz(dm.conf_size()), mx(dm.conf_size()),
+      my(dm.conf_size());
+  for (int k = 0; k < dm.np; ++k)
+    for (int j = 0; j < dm.nt; ++j)
+      for (int i = 0; i < dm.nr; ++i) {
+        const size_t q = (size_t)i + dm.nr * (j + (size_t)dm.nt * k);
+        const double st = std::sin(dm.th[j]), ct = std::cos(dm.th[j]), cp = std::cos(dm.ph[k]),
+                     sp = std::sin(dm.ph[k]);
+        sx[q] = st * cp;
+        sy[q] = st * sp;
+        sz[q] = ct;
+        const double rf = unit_m ? 1.0 : dm.R[q];
+        mx[q] = -rf * st * sp;
+        my[q] = rf * st * cp;
+      }
+  sm[0] = cached_const(d, "sx", sx, dm.std_base());
+  sm[1] = cached_const(d, "sy", sy, dm.std_base());
+  sm[2] = cached_const(d, "sz", sz, dm.anti_base());
+  mg[0] = cached_const(d, unit_m ? "mhx" : "mx", mx, dm.std_base());
+  mg[1] = cached_const(d, unit_m ? "mhy" : "my", my, dm.std_base());
+  mg[2] = TE{tr.zero(d, SP_VAL), tr.zero(d, SP_TAN)};
+}
+
 TE XctsSystem::t_const(int d, const std::vector<double>& v, const Base& b) {
   return {tr.const_conf(d, v, b), tr.zero(d, SP_TAN)};
 }
@@ -143,26 +171,9 @@ std::vector<TE> XctsSystem::expr(const EqRec& e, int part, Defs& D) {
   }
   // inner boundary (excision)
   if (w == "N") return {TE{tr.add_scal(D.N.val, -prm.n0), D.N.der}};
-  // s^i radial unit vector, m^i = (-y, x, 0); x/y components standard base, z anti-symmetric
-  std::vector<double> sx(dm.conf_size()), sy(dm.conf_size()), sz(dm.conf_size()), mx(dm.conf_size()),
-      my(dm.conf_size());
-  for (int k = 0; k < dm.np; ++k)
-    for (int j = 0; j < dm.nt; ++j)
-      for (int i = 0; i < dm.nr; ++i) {
-        const size_t q = (size_t)i + dm.nr * (j + (size_t)dm.nt * k);
-        const double st = std::sin(dm.th[j]), ct = std::cos(dm.th[j]), cp = std::cos(dm.ph[k]),
-                     sp = std::sin(dm.ph[k]);
-        sx[q] = st * cp;
-        sy[q] = st * sp;
-        sz[q] = ct;
-        mx[q] = -dm.R[q] * st * sp;
-        my[q] = dm.R[q] * st * cp;
-      }
-  TE sm[3] = {cached_const(d, "sx", sx, dm.std_base()), cached_const(d, "sy", sy, dm.std_base()),
-              cached_const(d, "sz", sz, dm.anti_base())};
+  TE sm[3], mg[3];
+  unit_vectors(d, sm, mg, false);
   if (w == "bet") {
-    TE mg[3] = {cached_const(d, "mx", mx, dm.std_base()), cached_const(d, "my", my, dm.std_base()),
-                TE{tr.zero(d, SP_VAL), tr.zero(d, SP_TAN)}};
     TE ome = cached_const(d, "ome", std::vector<double>(dm.conf_size(), prm.ome), dm.std_base());
     TE n0 = cached_const(d, "n0", std::vector<double>(dm.conf_size(), prm.n0), dm.std_base());
     TE nP2 = t_div(n0, t_pow(D.P, 2, d));
@@ -360,6 +371,55 @@ void XctsSystem::build() {
   n_unknowns = off;
   if (n_unknowns != n_rows) throw std::runtime_error("Number of unknowns is different from the number of equations");
   for (auto& L : layout) trace_tangent(L[0], L[1]);
+  trace_observables();
+}
+
+// Surface integrands of the global quantities at the outer boundary of the
+// compactified domain (codes/Par_version/Kerr/reader.cpp:114-125), in the Cartesian
+// basis: M_ADM = -integ(d_r P)/2pi, M_Komar = integ(d_r N)/4pi,
+// J = -integ(A^ij s_i m_j)/8pi, where integ (Domain_compact::integ,
+// domain_compact.cpp:89-121) takes r r sin(theta) so, evaluates it on the boundary and
+// weighs the theta coefficients.  Traced into a program of its own (value space),
+// run after the residual program on the current fields.
+void XctsSystem::trace_observables() {
+  const int d = prm.dmax;
+  Domain& dm = doms[d];
+  if (!dm.compact) throw std::runtime_error("observables need a compactified outer domain");
+  Program saved = std::move(tr.val);
+  tr.val = Program();
+  auto zd = zero_ders();
+  Defs D = defs(d, zd[d]);
+  TE sm[3], mh[3];
+  unit_vectors(d, sm, mh, true);
+  // A^ij s_i m_j = r A^ij s_i mh_j; the factor r is applied in coefficient space
+  // (mult_r of the compactified domain: div_xm1 / alpha, domain_compact_ope.cpp:173-178)
+  VDp j_int = nullptr;
+  for (int i = 0; i < 3; ++i)
+    for (int k = 0; k < 2; ++k) {
+      VDp t = tr.mul(tr.mul(D.A[i][k].val, sm[i].val), mh[k].val);
+      j_int = j_int ? tr.add(j_int, t) : t;
+    }
+  if (!j_int->zero) j_int = tr.scal(tr.div_xm1(j_int), 1.0 / dm.alpha);
+  const VDp so[3] = {tr.der_normal(D.P.val, OUTER_BC), tr.der_normal(D.N.val, OUTER_BC), j_int};
+  const double fac[3] = {-1.0 / (2 * M_PI), 1.0 / (4 * M_PI), -1.0 / (8 * M_PI)};
+  obs.clear();
+  for (int q = 0; q < 3; ++q) {
+    ObsRec r;
+    r.fac = fac[q];
+    if (so[q]->zero) {
+      obs.push_back(r);
+      continue;
+    }
+    // r r sin(theta) so; mult_r of the compactified domain is div_xm1 / alpha (domain_compact_ope.cpp:173-178)
+    VDp rr = tr.mult_sin_theta(so[q]);
+    for (int t = 0; t < 2; ++t) rr = tr.scal(tr.div_xm1(rr), 1.0 / dm.alpha);
+    tr.coef(rr);
+    r.slot = rr->cf;
+    r.bt0 = rr->base.bt[0];
+    obs.push_back(r);
+  }
+  obs_prog = std::move(tr.val);
+  tr.val = std::move(saved);
 }
 
 }  // namespace kadhost

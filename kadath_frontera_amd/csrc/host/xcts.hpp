@@ -69,6 +69,15 @@ class XctsSystem {
     std::vector<double> w;
   };
   std::vector<Csr> tau_maps;  // indexed like layout
+  // global quantities at the outer boundary: value-space program producing, per
+  // quantity, the coefficients of r r sin(theta) so; result = fac * integ weights
+  struct ObsRec {
+    int slot = -1;  // value slot of the coefficients (-1: identically zero)
+    int bt0 = 0;    // theta basis of phi coefficient 0
+    double fac = 0;
+  };
+  Program obs_prog;
+  std::vector<ObsRec> obs;  // M_ADM, M_Komar, J_ADM
 
   explicit XctsSystem(const XctsParams& p);
   Base field_base(int f, int d) const { return f == 4 ? doms[d].anti_base() : doms[d].std_base(); }
@@ -102,6 +111,8 @@ class XctsSystem {
   std::vector<TE> expr(const EqRec& e, int part, Defs& D);
   void trace_value();
   void reorder_rows();
+  void trace_observables();
+  void unit_vectors(int d, TE sm[3], TE mg[3], bool unit_m);
   void trace_tangent(int f, int d);
   std::vector<std::array<VDp, NF>> zero_ders();
 };

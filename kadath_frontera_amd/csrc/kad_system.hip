@@ -132,6 +132,33 @@ __global__ void absmax_kernel(const double* x, int n, double* out) {
   if (threadIdx.x == 0) *out = red[0];
 }
 
+// Surface integral of one quantity (Domain_compact::integ, domain_compact.cpp:89-121):
+// cf holds the coefficients of r r sin(theta) so; value on the boundary x = bsign of
+// phi coefficient 0, theta weights pi (COS_EVEN, j = 0) or 2/(2j+1) (SIN_ODD), times
+// 2 pi and fac.  One workgroup.
+__global__ void surface_integral_kernel(const double* __restrict__ cf, int nr, int nt, int bt0, int bsign,
+                                        double fac, double* __restrict__ out) {
+  __shared__ double red[64];
+  const int j = threadIdx.x;
+  double v = 0.0;
+  if (j < nt && (bt0 == COS_EVEN || bt0 == SIN_ODD)) {
+    double b = 0.0, sg = 1.0;
+    for (int i = 0; i < nr; ++i) {
+      b += sg * cf[i + (int64_t)nr * j];
+      sg *= bsign;
+    }
+    if (bt0 == COS_EVEN) v = (j == 0) ? M_PI * b : 0.0;
+    else v = 2.0 / (2.0 * j + 1.0) * b;
+  }
+  red[threadIdx.x] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int q = 0; q < 64; ++q) t += red[q];
+    *out = t * 2.0 * M_PI * fac;
+  }
+}
+
 struct DevPlanX {
   kad::Passes P;
   double* pool = nullptr;
@@ -606,6 +633,47 @@ int kad_xcts_get_coef(kad_system* s, int f, int d, double* out) {
                                (size_t)sy.doms[d].coef_size() * sizeof(double), hipMemcpyDeviceToHost,
                                S->ctx->stream));
   KAD_HIP_CHECK(hipStreamSynchronize(S->ctx->stream));
+  return KAD_OK;
+}
+
+// Overwrite the coefficients of field f in domain d (HOST in, nr*nt*(np+2)): the
+// counterpart of kad_xcts_get_coef, for fields prepared elsewhere (Scalar::set_domain).
+int kad_xcts_set_coef(kad_system* s, int f, int d, const double* in) {
+  if (!s || !in) return KAD_ERR_ARG;
+  auto* S = reinterpret_cast<kad_system_impl*>(s);
+  XctsSystem& sy = *S->sys;
+  if (f < 0 || f >= XctsSystem::NF || d < sy.prm.dmin || d > sy.prm.dmax) return KAD_ERR_ARG;
+  KAD_HIP_CHECK(hipMemcpyAsync(S->val_arena + (int64_t)sy.field_slot[f][d] * S->stride, in,
+                               (size_t)sy.doms[d].coef_size() * sizeof(double), hipMemcpyHostToDevice,
+                               S->ctx->stream));
+  KAD_HIP_CHECK(hipStreamSynchronize(S->ctx->stream));
+  return KAD_OK;
+}
+
+// Global quantities of the current fields at the outer boundary of the compactified
+// domain (codes/Par_version/Kerr/reader.cpp:114-125): out[0] ADM mass, out[1] Komar
+// mass, out[2] ADM angular momentum about z.  HOST out.
+int kad_xcts_observables(kad_system* s, double* out) {
+  if (!s || !out) return KAD_ERR_ARG;
+  auto* S = reinterpret_cast<kad_system_impl*>(s);
+  XctsSystem& sy = *S->sys;
+  hipStream_t st = S->ctx->stream;
+  int rc = S->value_pass();
+  if (rc) return rc;
+  if ((rc = S->run(sy.obs_prog.code, 1, 0, -1, S->F, 0, 0))) return rc;
+  const Domain& dm = sy.doms[sy.prm.dmax];
+  for (int q = 0; q < 3; ++q) {
+    const auto& r = sy.obs[q];
+    if (r.slot < 0) {
+      KAD_HIP_CHECK(hipMemsetAsync(S->scratch + q, 0, sizeof(double), st));
+      continue;
+    }
+    surface_integral_kernel<<<1, 64, 0, st>>>(S->val_arena + (int64_t)r.slot * S->stride, dm.nr, dm.nt, r.bt0, 1,
+                                              r.fac, S->scratch + q);
+    KAD_HIP_CHECK(hipGetLastError());
+  }
+  KAD_HIP_CHECK(hipMemcpyAsync(out, S->scratch, 3 * sizeof(double), hipMemcpyDeviceToHost, st));
+  KAD_HIP_CHECK(hipStreamSynchronize(st));
   return KAD_OK;
 }
 

@@ -97,6 +97,16 @@ class XctsSystem:
         self.ctx._enter()
         check(lib.kad_xcts_update(self.h, ctypes.c_void_p(X.data_ptr())), "kad_xcts_update")
 
+    def set_coef(self, f, d, cf):
+        cf = np.ascontiguousarray(cf, dtype=np.float64)
+        check(lib.kad_xcts_set_coef(self.h, f, d, _dp(cf)), "kad_xcts_set_coef")
+
+    def observables(self):
+        """(M_ADM, M_Komar, J_ADM) of the current fields (surface integrals at infinity)."""
+        out = np.zeros(3)
+        check(lib.kad_xcts_observables(self.h, _dp(out)), "kad_xcts_observables")
+        return dict(M_adm=out[0], M_komar=out[1], J_adm=out[2])
+
     def coef(self, f, d):
         nr, nt, npt = self.res
         out = np.zeros((npt + 2) * nt * nr)

@@ -353,6 +353,30 @@ class XCTSSystem:
             val = val - self.spin * self.cm * self.cm
         return val
 
+    # ---- global quantities (codes/Par_version/Kerr/reader.cpp:114-125, Cartesian basis)
+    def observables(self):
+        """M_ADM = -integ(d_r P)/2pi, M_Komar = integ(d_r N)/4pi, J_ADM = -integ(A^ij s_i m_j)/8pi
+        at the outer boundary of the compactified domain, integ as Domain_compact::integ
+        (domain_compact.cpp:89-121); d_r of the compactified domain is der_normal there
+        (domain_compact_ope.cpp:187-189, domain_compact.cpp:68-87)."""
+        d = self.dmax
+        D = self.defs(d, {})
+        dom = D["dom"]
+        sm, _ = self.consts[d]
+        # m = r mh with mh = sin(theta) (-sin(phi), cos(phi), 0) bounded at infinity; the
+        # factor r is applied in coefficient space, as mult_r(a13.mult_sin_theta()) does
+        st = np.sin(dom.TH)
+        mh = [dom.conf_field(-st * np.sin(dom.PH)), dom.conf_field(st * np.cos(dom.PH))]
+        m_adm = -dom.integ(dom.der_normal(D["P"].val, OUTER_BC), OUTER_BC) / (2 * np.pi)
+        m_komar = dom.integ(dom.der_normal(D["N"].val, OUTER_BC), OUTER_BC) / (4 * np.pi)
+        so = None
+        for i in range(3):
+            for j in range(2):
+                t = vmul(vmul(D["A"][i][j].val, sm[i]), mh[j])
+                so = t if so is None else vadd(so, t)
+        j_adm = -dom.integ(dom.mult_r(so), OUTER_BC) / (8 * np.pi)
+        return dict(M_adm=float(m_adm), M_komar=float(m_komar), J_adm=float(j_adm))
+
     # ---- residual (System_of_eqs::sec_member, solver.cpp:49-85)
     def residual(self):
         cache_d = {}
@@ -504,6 +528,25 @@ def initial_fields(sys, rexc, X0=None):
         for name in XCTSSystem.FIELDS:
             for d in range(dmin, dmax + 1):
                 fields[name][d] = vadd(fields[name][d], corr[name][d])
+    return fields
+
+
+def analytic_fields(sy, m, J):
+    """P = 1 + m/(2r), N = 1 - m/r, bet = (2J/r^3)(-y, x, 0): polynomials in 1/r in the
+    compactified domain, so the surface integrals at infinity are exact: M_ADM = m,
+    M_Komar = m, and, for m = 0 (A^ij = L bet / 2N with N = 1 stays polynomial),
+    J_ADM = J."""
+    fields = {n: {} for n in XCTSSystem.FIELDS}
+    for d in range(sy.dmin, sy.dmax + 1):
+        dom = sy.sp.doms[d]
+        with np.errstate(divide="ignore"):
+            u = np.where(np.isfinite(dom.R), 1.0 / dom.R, 0.0)
+        st = np.sin(dom.TH)
+        fields["P"][d] = dom.conf_field(1.0 + 0.5 * m * u)
+        fields["N"][d] = dom.conf_field(1.0 - m * u)
+        fields["bx"][d] = dom.conf_field(-2.0 * J * u * u * st * np.sin(dom.PH))
+        fields["by"][d] = dom.conf_field(2.0 * J * u * u * st * np.cos(dom.PH))
+        fields["bz"][d] = dom.conf_field(0.0, anti=True)
     return fields
 
 

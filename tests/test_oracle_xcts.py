@@ -74,3 +74,22 @@ def test_oracle_newton_is_quadratic():
     tail = [e for e in errs if e < 1e-2]
     for a, b in zip(tail, tail[1:]):
         assert b <= 50.0 * a * a + 1e-13, errs
+
+
+def test_oracle_observables_closed_forms():
+    """Domain_compact::integ surface integrals (reader.cpp:114-125) on fields whose
+    ADM mass, Komar mass and angular momentum are known exactly."""
+    sy = _system()
+    sy.set_fields(OX.analytic_fields(sy, 0.7, 0.0))
+    o = sy.observables()
+    assert abs(o["M_adm"] - 0.7) < 1e-12, o
+    assert abs(o["M_komar"] - 0.7) < 1e-12, o
+    assert abs(o["J_adm"]) < 1e-12, o
+    sy.set_fields(OX.analytic_fields(sy, 0.0, 0.3))
+    o = sy.observables()
+    assert abs(o["J_adm"] - 0.3) < 1e-12, o
+    assert abs(o["M_adm"]) < 1e-12 and abs(o["M_komar"]) < 1e-12, o
+    # the flat guess P = 1 + a/r has M_ADM = 2a, N = 1 and bet = 0 carry nothing
+    sy.set_fields(OX.initial_fields(sy, 1.0))
+    o = sy.observables()
+    assert abs(o["M_adm"] - 2.0) < 1e-12 and abs(o["M_komar"]) < 1e-12 and abs(o["J_adm"]) < 1e-12, o
