@@ -173,6 +173,13 @@ int kad_xcts_set_coef(kad_system* s, int f, int d, const double* in);
  * surface integrals at infinity (codes/Par_version/Kerr/reader.cpp:114-125 with
  * Domain_compact::integ, domain_compact.cpp:89-121), HOST out[3] */
 int kad_xcts_observables(kad_system* s, double* out);
+/* set a parameter: which 0 = Ome, the horizon rotation of the inner shift condition */
+int kad_xcts_set_param(kad_system* s, int which, double value);
+/* adjoint sensitivities at a solution: out[q] = dQ_q/dOme for Q = (M_ADM, M_Komar,
+ * J_ADM), from J^T lambda_q = dQ_q/du (transposed solve with the LU factors of J,
+ * refilled at the current fields) and dQ_q/dOme = -lambda_q . dF/dOme.  HOST out[3];
+ * dQdu (HOST 3 x n, may be NULL) receives dQ/du. */
+int kad_xcts_sensitivity(kad_system* s, double* out, double* dQdu);
 /* One Newton iteration.  solver: 0 fp64 LU, 1 fp32 LU + fp64 IR, 2 fp32 LU +
  * GMRES-IR, 3 bf16-update LU + GMRES-IR.  Returns 1 if ||F||_inf < precision at
  * entry (no step), 0 after a step, KAD_ERR_SINGULAR on an exactly singular J.

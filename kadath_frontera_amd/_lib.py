@@ -55,6 +55,8 @@ SIGNATURES = {
     "kad_xcts_get_coef": (c_int, [c_vp, c_int, c_int, c_dp]),
     "kad_xcts_set_coef": (c_int, [c_vp, c_int, c_int, c_dp]),
     "kad_xcts_observables": (c_int, [c_vp, c_dp]),
+    "kad_xcts_set_param": (c_int, [c_vp, c_int, c_dbl]),
+    "kad_xcts_sensitivity": (c_int, [c_vp, c_dp, c_dp]),
     "kad_xcts_do_newton": (c_int, [c_vp, c_dbl, c_dp, c_int, c_dp]),
     "kad_xcts_last_solve": (c_int, [c_vp, c_ip, c_dp]),
     "kad_xcts_perturbation": (c_int, [c_vp, ctypes.c_uint64, c_dbl, c_dp]),

@@ -624,6 +624,19 @@ int Tracer::export_inside(const VDp& v, int order, int row0, double sign, int ac
   return export_rows(v, rows, I_EXPORT_IN, 1, row0, sign, accumulate);
 }
 
+void Tracer::export_integral(const VDp& v, int q, double fac) {
+  if (v->zero) return;
+  coef(v);
+  Instr in;
+  in.kind = I_EXPORT_INT;
+  in.dom = v->dom;
+  in.a = {v->cf, v->space};
+  in.row0 = q;
+  in.s = fac;
+  in.aux = v->base.bt[0];
+  prog(v->space).code.push_back(in);
+}
+
 int Tracer::export_boundary(const VDp& v, int bound, int row0, double sign, int accumulate, int* nrows_out) {
   if (v->zero) return -1;
   coef(v);

@@ -36,7 +36,7 @@ struct Node {
 typedef std::shared_ptr<Node> VDp;
 
 // Instruction kinds.
-enum InstrKind { I_LIN = 0, I_EW = 1, I_EXPORT_IN = 2, I_EXPORT_BD = 3, I_SEED = 4, I_ZERO = 5 };
+enum InstrKind { I_LIN = 0, I_EW = 1, I_EXPORT_IN = 2, I_EXPORT_BD = 3, I_SEED = 4, I_ZERO = 5, I_EXPORT_INT = 6 };
 enum EwKind { EW_ADD = 0, EW_SUB = 1, EW_MUL = 2, EW_DIV = 3, EW_SCAL = 4, EW_ADDS = 5, EW_COPY = 6, EW_SDIV = 7 };
 
 struct Opnd {
@@ -65,6 +65,8 @@ struct Instr {
   int bsign = 1;   // boundary: +1 (x = 1) or -1 (x = -1)
   // SEED
   int seed_tab = -1;
+  // EXPORT_INT: theta basis of phi coefficient 0 (row0 = quantity index, s = factor)
+  int aux = 0;
 };
 
 // A chain of 1-D passes with host matrices (uploaded once per plan).
@@ -143,6 +145,9 @@ class Tracer {
   // returns the number of rows; emits nothing for a zero node
   int export_inside(const VDp& v, int order, int row0, double sign, int accumulate, int* nrows_out);
   int export_boundary(const VDp& v, int bound, int row0, double sign, int accumulate, int* nrows_out);
+  // surface integral at the outer boundary of the coefficients of v (Domain_compact::integ
+  // weights, times fac) into quantity q of the column
+  void export_integral(const VDp& v, int q, double fac);
   int nrows_inside(const Base& b, int dom, int order) { return (int)D(dom).tau_rows_inside(b.bt, order).size(); }
   int nrows_boundary(const Base& b, int dom) { return (int)D(dom).tau_rows_boundary(b.bt).size(); }
 

@@ -175,6 +175,7 @@ std::vector<TE> XctsSystem::expr(const EqRec& e, int part, Defs& D) {
   unit_vectors(d, sm, mg, false);
   if (w == "bet") {
     TE ome = cached_const(d, "ome", std::vector<double>(dm.conf_size(), prm.ome), dm.std_base());
+    if (ome_seed && ome_seed->dom == d) ome.der = ome_seed;
     TE n0 = cached_const(d, "n0", std::vector<double>(dm.conf_size(), prm.n0), dm.std_base());
     TE nP2 = t_div(n0, t_pow(D.P, 2, d));
     std::vector<TE> out;
@@ -278,6 +279,7 @@ void XctsSystem::trace_tangent(int f, int d) {
     for (int u = 0; u < cnt; ++u) doms[d].seed(bt, u, &st.pos[(size_t)u * 3], &st.val[(size_t)u * 3], &st.cnt[u]);
     seeds.push_back(st);
   }
+  seed_tab_of[{f, d}] = tab;
   tr.begin_tangent(&tprog.back());
   auto zd = zero_ders();
   zd[d][f] = tr.seed_node(d, field_base(f, d), tab);
@@ -383,43 +385,114 @@ void XctsSystem::build() {
 // run after the residual program on the current fields.
 void XctsSystem::trace_observables() {
   const int d = prm.dmax;
-  Domain& dm = doms[d];
-  if (!dm.compact) throw std::runtime_error("observables need a compactified outer domain");
+  if (!doms[d].compact) throw std::runtime_error("observables need a compactified outer domain");
   Program saved = std::move(tr.val);
   tr.val = Program();
   auto zd = zero_ders();
   Defs D = defs(d, zd[d]);
   TE sm[3], mh[3];
   unit_vectors(d, sm, mh, true);
-  // A^ij s_i m_j = r A^ij s_i mh_j; the factor r is applied in coefficient space
-  // (mult_r of the compactified domain: div_xm1 / alpha, domain_compact_ope.cpp:173-178)
-  VDp j_int = nullptr;
-  for (int i = 0; i < 3; ++i)
-    for (int k = 0; k < 2; ++k) {
-      VDp t = tr.mul(tr.mul(D.A[i][k].val, sm[i].val), mh[k].val);
-      j_int = j_int ? tr.add(j_int, t) : t;
-    }
-  if (!j_int->zero) j_int = tr.scal(tr.div_xm1(j_int), 1.0 / dm.alpha);
-  const VDp so[3] = {tr.der_normal(D.P.val, OUTER_BC), tr.der_normal(D.N.val, OUTER_BC), j_int};
   const double fac[3] = {-1.0 / (2 * M_PI), 1.0 / (4 * M_PI), -1.0 / (8 * M_PI)};
   obs.clear();
   for (int q = 0; q < 3; ++q) {
     ObsRec r;
     r.fac = fac[q];
-    if (so[q]->zero) {
-      obs.push_back(r);
-      continue;
+    VDp rr = obs_integrand(q, D, sm, mh, false);
+    if (!rr->zero) {
+      tr.coef(rr);
+      r.slot = rr->cf;
+      r.bt0 = rr->base.bt[0];
     }
-    // r r sin(theta) so; mult_r of the compactified domain is div_xm1 / alpha (domain_compact_ope.cpp:173-178)
-    VDp rr = tr.mult_sin_theta(so[q]);
-    for (int t = 0; t < 2; ++t) rr = tr.scal(tr.div_xm1(rr), 1.0 / dm.alpha);
-    tr.coef(rr);
-    r.slot = rr->cf;
-    r.bt0 = rr->base.bt[0];
     obs.push_back(r);
   }
   obs_prog = std::move(tr.val);
   tr.val = std::move(saved);
+  trace_obs_tangents();
+  trace_ome_tangent();
+}
+
+// r r sin(theta) so for quantity q (0 M_ADM: d_r P, 1 M_Komar: d_r N, 2 J_ADM:
+// A^ij s_i m_j = r A^ij s_i mh_j) of the value (der = false) or tangent (der = true)
+// fields.  d_r of the compactified domain is der_normal there; its mult_r is div_xm1 /
+// alpha (domain_compact_ope.cpp:173-189).
+VDp XctsSystem::obs_integrand(int q, Defs& D, const TE sm[3], const TE mh[3], bool der) {
+  const int d = prm.dmax;
+  Domain& dm = doms[d];
+  VDp so;
+  if (q == 0) {
+    so = tr.der_normal(der ? D.P.der : D.P.val, OUTER_BC);
+  } else if (q == 1) {
+    so = tr.der_normal(der ? D.N.der : D.N.val, OUTER_BC);
+  } else {
+    so = nullptr;
+    for (int i = 0; i < 3; ++i)
+      for (int k = 0; k < 2; ++k) {
+        VDp t = tr.mul(tr.mul(der ? D.A[i][k].der : D.A[i][k].val, sm[i].val), mh[k].val);
+        so = so ? tr.add(so, t) : t;
+      }
+    if (!so->zero) so = tr.scal(tr.div_xm1(so), 1.0 / dm.alpha);
+  }
+  if (so->zero) return so;
+  VDp rr = tr.mult_sin_theta(so);
+  for (int t = 0; t < 2; ++t) rr = tr.scal(tr.div_xm1(rr), 1.0 / dm.alpha);
+  return rr;
+}
+
+// d(M_ADM, M_Komar, J_ADM)/d(unknowns) of the outer domain's fields (the quantities
+// read no other domain): the Jacobian's seeds, the observables' tangents, one
+// integral per quantity and column.
+void XctsSystem::trace_obs_tangents() {
+  const int d = prm.dmax;
+  obs_tprog.clear();
+  obs_tprog_lay.clear();
+  obs_tprog_tab.clear();
+  TE sm[3], mh[3];
+  unit_vectors(d, sm, mh, true);
+  const double fac[3] = {-1.0 / (2 * M_PI), 1.0 / (4 * M_PI), -1.0 / (8 * M_PI)};
+  for (auto& L : layout) {
+    if (L[1] != d) continue;
+    const int f = L[0];
+    obs_tprog.emplace_back();
+    obs_tprog_lay.push_back(L);
+    const int tab = seed_tab_of.at({f, d});
+    obs_tprog_tab.push_back(tab);
+    tr.begin_tangent(&obs_tprog.back());
+    auto zd = zero_ders();
+    zd[d][f] = tr.seed_node(d, field_base(f, d), tab);
+    Defs D = defs(d, zd[d]);
+    for (int q = 0; q < 3; ++q) tr.export_integral(obs_integrand(q, D, sm, mh, true), q, fac[q]);
+    tr.tan = nullptr;
+  }
+}
+
+// dF/dOme: Ome enters the inner boundary condition of the shift,
+// bet^i = n0 / P^2 s^i + a Ome m^i (kerr.cpp:91-95); its tangent is the constant 1,
+// a single coefficient (T_0 cos(0 theta) a_0) seeded by a one-entry table.
+void XctsSystem::trace_ome_tangent() {
+  const int d = prm.dmin;
+  SeedTab st;
+  st.pos = {0, 0, 0};
+  st.val = {1.0, 0.0, 0.0};
+  st.cnt = {1};
+  ome_tab = (int)seeds.size();
+  seeds.push_back(st);
+  ome_prog = Program();
+  tr.begin_tangent(&ome_prog);
+  ome_seed = tr.seed_node(d, doms[d].std_base(), ome_tab);
+  auto zd = zero_ders();
+  Defs D = defs(d, zd[d]);
+  for (auto& e : eqs) {
+    if (e.kind != 3 || e.d != d || e.which != "bet") continue;
+    std::vector<TE> p = expr(e, 0, D);
+    int row = e.row0;
+    for (int c = 0; c < e.ncomp; ++c) {
+      int nr = 0;
+      tr.export_boundary(p[c].der, INNER_BC, row, 1.0, 0, &nr);
+      row += e.nrows[c];
+    }
+  }
+  ome_seed = nullptr;
+  tr.tan = nullptr;
 }
 
 }  // namespace kadhost

@@ -78,9 +78,22 @@ class XctsSystem {
   };
   Program obs_prog;
   std::vector<ObsRec> obs;  // M_ADM, M_Komar, J_ADM
+  // d(observables)/d(unknowns): one tangent program per field of the outer domain,
+  // seeded like the Jacobian programs, exporting the 3 integrals per column
+  std::vector<Program> obs_tprog;
+  std::vector<std::array<int, 4>> obs_tprog_lay;  // field, domain, first unknown, count
+  std::vector<int> obs_tprog_tab;                 // seed table
+  // dF/dOme: tangent program of the horizon rotation (a constant field seeded with 1)
+  Program ome_prog;
+  int ome_tab = -1;
 
   explicit XctsSystem(const XctsParams& p);
   Base field_base(int f, int d) const { return f == 4 ? doms[d].anti_base() : doms[d].std_base(); }
+  // value slot of the configuration values of a cached constant field (-1 if absent)
+  int const_slot(const std::string& name, int d) const {
+    auto it = consts.find(name + ":" + std::to_string(d));
+    return it == consts.end() ? -1 : it->second.val->c;
+  }
   void build();
 
  private:
@@ -112,6 +125,11 @@ class XctsSystem {
   void trace_value();
   void reorder_rows();
   void trace_observables();
+  void trace_obs_tangents();
+  void trace_ome_tangent();
+  VDp obs_integrand(int q, Defs& D, const TE sm[3], const TE mh[3], bool der);
+  std::map<std::pair<int, int>, int> seed_tab_of;  // (field, domain) -> seed table
+  VDp ome_seed = nullptr;                          // tangent of Ome while tracing ome_prog
   void unit_vectors(int d, TE sm[3], TE mg[3], bool unit_m);
   void trace_tangent(int f, int d);
   std::vector<std::array<VDp, NF>> zero_ders();

@@ -159,6 +159,27 @@ __global__ void surface_integral_kernel(const double* __restrict__ cf, int nr, i
   }
 }
 
+// Per column b: fac * 2 pi * sum_j w_j sum_i cf_b(i, j, k = 0) (the integ weights of
+// surface_integral_kernel at the outer boundary) into out[b + q ldo].
+__global__ void export_int_kernel(const double* __restrict__ src, int64_t sstride, int nr, int nt, int bt0,
+                                  double fac, double* __restrict__ out, int q, int64_t col0, int64_t ldo,
+                                  int nbatch) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbatch) return;
+  const double* x = src + (int64_t)b * sstride;
+  double tot = 0.0;
+  if (bt0 == COS_EVEN || bt0 == SIN_ODD) {
+    for (int j = 0; j < nt; ++j) {
+      const double w = bt0 == COS_EVEN ? (j == 0 ? M_PI : 0.0) : 2.0 / (2.0 * j + 1.0);
+      if (w == 0.0) continue;
+      double v = 0.0;
+      for (int i = 0; i < nr; ++i) v += x[i + (int64_t)nr * j];
+      tot += w * v;
+    }
+  }
+  out[col0 + b + (int64_t)q * ldo] = tot * 2.0 * M_PI * fac;
+}
+
 struct DevPlanX {
   kad::Passes P;
   double* pool = nullptr;
@@ -194,6 +215,10 @@ struct kad_system_impl {
   std::vector<int*> csr_rowptr, csr_col;
   std::vector<double*> csr_w;
   std::vector<DevProgram> tprogs;
+  std::vector<DevProgram> obs_tprogs;  // d(observables)/d(unknowns)
+  DevProgram ome_prog;                 // dF/dOme
+  double* dQ = nullptr;                // 3 x n, row q contiguous
+  double* dFdp = nullptr;              // n
   int nphys_max = 0;
   // dense linear algebra buffers (lazily allocated)
   double* J = nullptr;
@@ -355,6 +380,23 @@ int kad_system_impl::setup() {
     nphys_max = std::max(nphys_max, dp.nphys);
     tprogs.push_back(std::move(dp));
   }
+  for (size_t p = 0; p < sys->obs_tprog.size(); ++p) {
+    DevProgram dp;
+    dp.code = sys->obs_tprog[p].code;
+    dp.nphys = allocate_slots(dp.code, sys->obs_tprog[p].nslots);
+    dp.field = sys->obs_tprog_lay[p][0];
+    dp.dom = sys->obs_tprog_lay[p][1];
+    dp.off = sys->obs_tprog_lay[p][2];
+    dp.cnt = sys->obs_tprog_lay[p][3];
+    dp.seed_tab = sys->obs_tprog_tab[p];
+    nphys_max = std::max(nphys_max, dp.nphys);
+    obs_tprogs.push_back(std::move(dp));
+  }
+  ome_prog.code = sys->ome_prog.code;
+  ome_prog.nphys = allocate_slots(ome_prog.code, sys->ome_prog.nslots);
+  ome_prog.seed_tab = sys->ome_tab;
+  ome_prog.cnt = 1;
+  nphys_max = std::max(nphys_max, ome_prog.nphys);
   if ((rc = dmalloc((void**)&tan_arena, (size_t)std::max(1, nphys_max) * B * stride * sizeof(double)))) return rc;
   if ((rc = dmalloc((void**)&F, (size_t)sys->n_rows * sizeof(double)))) return rc;
   if ((rc = dmalloc((void**)&scratch, 64 * sizeof(double)))) return rc;
@@ -395,6 +437,13 @@ int kad_system_impl::run(const std::vector<Instr>& code, int nb, int u0, int see
           export_bd_kernel<<<(unsigned)kad::cdiv(tot, 256), 256, 0, s>>>(
               row_tabs[in.rows], in.nrows, slot_ptr(in.a), slot_stride(in.a), dm.nr, in.bsign, out, in.row0, col0,
               ldo, in.sign, in.accumulate, batch);
+        break;
+      }
+      case I_EXPORT_INT: {
+        const int batch = (in.a.space == SP_VAL) ? 1 : nb;
+        export_int_kernel<<<(unsigned)kad::cdiv(batch, 64), 64, 0, s>>>(slot_ptr(in.a), slot_stride(in.a), dm.nr,
+                                                                         dm.nt, in.aux, in.s, out, in.row0, col0,
+                                                                         ldo, batch);
         break;
       }
       case I_SEED: {
@@ -673,6 +722,74 @@ int kad_xcts_observables(kad_system* s, double* out) {
     KAD_HIP_CHECK(hipGetLastError());
   }
   KAD_HIP_CHECK(hipMemcpyAsync(out, S->scratch, 3 * sizeof(double), hipMemcpyDeviceToHost, st));
+  KAD_HIP_CHECK(hipStreamSynchronize(st));
+  return KAD_OK;
+}
+
+// Set a parameter of the system: which 0 = Ome (horizon rotation, kerr.cpp:91-95).
+int kad_xcts_set_param(kad_system* s, int which, double value) {
+  if (!s || which != 0) return KAD_ERR_ARG;
+  auto* S = reinterpret_cast<kad_system_impl*>(s);
+  XctsSystem& sy = *S->sys;
+  for (int d = sy.prm.dmin; d <= sy.prm.dmax; ++d) {
+    const int slot = sy.const_slot("ome", d);
+    if (slot < 0) continue;
+    std::vector<double> h(sy.doms[d].conf_size(), value);
+    KAD_HIP_CHECK(hipMemcpyAsync(S->val_arena + (int64_t)slot * S->stride, h.data(), h.size() * sizeof(double),
+                                 hipMemcpyHostToDevice, S->ctx->stream));
+    KAD_HIP_CHECK(hipStreamSynchronize(S->ctx->stream));
+  }
+  sy.prm.ome = value;
+  return KAD_OK;
+}
+
+// Sensitivities of the global quantities Q = (M_ADM, M_Komar, J_ADM) at a solution
+// F(u, Ome) = 0 with respect to Ome, by the adjoint method:
+//   J^T lambda_q = dQ_q/du   (the transposed solve reuses the LU factors of J),
+//   dQ_q/dOme = dQ_q/dOme|_u - lambda_q . dF/dOme   (the first term is 0 here).
+// dQ/du comes from tangent programs of the observables, dF/dOme from the tangent
+// program of Ome; J is refilled and factored at the current fields.  HOST out[3];
+// dQdu (HOST, 3 x n, may be NULL) receives dQ/du.
+int kad_xcts_sensitivity(kad_system* s, double* out, double* dQdu) {
+  if (!s || !out) return KAD_ERR_ARG;
+  auto* S = reinterpret_cast<kad_system_impl*>(s);
+  XctsSystem& sy = *S->sys;
+  hipStream_t st = S->ctx->stream;
+  const int n = sy.n_unknowns;
+  int rc;
+  if (!S->J) {
+    if ((rc = S->dmalloc((void**)&S->J, (size_t)n * n * sizeof(double)))) return rc;
+    if ((rc = S->dmalloc((void**)&S->ipiv, (size_t)n * sizeof(int)))) return rc;
+    if ((rc = S->dmalloc((void**)&S->perm, (size_t)n * sizeof(int)))) return rc;
+    if ((rc = S->dmalloc((void**)&S->X, (size_t)n * sizeof(double)))) return rc;
+  }
+  if (!S->dQ) {
+    if ((rc = S->dmalloc((void**)&S->dQ, (size_t)3 * n * sizeof(double)))) return rc;
+    if ((rc = S->dmalloc((void**)&S->dFdp, (size_t)n * sizeof(double)))) return rc;
+  }
+  if ((rc = S->jacobian(S->J, n))) return rc;  // includes the value pass at the current fields
+  if ((rc = S->run(sy.obs_prog.code, 1, 0, -1, S->F, 0, 0))) return rc;
+  KAD_HIP_CHECK(hipMemsetAsync(S->dQ, 0, (size_t)3 * n * sizeof(double), st));
+  KAD_HIP_CHECK(hipMemsetAsync(S->dFdp, 0, (size_t)n * sizeof(double), st));
+  for (auto& p : S->obs_tprogs)
+    for (int u0 = 0; u0 < p.cnt; u0 += S->B) {
+      const int nb = std::min(S->B, p.cnt - u0);
+      if ((rc = S->run(p.code, nb, u0, p.seed_tab, S->dQ, (int64_t)p.off + u0, n))) return rc;
+    }
+  if ((rc = S->run(S->ome_prog.code, 1, 0, S->ome_prog.seed_tab, S->dFdp, 0, n))) return rc;
+  if (dQdu) {
+    KAD_HIP_CHECK(hipMemcpyAsync(dQdu, S->dQ, (size_t)3 * n * sizeof(double), hipMemcpyDeviceToHost, st));
+  }
+  rc = kad_lu_factor(reinterpret_cast<kad_ctx*>(S->ctx), KAD_PREC_F64, n, S->J, n, S->ipiv, S->perm, 0);
+  if (rc < 0) return rc;
+  if (rc > 0) return KAD_ERR_SINGULAR;
+  for (int q = 0; q < 3; ++q) {
+    KAD_HIP_CHECK(hipMemcpyAsync(S->X, S->dQ + (size_t)q * n, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice,
+                                 st));
+    if ((rc = kad_lu_solve(reinterpret_cast<kad_ctx*>(S->ctx), KAD_PREC_F64, n, S->J, n, S->perm, S->X, 1)))
+      return rc;
+    out[q] = -S->ctx->dot(S->X, S->dFdp, n);
+  }
   KAD_HIP_CHECK(hipStreamSynchronize(st));
   return KAD_OK;
 }

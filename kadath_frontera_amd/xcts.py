@@ -107,6 +107,30 @@ class XctsSystem:
         check(lib.kad_xcts_observables(self.h, _dp(out)), "kad_xcts_observables")
         return dict(M_adm=out[0], M_komar=out[1], J_adm=out[2])
 
+    def set_ome(self, value):
+        """Horizon rotation Ome of the inner shift condition."""
+        check(lib.kad_xcts_set_param(self.h, 0, float(value)), "kad_xcts_set_param")
+
+    def sensitivity(self, with_dqdu=False):
+        """Adjoint dQ/dOme of Q = (M_ADM, M_Komar, J_ADM) at the current (converged) fields:
+        J^T lambda = dQ/du with the LU factors of J, dQ/dOme = -lambda . dF/dOme."""
+        out = np.zeros(3)
+        dq = np.zeros((3, self.n)) if with_dqdu else None
+        check(lib.kad_xcts_sensitivity(self.h, _dp(out), _dp(dq) if dq is not None else None),
+              "kad_xcts_sensitivity")
+        res = dict(M_adm=out[0], M_komar=out[1], J_adm=out[2])
+        return (res, dq) if with_dqdu else res
+
+    def solve(self, precision=1e-10, solver="f64", guess=None, max_iter=20):
+        """Newton iterations (kerr.cpp:102-112) until ||F||_inf < precision; returns the
+        number of iterations and the final error."""
+        self.set_guess(guess)
+        for it in range(max_iter + 1):
+            conv, err, _, _ = self.do_newton(precision, solver)
+            if conv:
+                return it, err
+        raise RuntimeError(f"Newton did not reach {precision} in {max_iter} iterations (|F| = {err:.3e})")
+
     def coef(self, f, d):
         nr, nt, npt = self.res
         out = np.zeros((npt + 2) * nt * nr)
