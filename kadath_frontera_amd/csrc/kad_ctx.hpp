@@ -44,6 +44,17 @@ struct LuWork {
   int* tile_list = nullptr;     // compacted active row tiles, column tiles, and the 2 counts
   int* swap_map = nullptr;      // row gathers of the deferred left interchanges
   size_t swap_cap = 0;          // ints
+  int* ext = nullptr;           // last nonzero row per (128-column tile, 2048-row chunk)
+  size_t cap_ext = 0;
+  int* mask_c2 = nullptr;       // column tiles of U12 beyond the next panel
+  int* mask_r2 = nullptr;       // second row-tile mask (alternate steps)
+  // look-ahead: the next panel is updated and factored on a second stream
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_panel[2] = {nullptr, nullptr}, ev_next[2] = {nullptr, nullptr}, ev_start = nullptr;
+  // triangular-inverse and GEMM scratch of the main stream (the panel stream uses tinv / tmp)
+  void* tinv_m = nullptr;
+  void* tmp_m = nullptr;
+  size_t tmp_m_bytes = 0;
 };
 
 // Optional live timing of one kernel class (HIP events on the launch stream) and
@@ -73,7 +84,7 @@ struct Ctx {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   LuWork lu;
-  size_t cap_redv = 0, cap_redi = 0, cap_tinv = 0, cap_map = 0, cap_mask = 0;
+  size_t cap_redv = 0, cap_redi = 0, cap_tinv = 0, cap_map = 0, cap_mask = 0, cap_tinv_m = 0;
   double* gemv_part = nullptr;
   size_t cap_part = 0;
   double* vwork = nullptr;

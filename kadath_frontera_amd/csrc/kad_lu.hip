@@ -172,6 +172,27 @@ __global__ __launch_bounds__(256) void row_tile_mask_kernel(const T* __restrict_
   any = __syncthreads_or(any);
   if (threadIdx.x == 0) mask[blockIdx.x] = any;
 }
+// Last nonzero row (or -1) of each 128-column tile within each 2048-row chunk:
+// out[chunk * ntc + tile].  One workgroup per (tile, chunk); rows coalesced.
+template <typename T>
+__global__ __launch_bounds__(256) void col_extent_kernel(const T* __restrict__ A, int64_t lda, int n,
+                                                         int* __restrict__ out) {
+  __shared__ int red[256];
+  const int c0 = blockIdx.x * 128, r0 = blockIdx.y * 2048;
+  const int ce = min(c0 + 128, n), re = min(r0 + 2048, n);
+  int last = -1;
+  for (int c = c0; c < ce; ++c)
+    for (int r = r0 + threadIdx.x; r < re; r += 256)
+      if (A[r + (int64_t)c * lda] != T(0)) last = max(last, r);
+  red[threadIdx.x] = last;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + st]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+}
+
 // Compacted lists of the active row / column tiles: list[0..) rows, list[stride..)
 // columns, list[2 stride] and list[2 stride + 1] their counts.  One thread (a few
 // hundred tiles).
@@ -219,12 +240,15 @@ int trsm_left_unit_lower(const T* L, int64_t lda, int r, T* A12, int64_t ld12, i
     return trsm_left_unit_lower<T>(L + h + (int64_t)h * lda, lda, r - h, A12 + h, ld12, ncols, w, gemm_prec, cmask);
   }
   T* linv = (T*)w.tinv;
+  // Column chunks so that the scratch holds r x chunk; with more than one chunk they
+  // are multiples of 128 (the mask granularity), else the masks are not used.
+  const int64_t cap = w.tmp ? (int64_t)(w.tmp_bytes / sizeof(T)) / r : 0;
+  if (!linv || cap < 1) return KAD_ERR_ALLOC;
+  int chunk = (int)std::min<int64_t>(cap, ncols);
+  if (chunk < ncols && chunk >= 128) chunk = chunk / 128 * 128;
+  if (chunk < ncols && chunk % 128 != 0) cmask = nullptr;
   int rc = trtri_rec<T>(L, lda, r, linv, r, w);
   if (rc) return rc;
-  // Column chunks (multiples of 128, the mask granularity) so that the scratch holds r x chunk.
-  const int64_t cap = (int64_t)(w.tmp_bytes / sizeof(T)) / r;
-  int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(cap, ncols));
-  if (chunk < ncols) chunk = std::max(128, chunk / 128 * 128);
   for (int c = 0; c < ncols; c += chunk) {
     const int nc = std::min(chunk, ncols - c);
     T* tmp = (T*)w.tmp;
@@ -238,10 +262,82 @@ int trsm_left_unit_lower(const T* L, int64_t lda, int r, T* A12, int64_t ld12, i
   return KAD_OK;
 }
 
+// Row interchanges of pivots [k1, k2) (at most 512) applied to COLS columns per
+// workgroup through LDS.  Slots 0 .. nsw-1 hold rows k1 .. k2-1; slot nsw + j holds the
+// pivot row of step j when that row lies below k2 and step j is its first occurrence.
+// All slots are loaded in parallel, the interchange sequence is replayed in LDS (one
+// thread per column, LAPACK order), and every touched row is written back: the
+// result equals the sequential interchanges.
+template <typename T, int NSW, int COLS>
+__global__ __launch_bounds__(256) void laswp_lds_kernel(T* __restrict__ A, int64_t lda, int c0, int c1,
+                                                        const int* __restrict__ ipiv, int k1, int k2) {
+  __shared__ double vals[2 * NSW * COLS];  // [slot][column]
+  __shared__ int slot_of[NSW];
+  __shared__ int prow[NSW];                // row of slot nsw + j, or -1
+  const int nsw = k2 - k1;                 // <= NSW (checked by the launcher)
+  const int t = threadIdx.x;
+  const int cb = c0 + blockIdx.x * COLS;
+  const int nc = min(COLS, c1 - cb);
+  for (int j = t; j < nsw; j += 256) {
+    const int p = ipiv[k1 + j];
+    if (p < k2) {
+      slot_of[j] = p - k1;
+      prow[j] = -1;
+    } else {
+      int first = j;
+      for (int q = 0; q < j; ++q)
+        if (ipiv[k1 + q] == p) {
+          first = q;
+          break;
+        }
+      slot_of[j] = nsw + first;
+      prow[j] = (first == j) ? p : -1;
+    }
+  }
+  __syncthreads();
+  // load: rows k1 .. k2-1 (coalesced along rows) and the first-occurrence pivot rows
+  for (int e = t; e < nsw * COLS; e += 256) {
+    const int r = e % nsw, c = e / nsw;
+    if (c < nc) vals[r * COLS + c] = (double)A[(k1 + r) + (int64_t)(cb + c) * lda];
+  }
+  for (int e = t; e < nsw * COLS; e += 256) {
+    const int j = e % nsw, c = e / nsw;
+    const int p = prow[j];
+    if (c < nc && p >= 0) vals[(nsw + j) * COLS + c] = (double)A[p + (int64_t)(cb + c) * lda];
+  }
+  __syncthreads();
+  if (t < nc)
+    for (int j = 0; j < nsw; ++j) {
+      const int b = slot_of[j];
+      if (b != j) {
+        const double x = vals[j * COLS + t];
+        vals[j * COLS + t] = vals[b * COLS + t];
+        vals[b * COLS + t] = x;
+      }
+    }
+  __syncthreads();
+  for (int e = t; e < nsw * COLS; e += 256) {
+    const int r = e % nsw, c = e / nsw;
+    if (c < nc) A[(k1 + r) + (int64_t)(cb + c) * lda] = (T)vals[r * COLS + c];
+  }
+  for (int e = t; e < nsw * COLS; e += 256) {
+    const int j = e % nsw, c = e / nsw;
+    const int p = prow[j];
+    if (c < nc && p >= 0) A[p + (int64_t)(cb + c) * lda] = (T)vals[(nsw + j) * COLS + c];
+  }
+}
+
 template <typename T>
 int laswp(T* A, int64_t lda, int c0, int c1, const int* ipiv, int k1, int k2, LuWork& w) {
   if (c1 <= c0 || k2 <= k1) return KAD_OK;
-  laswp_kernel<T><<<(unsigned)cdiv(c1 - c0, 256), 256, 0, w.stream>>>(A, lda, c0, c1, ipiv, k1, k2);
+  const int nsw = k2 - k1;
+  if (nsw <= 256) {
+    laswp_lds_kernel<T, 256, 32><<<(unsigned)cdiv(c1 - c0, 32), 256, 0, w.stream>>>(A, lda, c0, c1, ipiv, k1, k2);
+  } else if (nsw <= 512) {
+    laswp_lds_kernel<T, 512, 16><<<(unsigned)cdiv(c1 - c0, 16), 256, 0, w.stream>>>(A, lda, c0, c1, ipiv, k1, k2);
+  } else {
+    laswp_kernel<T><<<(unsigned)cdiv(c1 - c0, 256), 256, 0, w.stream>>>(A, lda, c0, c1, ipiv, k1, k2);
+  }
   KAD_HIP_CHECK(hipGetLastError());
   return KAD_OK;
 }
@@ -498,52 +594,123 @@ int apply_left_swaps(T* A, int64_t lda, int n, const int* ipiv, int nb, LuWork& 
 
 // Full factorisation.  gemm_prec selects the trailing-update MFMA precision
 // (F64 for double storage; F32 or BF16 for float storage).
+//
+// Look-ahead of depth one on two streams of the same priority.  For step p (panel
+// [k0, k1), next panel [k1, k2), the one after [k2, k3)):
+//   panel stream: interchanges, U12 and update of the next panel's columns
+//     [k1, k2), then the next panel's factorisation and its L21 row-tile mask;
+//   main stream: interchanges and U12 of columns [k2, n), the update of [k2, k3)
+//     (signalled: the panel stream needs it at step p+1), then the update of [k3, n).
+// Events (ping-pong by step parity so a new record never overtakes a pending
+// wait): ev_panel "panel p and its mask are done", ev_next "columns of the panel
+// after next are up to date".  The two streams work on disjoint columns, each with
+// its own triangular-inverse / GEMM scratch; the L21 row-tile mask alternates
+// between two buffers.
 template <typename T>
 int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w) {
-  std::vector<int> hmask;
-  for (int k0 = 0; k0 < n; k0 += nb) {
-    const int k1 = std::min(k0 + nb, n);
-    // rows below which the panel's columns are exactly zero: the panel (pivot
-    // search, interchanges, its internal updates) runs on rows [k0, R) only
-    const int ntile = (int)cdiv(n - k0, 128);
-    row_tile_mask_kernel<T><<<(unsigned)ntile, 256, 0, w.stream>>>(A + k0 + (int64_t)k0 * lda, lda, n - k0, k1 - k0,
-                                                                   w.mask_r);
-    KAD_HIP_CHECK(hipGetLastError());
-    hmask.resize(ntile);
-    KAD_HIP_CHECK(hipMemcpyAsync(hmask.data(), w.mask_r, ntile * sizeof(int), hipMemcpyDeviceToHost, w.stream));
-    KAD_HIP_CHECK(hipStreamSynchronize(w.stream));
-    int last = -1;
-    for (int t = 0; t < ntile; ++t)
-      if (hmask[t]) last = t;
-    const int R = std::max(k1, std::min(n, k0 + (last + 1) * 128));
-    int rc = panel_rec<T>(A, lda, R, k0, k1, ipiv, w);
+  // both streams' scratch must exist before anything is launched
+  if (!w.stream2 || !w.tinv || !w.tinv_m || !w.tmp || !w.tmp_m || w.tmp_m_bytes < w.tmp_bytes || !w.tmp2 ||
+      !w.mask_r || !w.mask_r2 || !w.ext)
+    return KAD_ERR_ALLOC;
+  LuWork wp = w;  // panel stream: tinv, tmp, tmp2 and the panel reductions
+  wp.stream = w.stream2;
+  LuWork wm = w;  // main stream: its own triangular inverse and GEMM scratch
+  wm.tinv = w.tinv_m;
+  wm.tmp = w.tmp_m;
+  wm.tmp_bytes = w.tmp_m_bytes;
+  int* mask_rows[2] = {w.mask_r, w.mask_r2};
+  const int stride = (int)cdiv(n, 128) + 1;
+  // Row extent of every 128-column tile of the input (one pass, one readback).
+  // A panel's rows at or below R are zero in its columns and stay zero: the
+  // interchanges and the update of an earlier panel q touch rows above R_q only,
+  // so R_p = max(initial extent of p's columns, R_q for q < p) bounds them.
+  const int ntc = (int)cdiv(n, 128), nrc = (int)cdiv(n, 2048);
+  col_extent_kernel<T><<<dim3(ntc, nrc), 256, 0, w.stream>>>(A, lda, n, w.ext);
+  KAD_HIP_CHECK(hipGetLastError());
+  std::vector<int> hext((size_t)ntc * nrc), ext(ntc, -1);
+  KAD_HIP_CHECK(hipMemcpyAsync(hext.data(), w.ext, hext.size() * sizeof(int), hipMemcpyDeviceToHost, w.stream));
+  KAD_HIP_CHECK(hipStreamSynchronize(w.stream));
+  for (int r = 0; r < nrc; ++r)
+    for (int t = 0; t < ntc; ++t) ext[t] = std::max(ext[t], hext[(size_t)r * ntc + t]);
+  int r_prev = 0;
+  // factor panel [k0, k1) on the panel stream (rows [k0, R)), then its L21 mask
+  auto panel = [&](int k0, int k1, int* mask) -> int {
+    int R = std::max(k1, r_prev);
+    for (int t = k0 / 128; t <= (k1 - 1) / 128; ++t) R = std::max(R, ext[t] + 1);
+    R = std::min(R, n);
+    r_prev = R;
+    int rc = panel_rec<T>(A, lda, R, k0, k1, ipiv, wp);
     if (rc) return rc;
     if (k1 < n) {
-      rc = laswp<T>(A, lda, k1, n, ipiv, k0, k1, w);
-      if (rc) return rc;
-      // structural zeros of L21 (row tiles) and A12 (column tiles; U12 = L11^{-1} A12
-      // keeps them): the tile pairs they touch are skipped exactly
-      row_tile_mask_kernel<T><<<(unsigned)cdiv(n - k1, 128), 256, 0, w.stream>>>(A + k1 + (int64_t)k0 * lda, lda,
-                                                                                n - k1, k1 - k0, w.mask_r);
-      col_tile_mask_kernel<T><<<(unsigned)cdiv(n - k1, 128), 256, 0, w.stream>>>(A + k0 + (int64_t)k1 * lda, lda,
-                                                                                k1 - k0, n - k1, w.mask_c);
+      row_tile_mask_kernel<T><<<(unsigned)cdiv(n - k1, 128), 256, 0, wp.stream>>>(A + k1 + (int64_t)k0 * lda, lda,
+                                                                                 n - k1, k1 - k0, mask);
       KAD_HIP_CHECK(hipGetLastError());
-      rc = trsm_left_unit_lower<T>(A + k0 + (int64_t)k0 * lda, lda, k1 - k0, A + k0 + (int64_t)k1 * lda, lda, n - k1,
-                                   w, gemm_prec, w.mask_c);
+    }
+    return KAD_OK;
+  };
+  // the caller's work on the main stream precedes the first panel
+  KAD_HIP_CHECK(hipEventRecord(w.ev_start, w.stream));
+  KAD_HIP_CHECK(hipStreamWaitEvent(wp.stream, w.ev_start, 0));
+  int rc = panel(0, std::min(nb, n), mask_rows[0]);
+  if (rc) return rc;
+  KAD_HIP_CHECK(hipEventRecord(w.ev_panel[0], wp.stream));
+  for (int p = 0, k0 = 0; k0 < n; ++p, k0 += nb) {
+    const int k1 = std::min(k0 + nb, n);
+    const int k2 = std::min(k1 + nb, n);
+    const int k3 = std::min(k2 + nb, n);
+    const int par = p & 1;
+    const int* mrows = mask_rows[par];
+    const T* L21 = A + k1 + (int64_t)k0 * lda;
+    // ---- main stream: columns [k2, n)
+    KAD_HIP_CHECK(hipStreamWaitEvent(w.stream, w.ev_panel[par], 0));
+    if (k1 >= n) break;
+    if (k2 < n) {
+      rc = laswp<T>(A, lda, k2, n, ipiv, k0, k1, wm);
       if (rc) return rc;
-      // active tile pairs compacted over the grid, so they spread over every XCD
-      const int stride = (int)cdiv(n, 128) + 1;
-      tile_lists_kernel<<<1, 64, 0, w.stream>>>(w.mask_r, (int)cdiv(n - k1, 128), w.mask_c, (int)cdiv(n - k1, 128),
-                                                w.tile_list, stride);
+      // structural zeros of A12 (column tiles; U12 = L11^{-1} A12 keeps them)
+      col_tile_mask_kernel<T><<<(unsigned)cdiv(n - k2, 128), 256, 0, w.stream>>>(A + k0 + (int64_t)k2 * lda, lda,
+                                                                                k1 - k0, n - k2, w.mask_c);
       KAD_HIP_CHECK(hipGetLastError());
+      rc = trsm_left_unit_lower<T>(A + k0 + (int64_t)k0 * lda, lda, k1 - k0, A + k0 + (int64_t)k2 * lda, lda, n - k2,
+                                   wm, gemm_prec, w.mask_c);
+      if (rc) return rc;
+      // active tile pairs of the update of [k3, n), compacted on the grid (all XCDs
+      // busy); built before ev_next, whose waiter rewrites this step's row mask
+      if (k3 < n) {
+        col_tile_mask_kernel<T><<<(unsigned)cdiv(n - k3, 128), 256, 0, w.stream>>>(A + k0 + (int64_t)k3 * lda, lda,
+                                                                                  k1 - k0, n - k3, w.mask_c2);
+        tile_lists_kernel<<<1, 64, 0, w.stream>>>(mrows, (int)cdiv(n - k1, 128), w.mask_c2, (int)cdiv(n - k3, 128),
+                                                  w.tile_list, stride);
+        KAD_HIP_CHECK(hipGetLastError());
+      }
+      // the panel after next first, so the panel stream can go on
+      rc = gemm_launch_masked(gemm_prec, GOP_SUB, n - k1, k3 - k2, k1 - k0, L21, lda, A + k0 + (int64_t)k2 * lda, lda,
+                              A + k1 + (int64_t)k2 * lda, lda, mrows, w.mask_c, w.stream);
+      if (rc) return rc;
+    }
+    KAD_HIP_CHECK(hipEventRecord(w.ev_next[par], w.stream));
+    if (k3 < n) {
       TileSel sel;
       sel.rlist = w.tile_list;
       sel.clist = w.tile_list + stride;
       sel.cnt = w.tile_list + 2 * stride;
-      rc = gemm_launch_sel(gemm_prec, GOP_SUB, n - k1, n - k1, k1 - k0, A + k1 + (int64_t)k0 * lda, lda,
-                           A + k0 + (int64_t)k1 * lda, lda, A + k1 + (int64_t)k1 * lda, lda, sel, w.stream);
+      rc = gemm_launch_sel(gemm_prec, GOP_SUB, n - k1, n - k3, k1 - k0, L21, lda, A + k0 + (int64_t)k3 * lda, lda,
+                           A + k1 + (int64_t)k3 * lda, lda, sel, w.stream);
       if (rc) return rc;
     }
+    // ---- panel stream: the next panel's columns [k1, k2), then the next panel
+    if (p > 0) KAD_HIP_CHECK(hipStreamWaitEvent(wp.stream, w.ev_next[par ^ 1], 0));
+    rc = laswp<T>(A, lda, k1, k2, ipiv, k0, k1, wp);
+    if (rc) return rc;
+    rc = trsm_left_unit_lower<T>(A + k0 + (int64_t)k0 * lda, lda, k1 - k0, A + k0 + (int64_t)k1 * lda, lda, k2 - k1,
+                                 wp, gemm_prec);
+    if (rc) return rc;
+    rc = gemm_launch_masked(gemm_prec, GOP_SUB, n - k1, k2 - k1, k1 - k0, L21, lda, A + k0 + (int64_t)k1 * lda, lda,
+                            A + k1 + (int64_t)k1 * lda, lda, mrows, nullptr, wp.stream);
+    if (rc) return rc;
+    rc = panel(k1, k2, mask_rows[par ^ 1]);
+    if (rc) return rc;
+    KAD_HIP_CHECK(hipEventRecord(w.ev_panel[par ^ 1], wp.stream));
   }
   return apply_left_swaps<T>(A, lda, n, ipiv, nb, w);
 }

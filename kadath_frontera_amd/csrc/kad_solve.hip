@@ -81,9 +81,21 @@ int Ctx::lu_work(int n, int nb) {
   if ((rc = ensure((void**)&lu.red_idx, &cap_redi, 2 * sizeof(int) * red))) return rc;
   if ((rc = ensure((void**)&lu.map, &cap_map, 2 * sizeof(int) * (size_t)n))) return rc;
   const size_t nt = (size_t)cdiv(n, 128) + 1;
-  if ((rc = ensure((void**)&lu.mask_r, &cap_mask, sizeof(int) * (4 * nt + 2)))) return rc;
+  if ((rc = ensure((void**)&lu.mask_r, &cap_mask, sizeof(int) * (7 * nt + 2)))) return rc;
   lu.mask_c = lu.mask_r + nt;
-  lu.tile_list = lu.mask_r + 2 * nt;
+  lu.tile_list = lu.mask_r + 2 * nt;  // 2 nt + 2
+  lu.mask_c2 = lu.mask_r + 4 * nt + 2;
+  lu.mask_r2 = lu.mask_r + 5 * nt + 2;
+  if ((rc = ensure((void**)&lu.ext, &lu.cap_ext, sizeof(int) * (size_t)cdiv(n, 128) * cdiv(n, 2048)))) return rc;
+  if (!lu.stream2) {
+    // same (default) priority as the main stream
+    KAD_HIP_CHECK(hipStreamCreateWithFlags(&lu.stream2, hipStreamNonBlocking));
+    for (int q = 0; q < 2; ++q) {
+      KAD_HIP_CHECK(hipEventCreateWithFlags(&lu.ev_panel[q], hipEventDisableTiming));
+      KAD_HIP_CHECK(hipEventCreateWithFlags(&lu.ev_next[q], hipEventDisableTiming));
+    }
+    KAD_HIP_CHECK(hipEventCreateWithFlags(&lu.ev_start, hipEventDisableTiming));
+  }
   lu.red_cap = red;
   if (!lu.counter) {
     KAD_HIP_CHECK(hipMalloc((void**)&lu.counter, 64));
@@ -94,6 +106,9 @@ int Ctx::lu_work(int n, int nb) {
   // GEMM scratch for L11^{-1} A12: nb x chunk (bounded at 64 MiB).
   const size_t want = std::min<size_t>((size_t)nb * (size_t)n * 8, (size_t)64 << 20);
   if ((rc = ensure(&lu.tmp, &lu.tmp_bytes, std::max<size_t>(want, (size_t)nb * nb * 8)))) return rc;
+  // the main stream's own copies (sized after tmp: the look-ahead runs both at once)
+  if ((rc = ensure(&lu.tinv_m, &cap_tinv_m, (size_t)nb * nb * 8))) return rc;
+  if ((rc = ensure(&lu.tmp_m, &lu.tmp_m_bytes, lu.tmp_bytes))) return rc;
   if ((rc = ensure(&lu.tmp2, &lu.tmp2_bytes, std::max<size_t>((size_t)nb * nb * 8, (size_t)n * 32 * 8)))) return rc;
   lu.stream = stream;
   return KAD_OK;
@@ -155,9 +170,17 @@ Ctx::~Ctx() {
   for (auto& kv : plans) delete kv.second;
   for (void* p : owned) (void)hipFree(p);
   void* bufs[] = {lu.red_val, lu.red_idx, lu.counter, lu.tinv,   lu.tmp,  lu.tmp2,
-                  lu.map,     lu.mask_r,  lu.swap_map, gemv_part, vwork,  krylov, trs_tab};
+                  lu.map,     lu.mask_r,  lu.swap_map, lu.ext,  gemv_part, vwork,  krylov, trs_tab};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
+  for (int q = 0; q < 2; ++q) {
+    if (lu.ev_panel[q]) (void)hipEventDestroy(lu.ev_panel[q]);
+    if (lu.ev_next[q]) (void)hipEventDestroy(lu.ev_next[q]);
+  }
+  if (lu.ev_start) (void)hipEventDestroy(lu.ev_start);
+  if (lu.tinv_m) (void)hipFree(lu.tinv_m);
+  if (lu.tmp_m) (void)hipFree(lu.tmp_m);
+  if (lu.stream2) (void)hipStreamDestroy(lu.stream2);
   if (own_stream && stream) (void)hipStreamDestroy(stream);
 }
 
