@@ -18,6 +18,53 @@
 
 namespace kad {
 
+// One pass along axis 0 (r) or 2 (phi) with NIN input points per line: each thread
+// owns one output index o and a strided set of lines; its matrix row stays in
+// registers while consecutive lines use the same matrix (the usual case along r and
+// phi), and the threads of one line read the same inputs (LDS broadcast).
+template <int NIN>
+__device__ __forceinline__ void pass_rows(const double* __restrict__ x, double* __restrict__ y,
+                                          const double* __restrict__ mats, const int* __restrict__ lm, int ax,
+                                          const int dims[3], const int od[3], int nout_ax) {
+  const int nout = od[0] * od[1] * od[2];
+  const int o = threadIdx.x % nout_ax;
+  const int per_sweep = blockDim.x / nout_ax;
+  const int l0 = threadIdx.x / nout_ax;
+  const int nline = nout / nout_ax;
+  const int stride = ax == 0 ? 1 : dims[0] * dims[1];
+  int cur_off = -2;
+  double m[NIN];
+  if (l0 >= per_sweep) return;
+  for (int li = l0; li < nline; li += per_sweep) {
+    int line, base_idx, e;
+    if (ax == 0) {
+      line = li;
+      base_idx = dims[0] * li;
+      e = o + od[0] * li;
+    } else {
+      const int i = li % od[0], j = li / od[0];
+      line = 0;
+      base_idx = i + dims[0] * j;
+      e = i + od[0] * (j + od[1] * o);
+    }
+    const int off = lm[line];
+    if (off != cur_off) {
+      cur_off = off;
+#pragma unroll
+      for (int q = 0; q < NIN; ++q) m[q] = off >= 0 ? mats[off + o * NIN + q] : 0.0;
+    }
+    const double* xv = x + base_idx;
+    double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+    for (int q = 0; q + 1 < NIN; q += 2) {
+      s0 += m[q] * xv[q * stride];
+      s1 += m[q + 1] * xv[(q + 1) * stride];
+    }
+    if (NIN % 2) s0 += m[NIN - 1] * xv[(NIN - 1) * stride];
+    y[e] = s0 + s1;
+  }
+}
+
 // One workgroup per batch element.  Dynamic LDS: two ping-pong buffers, then the
 // plan's matrix pool and per-line matrix tables (a few KB, staged once), so the
 // contraction of every output element reads only LDS.
@@ -56,6 +103,23 @@ __global__ __launch_bounds__(256) void apply3d_kernel(const double* __restrict__
     double* y = buf[cur ^ 1];
     const int* lm = ltab + loff;
     const int stride = ax == 0 ? 1 : (ax == 1 ? dims[0] : dims[0] * dims[1]);
+    if (ax != 1 && nin_ax >= 12 && nin_ax <= 17) {
+      switch (nin_ax) {
+        case 12: pass_rows<12>(x, y, mats, lm, ax, dims, od, nout_ax); break;
+        case 13: pass_rows<13>(x, y, mats, lm, ax, dims, od, nout_ax); break;
+        case 14: pass_rows<14>(x, y, mats, lm, ax, dims, od, nout_ax); break;
+        case 15: pass_rows<15>(x, y, mats, lm, ax, dims, od, nout_ax); break;
+        case 16: pass_rows<16>(x, y, mats, lm, ax, dims, od, nout_ax); break;
+        default: pass_rows<17>(x, y, mats, lm, ax, dims, od, nout_ax); break;
+      }
+      loff += P.nlines[p];
+      dims[0] = od[0];
+      dims[1] = od[1];
+      dims[2] = od[2];
+      cur ^= 1;
+      __syncthreads();
+      continue;
+    }
     for (int e = threadIdx.x; e < nout; e += blockDim.x) {
       const int i = e % od[0];
       const int t = e / od[0];
