@@ -30,6 +30,13 @@ BOUNDS = [1.0, 2.0, 4.0, 8.0, 16.0]
 N0, A, OME = 0.5, 1.0, 0.05
 SEED = 20261202
 PEAK = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0}  # MI355X dense matrix TFLOP/s (vendor figures)
+# HBM bytes per launch of the two kernel classes on this workload (FP64, domain-major),
+# from two rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE) over three Newton
+# iterations from the guess, averaged over every launch of the kernel.  FETCH_SIZE is
+# scaled by 2.0 and WRITE_SIZE by 1.0: the factors at which the same counters read a
+# 256 MiB device-to-device copy (131093.6 KiB fetched, 262144.0 KiB written).
+HBM_TRAFFIC_B = {"gemm_k4": (2.0 * 18463.96 + 23028.09) * 1024.0,   # both instantiations
+                 "apply3d": (2.0 * 17506.86 + 34924.91) * 1024.0}
 
 
 def cpu_baseline(budget_s=25.0):
@@ -178,12 +185,12 @@ def main():
     gw, gms, gn = meas[0]
     aw, ams, an = meas[1]
     roof_gemm = {"bound": "mfma", "achieved": gw / (gms / 1e3) / 1e12, "peak": PEAK[prec_of[args.solver]],
-                 "unit": "TFLOP/s", "traffic": None,
+                 "unit": "TFLOP/s", "traffic": HBM_TRAFFIC_B["gemm_k4"] if args.solver == "f64" else None,
                  "kernel": "gemm_k4_kernel: LU trailing update + blocked-TRSM updates, executed tile pairs "
                            "(%d launches, %.3e flop, %.1f ms per iteration)" % (gn, gw, gms)}
     roof_gemm["frac"] = roof_gemm["achieved"] / roof_gemm["peak"]
     roof_apply = {"bound": "hbm", "achieved": aw / (ams / 1e3) / 1e9, "peak": 8000.0, "unit": "GB/s",
-                  "traffic": None,
+                  "traffic": HBM_TRAFFIC_B["apply3d"], "algorithmic_bytes_per_launch": aw / max(an, 1),
                   "kernel": "apply3d_kernel: fused 3-D spectral passes of the residual and Jacobian programs "
                             "(%d launches, %.3e B, %.1f ms per iteration)" % (an, aw, ams)}
     roof_apply["frac"] = roof_apply["achieved"] / roof_apply["peak"]

@@ -45,11 +45,13 @@ int kad_ctx_destroy(kad_ctx* ctx);
 void* kad_ctx_stream(kad_ctx* ctx);
 int kad_sync(kad_ctx* ctx);
 
-/* Live timing of one kernel class with HIP events on the context stream: kind 0 =
- * the MFMA GEMM launches (work in flops), kind 1 = the fused 3-D spectral passes
- * of the residual / Jacobian programs (work in algorithmic HBM bytes).  on = 1
- * starts a new measurement; read returns total work, kernel milliseconds and
- * launches. */
+/* Live timing of one kernel class with HIP events on the stream of each launch: kind
+ * 0 = the MFMA GEMM launches (work in executed flops: tile pairs skipped as structural
+ * zeros are not counted; the device counts them, no host synchronisation), kind 1 =
+ * the fused 3-D spectral passes of the residual / Jacobian programs (work in
+ * algorithmic HBM bytes).  on = 1 starts a new measurement window (at most 2^18
+ * launches; read returns KAD_ERR_ALLOC when a window held more); read returns total
+ * work, kernel milliseconds and launches. */
 int kad_kernel_timing(kad_ctx* ctx, int kind, int on);
 int kad_kernel_timing_read(kad_ctx* ctx, int kind, double* work, double* ms, int* launches);
 

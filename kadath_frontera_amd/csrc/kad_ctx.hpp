@@ -61,10 +61,17 @@ struct LuWork {
 // its algorithmic work per launch: flops of the MFMA GEMMs, bytes of the fused
 // 3-D spectral passes.
 struct GemmTimer {
+  static constexpr size_t kCap = size_t(1) << 18;  // timed launches per window
   std::vector<hipEvent_t> ev;   // pairs
   std::vector<double> flops;    // work per pair (flops or bytes)
+  std::vector<char> scaled;     // pair's work is multiplied by dscale[pair] at read time
+  double* dscale = nullptr;     // device, kCap entries: executed fraction written by the device
   size_t used = 0;
-  int begin(hipStream_t s, double work);
+  bool overflow = false;        // more than kCap launches in the window (read reports an error)
+  // Device slot for the executed fraction of the next pair (nullptr when the window is full).
+  double* scale_slot() { return (dscale && used < kCap) ? dscale + used : nullptr; }
+  // Opens a pair; returns 1 (nothing recorded) when the window is full.
+  int begin(hipStream_t s, double work, bool scaled_by_device = false);
   int end(hipStream_t s);
 };
 extern GemmTimer* g_gemm_timer;   // GEMM launches (work = flops)

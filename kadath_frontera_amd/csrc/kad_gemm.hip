@@ -304,6 +304,29 @@ __global__ __launch_bounds__(GNT) void gemm_bf16_kernel(int m, int n, int k, con
   }
 }
 
+// Fraction of the (nbm x nbn) tile pairs of C that a launch with `sel` executes: the
+// list counts, or the product of the active fractions of the row and column masks.
+// One block; timing runs only.
+__global__ void tile_fraction_kernel(TileSel sel, int nbm, int nbn, double* out) {
+  __shared__ int cr, cc;
+  if (sel.cnt) {
+    if (threadIdx.x == 0) *out = ((double)sel.cnt[0] / nbm) * ((double)sel.cnt[1] / nbn);
+    return;
+  }
+  if (threadIdx.x == 0) {
+    cr = 0;
+    cc = 0;
+  }
+  __syncthreads();
+  int r = 0, c = 0;
+  for (int q = threadIdx.x; q < nbm; q += blockDim.x) r += sel.rmask ? (sel.rmask[q] != 0) : 1;
+  for (int q = threadIdx.x; q < nbn; q += blockDim.x) c += sel.cmask ? (sel.cmask[q] != 0) : 1;
+  atomicAdd(&cr, r);
+  atomicAdd(&cc, c);
+  __syncthreads();
+  if (threadIdx.x == 0) *out = ((double)cr / nbm) * ((double)cc / nbn);
+}
+
 // Host launcher: C (-)= A*B for the given precision.  Returns KAD_OK or an error.
 int gemm_launch_sel(int prec, int op, int m, int n, int k, const void* A, int64_t lda, const void* B, int64_t ldb,
                     void* C, int64_t ldc, const TileSel& sel, hipStream_t s);
@@ -337,27 +360,17 @@ int gemm_launch_sel(int prec, int op, int m, int n, int k, const void* A, int64_
   const int nblocks = (int)(cdiv(m, GBM) * cdiv(n, GBN));
   dim3 grid(nblocks), block(GNT);
   GemmTimer* tm = g_gemm_timer;
+  bool timed = false;
   if (tm) {
-    // executed work: the tile pairs that are not skipped (the masks are read back,
-    // which synchronises; timing runs only)
-    double frac = 1.0;
+    // executed work: the device counts the tile pairs that are not skipped, just before
+    // the launch on the same stream (outside the timed bracket, no host synchronisation)
     const int nbm = (int)cdiv(m, GBM), nbn = (int)cdiv(n, GBN);
-    if (sel.cnt) {
-      int hc[2];
-      KAD_HIP_CHECK(hipMemcpyAsync(hc, sel.cnt, sizeof(hc), hipMemcpyDeviceToHost, s));
-      KAD_HIP_CHECK(hipStreamSynchronize(s));
-      frac = ((double)hc[0] / nbm) * ((double)hc[1] / nbn);
-    } else if (sel.rmask || sel.cmask) {
-      std::vector<int> hr(nbm, 1), hc(nbn, 1);
-      if (sel.rmask) KAD_HIP_CHECK(hipMemcpyAsync(hr.data(), sel.rmask, nbm * sizeof(int), hipMemcpyDeviceToHost, s));
-      if (sel.cmask) KAD_HIP_CHECK(hipMemcpyAsync(hc.data(), sel.cmask, nbn * sizeof(int), hipMemcpyDeviceToHost, s));
-      KAD_HIP_CHECK(hipStreamSynchronize(s));
-      double ar = 0, ac = 0;
-      for (int v : hr) ar += (v != 0);
-      for (int v : hc) ac += (v != 0);
-      frac = (ar / nbm) * (ac / nbn);
-    }
-    if (tm->begin(s, 2.0 * m * n * (double)k * frac)) return KAD_ERR_HIP;
+    double* slot = tm->scale_slot();
+    const bool masked = sel.cnt || sel.rmask || sel.cmask;
+    if (slot && masked) tile_fraction_kernel<<<1, 256, 0, s>>>(sel, nbm, nbn, slot);
+    const int rc = tm->begin(s, 2.0 * m * n * (double)k, masked);
+    if (rc < 0) return rc;
+    timed = rc == 0;
   }
   if (prec == F64) {
     if (op == OP_SUB)
@@ -384,7 +397,7 @@ int gemm_launch_sel(int prec, int op, int m, int n, int k, const void* A, int64_
     return KAD_ERR_ARG;
   }
   KAD_HIP_CHECK(hipGetLastError());
-  if (tm && tm->end(s)) return KAD_ERR_HIP;
+  if (timed && tm->end(s)) return KAD_ERR_HIP;
   return KAD_OK;
 }
 

@@ -47,7 +47,11 @@ static constexpr int RED_BLOCKS = 256;
 GemmTimer* g_gemm_timer = nullptr;
 GemmTimer* g_apply_timer = nullptr;
 
-int GemmTimer::begin(hipStream_t s, double work) {
+int GemmTimer::begin(hipStream_t s, double work, bool scaled_by_device) {
+  if (used >= kCap) {
+    overflow = true;
+    return 1;
+  }
   if (2 * used + 2 > ev.size()) {
     const size_t old = ev.size();
     ev.resize(old + 2048);
@@ -55,6 +59,7 @@ int GemmTimer::begin(hipStream_t s, double work) {
       if (hipEventCreate(&ev[q]) != hipSuccess) return KAD_ERR_HIP;
   }
   flops.push_back(work);
+  scaled.push_back(scaled_by_device ? 1 : 0);
   return hipEventRecord(ev[2 * used], s) == hipSuccess ? KAD_OK : KAD_ERR_HIP;
 }
 
@@ -220,9 +225,15 @@ int kad_kernel_timing(kad_ctx* ctx, int kind, int on) {
   static kad::GemmTimer timers[2];
   kad::GemmTimer*& slot = kind == 0 ? kad::g_gemm_timer : kad::g_apply_timer;
   if (on) {
-    timers[kind].used = 0;
-    timers[kind].flops.clear();
-    slot = &timers[kind];
+    kad::GemmTimer& t = timers[kind];
+    KAD_HIP_CHECK(hipSetDevice(reinterpret_cast<Ctx*>(ctx)->device));
+    KAD_HIP_CHECK(hipDeviceSynchronize());  // earlier launches may still write dscale
+    if (!t.dscale) KAD_HIP_CHECK(hipMalloc((void**)&t.dscale, sizeof(double) * kad::GemmTimer::kCap));
+    t.used = 0;
+    t.overflow = false;
+    t.flops.clear();
+    t.scaled.clear();
+    slot = &t;
   } else {
     slot = nullptr;
   }
@@ -234,16 +245,22 @@ int kad_kernel_timing(kad_ctx* ctx, int kind, int on) {
 int kad_kernel_timing_read(kad_ctx* ctx, int kind, double* work, double* ms, int* launches) {
   if (!ctx || kind < 0 || kind > 1) return KAD_ERR_ARG;
   Ctx* c = reinterpret_cast<Ctx*>(ctx);
-  KAD_HIP_CHECK(hipStreamSynchronize(c->stream));
+  KAD_HIP_CHECK(hipSetDevice(c->device));
+  // launches of the window may sit on any of the context's streams
+  KAD_HIP_CHECK(hipDeviceSynchronize());
   kad::GemmTimer* tm = kind == 0 ? kad::g_gemm_timer : kad::g_apply_timer;
   double f = 0, tot = 0;
   int n = 0;
   if (tm) {
+    if (tm->overflow) return KAD_ERR_ALLOC;
+    std::vector<double> sc(tm->used, 1.0);
+    if (tm->used)
+      KAD_HIP_CHECK(hipMemcpy(sc.data(), tm->dscale, sizeof(double) * tm->used, hipMemcpyDeviceToHost));
     for (size_t q = 0; q < tm->used; ++q) {
       float e = 0;
       KAD_HIP_CHECK(hipEventElapsedTime(&e, tm->ev[2 * q], tm->ev[2 * q + 1]));
       tot += e;
-      f += tm->flops[q];
+      f += tm->flops[q] * (tm->scaled[q] ? sc[q] : 1.0);
       ++n;
     }
   }

@@ -203,17 +203,20 @@ int apply3d_launch(const double* in, int64_t in_stride, double* out, int64_t out
   const size_t lds = (2 * (size_t)P.maxsize + (size_t)P.pool_size) * sizeof(double) + nl * sizeof(int);
   if (lds > 160 * 1024) return KAD_ERR_ARG;
   GemmTimer* tm = g_apply_timer;
+  bool timed = false;
   if (tm) {
     // algorithmic HBM bytes: the input arrays read once, the output arrays written
     // once (accumulate also reads them); the per-line matrices are cache-resident
     int od[3] = {d0, d1, d2};
     for (int p = 0; p < P.npass; ++p) od[P.axis[p]] = P.nout[p];
     const double nin = (double)d0 * d1 * d2, nout = (double)od[0] * od[1] * od[2];
-    if (tm->begin(s, 8.0 * nbatch * (nin + nout * (accumulate ? 2 : 1)))) return KAD_ERR_HIP;
+    const int rc = tm->begin(s, 8.0 * nbatch * (nin + nout * (accumulate ? 2 : 1)));
+    if (rc < 0) return rc;
+    timed = rc == 0;
   }
   apply3d_kernel<<<nbatch, 256, lds, s>>>(in, in_stride, out, out_stride, d0, d1, d2, P, pool, accumulate, scale);
   KAD_HIP_CHECK(hipGetLastError());
-  if (tm && tm->end(s)) return KAD_ERR_HIP;
+  if (timed && tm->end(s)) return KAD_ERR_HIP;
   return KAD_OK;
 }
 
