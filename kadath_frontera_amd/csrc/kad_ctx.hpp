@@ -47,7 +47,8 @@ struct LuWork {
   int* ext = nullptr;           // last nonzero row per (128-column tile, 2048-row chunk)
   size_t cap_ext = 0;
   int* mask_c2 = nullptr;       // column tiles of U12 beyond the next panel
-  int* mask_r2 = nullptr;       // second row-tile mask (alternate steps)
+  int* mask_r2 = nullptr;       // second and third row-tile masks (rotating by step)
+  int* mask_r3 = nullptr;
   // look-ahead: the next panel is updated and factored on a second stream
   hipStream_t stream2 = nullptr;
   hipEvent_t ev_panel[2] = {nullptr, nullptr}, ev_next[2] = {nullptr, nullptr}, ev_start = nullptr;
@@ -55,6 +56,12 @@ struct LuWork {
   void* tinv_m = nullptr;
   void* tmp_m = nullptr;
   size_t tmp_m_bytes = 0;
+  // persistent base panel: candidate / row exchange buffers, grid-barrier words
+  // (arrivals, exits, abort) and the CU count that bounds its grid
+  void* ffp = nullptr;
+  int* ffp_sync = nullptr;
+  int ncu = 0;
+  bool ffp_off = false;         // KAD_LU_PANEL=percolumn selects the per-column kernels
 };
 
 // Optional live timing of one kernel class (HIP events on the launch stream) and

@@ -477,6 +477,316 @@ __global__ void ff_scatter_kernel(T* __restrict__ A, int64_t lda, int n, int c0,
   if (i < n && c < c1) A[i + (int64_t)c * lda] = tmp[(i - c0) + (int64_t)(c - c0) * (n - c0)];
 }
 
+// ---------------------------------------------------------------------------
+// Persistent base panel: one launch factors columns [c0, c1) (c1 - c0 <= W) of rows
+// [c0, n).  Workgroup g owns logical rows [c0 + g rpw, c0 + (g + 1) rpw) of the block
+// and keeps them in LDS for the whole launch; interchanges move row contents between
+// owners.  Per column j:
+//   1. every workgroup publishes its pivot candidate (|a|, logical row, the
+//      candidate's row of the block); the owner of row j publishes row j;
+//   2. one grid barrier (device-scope fence + arrival counter);
+//   3. every workgroup reads all candidates with their rows in one round trip,
+//      reduces them itself -- LAPACK's choice, the first index of max |a| -- and
+//      swaps, scales and updates its rows with the per-column kernel's operations.
+// At the end the interchanges are applied to the panel's other columns [k0, c0) and
+// [c1, k1) (exact: those columns are not read in between).
+// The barrier's cost grows with the number of arriving workgroups, so each one holds
+// as many rows as fits beside a GEMM workgroup on one CU (the trailing update runs
+// concurrently): at most FFP_ROW_BYTES of rows plus 18 KB of static LDS, 4 waves.
+// The grid must be resident at once: the host launches at most one workgroup per
+// CU, and nothing else on the device waits on this kernel.  Every wait is bounded: a
+// workgroup that reaches the spin limit raises the abort flag, every waiter leaves
+// on it, and the host reports the failure.
+// sync[0] = arrivals, sync[1] = exits (the last one resets both), sync[2] = abort.
+// ---------------------------------------------------------------------------
+constexpr int FFP_MAX_GRID = 64;
+constexpr int FFP_THREADS = 256;
+constexpr int FFP_ROW_BYTES = 72 * 1024;
+
+__device__ __forceinline__ void argmax_take(double& v, int& i, double ov, int oi) {
+  if (oi >= 0 && (i < 0 || ov > v || (ov == v && oi < i))) {
+    v = ov;
+    i = oi;
+  }
+}
+
+// Max |a| with the smallest index among ties (invalid entries carry index -1) over
+// the 256-thread block; the result reaches every thread.  sv / si: 4 entries in LDS.
+__device__ __forceinline__ void argmax_256(double& v, int& i, double* sv, int* si) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double ov = __shfl_xor(v, o);
+    const int oi = __shfl_xor(i, o);
+    argmax_take(v, i, ov, oi);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    sv[threadIdx.x >> 6] = v;
+    si[threadIdx.x >> 6] = i;
+  }
+  __syncthreads();
+  v = -1.0;
+  i = -1;
+#pragma unroll
+  for (int q = 0; q < FFP_THREADS / 64; ++q) argmax_take(v, i, sv[q], si[q]);
+  __syncthreads();
+}
+
+template <typename T, int W>
+__global__ __launch_bounds__(FFP_THREADS) void ffp_kernel(T* __restrict__ A, int64_t lda, int n, int c0, int c1,
+                                                          int k0, int k1, int rpw, double* __restrict__ cval,
+                                                          int* __restrict__ cidx, T* __restrict__ crow,
+                                                          T* __restrict__ rowj, int* __restrict__ ipiv,
+                                                          int* __restrict__ info, int* __restrict__ sync,
+                                                          int spin_limit) {
+  extern __shared__ __align__(16) unsigned char ffp_smem[];
+  T* L = reinterpret_cast<T*>(ffp_smem);  // rpw x (W + 1)
+  constexpr int LD = W + 1;
+  __shared__ T cand[FFP_MAX_GRID * W];     // every workgroup's candidate row of this column
+  __shared__ double candv[FFP_MAX_GRID];
+  __shared__ int candi[FFP_MAX_GRID];
+  __shared__ T jrow[W];
+  __shared__ double sv[FFP_THREADS / 64];
+  __shared__ int si[FFP_THREADS / 64];
+  __shared__ int s_piv[W];
+  __shared__ int s_rows[2 * W];
+  __shared__ int s_src[2 * W];
+  __shared__ int s_nrows;
+  __shared__ int s_abort;
+  const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x;
+  const int r0 = c0 + g * rpw;
+  const int nr = max(0, min(rpw, n - r0));
+  const int w = c1 - c0;
+  for (int e = tid; e < nr * w; e += FFP_THREADS) {
+    const int r = e % nr, c = e / nr;
+    L[r * LD + c] = A[(int64_t)(r0 + r) + (int64_t)(c0 + c) * lda];
+  }
+  __syncthreads();
+  bool live = true;  // uniform: false once a barrier was abandoned
+  for (int jj = 0; jj < w; ++jj) {
+    const int j = c0 + jj;
+    // 1. local candidate among the owned logical rows >= j, then publish
+    double v = -1.0;
+    int id = -1;
+    for (int r = tid; r < nr; r += FFP_THREADS)
+      if (r0 + r >= j) argmax_take(v, id, fabs((double)L[r * LD + jj]), r0 + r);
+    argmax_256(v, id, sv, si);
+    const size_t slot = (size_t)jj * G + g;
+    if (tid == 0) {
+      cval[slot] = v;
+      cidx[slot] = id;
+    }
+    if (id >= 0 && tid < w) crow[slot * W + tid] = L[(id - r0) * LD + tid];
+    if (j >= r0 && j < r0 + nr && tid < w) rowj[(size_t)jj * W + tid] = L[(j - r0) * LD + tid];
+    // 2. grid barrier
+    __syncthreads();
+    if (tid == 0) {
+      __threadfence();
+      atomicAdd(&sync[0], 1);
+      const int target = G * (jj + 1);
+      int ab = 0;
+      for (int it = 0; __hip_atomic_load(&sync[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target; ++it) {
+        if ((it & 255) == 255 && __hip_atomic_load(&sync[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+          ab = 1;
+          break;
+        }
+        if (it >= spin_limit) {
+          __hip_atomic_store(&sync[2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          ab = 1;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+      __threadfence();
+      s_abort = ab;
+    }
+    __syncthreads();
+    if (s_abort) {  // uniform; the host sees sync[2] and reports the failure
+      live = false;
+      break;
+    }
+    // 3. all candidates and row j in one round trip, then the pivot
+    for (int e = tid; e < G * w; e += FFP_THREADS) {
+      const int q = e / w, c = e - q * w;
+      cand[q * W + c] = crow[((size_t)jj * G + q) * W + c];
+    }
+    if (tid < G) {
+      candi[tid] = cidx[(size_t)jj * G + tid];
+      candv[tid] = cval[(size_t)jj * G + tid];
+    }
+    if (tid < w) jrow[tid] = rowj[(size_t)jj * W + tid];
+    __syncthreads();
+    v = -1.0;
+    id = -1;
+    if (tid < G && candi[tid] >= 0) {
+      v = candv[tid];
+      id = candi[tid];
+    }
+    argmax_256(v, id, sv, si);
+    int p = id;
+    if (p < j || p >= n) {
+      // only reachable with corrupt candidates: keep every index in range and fail
+      p = j;
+      if (tid == 0) __hip_atomic_store(&sync[2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    const T* urow = cand + ((p - c0) / rpw) * W;  // the row p's owner published
+    if (tid == 0) s_piv[jj] = p;
+    if (g == 0 && tid == 0) {
+      ipiv[j] = p;
+      if (v == 0.0 && *info == 0) *info = j + 1;
+    }
+    if (p != j && tid < w) {
+      if (j >= r0 && j < r0 + nr) L[(j - r0) * LD + tid] = urow[tid];
+      if (p >= r0 && p < r0 + nr) L[(p - r0) * LD + tid] = jrow[tid];
+    }
+    __syncthreads();
+    // the multipliers of the owned rows below j, then the rank-1 update in two halves
+    // of 16 columns with that half of the pivot row in registers (constant indices):
+    // every LDS read of a row is independent of its writes, and at most ~90 VGPRs
+    // are live, so a workgroup fits beside a GEMM workgroup on one CU
+    constexpr int RPT = 1;  // rows per thread (the host launches at most FFP_THREADS rows per workgroup)
+    const T piv = urow[jj];
+    T lq[RPT];
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      const int r = tid + q * FFP_THREADS;
+      lq[q] = T(0);
+      if (r < nr && r0 + r > j) {
+        T l = L[r * LD + jj];
+        if (piv != T(0)) l = l / piv;
+        L[r * LD + jj] = l;
+        lq[q] = l;
+      }
+    }
+#pragma unroll
+    for (int h = 0; h < W; h += 16) {
+      if (h + 16 > jj + 1 && h < w) {  // uniform: some column c of the half has jj < c < w
+        T uh[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) uh[c] = urow[h + c];
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+          const int r = tid + q * FFP_THREADS;
+          if (r < nr && r0 + r > j) {
+            T* row = L + r * LD + h;
+            T x[16];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) x[c] = row[c];
+#pragma unroll
+            for (int c = 0; c < 16; ++c)
+              if (h + c > jj && h + c < w) row[c] = x[c] - lq[q] * uh[c];
+          }
+        }
+      }
+    }
+    __syncthreads();  // cand / jrow / L are rewritten in the next column
+  }
+  if (!live) return;
+  for (int e = tid; e < nr * w; e += FFP_THREADS) {
+    const int r = e % nr, c = e / nr;
+    A[(int64_t)(r0 + r) + (int64_t)(c0 + c) * lda] = L[r * LD + c];
+  }
+  // the interchanges on the panel's other columns: the affected rows are rows
+  // c0 .. c1-1 and the pivot rows (2 w entries, duplicates harmless: they carry the
+  // same value); entry k's content comes from the row found by undoing the
+  // interchanges in reverse order
+  if (tid < 2 * w) {
+    const int r = tid < w ? c0 + tid : s_piv[tid - w];
+    int x = r;
+    for (int jj = w - 1; jj >= 0; --jj) {
+      const int a = c0 + jj, b = s_piv[jj];
+      x = (x == a) ? b : ((x == b) ? a : x);
+    }
+    s_rows[tid] = r;
+    s_src[tid] = x;
+  }
+  if (tid == 0) s_nrows = 2 * w;
+  __syncthreads();
+  // (column, affected row) pairs, columns split over the workgroups: every thread
+  // gathers up to FFP_TAIL of them into registers, the workgroup synchronises, then
+  // every thread scatters them -- one round trip per batch, and no row of a column is
+  // written before all of that column's reads (a column stays in one workgroup)
+  constexpr int FFP_TAIL = 16;
+  const int nrw = s_nrows, nleft = c0 - k0, nother = (c0 - k0) + (k1 - c1);
+  const int cpg = (int)cdiv(nother, G);  // columns of this workgroup: [g cpg, (g + 1) cpg)
+  const int qa = min(nother, g * cpg), qe = min(nother, qa + cpg);
+  const int npairs = (qe - qa) * nrw;
+  const int step = ((FFP_TAIL * FFP_THREADS) / nrw) * nrw;  // whole columns per batch (nrw <= 2 W)
+  for (int b0 = 0; b0 < npairs; b0 += step) {
+    const int bend = min(npairs, b0 + step);
+    T val[FFP_TAIL];
+#pragma unroll
+    for (int u = 0; u < FFP_TAIL; ++u) {
+      const int e = b0 + u * FFP_THREADS + tid;
+      if (e < bend) {
+        const int q = qa + e / nrw, k = e - (e / nrw) * nrw;
+        const int col = q < nleft ? k0 + q : c1 + (q - nleft);
+        val[u] = A[(int64_t)s_src[k] + (int64_t)col * lda];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < FFP_TAIL; ++u) {
+      const int e = b0 + u * FFP_THREADS + tid;
+      if (e < bend) {
+        const int q = qa + e / nrw, k = e - (e / nrw) * nrw;
+        const int col = q < nleft ? k0 + q : c1 + (q - nleft);
+        A[(int64_t)s_rows[k] + (int64_t)col * lda] = val[u];
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0 && atomicAdd(&sync[1], 1) == G - 1) {
+    // every workgroup has left its last wait: reset for the next launch
+    __hip_atomic_store(&sync[0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&sync[1], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// Bytes of the candidate / row exchange buffers of the persistent base panel.
+size_t ffp_bytes() {
+  return (size_t)FFP_MAX_GRID * PANEL_BASE * (sizeof(double) + sizeof(int)) +
+         ((size_t)FFP_MAX_GRID * PANEL_BASE * PANEL_BASE + (size_t)PANEL_BASE * PANEL_BASE) * sizeof(double);
+}
+
+// Launch shape of the persistent base panel for m rows of element type T: grid and
+// rows per workgroup, or false when it does not fit (the per-column kernels run).
+template <typename T>
+bool ffp_shape(int m, int ncu, int& grid, int& rpw) {
+  // one row per thread: a second row for a few threads would double the latency of
+  // every column step for the whole workgroup
+  const int rpw_max = std::min(FFP_THREADS, FFP_ROW_BYTES / (int)((PANEL_BASE + 1) * sizeof(T)));
+  grid = (int)cdiv(m, rpw_max);
+  if (m < 1 || grid > std::min(ncu, FFP_MAX_GRID)) return false;
+  rpw = (int)cdiv(m, grid);
+  grid = (int)cdiv(m, rpw);
+  return true;
+}
+
+// One persistent base panel: columns [c0, c1) of rows [c0, n), interchanges applied
+// across the panel's columns [k0, k1).
+template <typename T>
+int ffp_launch(T* A, int64_t lda, int n, int c0, int c1, int k0, int k1, int* ipiv, LuWork& w) {
+  int grid, rpw;
+  if (!ffp_shape<T>(n - c0, w.ncu, grid, rpw) || rpw > FFP_THREADS || c1 - c0 > PANEL_BASE || c1 > n)
+    return KAD_ERR_ARG;
+  static bool lds_set = false;  // one attribute call per element type (dynamic LDS above 64 KB)
+  if (!lds_set) {
+    KAD_HIP_CHECK(hipFuncSetAttribute((const void*)ffp_kernel<T, PANEL_BASE>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, FFP_ROW_BYTES));
+    lds_set = true;
+  }
+  char* base = (char*)w.ffp;
+  double* cval = (double*)base;
+  int* cidx = (int*)(base + FFP_MAX_GRID * PANEL_BASE * sizeof(double));
+  T* crow = (T*)(base + FFP_MAX_GRID * PANEL_BASE * (sizeof(double) + sizeof(int)));
+  T* rowj = crow + (size_t)FFP_MAX_GRID * PANEL_BASE * PANEL_BASE;
+  const size_t lds = (size_t)rpw * (PANEL_BASE + 1) * sizeof(T);
+  ffp_kernel<T, PANEL_BASE><<<grid, FFP_THREADS, lds, w.stream>>>(A, lda, n, c0, c1, k0, k1, rpw, cval, cidx, crow,
+                                                                 rowj, ipiv, w.info, w.ffp_sync, 1 << 22);
+  KAD_HIP_CHECK(hipGetLastError());
+  return KAD_OK;
+}
+
 // Base panel: columns [c0, c1) (c1 - c0 <= PANEL_BASE), rows [c0, n).
 template <typename T>
 int panel_base(T* A, int64_t lda, int n, int c0, int c1, int* ipiv, LuWork& w) {
@@ -507,27 +817,29 @@ int panel_base(T* A, int64_t lda, int n, int c0, int c1, int* ipiv, LuWork& w) {
   return KAD_OK;
 }
 
-// Recursive panel factorisation of columns [c0, c1), rows [c0, n).
+// Recursive panel factorisation of columns [c0, c1), rows [c0, n), inside the panel
+// [pk0, pk1).  fused: the base panels are the persistent kernel, which applies its
+// interchanges across [pk0, pk1) itself, so the recursion swaps no rows.
 template <typename T>
-int panel_rec(T* A, int64_t lda, int n, int c0, int c1, int* ipiv, LuWork& w) {
+int panel_rec(T* A, int64_t lda, int n, int c0, int c1, int* ipiv, LuWork& w, int pk0, int pk1, bool fused) {
   const int wdt = c1 - c0;
-  if (wdt <= PANEL_BASE) return panel_base<T>(A, lda, n, c0, c1, ipiv, w);
+  if (wdt <= PANEL_BASE)
+    return fused ? ffp_launch<T>(A, lda, n, c0, c1, pk0, pk1, ipiv, w) : panel_base<T>(A, lda, n, c0, c1, ipiv, w);
   int h = wdt / 2;
   h = (h + 15) / 16 * 16;
   const int cm = c0 + h;
-  int rc = panel_rec<T>(A, lda, n, c0, cm, ipiv, w);
+  int rc = panel_rec<T>(A, lda, n, c0, cm, ipiv, w, pk0, pk1, fused);
   if (rc) return rc;
-  rc = laswp<T>(A, lda, cm, c1, ipiv, c0, cm, w);
-  if (rc) return rc;
+  if (!fused && (rc = laswp<T>(A, lda, cm, c1, ipiv, c0, cm, w))) return rc;
   rc = trsm_left_unit_lower<T>(A + c0 + (int64_t)c0 * lda, lda, h, A + c0 + (int64_t)cm * lda, lda, c1 - cm, w,
                                prec_of<T>());
   if (rc) return rc;
   rc = gemm_launch(prec_of<T>(), GOP_SUB, n - cm, c1 - cm, h, A + cm + (int64_t)c0 * lda, lda,
                    A + c0 + (int64_t)cm * lda, lda, A + cm + (int64_t)cm * lda, lda, w.stream);
   if (rc) return rc;
-  rc = panel_rec<T>(A, lda, n, cm, c1, ipiv, w);
+  rc = panel_rec<T>(A, lda, n, cm, c1, ipiv, w, pk0, pk1, fused);
   if (rc) return rc;
-  return laswp<T>(A, lda, c0, cm, ipiv, cm, c1, w);
+  return fused ? KAD_OK : laswp<T>(A, lda, c0, cm, ipiv, cm, c1, w);
 }
 
 // dst[i + c ldt] = A[map[i] + c lda] for i < nrows, c < ncols (row gather of a column block).
@@ -599,18 +911,21 @@ int apply_left_swaps(T* A, int64_t lda, int n, const int* ipiv, int nb, LuWork& 
 // [k0, k1), next panel [k1, k2), the one after [k2, k3)):
 //   panel stream: interchanges, U12 and update of the next panel's columns
 //     [k1, k2), then the next panel's factorisation and its L21 row-tile mask;
-//   main stream: interchanges and U12 of columns [k2, n), the update of [k2, k3)
-//     (signalled: the panel stream needs it at step p+1), then the update of [k3, n).
+//   main stream, part A: interchanges, U12 and update of [k2, k3) only, then the
+//     signal the panel stream waits for at step p+1;
+//   main stream, part B: interchanges, U12 and update of [k3, n) -- the bulk of the
+//     step, which runs beside the next panel instead of ahead of it.
 // Events (ping-pong by step parity so a new record never overtakes a pending
 // wait): ev_panel "panel p and its mask are done", ev_next "columns of the panel
 // after next are up to date".  The two streams work on disjoint columns, each with
-// its own triangular-inverse / GEMM scratch; the L21 row-tile mask alternates
-// between two buffers.
+// its own triangular-inverse / GEMM scratch.  The L21 row-tile masks rotate over
+// three buffers: part B of step p still reads panel p's mask while the panel stream
+// writes panel p+2's.
 template <typename T>
 int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w) {
   // both streams' scratch must exist before anything is launched
   if (!w.stream2 || !w.tinv || !w.tinv_m || !w.tmp || !w.tmp_m || w.tmp_m_bytes < w.tmp_bytes || !w.tmp2 ||
-      !w.mask_r || !w.mask_r2 || !w.ext)
+      !w.mask_r || !w.mask_r2 || !w.mask_r3 || !w.ext)
     return KAD_ERR_ALLOC;
   LuWork wp = w;  // panel stream: tinv, tmp, tmp2 and the panel reductions
   wp.stream = w.stream2;
@@ -618,7 +933,7 @@ int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w)
   wm.tinv = w.tinv_m;
   wm.tmp = w.tmp_m;
   wm.tmp_bytes = w.tmp_m_bytes;
-  int* mask_rows[2] = {w.mask_r, w.mask_r2};
+  int* mask_rows[3] = {w.mask_r, w.mask_r2, w.mask_r3};
   const int stride = (int)cdiv(n, 128) + 1;
   // Row extent of every 128-column tile of the input (one pass, one readback).
   // A panel's rows at or below R are zero in its columns and stay zero: the
@@ -639,7 +954,9 @@ int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w)
     for (int t = k0 / 128; t <= (k1 - 1) / 128; ++t) R = std::max(R, ext[t] + 1);
     R = std::min(R, n);
     r_prev = R;
-    int rc = panel_rec<T>(A, lda, R, k0, k1, ipiv, wp);
+    int fg, frpw;
+    const bool fused = wp.ffp && !wp.ffp_off && ffp_shape<T>(R - k0, wp.ncu, fg, frpw);
+    int rc = panel_rec<T>(A, lda, R, k0, k1, ipiv, wp, k0, k1, fused);
     if (rc) return rc;
     if (k1 < n) {
       row_tile_mask_kernel<T><<<(unsigned)cdiv(n - k1, 128), 256, 0, wp.stream>>>(A + k1 + (int64_t)k0 * lda, lda,
@@ -659,37 +976,41 @@ int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w)
     const int k2 = std::min(k1 + nb, n);
     const int k3 = std::min(k2 + nb, n);
     const int par = p & 1;
-    const int* mrows = mask_rows[par];
+    const int* mrows = mask_rows[p % 3];
     const T* L21 = A + k1 + (int64_t)k0 * lda;
-    // ---- main stream: columns [k2, n)
+    const T* L11 = A + k0 + (int64_t)k0 * lda;
+    // ---- main stream, part A: columns [k2, k3)
     KAD_HIP_CHECK(hipStreamWaitEvent(w.stream, w.ev_panel[par], 0));
     if (k1 >= n) break;
     if (k2 < n) {
-      rc = laswp<T>(A, lda, k2, n, ipiv, k0, k1, wm);
+      rc = laswp<T>(A, lda, k2, k3, ipiv, k0, k1, wm);
       if (rc) return rc;
       // structural zeros of A12 (column tiles; U12 = L11^{-1} A12 keeps them)
-      col_tile_mask_kernel<T><<<(unsigned)cdiv(n - k2, 128), 256, 0, w.stream>>>(A + k0 + (int64_t)k2 * lda, lda,
-                                                                                k1 - k0, n - k2, w.mask_c);
+      col_tile_mask_kernel<T><<<(unsigned)cdiv(k3 - k2, 128), 256, 0, w.stream>>>(A + k0 + (int64_t)k2 * lda, lda,
+                                                                                 k1 - k0, k3 - k2, w.mask_c);
       KAD_HIP_CHECK(hipGetLastError());
-      rc = trsm_left_unit_lower<T>(A + k0 + (int64_t)k0 * lda, lda, k1 - k0, A + k0 + (int64_t)k2 * lda, lda, n - k2,
-                                   wm, gemm_prec, w.mask_c);
+      rc = trsm_left_unit_lower<T>(L11, lda, k1 - k0, A + k0 + (int64_t)k2 * lda, lda, k3 - k2, wm, gemm_prec,
+                                   w.mask_c);
       if (rc) return rc;
-      // active tile pairs of the update of [k3, n), compacted on the grid (all XCDs
-      // busy); built before ev_next, whose waiter rewrites this step's row mask
-      if (k3 < n) {
-        col_tile_mask_kernel<T><<<(unsigned)cdiv(n - k3, 128), 256, 0, w.stream>>>(A + k0 + (int64_t)k3 * lda, lda,
-                                                                                  k1 - k0, n - k3, w.mask_c2);
-        tile_lists_kernel<<<1, 64, 0, w.stream>>>(mrows, (int)cdiv(n - k1, 128), w.mask_c2, (int)cdiv(n - k3, 128),
-                                                  w.tile_list, stride);
-        KAD_HIP_CHECK(hipGetLastError());
-      }
-      // the panel after next first, so the panel stream can go on
       rc = gemm_launch_masked(gemm_prec, GOP_SUB, n - k1, k3 - k2, k1 - k0, L21, lda, A + k0 + (int64_t)k2 * lda, lda,
                               A + k1 + (int64_t)k2 * lda, lda, mrows, w.mask_c, w.stream);
       if (rc) return rc;
     }
     KAD_HIP_CHECK(hipEventRecord(w.ev_next[par], w.stream));
+    // ---- main stream, part B: columns [k3, n), beside the next panel
     if (k3 < n) {
+      rc = laswp<T>(A, lda, k3, n, ipiv, k0, k1, wm);
+      if (rc) return rc;
+      col_tile_mask_kernel<T><<<(unsigned)cdiv(n - k3, 128), 256, 0, w.stream>>>(A + k0 + (int64_t)k3 * lda, lda,
+                                                                                k1 - k0, n - k3, w.mask_c2);
+      KAD_HIP_CHECK(hipGetLastError());
+      rc = trsm_left_unit_lower<T>(L11, lda, k1 - k0, A + k0 + (int64_t)k3 * lda, lda, n - k3, wm, gemm_prec,
+                                   w.mask_c2);
+      if (rc) return rc;
+      // active tile pairs of the update, compacted on the grid (all XCDs busy)
+      tile_lists_kernel<<<1, 64, 0, w.stream>>>(mrows, (int)cdiv(n - k1, 128), w.mask_c2, (int)cdiv(n - k3, 128),
+                                                w.tile_list, stride);
+      KAD_HIP_CHECK(hipGetLastError());
       TileSel sel;
       sel.rlist = w.tile_list;
       sel.clist = w.tile_list + stride;
@@ -702,13 +1023,12 @@ int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w)
     if (p > 0) KAD_HIP_CHECK(hipStreamWaitEvent(wp.stream, w.ev_next[par ^ 1], 0));
     rc = laswp<T>(A, lda, k1, k2, ipiv, k0, k1, wp);
     if (rc) return rc;
-    rc = trsm_left_unit_lower<T>(A + k0 + (int64_t)k0 * lda, lda, k1 - k0, A + k0 + (int64_t)k1 * lda, lda, k2 - k1,
-                                 wp, gemm_prec);
+    rc = trsm_left_unit_lower<T>(L11, lda, k1 - k0, A + k0 + (int64_t)k1 * lda, lda, k2 - k1, wp, gemm_prec);
     if (rc) return rc;
     rc = gemm_launch_masked(gemm_prec, GOP_SUB, n - k1, k2 - k1, k1 - k0, L21, lda, A + k0 + (int64_t)k1 * lda, lda,
                             A + k1 + (int64_t)k1 * lda, lda, mrows, nullptr, wp.stream);
     if (rc) return rc;
-    rc = panel(k1, k2, mask_rows[par ^ 1]);
+    rc = panel(k1, k2, mask_rows[(p + 1) % 3]);
     if (rc) return rc;
     KAD_HIP_CHECK(hipEventRecord(w.ev_panel[par ^ 1], wp.stream));
   }

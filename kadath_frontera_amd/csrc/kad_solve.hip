@@ -12,12 +12,14 @@
 #include <cmath>
 #include <algorithm>
 #include <cstring>
+#include <cstdlib>
 
 namespace kad {
 int gemm_launch(int prec, int op, int m, int n, int k, const void* A, int64_t lda, const void* B, int64_t ldb, void* C,
                 int64_t ldc, hipStream_t s);
 template <typename T> int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w);
 size_t trsv_table_doubles(int n);
+size_t ffp_bytes();
 template <typename T> int trsv_prepare(const T* LU, int64_t lda, int n, int trans, double* W, hipStream_t s);
 template <typename T>
 int getrs_fast(const T* LU, int64_t lda, int n, const int* perm, const double* W, double* b, int trans,
@@ -91,6 +93,7 @@ int Ctx::lu_work(int n, int nb) {
   lu.tile_list = lu.mask_r + 2 * nt;  // 2 nt + 2
   lu.mask_c2 = lu.mask_r + 4 * nt + 2;
   lu.mask_r2 = lu.mask_r + 5 * nt + 2;
+  lu.mask_r3 = lu.mask_r + 6 * nt + 2;  // [6 nt + 2, 7 nt + 2)
   if ((rc = ensure((void**)&lu.ext, &lu.cap_ext, sizeof(int) * (size_t)cdiv(n, 128) * cdiv(n, 2048)))) return rc;
   if (!lu.stream2) {
     // same (default) priority as the main stream
@@ -106,6 +109,13 @@ int Ctx::lu_work(int n, int nb) {
     KAD_HIP_CHECK(hipMalloc((void**)&lu.counter, 64));
     KAD_HIP_CHECK(hipMemsetAsync(lu.counter, 0, 64, stream));
     lu.info = (int*)((char*)lu.counter + 16);
+  }
+  lu.ffp_sync = (int*)((char*)lu.counter + 32);  // arrivals, exits, abort (zeroed above)
+  if (!lu.ffp) {
+    KAD_HIP_CHECK(hipMalloc(&lu.ffp, ffp_bytes()));
+    KAD_HIP_CHECK(hipDeviceGetAttribute(&lu.ncu, hipDeviceAttributeMultiprocessorCount, device));
+    const char* e = std::getenv("KAD_LU_PANEL");
+    lu.ffp_off = e && std::strcmp(e, "percolumn") == 0;
   }
   if ((rc = ensure(&lu.tinv, &cap_tinv, (size_t)nb * nb * 8))) return rc;
   // GEMM scratch for L11^{-1} A12: nb x chunk (bounded at 64 MiB).
@@ -174,8 +184,8 @@ int Ctx::upload(const void* host, size_t bytes, void** dev) {
 Ctx::~Ctx() {
   for (auto& kv : plans) delete kv.second;
   for (void* p : owned) (void)hipFree(p);
-  void* bufs[] = {lu.red_val, lu.red_idx, lu.counter, lu.tinv,   lu.tmp,  lu.tmp2,
-                  lu.map,     lu.mask_r,  lu.swap_map, lu.ext,  gemv_part, vwork,  krylov, trs_tab};
+  void* bufs[] = {lu.red_val, lu.red_idx,   lu.counter, lu.tinv,   lu.tmp,    lu.tmp2, lu.map,  lu.mask_r,
+                  lu.swap_map, lu.ext,      lu.ffp,     gemv_part, vwork,     krylov,  trs_tab};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (int q = 0; q < 2; ++q) {
@@ -303,8 +313,19 @@ int kad_lu_factor(kad_ctx* ctx, int prec, int n, void* A, int64_t lda, int* ipiv
   else
     rc = kad::getrf<float>((float*)A, lda, n, ipiv, nb, prec, c->lu);
   if (rc) return rc;
-  int info = 0;
-  KAD_HIP_CHECK(hipMemcpyAsync(&info, c->lu.info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  // info (counter + 16 B) and the persistent panel's barrier words (counter + 32 B)
+  int words[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  KAD_HIP_CHECK(hipMemcpyAsync(words, (char*)c->lu.counter + 16, sizeof(words), hipMemcpyDeviceToHost, c->stream));
+  KAD_HIP_CHECK(hipStreamSynchronize(c->stream));
+  const int info = words[0];
+  if (words[6] != 0) {
+    // a grid barrier of the persistent panel reached its spin limit: the factors are
+    // not valid; clear the barrier words for the next call
+    std::fprintf(stderr, "kad_lu_factor: persistent panel barrier timed out\n");
+    KAD_HIP_CHECK(hipMemsetAsync(c->lu.ffp_sync, 0, 4 * sizeof(int), c->stream));
+    KAD_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return KAD_ERR_HIP;
+  }
   if (perm) {
     std::vector<int> hp(n), hperm(n);
     KAD_HIP_CHECK(hipMemcpyAsync(hp.data(), ipiv, n * sizeof(int), hipMemcpyDeviceToHost, c->stream));

@@ -117,6 +117,64 @@ def test_lu_f64_matches_lapack(ctx, n, nb):
     assert be < 1e-13 * n
 
 
+def _per_column_factor(monkeypatch, prec, A):
+    """Factor A with the per-column base-panel kernels (a context created with
+    KAD_LU_PANEL=percolumn) instead of the persistent one."""
+    from kadath_frontera_amd.device import Context
+    monkeypatch.setenv("KAD_LU_PANEL", "percolumn")
+    c = Context(0)
+    try:
+        ipiv, perm, info = c.lu_factor(prec, A)
+        c.sync()
+    finally:
+        c.close()
+        monkeypatch.delenv("KAD_LU_PANEL")
+    return ipiv, info
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_persistent_panel_ties_match_per_column(ctx, prec, monkeypatch):
+    """The persistent base panel (one launch per 32 columns, one grid barrier per
+    column, rows held in LDS by 13 workgroups here) performs the per-column kernels'
+    operations in the same order: pivots and factors agree bit for bit.  Integer
+    entries make exact ties in |a|, where the choice must be the first index (the
+    first column's pivot is checked against argmax directly)."""
+    print(f"persistent panel ties {prec}", flush=True)
+    n = 3000
+    A = np.random.default_rng(11).integers(-2, 3, (n, n)).astype(np.float64)
+    dt = torch.float64 if prec == "f64" else torch.float32
+    d1, d2 = colmajor(A, dt), colmajor(A, dt)
+    ipiv1, _, info1 = ctx.lu_factor(prec, d1)
+    ctx.sync()
+    ipiv2, info2 = _per_column_factor(monkeypatch, prec, d2)
+    assert info1 == info2 == 0
+    assert int(ipiv1[0]) == int(np.argmax(np.abs(A[:, 0])))
+    assert torch.equal(ipiv1, ipiv2)
+    assert torch.equal(d1, d2)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_persistent_panel_large_grid_matches_per_column(ctx, prec, monkeypatch):
+    """Dense n = 40000: the first panels span 40000 rows, 173 workgroups in the grid
+    barrier.  Bitwise agreement with the per-column kernels."""
+    print(f"persistent panel n=40000 {prec}", flush=True)
+    n = 40000
+    dt = torch.float64 if prec == "f64" else torch.float32
+    g = torch.Generator(device="cuda").manual_seed(n)
+    d1 = torch.randn(n, n, generator=g, dtype=dt, device="cuda").T  # column-major view
+    d2 = torch.empty_like(d1)
+    d2.copy_(d1)
+    assert d2.stride() == d1.stride() == (1, n)
+    ipiv1, _, info1 = ctx.lu_factor(prec, d1)
+    ctx.sync()
+    ipiv2, info2 = _per_column_factor(monkeypatch, prec, d2)
+    assert info1 == info2 == 0
+    assert torch.equal(ipiv1, ipiv2)
+    assert torch.equal(d1, d2)
+    del d1, d2
+    torch.cuda.empty_cache()
+
+
 @pytest.mark.parametrize("trans", [0, 1])
 def test_lu_solve_f64(ctx, trans):
     n = 777
