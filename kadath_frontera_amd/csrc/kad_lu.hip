@@ -510,18 +510,42 @@ __device__ __forceinline__ void argmax_take(double& v, int& i, double ov, int oi
   }
 }
 
+// One step of a wave-level argmax: combine with the lane that DPP control CTRL reads
+// (a permutation inside each 16-lane row, so every source lane is valid).
+template <int CTRL>
+__device__ __forceinline__ void argmax_dpp(double& v, int& i) {
+  const long long b = __double_as_longlong(v);
+  const int hi = (int)(b >> 32), lo = (int)b;
+  const int ohi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false);
+  const int olo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false);
+  const int oi = __builtin_amdgcn_update_dpp(i, i, CTRL, 0xF, 0xF, false);
+  argmax_take(v, i, __longlong_as_double(((long long)ohi << 32) | (unsigned)olo), oi);
+}
+
 // Max |a| with the smallest index among ties (invalid entries carry index -1) over
-// the 256-thread block; the result reaches every thread.  sv / si: 4 entries in LDS.
+// the 256-thread block; the result reaches every thread.  Inside a wave: quad
+// swaps, then half-row and row mirrors (every lane of a 16-lane row ends with the
+// row's best), then the four rows' results are read from lanes 0, 16, 32, 48.  The
+// selection is a total order, so the result does not depend on the reduction order.
+// sv / si: 4 entries in LDS.  Every thread of the block must call it.
 __device__ __forceinline__ void argmax_256(double& v, int& i, double* sv, int* si) {
+  argmax_dpp<0xB1>(v, i);   // quad_perm [1,0,3,2]
+  argmax_dpp<0x4E>(v, i);   // quad_perm [2,3,0,1]
+  argmax_dpp<0x141>(v, i);  // row_half_mirror
+  argmax_dpp<0x140>(v, i);  // row_mirror
+  const long long b = __double_as_longlong(v);
+  const int hi = (int)(b >> 32), lo = (int)b;
+  double wv = -1.0;
+  int wi = -1;
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    const double ov = __shfl_xor(v, o);
-    const int oi = __shfl_xor(i, o);
-    argmax_take(v, i, ov, oi);
+  for (int r = 0; r < 4; ++r) {
+    const int rhi = __builtin_amdgcn_readlane(hi, 16 * r), rlo = __builtin_amdgcn_readlane(lo, 16 * r);
+    argmax_take(wv, wi, __longlong_as_double(((long long)rhi << 32) | (unsigned)rlo),
+                __builtin_amdgcn_readlane(i, 16 * r));
   }
   if ((threadIdx.x & 63) == 0) {
-    sv[threadIdx.x >> 6] = v;
-    si[threadIdx.x >> 6] = i;
+    sv[threadIdx.x >> 6] = wv;
+    si[threadIdx.x >> 6] = wi;
   }
   __syncthreads();
   v = -1.0;
@@ -561,14 +585,16 @@ __global__ __launch_bounds__(FFP_THREADS) void ffp_kernel(T* __restrict__ A, int
     L[r * LD + c] = A[(int64_t)(r0 + r) + (int64_t)(c0 + c) * lda];
   }
   __syncthreads();
+  // this thread's row (one per thread: the host launches at most FFP_THREADS rows)
+  const int r = tid;
+  const bool own = r < nr;
+  // local candidate of the first column; later columns' come out of the update
+  double v = own ? fabs((double)L[r * LD]) : -1.0;
+  int id = own ? r0 + r : -1;
   bool live = true;  // uniform: false once a barrier was abandoned
   for (int jj = 0; jj < w; ++jj) {
     const int j = c0 + jj;
-    // 1. local candidate among the owned logical rows >= j, then publish
-    double v = -1.0;
-    int id = -1;
-    for (int r = tid; r < nr; r += FFP_THREADS)
-      if (r0 + r >= j) argmax_take(v, id, fabs((double)L[r * LD + jj]), r0 + r);
+    // 1. the workgroup's candidate among its logical rows >= j, then publish
     argmax_256(v, id, sv, si);
     const size_t slot = (size_t)jj * G + g;
     if (tid == 0) {
@@ -604,10 +630,12 @@ __global__ __launch_bounds__(FFP_THREADS) void ffp_kernel(T* __restrict__ A, int
       live = false;
       break;
     }
-    // 3. all candidates and row j in one round trip, then the pivot
-    for (int e = tid; e < G * w; e += FFP_THREADS) {
-      const int q = e / w, c = e - q * w;
-      cand[q * W + c] = crow[((size_t)jj * G + q) * W + c];
+    // 3. all candidates and row j in one round trip (the G x W block of this column
+    //    is contiguous; entries c >= w of a row are never read), then the pivot
+#pragma unroll
+    for (int u = 0; u < (FFP_MAX_GRID * W) / FFP_THREADS; ++u) {
+      const int e = u * FFP_THREADS + tid;
+      if (e < G * W) cand[e] = crow[(size_t)jj * G * W + e];
     }
     if (tid < G) {
       candi[tid] = cidx[(size_t)jj * G + tid];
@@ -639,23 +667,17 @@ __global__ __launch_bounds__(FFP_THREADS) void ffp_kernel(T* __restrict__ A, int
       if (p >= r0 && p < r0 + nr) L[(p - r0) * LD + tid] = jrow[tid];
     }
     __syncthreads();
-    // the multipliers of the owned rows below j, then the rank-1 update in two halves
-    // of 16 columns with that half of the pivot row in registers (constant indices):
-    // every LDS read of a row is independent of its writes, and at most ~90 VGPRs
-    // are live, so a workgroup fits beside a GEMM workgroup on one CU
-    constexpr int RPT = 1;  // rows per thread (the host launches at most FFP_THREADS rows per workgroup)
+    // the multiplier of this thread's row (if below j), then the rank-1 update in
+    // two halves of 16 columns with that half of the pivot row in registers
+    // (constant indices): every LDS read of the row is independent of its writes,
+    // and ~90 VGPRs are live, so a workgroup fits beside a GEMM workgroup on one CU
+    const bool below = own && r0 + r > j;
     const T piv = urow[jj];
-    T lq[RPT];
-#pragma unroll
-    for (int q = 0; q < RPT; ++q) {
-      const int r = tid + q * FFP_THREADS;
-      lq[q] = T(0);
-      if (r < nr && r0 + r > j) {
-        T l = L[r * LD + jj];
-        if (piv != T(0)) l = l / piv;
-        L[r * LD + jj] = l;
-        lq[q] = l;
-      }
+    T l = T(0);
+    if (below) {
+      l = L[r * LD + jj];
+      if (piv != T(0)) l = l / piv;
+      L[r * LD + jj] = l;
     }
 #pragma unroll
     for (int h = 0; h < W; h += 16) {
@@ -663,22 +685,25 @@ __global__ __launch_bounds__(FFP_THREADS) void ffp_kernel(T* __restrict__ A, int
         T uh[16];
 #pragma unroll
         for (int c = 0; c < 16; ++c) uh[c] = urow[h + c];
+        if (below) {
+          T* row = L + r * LD + h;
+          T x[16];
 #pragma unroll
-        for (int q = 0; q < RPT; ++q) {
-          const int r = tid + q * FFP_THREADS;
-          if (r < nr && r0 + r > j) {
-            T* row = L + r * LD + h;
-            T x[16];
+          for (int c = 0; c < 16; ++c) x[c] = row[c];
 #pragma unroll
-            for (int c = 0; c < 16; ++c) x[c] = row[c];
-#pragma unroll
-            for (int c = 0; c < 16; ++c)
-              if (h + c > jj && h + c < w) row[c] = x[c] - lq[q] * uh[c];
-          }
+          for (int c = 0; c < 16; ++c)
+            if (h + c > jj && h + c < w) row[c] = x[c] - l * uh[c];
         }
       }
     }
-    __syncthreads();  // cand / jrow / L are rewritten in the next column
+    // this thread's candidate for the next column (its own row, just written); the
+    // next argmax_256 synchronises before any other thread reads the row
+    v = -1.0;
+    id = -1;
+    if (below && jj + 1 < w) {
+      v = fabs((double)L[r * LD + jj + 1]);
+      id = r0 + r;
+    }
   }
   if (!live) return;
   for (int e = tid; e < nr * w; e += FFP_THREADS) {

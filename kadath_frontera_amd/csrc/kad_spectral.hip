@@ -18,50 +18,74 @@
 
 namespace kad {
 
-// One pass along axis 0 (r) or 2 (phi) with NIN input points per line: each thread
-// owns one output index o and a strided set of lines; its matrix row stays in
-// registers while consecutive lines use the same matrix (the usual case along r and
-// phi), and the threads of one line read the same inputs (LDS broadcast).
+// One pass along any axis with NIN (12..17) input points per line, one line per
+// thread: the thread reads its line into registers once and produces all of the
+// line's outputs; the line's matrix is read with wave-uniform (scalar) loads from
+// the read-only matrix pool in global memory, so each FMA takes one operand from
+// SGPRs and one from VGPRs and LDS carries only the line in and the outputs out.
+// Waves whose lines use different matrices (e.g. theta lines of even and odd phi
+// index) are served one matrix at a time (a waterfall over the distinct offsets).
 template <int NIN>
-__device__ __forceinline__ void pass_rows(const double* __restrict__ x, double* __restrict__ y,
-                                          const double* __restrict__ mats, const int* __restrict__ lm, int ax,
-                                          const int dims[3], const int od[3], int nout_ax) {
-  const int nout = od[0] * od[1] * od[2];
-  const int o = threadIdx.x % nout_ax;
-  const int per_sweep = blockDim.x / nout_ax;
-  const int l0 = threadIdx.x / nout_ax;
-  const int nline = nout / nout_ax;
-  const int stride = ax == 0 ? 1 : dims[0] * dims[1];
-  int cur_off = -2;
-  double m[NIN];
-  if (l0 >= per_sweep) return;
-  for (int li = l0; li < nline; li += per_sweep) {
-    int line, base_idx, e;
-    if (ax == 0) {
-      line = li;
-      base_idx = dims[0] * li;
-      e = o + od[0] * li;
-    } else {
-      const int i = li % od[0], j = li / od[0];
-      line = 0;
-      base_idx = i + dims[0] * j;
-      e = i + od[0] * (j + od[1] * o);
+__device__ __forceinline__ void pass_lines(const double* __restrict__ x, double* __restrict__ y,
+                                           const double* __restrict__ pool, const int* __restrict__ lm, int ax,
+                                           const int dims[3], const int od[3], int nout_ax) {
+  const int nl0 = ax == 0 ? od[1] : od[0];
+  const int nlines = ax == 0 ? od[1] * od[2] : (ax == 1 ? od[0] * od[2] : od[0] * od[1]);
+  const int sin_ = ax == 0 ? 1 : (ax == 1 ? dims[0] : dims[0] * dims[1]);
+  const int sout = ax == 0 ? 1 : (ax == 1 ? od[0] : od[0] * od[1]);
+  for (int li = threadIdx.x; li - (int)(threadIdx.x & 63) < nlines; li += blockDim.x) {
+    // every lane of a wave runs the same iterations (the vote below needs them all)
+    const bool act = li < nlines;
+    const int a = act ? li % nl0 : 0, bq = act ? li / nl0 : 0;
+    int xb, yb, lmi;
+    if (ax == 0) {         // line (j, k) = (a, bq)
+      xb = dims[0] * (a + dims[1] * bq);
+      yb = od[0] * (a + od[1] * bq);
+      lmi = a + od[1] * bq;
+    } else if (ax == 1) {  // line (i, k) = (a, bq)
+      xb = a + dims[0] * dims[1] * bq;
+      yb = a + od[0] * od[1] * bq;
+      lmi = bq;
+    } else {               // line (i, j) = (a, bq)
+      xb = a + dims[0] * bq;
+      yb = a + od[0] * bq;
+      lmi = 0;
     }
-    const int off = lm[line];
-    if (off != cur_off) {
-      cur_off = off;
+    const int off = act ? lm[lmi] : -1;
+    double xv[NIN];
 #pragma unroll
-      for (int q = 0; q < NIN; ++q) m[q] = off >= 0 ? mats[off + o * NIN + q] : 0.0;
-    }
-    const double* xv = x + base_idx;
-    double s0 = 0.0, s1 = 0.0;
+    for (int q = 0; q < NIN; ++q) xv[q] = act ? x[xb + q * sin_] : 0.0;
+    // waterfall over the distinct matrices of the wave, in uniform control flow (so
+    // the matrix address is uniform and its loads are scalar): every lane computes
+    // with the chosen matrix, the lanes whose line uses it store
+    bool todo = act;
+    while (true) {
+      const unsigned long long pend = __ballot(todo);
+      if (pend == 0) break;
+      const int off0 = __builtin_amdgcn_readlane(off, __ffsll((long long)pend) - 1);
+      const bool mine = todo && off == off0;
+      todo = todo && !mine;
+      if (off0 < 0) {
+        for (int o = 0; o < nout_ax; ++o)
+          if (mine) y[yb + o * sout] = 0.0;
+        continue;
+      }
+      // the pool is read-only for the whole kernel: read it through the constant
+      // address space, so the uniform address gives scalar loads
+      typedef const double __attribute__((address_space(4)))* cdptr;
+      const cdptr M = (cdptr)(pool + off0);
+#pragma unroll 2
+      for (int o = 0; o < nout_ax; ++o) {
+        double s0 = 0.0, s1 = 0.0;
 #pragma unroll
-    for (int q = 0; q + 1 < NIN; q += 2) {
-      s0 += m[q] * xv[q * stride];
-      s1 += m[q + 1] * xv[(q + 1) * stride];
+        for (int q = 0; q + 1 < NIN; q += 2) {
+          s0 += M[o * NIN + q] * xv[q];
+          s1 += M[o * NIN + q + 1] * xv[q + 1];
+        }
+        if (NIN % 2) s0 += M[o * NIN + NIN - 1] * xv[NIN - 1];
+        if (mine) y[yb + o * sout] = s0 + s1;
+      }
     }
-    if (NIN % 2) s0 += m[NIN - 1] * xv[(NIN - 1) * stride];
-    y[e] = s0 + s1;
   }
 }
 
@@ -73,13 +97,17 @@ __global__ __launch_bounds__(256) void apply3d_kernel(const double* __restrict__
                                                       int d2, Passes P, const double* __restrict__ pool,
                                                       int accumulate, double scale) {
   extern __shared__ double lds[];
-  double* buf[2] = {lds, lds + P.maxsize};
+  // the two ping-pong buffers are selected by value (not through an array of
+  // pointers), so every access stays an LDS access and the matrix-pool loads can be
+  // proven unclobbered (scalar loads)
+  double* const buf0 = lds;
+  double* const buf1 = lds + P.maxsize;
   double* mats = lds + 2 * P.maxsize;
   int* ltab = reinterpret_cast<int*>(mats + P.pool_size);
   const int64_t b = blockIdx.x;
   const int nin = d0 * d1 * d2;
   const double* src = in + b * in_stride;
-  for (int e = threadIdx.x; e < nin; e += blockDim.x) buf[0][e] = src[e];
+  for (int e = threadIdx.x; e < nin; e += blockDim.x) buf0[e] = src[e];
   for (int e = threadIdx.x; e < P.pool_size; e += blockDim.x) mats[e] = pool[e];
   {
     int o = 0;
@@ -99,18 +127,18 @@ __global__ __launch_bounds__(256) void apply3d_kernel(const double* __restrict__
     int od[3] = {dims[0], dims[1], dims[2]};
     od[ax] = nout_ax;
     const int nout = od[0] * od[1] * od[2];
-    const double* x = buf[cur];
-    double* y = buf[cur ^ 1];
+    const double* x = cur ? buf1 : buf0;
+    double* y = cur ? buf0 : buf1;
     const int* lm = ltab + loff;
     const int stride = ax == 0 ? 1 : (ax == 1 ? dims[0] : dims[0] * dims[1]);
-    if (ax != 1 && nin_ax >= 12 && nin_ax <= 17) {
+    if (nin_ax >= 12 && nin_ax <= 17) {
       switch (nin_ax) {
-        case 12: pass_rows<12>(x, y, mats, lm, ax, dims, od, nout_ax); break;
-        case 13: pass_rows<13>(x, y, mats, lm, ax, dims, od, nout_ax); break;
-        case 14: pass_rows<14>(x, y, mats, lm, ax, dims, od, nout_ax); break;
-        case 15: pass_rows<15>(x, y, mats, lm, ax, dims, od, nout_ax); break;
-        case 16: pass_rows<16>(x, y, mats, lm, ax, dims, od, nout_ax); break;
-        default: pass_rows<17>(x, y, mats, lm, ax, dims, od, nout_ax); break;
+        case 12: pass_lines<12>(x, y, pool, lm, ax, dims, od, nout_ax); break;
+        case 13: pass_lines<13>(x, y, pool, lm, ax, dims, od, nout_ax); break;
+        case 14: pass_lines<14>(x, y, pool, lm, ax, dims, od, nout_ax); break;
+        case 15: pass_lines<15>(x, y, pool, lm, ax, dims, od, nout_ax); break;
+        case 16: pass_lines<16>(x, y, pool, lm, ax, dims, od, nout_ax); break;
+        default: pass_lines<17>(x, y, pool, lm, ax, dims, od, nout_ax); break;
       }
       loff += P.nlines[p];
       dims[0] = od[0];
@@ -164,10 +192,11 @@ __global__ __launch_bounds__(256) void apply3d_kernel(const double* __restrict__
   }
   const int nfin = dims[0] * dims[1] * dims[2];
   double* dst = out + b * out_stride;
+  const double* fin = cur ? buf1 : buf0;
   if (accumulate)
-    for (int e = threadIdx.x; e < nfin; e += blockDim.x) dst[e] += scale * buf[cur][e];
+    for (int e = threadIdx.x; e < nfin; e += blockDim.x) dst[e] += scale * fin[e];
   else
-    for (int e = threadIdx.x; e < nfin; e += blockDim.x) dst[e] = scale * buf[cur][e];
+    for (int e = threadIdx.x; e < nfin; e += blockDim.x) dst[e] = scale * fin[e];
 }
 
 // Contiguous 1-D lines: in [L][nin] -> out [L][nout], one matrix for all lines.
