@@ -226,11 +226,49 @@ __global__ __launch_bounds__(256) void col_tile_mask_kernel(const T* __restrict_
 // leaf's condition number.
 constexpr int TRSM_LEAF = 64;
 
+// L X = B in place for a unit-lower L of order 32 and B of 32 rows: one lane per
+// column of B, 64 columns per wave.  The lane keeps the rows not yet final in a
+// window y[0..31] of registers: at step k, y[0] = B(k) is final (stored), and every
+// other entry is updated and shifted down, y[i-1] = y[i] - L(k+i, k) y[0] -- the
+// indices into y are constants, so the step loop needs no unrolling and no dynamic
+// register indexing; entries past the rows still open compute values that are
+// never used.  L is staged in LDS once and read with broadcast loads.  The order of
+// operations on every B(i) is the column-oriented substitution of LAPACK's xTRSM
+// ('L', 'L', 'N', 'U').  cmask (optional): 128-column tiles of B that are
+// structurally zero stay untouched.
+constexpr int TRSM_FUSED = 32;
+
+template <typename T>
+__global__ __launch_bounds__(64, 4) void trsm_leaf_kernel(const T* __restrict__ L, int64_t lda, T* __restrict__ B,
+                                                       int64_t ldb, int ncols, const int* __restrict__ cmask) {
+  constexpr int R = TRSM_FUSED;
+  __shared__ T Ls[R * R + R];  // column-major, Ls[i + R k] = L(i, k); R entries of padding read past the end
+  const int c0 = blockIdx.x * 64;
+  if (cmask && !cmask[c0 / 128]) return;  // uniform: the block lies in one 128-column tile
+  for (int e = threadIdx.x; e < R * R; e += 64) Ls[e] = L[(int64_t)(e % R) + (int64_t)(e / R) * lda];
+  if (threadIdx.x < R) Ls[R * R + threadIdx.x] = T(0);
+  const int c = c0 + threadIdx.x;
+  const bool act = c < ncols;
+  T* b = B + (int64_t)(act ? c : c0) * ldb;
+  T y[R];
+#pragma unroll
+  for (int i = 0; i < R; ++i) y[i] = act ? b[i] : T(0);
+  __syncthreads();
+#pragma unroll 1
+  for (int k = 0; k < R; ++k) {
+    const T xk = y[0];
+    if (act) b[k] = xk;
+    const T* lk = Ls + k + R * k;  // lk[i] = L(k + i, k); i + k < 2 R stays inside Ls
+#pragma unroll
+    for (int i = 1; i < R; ++i) y[i - 1] = y[i] - lk[i] * xk;
+  }
+}
+
 template <typename T>
 int trsm_left_unit_lower(const T* L, int64_t lda, int r, T* A12, int64_t ld12, int ncols, LuWork& w, int gemm_prec,
                          const int* cmask = nullptr) {
   if (ncols <= 0 || r <= 0) return KAD_OK;
-  if (r > TRSM_LEAF) {
+  if (r > TRSM_LEAF || r == 2 * TRSM_FUSED) {
     const int h = std::max(32, (r / 2) / 32 * 32);
     int rc = trsm_left_unit_lower<T>(L, lda, h, A12, ld12, ncols, w, gemm_prec, cmask);
     if (rc) return rc;
@@ -238,6 +276,11 @@ int trsm_left_unit_lower(const T* L, int64_t lda, int r, T* A12, int64_t ld12, i
                             A12 + h, ld12, nullptr, cmask, w.stream);
     if (rc) return rc;
     return trsm_left_unit_lower<T>(L + h + (int64_t)h * lda, lda, r - h, A12 + h, ld12, ncols, w, gemm_prec, cmask);
+  }
+  if (r == TRSM_FUSED) {
+    trsm_leaf_kernel<T><<<(unsigned)cdiv(ncols, 64), 64, 0, w.stream>>>(L, lda, A12, ld12, ncols, cmask);
+    KAD_HIP_CHECK(hipGetLastError());
+    return KAD_OK;
   }
   T* linv = (T*)w.tinv;
   // Column chunks so that the scratch holds r x chunk; with more than one chunk they
