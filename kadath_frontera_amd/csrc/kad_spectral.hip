@@ -95,7 +95,7 @@ __device__ __forceinline__ void pass_lines(const double* __restrict__ x, double*
 __global__ __launch_bounds__(256) void apply3d_kernel(const double* __restrict__ in, int64_t in_stride,
                                                       double* __restrict__ out, int64_t out_stride, int d0, int d1,
                                                       int d2, Passes P, const double* __restrict__ pool,
-                                                      int accumulate, double scale) {
+                                                      int accumulate, double scale, int stage_pool) {
   extern __shared__ double lds[];
   // the two ping-pong buffers are selected by value (not through an array of
   // pointers), so every access stays an LDS access and the matrix-pool loads can be
@@ -103,12 +103,13 @@ __global__ __launch_bounds__(256) void apply3d_kernel(const double* __restrict__
   double* const buf0 = lds;
   double* const buf1 = lds + P.maxsize;
   double* mats = lds + 2 * P.maxsize;
-  int* ltab = reinterpret_cast<int*>(mats + P.pool_size);
+  int* ltab = reinterpret_cast<int*>(mats + (stage_pool ? P.pool_size : 0));
   const int64_t b = blockIdx.x;
   const int nin = d0 * d1 * d2;
   const double* src = in + b * in_stride;
   for (int e = threadIdx.x; e < nin; e += blockDim.x) buf0[e] = src[e];
-  for (int e = threadIdx.x; e < P.pool_size; e += blockDim.x) mats[e] = pool[e];
+  if (stage_pool)  // only passes outside 12..17 points per line read the matrices from LDS
+    for (int e = threadIdx.x; e < P.pool_size; e += blockDim.x) mats[e] = pool[e];
   {
     int o = 0;
     for (int p = 0; p < P.npass; ++p) {
@@ -229,7 +230,19 @@ int apply3d_launch(const double* in, int64_t in_stride, double* out, int64_t out
   if (nbatch <= 0) return KAD_OK;
   size_t nl = 0;
   for (int p = 0; p < P.npass; ++p) nl += (size_t)P.nlines[p];
-  const size_t lds = (2 * (size_t)P.maxsize + (size_t)P.pool_size) * sizeof(double) + nl * sizeof(int);
+  // the matrix pool is staged in LDS only if some pass has a line length outside
+  // 12..17 (the other passes read it with scalar loads)
+  int stage_pool = 0;
+  {
+    int dm[3] = {d0, d1, d2};
+    for (int p = 0; p < P.npass; ++p) {
+      const int nin_ax = dm[P.axis[p]];
+      if (nin_ax < 12 || nin_ax > 17) stage_pool = 1;
+      dm[P.axis[p]] = P.nout[p];
+    }
+  }
+  const size_t lds =
+      (2 * (size_t)P.maxsize + (stage_pool ? (size_t)P.pool_size : 0)) * sizeof(double) + nl * sizeof(int);
   if (lds > 160 * 1024) return KAD_ERR_ARG;
   GemmTimer* tm = g_apply_timer;
   bool timed = false;
@@ -243,7 +256,8 @@ int apply3d_launch(const double* in, int64_t in_stride, double* out, int64_t out
     if (rc < 0) return rc;
     timed = rc == 0;
   }
-  apply3d_kernel<<<nbatch, 256, lds, s>>>(in, in_stride, out, out_stride, d0, d1, d2, P, pool, accumulate, scale);
+  apply3d_kernel<<<nbatch, 256, lds, s>>>(in, in_stride, out, out_stride, d0, d1, d2, P, pool, accumulate, scale,
+                                          stage_pool);
   KAD_HIP_CHECK(hipGetLastError());
   if (timed && tm->end(s)) return KAD_ERR_HIP;
   return KAD_OK;

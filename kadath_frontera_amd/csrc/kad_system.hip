@@ -33,26 +33,31 @@ struct DevRow {
 };
 
 // dst[b][e] = op(a[b or 0][e], b[b or 0][e], s)
+// Element-wise instruction over a batch: blockIdx.y walks the batch rows (a stride of
+// 0 broadcasts one array to every row), blockIdx.x / threadIdx.x the elements.
 __global__ void ew_kernel(int kind, double* __restrict__ dst, int64_t dstride, const double* __restrict__ a,
                           int64_t astride, const double* __restrict__ bsrc, int64_t bstride, double s, int size,
                           int nbatch) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)size * nbatch) return;
-  const int64_t bb = t / size;
-  const int e = (int)(t % size);
-  const double x = a[bb * astride + e];
-  double r;
-  switch (kind) {
-    case EW_ADD: r = x + bsrc[bb * bstride + e]; break;
-    case EW_SUB: r = x - bsrc[bb * bstride + e]; break;
-    case EW_MUL: r = x * bsrc[bb * bstride + e]; break;
-    case EW_DIV: r = x / bsrc[bb * bstride + e]; break;
-    case EW_SCAL: r = x * s; break;
-    case EW_SDIV: r = x / s; break;
-    case EW_ADDS: r = x + s; break;
-    default: r = x; break;
+  for (int bb = blockIdx.y; bb < nbatch; bb += gridDim.y) {
+    const double* ar = a + (int64_t)bb * astride;
+    const double* br = bsrc ? bsrc + (int64_t)bb * bstride : nullptr;
+    double* dr = dst + (int64_t)bb * dstride;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < size; e += gridDim.x * blockDim.x) {
+      const double x = ar[e];
+      double r;
+      switch (kind) {
+        case EW_ADD: r = x + br[e]; break;
+        case EW_SUB: r = x - br[e]; break;
+        case EW_MUL: r = x * br[e]; break;
+        case EW_DIV: r = x / br[e]; break;
+        case EW_SCAL: r = x * s; break;
+        case EW_SDIV: r = x / s; break;
+        case EW_ADDS: r = x + s; break;
+        default: r = x; break;
+      }
+      dr[e] = r;
+    }
   }
-  dst[bb * dstride + e] = r;
 }
 
 // Inside rows: out[(row0 + r) + (col0 + b) ldo] (+)= sign (src[pos] - gw src[gpos])
@@ -419,8 +424,8 @@ int kad_system_impl::run(const std::vector<Instr>& code, int nb, int u0, int see
       }
       case I_EW: {
         const int batch = (in.dst.space == SP_VAL) ? 1 : nb;
-        const int64_t tot = (int64_t)in.size * batch;
-        ew_kernel<<<(unsigned)kad::cdiv(tot, 256), 256, 0, s>>>(
+        const dim3 grid((unsigned)kad::cdiv(in.size, 256), (unsigned)std::min(batch, 65535));
+        ew_kernel<<<grid, 256, 0, s>>>(
             in.ew, slot_ptr(in.dst), slot_stride(in.dst), slot_ptr(in.a), slot_stride(in.a),
             in.b.slot >= 0 ? slot_ptr(in.b) : nullptr, in.b.slot >= 0 ? slot_stride(in.b) : 0, in.s, in.size, batch);
         break;
