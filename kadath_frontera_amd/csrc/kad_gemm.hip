@@ -45,6 +45,9 @@ __device__ inline void tile_coords(int bid, int nbm, int nbn, int& tm, int& tn) 
 
 // Tile of this workgroup under a TileSel; false: nothing to compute (the caller
 // zeroes the tile for C = AB in mask mode, returns otherwise).
+// Masks and lists are per 128-row / 128-column tile; a kernel with BM-wide tiles
+// (BM = 64 or 128) maps its tile to the 128-tile containing it (lists: BM = 128 only).
+template <int BM = GBM>
 __device__ inline bool select_tile(const TileSel& sel, int nbm, int nbn, int& tm, int& tn, bool& zero_fill) {
   zero_fill = false;
   if (sel.cnt) {
@@ -57,16 +60,16 @@ __device__ inline bool select_tile(const TileSel& sel, int nbm, int nbn, int& tm
     return true;
   }
   tile_coords(blockIdx.x, nbm, nbn, tm, tn);
-  if ((sel.rmask && sel.rmask[tm] == 0) || (sel.cmask && sel.cmask[tn] == 0)) {
+  if ((sel.rmask && sel.rmask[tm * BM / 128] == 0) || (sel.cmask && sel.cmask[tn * BM / 128] == 0)) {
     zero_fill = true;
     return false;
   }
   return true;
 }
-template <typename T>
+template <typename T, int BM = GBM>
 __device__ inline void zero_tile(T* C, int64_t ldc, int m, int n, int row0, int col0, int tid) {
-  for (int e = tid; e < GBM * GBN; e += GNT) {
-    const int r = row0 + e % GBM, c = col0 + e / GBM;
+  for (int e = tid; e < BM * BM; e += GNT) {
+    const int r = row0 + e % BM, c = col0 + e / BM;
     if (r < m && c < n) C[r + (int64_t)c * ldc] = T(0);
   }
 }
@@ -74,7 +77,10 @@ __device__ inline void zero_tile(T* C, int64_t ldc, int m, int n, int row0, int 
 // ---------------------------------------------------------------------------
 // f64 / f32 path: 16x16x4 MFMA, LDS tiles stored k-major: As[k][m], Bs[k][n].
 // ---------------------------------------------------------------------------
-template <int P, int OP>
+// BM x BM tile per workgroup (BM = 128: the bulk updates; BM = 64: small and narrow
+// products -- 4x the workgroups, and at most ~100 VGPRs and 35 KB of LDS, so they run
+// beside the bulk update's 128-tiles on the same CUs instead of queueing behind them).
+template <int P, int OP, int BM = GBM>
 __global__ __launch_bounds__(GNT) void gemm_k4_kernel(int m, int n, int k,
                                                       const typename GemmCfg<P>::T* __restrict__ A, int64_t lda,
                                                       const typename GemmCfg<P>::T* __restrict__ B, int64_t ldb,
@@ -83,38 +89,39 @@ __global__ __launch_bounds__(GNT) void gemm_k4_kernel(int m, int n, int k,
   using T = typename GemmCfg<P>::T;
   constexpr int BK = GemmCfg<P>::BK;
   constexpr int PADA = (sizeof(T) == 8) ? 4 : 8;
-  constexpr int LDA_S = GBM + PADA, LDB_S = GBN + PADA;
-  constexpr int NA = GBM * BK / GNT;  // elements of A per thread per stage
-  constexpr int NB = GBN * BK / GNT;
+  constexpr int LDA_S = BM + PADA, LDB_S = BM + PADA;
+  constexpr int NA = BM * BK / GNT;  // elements of A per thread per stage
+  constexpr int NB = BM * BK / GNT;
+  constexpr int WT = BM / 2, TI = WT / 16;  // wave tile and its 16x16 MFMA tiles per side
   using acc_t = typename std::conditional<sizeof(T) == 8, d4v, f4v>::type;
 
   __shared__ T As[2][BK * LDA_S];
   __shared__ T Bs[2][BK * LDB_S];
 
-  const int nbm = (int)cdiv(m, GBM), nbn = (int)cdiv(n, GBN);
+  const int nbm = (int)cdiv(m, BM), nbn = (int)cdiv(n, BM);
   int tm = 0, tn = 0;
   bool zero_fill;
-  const bool active = select_tile(sel, nbm, nbn, tm, tn, zero_fill);
-  const int row0 = tm * GBM, col0 = tn * GBN;
+  const bool active = select_tile<BM>(sel, nbm, nbn, tm, tn, zero_fill);
+  const int row0 = tm * BM, col0 = tn * BM;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+  const int wm = (wave & 1) * WT, wn = (wave >> 1) * WT;
   if (!active) {
-    if (OP == OP_SET && zero_fill) zero_tile(C, ldc, m, n, row0, col0, tid);
+    if (OP == OP_SET && zero_fill) zero_tile<T, BM>(C, ldc, m, n, row0, col0, tid);
     return;
   }
 
-  acc_t acc[4][4];
+  acc_t acc[TI][TI];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < TI; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+    for (int j = 0; j < TI; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
   T ra[NA], rb[NB];
   auto gload = [&](int k0) {
 #pragma unroll
     for (int e = 0; e < NA; ++e) {
       const int idx = tid + e * GNT;
-      const int r = idx % GBM, kk = idx / GBM;
+      const int r = idx % BM, kk = idx / BM;
       const int gr = row0 + r, gk = k0 + kk;
       ra[e] = (gr < m && gk < k) ? A[gr + (int64_t)gk * lda] : T(0);
     }
@@ -130,7 +137,7 @@ __global__ __launch_bounds__(GNT) void gemm_k4_kernel(int m, int n, int k,
 #pragma unroll
     for (int e = 0; e < NA; ++e) {
       const int idx = tid + e * GNT;
-      As[buf][(idx / GBM) * LDA_S + (idx % GBM)] = ra[e];
+      As[buf][(idx / BM) * LDA_S + (idx % BM)] = ra[e];
     }
 #pragma unroll
     for (int e = 0; e < NB; ++e) {
@@ -151,16 +158,16 @@ __global__ __launch_bounds__(GNT) void gemm_k4_kernel(int m, int n, int k,
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       const int kr = kk + (lane >> 4);
-      T af[4], bf[4];
+      T af[TI], bf[TI];
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
+      for (int t = 0; t < TI; ++t) {
         af[t] = as[kr * LDA_S + wm + t * 16 + (lane & 15)];
         bf[t] = bs[kr * LDB_S + wn + t * 16 + (lane & 15)];
       }
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < TI; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < TI; ++j) {
           if constexpr (sizeof(T) == 8)
             acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j], af[i], acc[i][j], 0, 0, 0);
           else
@@ -177,11 +184,11 @@ __global__ __launch_bounds__(GNT) void gemm_k4_kernel(int m, int n, int k,
   // f32 16x16x4 accumulators are D'[4*(lane>>4)+r][lane&15], f64 16x16x4 ones
   // D'[4*r+(lane>>4)][lane&15] (measured lane layouts of the two instructions).
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < TI; ++i) {
     const int gr = row0 + wm + i * 16 + (lane & 15);
     if (gr >= m) continue;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < TI; ++j) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int gc = col0 + wn + j * 16 + (sizeof(T) == 8 ? 4 * r + (lane >> 4) : 4 * (lane >> 4) + r);
@@ -358,7 +365,9 @@ int gemm_launch_sel(int prec, int op, int m, int n, int k, const void* A, int64_
     return KAD_OK;
   }
   const int nblocks = (int)(cdiv(m, GBM) * cdiv(n, GBN));
-  dim3 grid(nblocks), block(GNT);
+  // fewer 128-tiles than two per CU (and not a compacted tile list): 64-tiles
+  const bool small = !sel.cnt && prec != BF16 && nblocks < 512;
+  dim3 grid(small ? (unsigned)(cdiv(m, 64) * cdiv(n, 64)) : (unsigned)nblocks), block(GNT);
   GemmTimer* tm = g_gemm_timer;
   bool timed = false;
   if (tm) {
@@ -373,19 +382,33 @@ int gemm_launch_sel(int prec, int op, int m, int n, int k, const void* A, int64_
     timed = rc == 0;
   }
   if (prec == F64) {
-    if (op == OP_SUB)
-      gemm_k4_kernel<F64, OP_SUB><<<grid, block, 0, s>>>(m, n, k, (const double*)A, lda, (const double*)B, ldb,
-                                                          (double*)C, ldc, sel);
-    else
-      gemm_k4_kernel<F64, OP_SET><<<grid, block, 0, s>>>(m, n, k, (const double*)A, lda, (const double*)B, ldb,
-                                                          (double*)C, ldc, sel);
+    const double *a = (const double*)A, *b = (const double*)B;
+    double* c = (double*)C;
+    if (op == OP_SUB) {
+      if (small)
+        gemm_k4_kernel<F64, OP_SUB, 64><<<grid, block, 0, s>>>(m, n, k, a, lda, b, ldb, c, ldc, sel);
+      else
+        gemm_k4_kernel<F64, OP_SUB><<<grid, block, 0, s>>>(m, n, k, a, lda, b, ldb, c, ldc, sel);
+    } else {
+      if (small)
+        gemm_k4_kernel<F64, OP_SET, 64><<<grid, block, 0, s>>>(m, n, k, a, lda, b, ldb, c, ldc, sel);
+      else
+        gemm_k4_kernel<F64, OP_SET><<<grid, block, 0, s>>>(m, n, k, a, lda, b, ldb, c, ldc, sel);
+    }
   } else if (prec == F32) {
-    if (op == OP_SUB)
-      gemm_k4_kernel<F32, OP_SUB><<<grid, block, 0, s>>>(m, n, k, (const float*)A, lda, (const float*)B, ldb,
-                                                          (float*)C, ldc, sel);
-    else
-      gemm_k4_kernel<F32, OP_SET><<<grid, block, 0, s>>>(m, n, k, (const float*)A, lda, (const float*)B, ldb,
-                                                          (float*)C, ldc, sel);
+    const float *a = (const float*)A, *b = (const float*)B;
+    float* c = (float*)C;
+    if (op == OP_SUB) {
+      if (small)
+        gemm_k4_kernel<F32, OP_SUB, 64><<<grid, block, 0, s>>>(m, n, k, a, lda, b, ldb, c, ldc, sel);
+      else
+        gemm_k4_kernel<F32, OP_SUB><<<grid, block, 0, s>>>(m, n, k, a, lda, b, ldb, c, ldc, sel);
+    } else {
+      if (small)
+        gemm_k4_kernel<F32, OP_SET, 64><<<grid, block, 0, s>>>(m, n, k, a, lda, b, ldb, c, ldc, sel);
+      else
+        gemm_k4_kernel<F32, OP_SET><<<grid, block, 0, s>>>(m, n, k, a, lda, b, ldb, c, ldc, sel);
+    }
   } else if (prec == BF16) {
     if (op == OP_SUB)
       gemm_bf16_kernel<OP_SUB><<<grid, block, 0, s>>>(m, n, k, (const float*)A, lda, (const float*)B, ldb, (float*)C,
