@@ -164,7 +164,12 @@ int trtri_rec(const T* L, int64_t lda, int nb, T* Linv, int64_t ldo, LuWork& w) 
 // tile holds any entry != 0 (NaN counts as nonzero), else 0.
 template <typename T>
 __global__ __launch_bounds__(256) void row_tile_mask_kernel(const T* __restrict__ A, int64_t lda, int nrows,
-                                                            int ncols, int* __restrict__ mask) {
+                                                            int ncols, int* __restrict__ mask, int nzrows = -1) {
+  // rows at or past nzrows (when given) are known to be zero: those tiles are not read
+  if (nzrows >= 0 && (int)blockIdx.x * 128 >= nzrows) {
+    if (threadIdx.x == 0) mask[blockIdx.x] = 0;
+    return;
+  }
   const int r = blockIdx.x * 128 + (threadIdx.x & 127);
   int any = 0;
   if (r < nrows)
@@ -317,19 +322,22 @@ __global__ __launch_bounds__(256) void laswp_lds_kernel(T* __restrict__ A, int64
   __shared__ double vals[2 * NSW * COLS];  // [slot][column]
   __shared__ int slot_of[NSW];
   __shared__ int prow[NSW];                // row of slot nsw + j, or -1
+  __shared__ int piv[NSW];                 // ipiv[k1 .. k2), staged once (the searches below read it)
   const int nsw = k2 - k1;                 // <= NSW (checked by the launcher)
   const int t = threadIdx.x;
   const int cb = c0 + blockIdx.x * COLS;
   const int nc = min(COLS, c1 - cb);
+  for (int j = t; j < nsw; j += 256) piv[j] = ipiv[k1 + j];
+  __syncthreads();
   for (int j = t; j < nsw; j += 256) {
-    const int p = ipiv[k1 + j];
+    const int p = piv[j];
     if (p < k2) {
       slot_of[j] = p - k1;
       prow[j] = -1;
     } else {
       int first = j;
       for (int q = 0; q < j; ++q)
-        if (ipiv[k1 + q] == p) {
+        if (piv[q] == p) {
           first = q;
           break;
         }
@@ -374,10 +382,12 @@ template <typename T>
 int laswp(T* A, int64_t lda, int c0, int c1, const int* ipiv, int k1, int k2, LuWork& w) {
   if (c1 <= c0 || k2 <= k1) return KAD_OK;
   const int nsw = k2 - k1;
+  // 8 (4) columns per workgroup: ~33 KB of LDS, so the workgroups fit beside the
+  // concurrent update's GEMM workgroups instead of waiting for a whole CU
   if (nsw <= 256) {
-    laswp_lds_kernel<T, 256, 32><<<(unsigned)cdiv(c1 - c0, 32), 256, 0, w.stream>>>(A, lda, c0, c1, ipiv, k1, k2);
+    laswp_lds_kernel<T, 256, 8><<<(unsigned)cdiv(c1 - c0, 8), 256, 0, w.stream>>>(A, lda, c0, c1, ipiv, k1, k2);
   } else if (nsw <= 512) {
-    laswp_lds_kernel<T, 512, 16><<<(unsigned)cdiv(c1 - c0, 16), 256, 0, w.stream>>>(A, lda, c0, c1, ipiv, k1, k2);
+    laswp_lds_kernel<T, 512, 4><<<(unsigned)cdiv(c1 - c0, 4), 256, 0, w.stream>>>(A, lda, c0, c1, ipiv, k1, k2);
   } else {
     laswp_kernel<T><<<(unsigned)cdiv(c1 - c0, 256), 256, 0, w.stream>>>(A, lda, c0, c1, ipiv, k1, k2);
   }
@@ -1027,8 +1037,9 @@ int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w)
     int rc = panel_rec<T>(A, lda, R, k0, k1, ipiv, wp, k0, k1, fused);
     if (rc) return rc;
     if (k1 < n) {
+      // the panel's rows at or below R are zero in its columns
       row_tile_mask_kernel<T><<<(unsigned)cdiv(n - k1, 128), 256, 0, wp.stream>>>(A + k1 + (int64_t)k0 * lda, lda,
-                                                                                 n - k1, k1 - k0, mask);
+                                                                                 n - k1, k1 - k0, mask, R - k1);
       KAD_HIP_CHECK(hipGetLastError());
     }
     return KAD_OK;
