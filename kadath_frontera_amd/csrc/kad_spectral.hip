@@ -107,7 +107,22 @@ __global__ __launch_bounds__(256) void apply3d_kernel(const double* __restrict__
   const int64_t b = blockIdx.x;
   const int nin = d0 * d1 * d2;
   const double* src = in + b * in_stride;
-  for (int e = threadIdx.x; e < nin; e += blockDim.x) buf0[e] = src[e];
+  // 8 loads in flight per thread (one load per iteration would wait out the HBM
+  // latency ~10 times per array)
+  constexpr int UL = 8;
+  for (int e0 = 0; e0 < nin; e0 += UL * 256) {
+    double v[UL];
+#pragma unroll
+    for (int u = 0; u < UL; ++u) {
+      const int e = e0 + u * 256 + (int)threadIdx.x;
+      v[u] = e < nin ? src[e] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < UL; ++u) {
+      const int e = e0 + u * 256 + (int)threadIdx.x;
+      if (e < nin) buf0[e] = v[u];
+    }
+  }
   if (stage_pool)  // only passes outside 12..17 points per line read the matrices from LDS
     for (int e = threadIdx.x; e < P.pool_size; e += blockDim.x) mats[e] = pool[e];
   {
@@ -194,10 +209,23 @@ __global__ __launch_bounds__(256) void apply3d_kernel(const double* __restrict__
   const int nfin = dims[0] * dims[1] * dims[2];
   double* dst = out + b * out_stride;
   const double* fin = cur ? buf1 : buf0;
-  if (accumulate)
-    for (int e = threadIdx.x; e < nfin; e += blockDim.x) dst[e] += scale * fin[e];
-  else
+  if (accumulate) {
+    for (int e0 = 0; e0 < nfin; e0 += UL * 256) {
+      double v[UL];
+#pragma unroll
+      for (int u = 0; u < UL; ++u) {
+        const int e = e0 + u * 256 + (int)threadIdx.x;
+        v[u] = e < nfin ? dst[e] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < UL; ++u) {
+        const int e = e0 + u * 256 + (int)threadIdx.x;
+        if (e < nfin) dst[e] = v[u] + scale * fin[e];
+      }
+    }
+  } else {
     for (int e = threadIdx.x; e < nfin; e += blockDim.x) dst[e] = scale * fin[e];
+  }
 }
 
 // Contiguous 1-D lines: in [L][nin] -> out [L][nout], one matrix for all lines.
