@@ -25,8 +25,10 @@ namespace kad {
 // SGPRs and one from VGPRs and LDS carries only the line in and the outputs out.
 // Waves whose lines use different matrices (e.g. theta lines of even and odd phi
 // index) are served one matrix at a time (a waterfall over the distinct offsets).
+// x and y may be the same buffer when the pass keeps the line length: the thread
+// reads its whole line before writing the line's outputs, and lines are disjoint.
 template <int NIN>
-__device__ __forceinline__ void pass_lines(const double* __restrict__ x, double* __restrict__ y,
+__device__ __forceinline__ void pass_lines(const double* x, double* y,
                                            const double* __restrict__ pool, const int* __restrict__ lm, int ax,
                                            const int dims[3], const int od[3], int nout_ax) {
   const int nl0 = ax == 0 ? od[1] : od[0];
@@ -95,14 +97,15 @@ __device__ __forceinline__ void pass_lines(const double* __restrict__ x, double*
 __global__ __launch_bounds__(256) void apply3d_kernel(const double* __restrict__ in, int64_t in_stride,
                                                       double* __restrict__ out, int64_t out_stride, int d0, int d1,
                                                       int d2, Passes P, const double* __restrict__ pool,
-                                                      int accumulate, double scale, int stage_pool) {
+                                                      int accumulate, double scale, int stage_pool, int inplace) {
   extern __shared__ double lds[];
   // the two ping-pong buffers are selected by value (not through an array of
   // pointers), so every access stays an LDS access and the matrix-pool loads can be
-  // proven unclobbered (scalar loads)
+  // proven unclobbered (scalar loads); inplace: every pass keeps its line length and
+  // runs line by line, so one buffer serves as input and output (half the LDS)
   double* const buf0 = lds;
-  double* const buf1 = lds + P.maxsize;
-  double* mats = lds + 2 * P.maxsize;
+  double* const buf1 = inplace ? lds : lds + P.maxsize;
+  double* mats = lds + (inplace ? 1 : 2) * P.maxsize;
   int* ltab = reinterpret_cast<int*>(mats + (stage_pool ? P.pool_size : 0));
   const int64_t b = blockIdx.x;
   const int nin = d0 * d1 * d2;
@@ -260,17 +263,20 @@ int apply3d_launch(const double* in, int64_t in_stride, double* out, int64_t out
   for (int p = 0; p < P.npass; ++p) nl += (size_t)P.nlines[p];
   // the matrix pool is staged in LDS only if some pass has a line length outside
   // 12..17 (the other passes read it with scalar loads)
-  int stage_pool = 0;
+  int stage_pool = 0, square = 1;
   {
     int dm[3] = {d0, d1, d2};
     for (int p = 0; p < P.npass; ++p) {
       const int nin_ax = dm[P.axis[p]];
       if (nin_ax < 12 || nin_ax > 17) stage_pool = 1;
+      if (P.nout[p] != nin_ax) square = 0;
       dm[P.axis[p]] = P.nout[p];
     }
   }
-  const size_t lds =
-      (2 * (size_t)P.maxsize + (stage_pool ? (size_t)P.pool_size : 0)) * sizeof(double) + nl * sizeof(int);
+  // one LDS buffer when every pass is a line-by-line pass that keeps its line length
+  const int inplace = (!stage_pool && square) ? 1 : 0;
+  const size_t lds = ((inplace ? 1 : 2) * (size_t)P.maxsize + (stage_pool ? (size_t)P.pool_size : 0)) *
+                         sizeof(double) + nl * sizeof(int);
   if (lds > 160 * 1024) return KAD_ERR_ARG;
   GemmTimer* tm = g_apply_timer;
   bool timed = false;
@@ -285,7 +291,7 @@ int apply3d_launch(const double* in, int64_t in_stride, double* out, int64_t out
     timed = rc == 0;
   }
   apply3d_kernel<<<nbatch, 256, lds, s>>>(in, in_stride, out, out_stride, d0, d1, d2, P, pool, accumulate, scale,
-                                          stage_pool);
+                                          stage_pool, inplace);
   KAD_HIP_CHECK(hipGetLastError());
   if (timed && tm->end(s)) return KAD_ERR_HIP;
   return KAD_OK;
