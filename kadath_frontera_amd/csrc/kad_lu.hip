@@ -645,6 +645,7 @@ __global__ __launch_bounds__(FFP_THREADS) void ffp_kernel(T* __restrict__ A, int
   double v = own ? fabs((double)L[r * LD]) : -1.0;
   int id = own ? r0 + r : -1;
   bool live = true;  // uniform: false once a barrier was abandoned
+  int jfail = w;     // first column not factored (after an abandoned barrier)
   for (int jj = 0; jj < w; ++jj) {
     const int j = c0 + jj;
     // 1. the workgroup's candidate among its logical rows >= j, then publish
@@ -681,6 +682,7 @@ __global__ __launch_bounds__(FFP_THREADS) void ffp_kernel(T* __restrict__ A, int
     __syncthreads();
     if (s_abort) {  // uniform; the host sees sync[2] and reports the failure
       live = false;
+      jfail = jj;
       break;
     }
     // 3. all candidates and row j in one round trip (the G x W block of this column
@@ -758,7 +760,13 @@ __global__ __launch_bounds__(FFP_THREADS) void ffp_kernel(T* __restrict__ A, int
       id = r0 + r;
     }
   }
-  if (!live) return;
+  if (!live) {
+    // the columns not factored get identity interchanges: later kernels index rows
+    // with ipiv, which must stay within the matrix (the factors are reported invalid)
+    if (g == 0)
+      for (int jj = jfail + tid; jj < w; jj += FFP_THREADS) ipiv[c0 + jj] = c0 + jj;
+    return;
+  }
   for (int e = tid; e < nr * w; e += FFP_THREADS) {
     const int r = e % nr, c = e / nr;
     A[(int64_t)(r0 + r) + (int64_t)(c0 + c) * lda] = L[r * LD + c];
@@ -860,7 +868,7 @@ int ffp_launch(T* A, int64_t lda, int n, int c0, int c1, int k0, int k1, int* ip
   T* rowj = crow + (size_t)FFP_MAX_GRID * PANEL_BASE * PANEL_BASE;
   const size_t lds = (size_t)rpw * (PANEL_BASE + 1) * sizeof(T);
   ffp_kernel<T, PANEL_BASE><<<grid, FFP_THREADS, lds, w.stream>>>(A, lda, n, c0, c1, k0, k1, rpw, cval, cidx, crow,
-                                                                 rowj, ipiv, w.info, w.ffp_sync, 1 << 22);
+                                                                 rowj, ipiv, w.info, w.ffp_sync, w.ffp_spin);
   KAD_HIP_CHECK(hipGetLastError());
   return KAD_OK;
 }
@@ -940,6 +948,10 @@ int apply_left_swaps(T* A, int64_t lda, int n, const int* ipiv, int nb, LuWork& 
   std::vector<int> hp(n);
   KAD_HIP_CHECK(hipMemcpyAsync(hp.data(), ipiv, n * sizeof(int), hipMemcpyDeviceToHost, w.stream));
   KAD_HIP_CHECK(hipStreamSynchronize(w.stream));
+  // the interchanges index host memory below: a factorisation that did not complete
+  // (an abandoned panel barrier) leaves entries unwritten
+  for (int i = 0; i < n; ++i)
+    if (hp[i] < i || hp[i] >= n) return KAD_ERR_HIP;
   size_t tot = 0;
   for (int p = 0; p + 1 < np; ++p) tot += (size_t)(n - (p + 1) * nb);
   if (tot > w.swap_cap) {
@@ -1118,12 +1130,16 @@ template int getrf<double>(double*, int64_t, int, int*, int, int, LuWork&);
 template int getrf<float>(float*, int64_t, int, int*, int, int, LuWork&);
 
 // perm from LAPACK-style ipiv (host side, O(n)).
-void ipiv_to_perm(const int* ipiv, int n, int* perm) {
+// Returns false (perm unspecified) if some ipiv[i] is outside [i, n): interchanges read
+// back from the device are checked before they index host memory.
+bool ipiv_to_perm(const int* ipiv, int n, int* perm) {
   for (int i = 0; i < n; ++i) perm[i] = i;
   for (int i = 0; i < n; ++i) {
     const int p = ipiv[i];
+    if (p < i || p >= n) return false;
     if (p != i) std::swap(perm[i], perm[p]);
   }
+  return true;
 }
 
 // ---------------------------------------------------------------------------

@@ -24,7 +24,7 @@ template <typename T> int trsv_prepare(const T* LU, int64_t lda, int n, int tran
 template <typename T>
 int getrs_fast(const T* LU, int64_t lda, int n, const int* perm, const double* W, double* b, int trans,
                double* work, hipStream_t s);
-void ipiv_to_perm(const int* ipiv, int n, int* perm);
+bool ipiv_to_perm(const int* ipiv, int n, int* perm);
 int gemv64(const double* A, int64_t lda, int n, const double* x, const double* b, double* y, int trans, double* part,
            hipStream_t s);
 __global__ void axpy_kernel(int n, double a, const double* x, double* y);
@@ -116,6 +116,10 @@ int Ctx::lu_work(int n, int nb) {
     KAD_HIP_CHECK(hipDeviceGetAttribute(&lu.ncu, hipDeviceAttributeMultiprocessorCount, device));
     const char* e = std::getenv("KAD_LU_PANEL");
     lu.ffp_off = e && std::strcmp(e, "percolumn") == 0;
+    // a small limit makes the persistent panel abandon its barriers (and the
+    // factorisation report an error) instead of waiting: the failure path's test
+    const char* sp = std::getenv("KAD_LU_SPIN");
+    if (sp) lu.ffp_spin = std::max(0, std::atoi(sp));
   }
   if ((rc = ensure(&lu.tinv, &cap_tinv, (size_t)nb * nb * 8))) return rc;
   // GEMM scratch for L11^{-1} A12: nb x chunk (bounded at 64 MiB).
@@ -312,7 +316,9 @@ int kad_lu_factor(kad_ctx* ctx, int prec, int n, void* A, int64_t lda, int* ipiv
     rc = kad::getrf<double>((double*)A, lda, n, ipiv, nb, kad::F64, c->lu);
   else
     rc = kad::getrf<float>((float*)A, lda, n, ipiv, nb, prec, c->lu);
-  if (rc) return rc;
+  // both streams of the factorisation have finished before its words are read (also
+  // when getrf returned early)
+  KAD_HIP_CHECK(hipStreamSynchronize(c->lu.stream2));
   // info (counter + 16 B) and the persistent panel's barrier words (counter + 32 B)
   int words[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   KAD_HIP_CHECK(hipMemcpyAsync(words, (char*)c->lu.counter + 16, sizeof(words), hipMemcpyDeviceToHost, c->stream));
@@ -326,11 +332,12 @@ int kad_lu_factor(kad_ctx* ctx, int prec, int n, void* A, int64_t lda, int* ipiv
     KAD_HIP_CHECK(hipStreamSynchronize(c->stream));
     return KAD_ERR_HIP;
   }
+  if (rc) return rc;
   if (perm) {
     std::vector<int> hp(n), hperm(n);
     KAD_HIP_CHECK(hipMemcpyAsync(hp.data(), ipiv, n * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     KAD_HIP_CHECK(hipStreamSynchronize(c->stream));
-    kad::ipiv_to_perm(hp.data(), n, hperm.data());
+    if (!kad::ipiv_to_perm(hp.data(), n, hperm.data())) return KAD_ERR_HIP;
     KAD_HIP_CHECK(hipMemcpyAsync(perm, hperm.data(), n * sizeof(int), hipMemcpyHostToDevice, c->stream));
   }
   KAD_HIP_CHECK(hipStreamSynchronize(c->stream));

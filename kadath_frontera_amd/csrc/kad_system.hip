@@ -20,7 +20,7 @@
 
 namespace kad {
 template <typename T> int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w);
-void ipiv_to_perm(const int* ipiv, int n, int* perm);
+bool ipiv_to_perm(const int* ipiv, int n, int* perm);
 }  // namespace kad
 
 using namespace kadhost;

@@ -175,6 +175,32 @@ def test_persistent_panel_large_grid_matches_per_column(ctx, prec, monkeypatch):
     torch.cuda.empty_cache()
 
 
+def test_persistent_panel_abandoned_barrier_reports_error(ctx, monkeypatch):
+    """The persistent panel's waits are bounded: with a spin limit of 0 (set from the
+    host, KAD_LU_SPIN) its grid barriers are abandoned at once, every workgroup exits,
+    and kad_lu_factor reports an error instead of waiting.  A context with the default
+    limit then factors the same matrix with LAPACK's pivots."""
+    from kadath_frontera_amd.device import Context
+    from kadath_frontera_amd._lib import KadError
+    print("persistent panel abandoned barrier", flush=True)
+    n = 3000
+    A = np.random.default_rng(12).standard_normal((n, n))
+    monkeypatch.setenv("KAD_LU_SPIN", "0")
+    c = Context(0)
+    try:
+        with pytest.raises(KadError):
+            c.lu_factor("f64", colmajor(A, torch.float64))
+        c.sync()
+    finally:
+        c.close()
+        monkeypatch.delenv("KAD_LU_SPIN")
+    dA = colmajor(A, torch.float64)
+    ipiv, perm, info = ctx.lu_factor("f64", dA)
+    assert info == 0
+    _, piv_ref = scipy.linalg.lu_factor(A)
+    assert np.array_equal(ipiv.cpu().numpy(), piv_ref)
+
+
 @pytest.mark.parametrize("trans", [0, 1])
 def test_lu_solve_f64(ctx, trans):
     n = 777
