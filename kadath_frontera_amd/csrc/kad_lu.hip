@@ -537,7 +537,7 @@ __global__ void ff_scatter_kernel(T* __restrict__ A, int64_t lda, int n, int c0,
 // owners.  Per column j:
 //   1. every workgroup publishes its pivot candidate (|a|, logical row, the
 //      candidate's row of the block); the owner of row j publishes row j;
-//   2. one grid barrier (device-scope fence + arrival counter);
+//   2. one grid barrier (arrival counter; the exchanged words are agent-scope atomics);
 //   3. every workgroup reads all candidates with their rows in one round trip,
 //      reduces them itself -- LAPACK's choice, the first index of max |a| -- and
 //      swaps, scales and updates its rows with the per-column kernel's operations.
@@ -555,6 +555,32 @@ __global__ void ff_scatter_kernel(T* __restrict__ A, int64_t lda, int n, int c0,
 constexpr int FFP_MAX_GRID = 64;
 constexpr int FFP_THREADS = 256;
 constexpr int FFP_ROW_BYTES = 72 * 1024;
+
+// The candidate exchange goes through agent-scope relaxed atomics: their stores are
+// written through to the level the XCDs share (sc1) and their loads read it, so no
+// XCD's L2 holds a stale or dirty copy.  A writer waits for its stores to complete
+// (s_waitcnt) before it arrives at the barrier, and a reader loads after it has seen
+// every arrival; neither needs a device-scope fence, which would write back and
+// invalidate its XCD's whole L2 (the concurrent GEMM's working set).
+template <typename T>
+__device__ __forceinline__ void pub_store(T* p, T v) {
+  static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32- or 64-bit words");
+  if constexpr (sizeof(T) == 8)
+    __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), __builtin_bit_cast(unsigned long long, v),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else
+    __hip_atomic_store(reinterpret_cast<unsigned*>(p), __builtin_bit_cast(unsigned, v), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T pub_load(const T* p) {
+  if constexpr (sizeof(T) == 8)
+    return __builtin_bit_cast(T, __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT));
+  else
+    return __builtin_bit_cast(T, __hip_atomic_load(reinterpret_cast<const unsigned*>(p), __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT));
+}
 
 __device__ __forceinline__ void argmax_take(double& v, int& i, double ov, int oi) {
   if (oi >= 0 && (i < 0 || ov > v || (ov == v && oi < i))) {
@@ -652,16 +678,17 @@ __global__ __launch_bounds__(FFP_THREADS) void ffp_kernel(T* __restrict__ A, int
     argmax_256(v, id, sv, si);
     const size_t slot = (size_t)jj * G + g;
     if (tid == 0) {
-      cval[slot] = v;
-      cidx[slot] = id;
+      pub_store(&cval[slot], v);
+      pub_store(&cidx[slot], id);
     }
-    if (id >= 0 && tid < w) crow[slot * W + tid] = L[(id - r0) * LD + tid];
-    if (j >= r0 && j < r0 + nr && tid < w) rowj[(size_t)jj * W + tid] = L[(j - r0) * LD + tid];
-    // 2. grid barrier
+    if (id >= 0 && tid < w) pub_store(&crow[slot * W + tid], L[(id - r0) * LD + tid]);
+    if (j >= r0 && j < r0 + nr && tid < w) pub_store(&rowj[(size_t)jj * W + tid], L[(j - r0) * LD + tid]);
+    // 2. grid barrier: every thread's published words are complete before thread 0
+    //    arrives
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (tid == 0) {
-      __threadfence();
-      atomicAdd(&sync[0], 1);
+      __hip_atomic_fetch_add(&sync[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       const int target = G * (jj + 1);
       int ab = 0;
       for (int it = 0; __hip_atomic_load(&sync[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target; ++it) {
@@ -676,7 +703,6 @@ __global__ __launch_bounds__(FFP_THREADS) void ffp_kernel(T* __restrict__ A, int
         }
         __builtin_amdgcn_s_sleep(1);
       }
-      __threadfence();
       s_abort = ab;
     }
     __syncthreads();
@@ -690,13 +716,13 @@ __global__ __launch_bounds__(FFP_THREADS) void ffp_kernel(T* __restrict__ A, int
 #pragma unroll
     for (int u = 0; u < (FFP_MAX_GRID * W) / FFP_THREADS; ++u) {
       const int e = u * FFP_THREADS + tid;
-      if (e < G * W) cand[e] = crow[(size_t)jj * G * W + e];
+      if (e < G * W) cand[e] = pub_load(&crow[(size_t)jj * G * W + e]);
     }
     if (tid < G) {
-      candi[tid] = cidx[(size_t)jj * G + tid];
-      candv[tid] = cval[(size_t)jj * G + tid];
+      candi[tid] = pub_load(&cidx[(size_t)jj * G + tid]);
+      candv[tid] = pub_load(&cval[(size_t)jj * G + tid]);
     }
-    if (tid < w) jrow[tid] = rowj[(size_t)jj * W + tid];
+    if (tid < w) jrow[tid] = pub_load(&rowj[(size_t)jj * W + tid]);
     __syncthreads();
     v = -1.0;
     id = -1;
