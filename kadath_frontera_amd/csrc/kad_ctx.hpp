@@ -62,7 +62,8 @@ struct LuWork {
   int* ffp_sync = nullptr;
   int ncu = 0;
   bool ffp_off = false;         // KAD_LU_PANEL=percolumn selects the per-column kernels
-  int ffp_spin = 1 << 22;       // spin limit of the grid barrier (KAD_LU_SPIN overrides)
+  int ffp_spin = 1 << 22;       // spin limit of the exchange's waits (KAD_LU_SPIN overrides)
+  unsigned ffp_seq = 0;         // persistent-panel launches since the slots were cleared
 };
 
 // Optional live timing of one kernel class (HIP events on the launch stream) and

@@ -535,51 +535,56 @@ __global__ void ff_scatter_kernel(T* __restrict__ A, int64_t lda, int n, int c0,
 // [c0, n).  Workgroup g owns logical rows [c0 + g rpw, c0 + (g + 1) rpw) of the block
 // and keeps them in LDS for the whole launch; interchanges move row contents between
 // owners.  Per column j:
-//   1. every workgroup publishes its pivot candidate (|a|, logical row, the
-//      candidate's row of the block); the owner of row j publishes row j;
-//   2. one grid barrier (arrival counter; the exchanged words are agent-scope atomics);
-//   3. every workgroup reads all candidates with their rows in one round trip,
-//      reduces them itself -- LAPACK's choice, the first index of max |a| -- and
+//   1. every workgroup publishes its pivot candidate (logical row and the candidate's
+//      row of the block); the owner of row j publishes row j;
+//   2. every workgroup reads all candidates with their rows in one round trip, each
+//      word flagged with its column (no grid barrier: a word is current when its flag
+//      is), reduces them itself -- LAPACK's choice, the first index of max |a| -- and
 //      swaps, scales and updates its rows with the per-column kernel's operations.
 // At the end the interchanges are applied to the panel's other columns [k0, c0) and
 // [c1, k1) (exact: those columns are not read in between).
-// The barrier's cost grows with the number of arriving workgroups, so each one holds
-// as many rows as fits beside a GEMM workgroup on one CU (the trailing update runs
+// The exchange's cost grows with the number of workgroups, so each one holds as many
+// rows as fits beside a GEMM workgroup on one CU (the trailing update runs
 // concurrently): at most FFP_ROW_BYTES of rows plus 18 KB of static LDS, 4 waves.
 // The grid must be resident at once: the host launches at most one workgroup per
 // CU, and nothing else on the device waits on this kernel.  Every wait is bounded: a
-// workgroup that reaches the spin limit raises the abort flag, every waiter leaves
-// on it, and the host reports the failure.
-// sync[0] = arrivals, sync[1] = exits (the last one resets both), sync[2] = abort.
+// workgroup that reaches the spin limit raises the abort flag (sync[2]), every waiter
+// leaves on it, and the host reports the failure.
 // ---------------------------------------------------------------------------
 constexpr int FFP_MAX_GRID = 64;
 constexpr int FFP_THREADS = 256;
 constexpr int FFP_ROW_BYTES = 72 * 1024;
 
-// The candidate exchange goes through agent-scope relaxed atomics: their stores are
-// written through to the level the XCDs share (sc1) and their loads read it, so no
-// XCD's L2 holds a stale or dirty copy.  A writer waits for its stores to complete
-// (s_waitcnt) before it arrives at the barrier, and a reader loads after it has seen
-// every arrival; neither needs a device-scope fence, which would write back and
-// invalidate its XCD's whole L2 (the concurrent GEMM's working set).
-template <typename T>
-__device__ __forceinline__ void pub_store(T* p, T v) {
-  static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32- or 64-bit words");
-  if constexpr (sizeof(T) == 8)
-    __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), __builtin_bit_cast(unsigned long long, v),
-                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  else
-    __hip_atomic_store(reinterpret_cast<unsigned*>(p), __builtin_bit_cast(unsigned, v), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
+// Exchange slots.  Every 64-bit slot holds one 32-bit data word and, in its high half,
+// the flag of the column that wrote it (launch sequence x 32 + column).  Slots are
+// read and written as agent-scope relaxed atomics (sc1: written through to, and read
+// from, the level the XCDs share), so a reader that sees the flag it expects holds
+// that column's word: each slot validates itself, no slot is ordered against another,
+// and no device-scope fence (a write-back and invalidate of the XCD's whole L2, the
+// concurrent GEMM's working set) or arrival counter is needed.  A column's slots are
+// one record per workgroup (its candidate's logical row, then the W values of that
+// row, one word each for FP32 and two for FP64) and the W values of row j.
+template <typename T, int W>
+struct FfpSlots {
+  static constexpr int VW = (int)(sizeof(T) / 4);  // words per value
+  static constexpr int REC = 1 + VW * W;           // one workgroup's record
+  static constexpr int COL = FFP_MAX_GRID * REC + VW * W;  // one column (G records, then row j)
+};
+
+__device__ __forceinline__ void slot_store(unsigned long long* p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long slot_load(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 template <typename T>
-__device__ __forceinline__ T pub_load(const T* p) {
-  if constexpr (sizeof(T) == 8)
-    return __builtin_bit_cast(T, __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_AGENT));
-  else
-    return __builtin_bit_cast(T, __hip_atomic_load(reinterpret_cast<const unsigned*>(p), __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_AGENT));
+__device__ __forceinline__ unsigned word_of(T v, int k) {
+  if constexpr (sizeof(T) == 8) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    return k ? (unsigned)(b >> 32) : (unsigned)b;
+  } else {
+    return __builtin_bit_cast(unsigned, v);
+  }
 }
 
 __device__ __forceinline__ void argmax_take(double& v, int& i, double ov, int oi) {
@@ -636,16 +641,16 @@ __device__ __forceinline__ void argmax_256(double& v, int& i, double* sv, int* s
 
 template <typename T, int W>
 __global__ __launch_bounds__(FFP_THREADS) void ffp_kernel(T* __restrict__ A, int64_t lda, int n, int c0, int c1,
-                                                          int k0, int k1, int rpw, double* __restrict__ cval,
-                                                          int* __restrict__ cidx, T* __restrict__ crow,
-                                                          T* __restrict__ rowj, int* __restrict__ ipiv,
-                                                          int* __restrict__ info, int* __restrict__ sync,
-                                                          int spin_limit) {
+                                                          int k0, int k1, int rpw,
+                                                          unsigned long long* __restrict__ xbuf, unsigned flag0,
+                                                          int* __restrict__ ipiv, int* __restrict__ info,
+                                                          int* __restrict__ sync, int spin_limit) {
+  using S = FfpSlots<T, W>;
+  constexpr int NU = (S::COL + FFP_THREADS - 1) / FFP_THREADS;  // slots per thread
   extern __shared__ __align__(16) unsigned char ffp_smem[];
   T* L = reinterpret_cast<T*>(ffp_smem);  // rpw x (W + 1)
   constexpr int LD = W + 1;
   __shared__ T cand[FFP_MAX_GRID * W];     // every workgroup's candidate row of this column
-  __shared__ double candv[FFP_MAX_GRID];
   __shared__ int candi[FFP_MAX_GRID];
   __shared__ T jrow[W];
   __shared__ double sv[FFP_THREADS / 64];
@@ -654,7 +659,8 @@ __global__ __launch_bounds__(FFP_THREADS) void ffp_kernel(T* __restrict__ A, int
   __shared__ int s_rows[2 * W];
   __shared__ int s_src[2 * W];
   __shared__ int s_nrows;
-  __shared__ int s_abort;
+  unsigned* const candw = reinterpret_cast<unsigned*>(cand);
+  unsigned* const jroww = reinterpret_cast<unsigned*>(jrow);
   const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x;
   const int r0 = c0 + g * rpw;
   const int nr = max(0, min(rpw, n - r0));
@@ -674,60 +680,86 @@ __global__ __launch_bounds__(FFP_THREADS) void ffp_kernel(T* __restrict__ A, int
   int jfail = w;     // first column not factored (after an abandoned barrier)
   for (int jj = 0; jj < w; ++jj) {
     const int j = c0 + jj;
-    // 1. the workgroup's candidate among its logical rows >= j, then publish
+    // 1. the workgroup's candidate among its logical rows >= j, then publish its
+    //    record (logical row -1: none) and, from row j's owner, row j
     argmax_256(v, id, sv, si);
-    const size_t slot = (size_t)jj * G + g;
-    if (tid == 0) {
-      pub_store(&cval[slot], v);
-      pub_store(&cidx[slot], id);
+    const unsigned flag = flag0 + (unsigned)jj;
+    const unsigned long long tag = (unsigned long long)flag << 32;
+    unsigned long long* const col = xbuf + (size_t)jj * S::COL;
+    if (tid == 0) slot_store(col + g * S::REC, tag | (unsigned)id);
+    if (tid < S::VW * w) {
+      const int c = tid / S::VW, k = tid - c * S::VW;
+      const unsigned x = id >= 0 ? word_of(L[(id - r0) * LD + c], k) : 0u;
+      slot_store(col + g * S::REC + 1 + tid, tag | x);
+      if (j >= r0 && j < r0 + nr) slot_store(col + G * S::REC + tid, tag | word_of(L[(j - r0) * LD + c], k));
     }
-    if (id >= 0 && tid < w) pub_store(&crow[slot * W + tid], L[(id - r0) * LD + tid]);
-    if (j >= r0 && j < r0 + nr && tid < w) pub_store(&rowj[(size_t)jj * W + tid], L[(j - r0) * LD + tid]);
-    // 2. grid barrier: every thread's published words are complete before thread 0
-    //    arrives
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    if (tid == 0) {
-      __hip_atomic_fetch_add(&sync[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const int target = G * (jj + 1);
-      int ab = 0;
-      for (int it = 0; __hip_atomic_load(&sync[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target; ++it) {
-        if ((it & 255) == 255 && __hip_atomic_load(&sync[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-          ab = 1;
-          break;
-        }
-        if (it >= spin_limit) {
-          __hip_atomic_store(&sync[2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          ab = 1;
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
+    // 2. every record and row j, in one round trip: all of a thread's slots are loaded
+    //    at once, then each is read again until it carries this column's flag.  Bounded:
+    //    at the spin limit the workgroup raises the abort flag, and every waiter leaves
+    //    on it; the host then reports the failure.
+    // slot e = u * FFP_THREADS + tid of this column: a record word (index or one of the
+    // w values) or, from G records on, a word of row j
+    const int nrec = G * S::REC, nall = nrec + S::VW * w;
+    const char* const colb = reinterpret_cast<const char*>(col) + (size_t)tid * 8u;
+    auto need = [&](int u) -> bool {
+      const int e = u * FFP_THREADS + tid;
+      return e < nall && (e >= nrec || e - (e / S::REC) * S::REC <= S::VW * w);
+    };
+    auto ptr = [&](int u) {
+      return reinterpret_cast<const unsigned long long*>(colb + (unsigned)(u * FFP_THREADS * 8));
+    };
+    unsigned long long xs[NU];
+#pragma unroll
+    for (int u = 0; u < NU; ++u) xs[u] = need(u) ? slot_load(ptr(u)) : tag;  // nothing to read: current
+    int bad = 0;
+    for (int it = 0;; ++it) {
+      unsigned pend = 0;
+#pragma unroll
+      for (int u = 0; u < NU; ++u)
+        if ((unsigned)(xs[u] >> 32) != flag) pend |= 1u << u;
+      if (!pend) break;
+      if (it >= spin_limit) {
+        __hip_atomic_store(&sync[2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        bad = 1;
+        break;
       }
-      s_abort = ab;
+      if ((it & 255) == 255 && __hip_atomic_load(&sync[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+        bad = 1;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+      for (int u = 0; u < NU; ++u)
+        if ((pend >> u) & 1u) xs[u] = slot_load(ptr(u));
     }
-    __syncthreads();
-    if (s_abort) {  // uniform; the host sees sync[2] and reports the failure
+    if (!bad) {
+#pragma unroll
+      for (int u = 0; u < NU; ++u) {
+        const int e = u * FFP_THREADS + tid;
+        if (!need(u)) continue;
+        const unsigned d = (unsigned)xs[u];
+        if (e >= nrec) {
+          jroww[e - nrec] = d;
+        } else {
+          const int gg = e / S::REC, q = e - gg * S::REC;
+          if (q == 0)
+            candi[gg] = (int)d;
+          else
+            candw[gg * W * S::VW + q - 1] = d;
+        }
+      }
+    }
+    if (__syncthreads_or(bad)) {  // uniform; the host sees sync[2] and reports the failure
       live = false;
       jfail = jj;
       break;
     }
-    // 3. all candidates and row j in one round trip (the G x W block of this column
-    //    is contiguous; entries c >= w of a row are never read), then the pivot
-#pragma unroll
-    for (int u = 0; u < (FFP_MAX_GRID * W) / FFP_THREADS; ++u) {
-      const int e = u * FFP_THREADS + tid;
-      if (e < G * W) cand[e] = pub_load(&crow[(size_t)jj * G * W + e]);
-    }
-    if (tid < G) {
-      candi[tid] = pub_load(&cidx[(size_t)jj * G + tid]);
-      candv[tid] = pub_load(&cval[(size_t)jj * G + tid]);
-    }
-    if (tid < w) jrow[tid] = pub_load(&rowj[(size_t)jj * W + tid]);
-    __syncthreads();
+    // 3. the pivot: LAPACK's choice, the first logical row of max |a| (a candidate's
+    //    |a| is its row's entry in this column)
     v = -1.0;
     id = -1;
     if (tid < G && candi[tid] >= 0) {
-      v = candv[tid];
+      v = fabs((double)cand[tid * W + jj]);
       id = candi[tid];
     }
     argmax_256(v, id, sv, si);
@@ -750,8 +782,9 @@ __global__ __launch_bounds__(FFP_THREADS) void ffp_kernel(T* __restrict__ A, int
     __syncthreads();
     // the multiplier of this thread's row (if below j), then the rank-1 update in
     // two halves of 16 columns with that half of the pivot row in registers
-    // (constant indices): every LDS read of the row is independent of its writes,
-    // and ~90 VGPRs are live, so a workgroup fits beside a GEMM workgroup on one CU
+    // (constant indices): every LDS read of the row is independent of its writes.
+    // The kernel stays within 144 VGPRs (FP64), so a workgroup fits beside a GEMM
+    // workgroup (368) on one CU
     const bool below = own && r0 + r > j;
     const T piv = urow[jj];
     T l = T(0);
@@ -847,18 +880,10 @@ __global__ __launch_bounds__(FFP_THREADS) void ffp_kernel(T* __restrict__ A, int
     }
     __syncthreads();
   }
-  if (tid == 0 && atomicAdd(&sync[1], 1) == G - 1) {
-    // every workgroup has left its last wait: reset for the next launch
-    __hip_atomic_store(&sync[0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(&sync[1], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
 }
 
-// Bytes of the candidate / row exchange buffers of the persistent base panel.
-size_t ffp_bytes() {
-  return (size_t)FFP_MAX_GRID * PANEL_BASE * (sizeof(double) + sizeof(int)) +
-         ((size_t)FFP_MAX_GRID * PANEL_BASE * PANEL_BASE + (size_t)PANEL_BASE * PANEL_BASE) * sizeof(double);
-}
+// Bytes of the exchange slots of the persistent base panel (W columns, FP64 records).
+size_t ffp_bytes() { return (size_t)PANEL_BASE * FfpSlots<double, PANEL_BASE>::COL * sizeof(unsigned long long); }
 
 // Launch shape of the persistent base panel for m rows of element type T: grid and
 // rows per workgroup, or false when it does not fit (the per-column kernels run).
@@ -887,14 +912,16 @@ int ffp_launch(T* A, int64_t lda, int n, int c0, int c1, int k0, int k1, int* ip
                                       hipFuncAttributeMaxDynamicSharedMemorySize, FFP_ROW_BYTES));
     lds_set = true;
   }
-  char* base = (char*)w.ffp;
-  double* cval = (double*)base;
-  int* cidx = (int*)(base + FFP_MAX_GRID * PANEL_BASE * sizeof(double));
-  T* crow = (T*)(base + FFP_MAX_GRID * PANEL_BASE * (sizeof(double) + sizeof(int)));
-  T* rowj = crow + (size_t)FFP_MAX_GRID * PANEL_BASE * PANEL_BASE;
+  // flags: PANEL_BASE per launch, never one the slots still hold (the buffer starts
+  // zeroed and the sequence at 1; before the 32-bit flags wrap it is cleared again)
+  if (++w.ffp_seq >= (1u << 26)) {
+    KAD_HIP_CHECK(hipMemsetAsync(w.ffp, 0, ffp_bytes(), w.stream));
+    w.ffp_seq = 1;
+  }
+  const unsigned flag0 = w.ffp_seq * (unsigned)PANEL_BASE;
   const size_t lds = (size_t)rpw * (PANEL_BASE + 1) * sizeof(T);
-  ffp_kernel<T, PANEL_BASE><<<grid, FFP_THREADS, lds, w.stream>>>(A, lda, n, c0, c1, k0, k1, rpw, cval, cidx, crow,
-                                                                 rowj, ipiv, w.info, w.ffp_sync, w.ffp_spin);
+  ffp_kernel<T, PANEL_BASE><<<grid, FFP_THREADS, lds, w.stream>>>(
+      A, lda, n, c0, c1, k0, k1, rpw, (unsigned long long*)w.ffp, flag0, ipiv, w.info, w.ffp_sync, w.ffp_spin);
   KAD_HIP_CHECK(hipGetLastError());
   return KAD_OK;
 }
@@ -1045,6 +1072,13 @@ int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w)
     return KAD_ERR_ALLOC;
   LuWork wp = w;  // panel stream: tinv, tmp, tmp2 and the panel reductions
   wp.stream = w.stream2;
+  // the persistent panel's launch sequence advances in wp; it outlives this call on
+  // every return path (reused flags would let stale exchange slots look current)
+  struct SeqBack {
+    LuWork& to;
+    const LuWork& from;
+    ~SeqBack() { to.ffp_seq = from.ffp_seq; }
+  } seq_back{w, wp};
   LuWork wm = w;  // main stream: its own triangular inverse and GEMM scratch
   wm.tinv = w.tinv_m;
   wm.tmp = w.tmp_m;

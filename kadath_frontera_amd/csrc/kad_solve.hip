@@ -110,13 +110,15 @@ int Ctx::lu_work(int n, int nb) {
     KAD_HIP_CHECK(hipMemsetAsync(lu.counter, 0, 64, stream));
     lu.info = (int*)((char*)lu.counter + 16);
   }
-  lu.ffp_sync = (int*)((char*)lu.counter + 32);  // arrivals, exits, abort (zeroed above)
+  lu.ffp_sync = (int*)((char*)lu.counter + 32);  // [2]: abort flag (zeroed above)
   if (!lu.ffp) {
     KAD_HIP_CHECK(hipMalloc(&lu.ffp, ffp_bytes()));
+    KAD_HIP_CHECK(hipMemsetAsync(lu.ffp, 0, ffp_bytes(), stream));  // flag 0: no column
+    lu.ffp_seq = 0;
     KAD_HIP_CHECK(hipDeviceGetAttribute(&lu.ncu, hipDeviceAttributeMultiprocessorCount, device));
     const char* e = std::getenv("KAD_LU_PANEL");
     lu.ffp_off = e && std::strcmp(e, "percolumn") == 0;
-    // a small limit makes the persistent panel abandon its barriers (and the
+    // a small limit makes the persistent panel abandon its waits (and the
     // factorisation report an error) instead of waiting: the failure path's test
     const char* sp = std::getenv("KAD_LU_SPIN");
     if (sp) lu.ffp_spin = std::max(0, std::atoi(sp));
