@@ -607,12 +607,11 @@ __device__ __forceinline__ void argmax_dpp(double& v, int& i) {
 }
 
 // Max |a| with the smallest index among ties (invalid entries carry index -1) over
-// the 256-thread block; the result reaches every thread.  Inside a wave: quad
-// swaps, then half-row and row mirrors (every lane of a 16-lane row ends with the
-// row's best), then the four rows' results are read from lanes 0, 16, 32, 48.  The
-// selection is a total order, so the result does not depend on the reduction order.
-// sv / si: 4 entries in LDS.  Every thread of the block must call it.
-__device__ __forceinline__ void argmax_256(double& v, int& i, double* sv, int* si) {
+// one wave; the result reaches every lane.  Quad swaps, then half-row and row mirrors
+// (every lane of a 16-lane row ends with the row's best), then the four rows' results
+// are read from lanes 0, 16, 32, 48.  The selection is a total order, so the result
+// does not depend on the reduction order.
+__device__ __forceinline__ void argmax_wave(double& v, int& i) {
   argmax_dpp<0xB1>(v, i);   // quad_perm [1,0,3,2]
   argmax_dpp<0x4E>(v, i);   // quad_perm [2,3,0,1]
   argmax_dpp<0x141>(v, i);  // row_half_mirror
@@ -627,16 +626,24 @@ __device__ __forceinline__ void argmax_256(double& v, int& i, double* sv, int* s
     argmax_take(wv, wi, __longlong_as_double(((long long)rhi << 32) | (unsigned)rlo),
                 __builtin_amdgcn_readlane(i, 16 * r));
   }
+  v = wv;
+  i = wi;
+}
+
+// The same over the 256-thread block, through 4 entries of LDS (sv / si) and one
+// barrier.  Every thread of the block must call it, and the caller must pass another
+// barrier before the next call writes sv / si.
+__device__ __forceinline__ void argmax_256(double& v, int& i, double* sv, int* si) {
+  argmax_wave(v, i);
   if ((threadIdx.x & 63) == 0) {
-    sv[threadIdx.x >> 6] = wv;
-    si[threadIdx.x >> 6] = wi;
+    sv[threadIdx.x >> 6] = v;
+    si[threadIdx.x >> 6] = i;
   }
   __syncthreads();
   v = -1.0;
   i = -1;
 #pragma unroll
   for (int q = 0; q < FFP_THREADS / 64; ++q) argmax_take(v, i, sv[q], si[q]);
-  __syncthreads();
 }
 
 template <typename T, int W>
@@ -645,8 +652,12 @@ __global__ __launch_bounds__(FFP_THREADS) void ffp_kernel(T* __restrict__ A, int
                                                           unsigned long long* __restrict__ xbuf, unsigned flag0,
                                                           int* __restrict__ ipiv, int* __restrict__ info,
                                                           int* __restrict__ sync, int spin_limit) {
+  static_assert(FFP_MAX_GRID <= 64, "one candidate per lane of a wave");
   using S = FfpSlots<T, W>;
   constexpr int NU = (S::COL + FFP_THREADS - 1) / FFP_THREADS;  // slots per thread
+  // issue priority over the GEMM waves that share the SIMDs: every column waits for
+  // the slowest workgroup, and the GEMM's MFMA chains leave issue slots to spare
+  __builtin_amdgcn_s_setprio(3);
   extern __shared__ __align__(16) unsigned char ffp_smem[];
   T* L = reinterpret_cast<T*>(ffp_smem);  // rpw x (W + 1)
   constexpr int LD = W + 1;
@@ -755,14 +766,16 @@ __global__ __launch_bounds__(FFP_THREADS) void ffp_kernel(T* __restrict__ A, int
       break;
     }
     // 3. the pivot: LAPACK's choice, the first logical row of max |a| (a candidate's
-    //    |a| is its row's entry in this column)
+    //    |a| is its row's entry in this column); every wave reduces the G <= 64
+    //    candidates itself
     v = -1.0;
     id = -1;
-    if (tid < G && candi[tid] >= 0) {
-      v = fabs((double)cand[tid * W + jj]);
-      id = candi[tid];
+    const int lane = tid & 63;
+    if (lane < G && candi[lane] >= 0) {
+      v = fabs((double)cand[lane * W + jj]);
+      id = candi[lane];
     }
-    argmax_256(v, id, sv, si);
+    argmax_wave(v, id);
     int p = id;
     if (p < j || p >= n) {
       // only reachable with corrupt candidates: keep every index in range and fail
