@@ -562,13 +562,20 @@ constexpr int FFP_ROW_BYTES = 72 * 1024;
 // that column's word: each slot validates itself, no slot is ordered against another,
 // and no device-scope fence (a write-back and invalidate of the XCD's whole L2, the
 // concurrent GEMM's working set) or arrival counter is needed.  A column's slots are
-// one record per workgroup (its candidate's logical row, then the W values of that
-// row, one word each for FP32 and two for FP64) and the W values of row j.
+// one record per workgroup (its candidate's logical row, for FP64 a pad word that
+// keeps the values 8-byte aligned, then the W values of that row, one word each for
+// FP32 and two for FP64) and, after the G records, the W values of row j.  Every
+// record and row j carry all W values (those of columns past the panel's end are
+// never read).  A column's stride covers whole 256-slot strips, so every thread reads
+// NU slots unconditionally and keeps those below G records + row j.
 template <typename T, int W>
 struct FfpSlots {
-  static constexpr int VW = (int)(sizeof(T) / 4);  // words per value
-  static constexpr int REC = 1 + VW * W;           // one workgroup's record
-  static constexpr int COL = FFP_MAX_GRID * REC + VW * W;  // one column (G records, then row j)
+  static constexpr int VW = (int)(sizeof(T) / 4);        // words per value
+  static constexpr int VOFF = VW;                        // first value word of a record
+  static constexpr int REC = VOFF + VW * W;              // one workgroup's record
+  static constexpr int USED = FFP_MAX_GRID * REC + VW * W;  // G <= FFP_MAX_GRID records, row j
+  static constexpr int NU = (USED + 255) / 256;          // slots per thread (256 threads)
+  static constexpr int COL = NU * 256;                   // one column's stride
 };
 
 __device__ __forceinline__ void slot_store(unsigned long long* p, unsigned long long v) {
@@ -653,26 +660,25 @@ __global__ __launch_bounds__(FFP_THREADS) void ffp_kernel(T* __restrict__ A, int
                                                           int* __restrict__ ipiv, int* __restrict__ info,
                                                           int* __restrict__ sync, int spin_limit) {
   static_assert(FFP_MAX_GRID <= 64, "one candidate per lane of a wave");
+  static_assert(FFP_THREADS == 256, "256-slot strips");
   using S = FfpSlots<T, W>;
-  constexpr int NU = (S::COL + FFP_THREADS - 1) / FFP_THREADS;  // slots per thread
+  constexpr int NU = S::NU;
   // issue priority over the GEMM waves that share the SIMDs: every column waits for
   // the slowest workgroup, and the GEMM's MFMA chains leave issue slots to spare
   __builtin_amdgcn_s_setprio(3);
   extern __shared__ __align__(16) unsigned char ffp_smem[];
   T* L = reinterpret_cast<T*>(ffp_smem);  // rpw x (W + 1)
   constexpr int LD = W + 1;
-  __shared__ T cand[FFP_MAX_GRID * W];     // every workgroup's candidate row of this column
-  __shared__ int candi[FFP_MAX_GRID];
-  __shared__ T jrow[W];
+  // this column's slots, data words only: G records, then row j
+  __shared__ __align__(16) unsigned xim[S::COL];
   __shared__ double sv[FFP_THREADS / 64];
   __shared__ int si[FFP_THREADS / 64];
   __shared__ int s_piv[W];
   __shared__ int s_rows[2 * W];
   __shared__ int s_src[2 * W];
   __shared__ int s_nrows;
-  unsigned* const candw = reinterpret_cast<unsigned*>(cand);
-  unsigned* const jroww = reinterpret_cast<unsigned*>(jrow);
   const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x;
+  const T* const jrow = reinterpret_cast<const T*>(xim + G * S::REC);
   const int r0 = c0 + g * rpw;
   const int nr = max(0, min(rpw, n - r0));
   const int w = c1 - c0;
@@ -697,37 +703,31 @@ __global__ __launch_bounds__(FFP_THREADS) void ffp_kernel(T* __restrict__ A, int
     const unsigned flag = flag0 + (unsigned)jj;
     const unsigned long long tag = (unsigned long long)flag << 32;
     unsigned long long* const col = xbuf + (size_t)jj * S::COL;
-    if (tid == 0) slot_store(col + g * S::REC, tag | (unsigned)id);
-    if (tid < S::VW * w) {
+    if (tid < S::VOFF) slot_store(col + g * S::REC + tid, tag | (tid == 0 ? (unsigned)id : 0u));
+    if (tid < S::VW * W) {
       const int c = tid / S::VW, k = tid - c * S::VW;
       const unsigned x = id >= 0 ? word_of(L[(id - r0) * LD + c], k) : 0u;
-      slot_store(col + g * S::REC + 1 + tid, tag | x);
+      slot_store(col + g * S::REC + S::VOFF + tid, tag | x);
       if (j >= r0 && j < r0 + nr) slot_store(col + G * S::REC + tid, tag | word_of(L[(j - r0) * LD + c], k));
     }
-    // 2. every record and row j, in one round trip: all of a thread's slots are loaded
-    //    at once, then each is read again until it carries this column's flag.  Bounded:
-    //    at the spin limit the workgroup raises the abort flag, and every waiter leaves
-    //    on it; the host then reports the failure.
-    // slot e = u * FFP_THREADS + tid of this column: a record word (index or one of the
-    // w values) or, from G records on, a word of row j
-    const int nrec = G * S::REC, nall = nrec + S::VW * w;
-    const char* const colb = reinterpret_cast<const char*>(col) + (size_t)tid * 8u;
-    auto need = [&](int u) -> bool {
-      const int e = u * FFP_THREADS + tid;
-      return e < nall && (e >= nrec || e - (e / S::REC) * S::REC <= S::VW * w);
-    };
-    auto ptr = [&](int u) {
-      return reinterpret_cast<const unsigned long long*>(colb + (unsigned)(u * FFP_THREADS * 8));
-    };
+    // 2. every record and row j, in one round trip: a thread loads its NU slots
+    //    (strip u, slot u * 256 + tid) at once, then reads again those below the used
+    //    slots until they carry this column's flag.  Bounded: at the spin limit the
+    //    workgroup raises the abort flag, and every waiter leaves on it; the host then
+    //    reports the failure.
+    const int nused = G * S::REC + S::VW * W;
+    const int nv = min(NU, max(0, (nused - tid + 255) >> 8));  // strips below nused
+    const unsigned valid = nv >= 32 ? ~0u : (1u << nv) - 1u;
+    const unsigned long long* const colt = col + tid;
     unsigned long long xs[NU];
 #pragma unroll
-    for (int u = 0; u < NU; ++u) xs[u] = need(u) ? slot_load(ptr(u)) : tag;  // nothing to read: current
+    for (int u = 0; u < NU; ++u) xs[u] = slot_load(colt + u * 256);
     int bad = 0;
     for (int it = 0;; ++it) {
       unsigned pend = 0;
 #pragma unroll
-      for (int u = 0; u < NU; ++u)
-        if ((unsigned)(xs[u] >> 32) != flag) pend |= 1u << u;
+      for (int u = 0; u < NU; ++u) pend |= (unsigned)((unsigned)(xs[u] >> 32) != flag) << u;
+      pend &= valid;
       if (!pend) break;
       if (it >= spin_limit) {
         __hip_atomic_store(&sync[2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -741,25 +741,10 @@ __global__ __launch_bounds__(FFP_THREADS) void ffp_kernel(T* __restrict__ A, int
       __builtin_amdgcn_s_sleep(1);
 #pragma unroll
       for (int u = 0; u < NU; ++u)
-        if ((pend >> u) & 1u) xs[u] = slot_load(ptr(u));
+        if ((pend >> u) & 1u) xs[u] = slot_load(colt + u * 256);
     }
-    if (!bad) {
 #pragma unroll
-      for (int u = 0; u < NU; ++u) {
-        const int e = u * FFP_THREADS + tid;
-        if (!need(u)) continue;
-        const unsigned d = (unsigned)xs[u];
-        if (e >= nrec) {
-          jroww[e - nrec] = d;
-        } else {
-          const int gg = e / S::REC, q = e - gg * S::REC;
-          if (q == 0)
-            candi[gg] = (int)d;
-          else
-            candw[gg * W * S::VW + q - 1] = d;
-        }
-      }
-    }
+    for (int u = 0; u < NU; ++u) xim[u * 256 + tid] = (unsigned)xs[u];
     if (__syncthreads_or(bad)) {  // uniform; the host sees sync[2] and reports the failure
       live = false;
       jfail = jj;
@@ -771,9 +756,9 @@ __global__ __launch_bounds__(FFP_THREADS) void ffp_kernel(T* __restrict__ A, int
     v = -1.0;
     id = -1;
     const int lane = tid & 63;
-    if (lane < G && candi[lane] >= 0) {
-      v = fabs((double)cand[lane * W + jj]);
-      id = candi[lane];
+    if (lane < G && (int)xim[lane * S::REC] >= 0) {
+      v = fabs((double)reinterpret_cast<const T*>(xim + lane * S::REC + S::VOFF)[jj]);
+      id = (int)xim[lane * S::REC];
     }
     argmax_wave(v, id);
     int p = id;
@@ -782,7 +767,7 @@ __global__ __launch_bounds__(FFP_THREADS) void ffp_kernel(T* __restrict__ A, int
       p = j;
       if (tid == 0) __hip_atomic_store(&sync[2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    const T* urow = cand + ((p - c0) / rpw) * W;  // the row p's owner published
+    const T* urow = reinterpret_cast<const T*>(xim + ((p - c0) / rpw) * S::REC + S::VOFF);  // p's owner's
     if (tid == 0) s_piv[jj] = p;
     if (g == 0 && tid == 0) {
       ipiv[j] = p;
