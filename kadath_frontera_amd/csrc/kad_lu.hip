@@ -532,9 +532,9 @@ __global__ void ff_scatter_kernel(T* __restrict__ A, int64_t lda, int n, int c0,
 
 // ---------------------------------------------------------------------------
 // Persistent base panel: one launch factors columns [c0, c1) (c1 - c0 <= W) of rows
-// [c0, n).  Workgroup g owns logical rows [c0 + g rpw, c0 + (g + 1) rpw) of the block
-// and keeps them in LDS for the whole launch; interchanges move row contents between
-// owners.  Per column j:
+// [c0, n).  Workgroup g owns logical rows [c0 + g rpw, c0 + (g + 1) rpw) of the block,
+// one per thread, and keeps them in registers for the whole launch; interchanges move
+// row contents between owners.  Per column j:
 //   1. every workgroup publishes its pivot candidate (logical row and the candidate's
 //      row of the block); the owner of row j publishes row j;
 //   2. every workgroup reads all candidates with their rows in one round trip, each
@@ -543,9 +543,9 @@ __global__ void ff_scatter_kernel(T* __restrict__ A, int64_t lda, int n, int c0,
 //      swaps, scales and updates its rows with the per-column kernel's operations.
 // At the end the interchanges are applied to the panel's other columns [k0, c0) and
 // [c1, k1) (exact: those columns are not read in between).
-// The exchange's cost grows with the number of workgroups, so each one holds as many
-// rows as fits beside a GEMM workgroup on one CU (the trailing update runs
-// concurrently): at most FFP_ROW_BYTES of rows plus 18 KB of static LDS, 4 waves.
+// The exchange's cost grows with the number of workgroups, so each one holds 256 rows,
+// and it fits beside a GEMM workgroup on one CU (the trailing update runs
+// concurrently): 144 VGPRs (FP64) and 19 KB of LDS, 4 waves.
 // The grid must be resident at once: the host launches at most one workgroup per
 // CU, and nothing else on the device waits on this kernel.  Every wait is bounded: a
 // workgroup that reaches the spin limit raises the abort flag (sync[2]), every waiter
@@ -553,7 +553,6 @@ __global__ void ff_scatter_kernel(T* __restrict__ A, int64_t lda, int n, int c0,
 // ---------------------------------------------------------------------------
 constexpr int FFP_MAX_GRID = 64;
 constexpr int FFP_THREADS = 256;
-constexpr int FFP_ROW_BYTES = 72 * 1024;
 
 // Exchange slots.  Every 64-bit slot holds one 32-bit data word and, in its high half,
 // the flag of the column that wrote it (launch sequence x 32 + column).  Slots are
@@ -576,6 +575,13 @@ struct FfpSlots {
   static constexpr int USED = FFP_MAX_GRID * REC + VW * W;  // G <= FFP_MAX_GRID records, row j
   static constexpr int NU = (USED + 255) / 256;          // slots per thread (256 threads)
   static constexpr int COL = NU * 256;                   // one column's stride
+  // where logical slot e (strip e / 256, thread e % 256) lies: wave-major, so one
+  // thread's NU slots are 512 bytes apart (immediate offsets from one or two bases)
+  // while each wave's load of a strip stays 64 consecutive slots
+  static __device__ __forceinline__ int pos(int e) {
+    const int t = e & 255;
+    return (t >> 6) * (NU * 64) + (e >> 8) * 64 + (t & 63);
+  }
 };
 
 __device__ __forceinline__ void slot_store(unsigned long long* p, unsigned long long v) {
@@ -666,9 +672,6 @@ __global__ __launch_bounds__(FFP_THREADS) void ffp_kernel(T* __restrict__ A, int
   // issue priority over the GEMM waves that share the SIMDs: every column waits for
   // the slowest workgroup, and the GEMM's MFMA chains leave issue slots to spare
   __builtin_amdgcn_s_setprio(3);
-  extern __shared__ __align__(16) unsigned char ffp_smem[];
-  T* L = reinterpret_cast<T*>(ffp_smem);  // rpw x (W + 1)
-  constexpr int LD = W + 1;
   // this column's slots, data words only: G records, then row j
   __shared__ __align__(16) unsigned xim[S::COL];
   __shared__ double sv[FFP_THREADS / 64];
@@ -677,24 +680,27 @@ __global__ __launch_bounds__(FFP_THREADS) void ffp_kernel(T* __restrict__ A, int
   __shared__ int s_rows[2 * W];
   __shared__ int s_src[2 * W];
   __shared__ int s_nrows;
+  __shared__ __align__(16) T stage[2][W];  // the candidate row and row j, for their waves
   const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x;
   const T* const jrow = reinterpret_cast<const T*>(xim + G * S::REC);
   const int r0 = c0 + g * rpw;
   const int nr = max(0, min(rpw, n - r0));
   const int w = c1 - c0;
-  for (int e = tid; e < nr * w; e += FFP_THREADS) {
-    const int r = e % nr, c = e / nr;
-    L[r * LD + c] = A[(int64_t)(r0 + r) + (int64_t)(c0 + c) * lda];
-  }
-  __syncthreads();
-  // this thread's row (one per thread: the host launches at most FFP_THREADS rows)
+  // this thread's row of the block, in registers for the whole launch (one row per
+  // thread: the host launches at most FFP_THREADS rows per workgroup); columns past
+  // the panel's end hold zeros
+  // (every index into it is a compile-time constant after unrolling, so it stays in
+  // registers; a uniform column index is reached by value selects)
   const int r = tid;
   const bool own = r < nr;
+  T row[W];
+#pragma unroll
+  for (int c = 0; c < W; ++c) row[c] = own && c < w ? A[(int64_t)(r0 + r) + (int64_t)(c0 + c) * lda] : T(0);
   // local candidate of the first column; later columns' come out of the update
-  double v = own ? fabs((double)L[r * LD]) : -1.0;
+  double v = own ? fabs((double)row[0]) : -1.0;
   int id = own ? r0 + r : -1;
-  bool live = true;  // uniform: false once a barrier was abandoned
-  int jfail = w;     // first column not factored (after an abandoned barrier)
+  bool live = true;  // uniform: false once a wait was abandoned
+  int jfail = w;     // first column not factored (after an abandoned wait)
   for (int jj = 0; jj < w; ++jj) {
     const int j = c0 + jj;
     // 1. the workgroup's candidate among its logical rows >= j, then publish its
@@ -703,32 +709,59 @@ __global__ __launch_bounds__(FFP_THREADS) void ffp_kernel(T* __restrict__ A, int
     const unsigned flag = flag0 + (unsigned)jj;
     const unsigned long long tag = (unsigned long long)flag << 32;
     unsigned long long* const col = xbuf + (size_t)jj * S::COL;
-    if (tid < S::VOFF) slot_store(col + g * S::REC + tid, tag | (tid == 0 ? (unsigned)id : 0u));
-    if (tid < S::VW * W) {
-      const int c = tid / S::VW, k = tid - c * S::VW;
-      const unsigned x = id >= 0 ? word_of(L[(id - r0) * LD + c], k) : 0u;
-      slot_store(col + g * S::REC + S::VOFF + tid, tag | x);
-      if (j >= r0 && j < r0 + nr) slot_store(col + G * S::REC + tid, tag | word_of(L[(j - r0) * LD + c], k));
+    // the owning lane stages the row in LDS, then its wave stores the row's words
+    // (one slot per lane) without a workgroup barrier
+    const int lane = tid & 63;
+    const bool cand_wave = id >= 0 && (tid >> 6) == ((id - r0) >> 6);
+    const bool rowj_wave = j >= r0 && j < r0 + nr && (tid >> 6) == ((j - r0) >> 6);
+    if (cand_wave && tid == id - r0) {
+#pragma unroll
+      for (int c = 0; c < W; ++c) stage[0][c] = row[c];
     }
+    if (rowj_wave && tid == j - r0) {
+#pragma unroll
+      for (int c = 0; c < W; ++c) stage[1][c] = row[c];
+    }
+    if (cand_wave || rowj_wave) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    static_assert(S::VW * W <= 64, "one word per lane");
+    if (cand_wave) {
+      if (lane < S::VOFF) slot_store(col + S::pos(g * S::REC + lane), tag | (lane == 0 ? (unsigned)id : 0u));
+      if (lane < S::VW * W)
+        slot_store(col + S::pos(g * S::REC + S::VOFF + lane),
+                   tag | reinterpret_cast<const unsigned*>(stage[0])[lane]);
+    } else if (id < 0 && tid < S::REC) {
+      slot_store(col + S::pos(g * S::REC + tid), tag | (tid == 0 ? (unsigned)id : 0u));
+    }
+    if (rowj_wave && lane < S::VW * W)
+      slot_store(col + S::pos(G * S::REC + lane), tag | reinterpret_cast<const unsigned*>(stage[1])[lane]);
     // 2. every record and row j, in one round trip: a thread loads its NU slots
-    //    (strip u, slot u * 256 + tid) at once, then reads again those below the used
+    //    (strip u, logical slot u * 256 + tid) at once, then reads again those below the used
     //    slots until they carry this column's flag.  Bounded: at the spin limit the
     //    workgroup raises the abort flag, and every waiter leaves on it; the host then
     //    reports the failure.
     const int nused = G * S::REC + S::VW * W;
     const int nv = min(NU, max(0, (nused - tid + 255) >> 8));  // strips below nused
     const unsigned valid = nv >= 32 ? ~0u : (1u << nv) - 1u;
-    const unsigned long long* const colt = col + tid;
-    unsigned long long xs[NU];
+    const unsigned long long* const colt = col + (tid >> 6) * (NU * 64) + lane;  // strip u: + 64 u
+    // every word goes to the LDS image as it arrives; only the pending mask is carried
+    unsigned pend = 0;
+    {
+      unsigned long long xs[NU];
 #pragma unroll
-    for (int u = 0; u < NU; ++u) xs[u] = slot_load(colt + u * 256);
+      for (int u = 0; u < NU; ++u) xs[u] = slot_load(colt + u * 64);
+#pragma unroll
+      for (int u = 0; u < NU; ++u) {
+        xim[u * 256 + tid] = (unsigned)xs[u];
+        pend |= (unsigned)((unsigned)(xs[u] >> 32) != flag) << u;
+      }
+    }
+    pend &= valid;
     int bad = 0;
-    for (int it = 0;; ++it) {
-      unsigned pend = 0;
-#pragma unroll
-      for (int u = 0; u < NU; ++u) pend |= (unsigned)((unsigned)(xs[u] >> 32) != flag) << u;
-      pend &= valid;
-      if (!pend) break;
+    for (int it = 0; pend; ++it) {
       if (it >= spin_limit) {
         __hip_atomic_store(&sync[2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         bad = 1;
@@ -739,12 +772,26 @@ __global__ __launch_bounds__(FFP_THREADS) void ffp_kernel(T* __restrict__ A, int
         break;
       }
       __builtin_amdgcn_s_sleep(1);
+      // again the groups of 4 strips that hold a pending slot
 #pragma unroll
-      for (int u = 0; u < NU; ++u)
-        if ((pend >> u) & 1u) xs[u] = slot_load(colt + u * 256);
+      for (int u0 = 0; u0 < NU; u0 += 4) {
+        if ((pend >> u0) & 15u) {
+          constexpr int GU = 4;
+          unsigned long long y[GU];
+#pragma unroll
+          for (int q = 0; q < GU; ++q)
+            if (u0 + q < NU) y[q] = slot_load(colt + (u0 + q) * 64);
+          unsigned nb = 0;
+#pragma unroll
+          for (int q = 0; q < GU; ++q)
+            if (u0 + q < NU) {
+              xim[(u0 + q) * 256 + tid] = (unsigned)y[q];
+              nb |= (unsigned)((unsigned)(y[q] >> 32) != flag) << q;
+            }
+          pend = (pend & ~(15u << u0)) | ((nb << u0) & valid);
+        }
+      }
     }
-#pragma unroll
-    for (int u = 0; u < NU; ++u) xim[u * 256 + tid] = (unsigned)xs[u];
     if (__syncthreads_or(bad)) {  // uniform; the host sees sync[2] and reports the failure
       live = false;
       jfail = jj;
@@ -755,7 +802,6 @@ __global__ __launch_bounds__(FFP_THREADS) void ffp_kernel(T* __restrict__ A, int
     //    candidates itself
     v = -1.0;
     id = -1;
-    const int lane = tid & 63;
     if (lane < G && (int)xim[lane * S::REC] >= 0) {
       v = fabs((double)reinterpret_cast<const T*>(xim + lane * S::REC + S::VOFF)[jj]);
       id = (int)xim[lane * S::REC];
@@ -773,47 +819,52 @@ __global__ __launch_bounds__(FFP_THREADS) void ffp_kernel(T* __restrict__ A, int
       ipiv[j] = p;
       if (v == 0.0 && *info == 0) *info = j + 1;
     }
-    if (p != j && tid < w) {
-      if (j >= r0 && j < r0 + nr) L[(j - r0) * LD + tid] = urow[tid];
-      if (p >= r0 && p < r0 + nr) L[(p - r0) * LD + tid] = jrow[tid];
+    // 4. the interchange: row j's owner takes the pivot row, row p's owner row j
+    //    (registers of one thread each: no barrier)
+    if (p != j) {
+      if (j >= r0 && j < r0 + nr && tid == j - r0) {
+#pragma unroll
+        for (int c = 0; c < W; ++c) row[c] = urow[c];
+      } else if (p >= r0 && p < r0 + nr && tid == p - r0) {
+#pragma unroll
+        for (int c = 0; c < W; ++c) row[c] = jrow[c];
+      }
     }
-    __syncthreads();
-    // the multiplier of this thread's row (if below j), then the rank-1 update in
-    // two halves of 16 columns with that half of the pivot row in registers
-    // (constant indices): every LDS read of the row is independent of its writes.
-    // The kernel stays within 144 VGPRs (FP64), so a workgroup fits beside a GEMM
-    // workgroup (368) on one CU
+    // 5. rows below j: the multiplier (column jj; jj is uniform, so the register
+    //    index is too), the rank-1 update of columns jj < c < w with the pivot row,
+    //    and the candidate of the next column
     const bool below = own && r0 + r > j;
     const T piv = urow[jj];
     T l = T(0);
     if (below) {
-      l = L[r * LD + jj];
+      T a = T(0);
+#pragma unroll
+      for (int c = 0; c < W; ++c) a = c == jj ? row[c] : a;
+      l = a;
       if (piv != T(0)) l = l / piv;
-      L[r * LD + jj] = l;
+#pragma unroll
+      for (int c = 0; c < W; ++c) row[c] = c == jj ? l : row[c];
     }
 #pragma unroll
-    for (int h = 0; h < W; h += 16) {
-      if (h + 16 > jj + 1 && h < w) {  // uniform: some column c of the half has jj < c < w
-        T uh[16];
+    for (int h = 0; h < W; h += 8) {
+      if (h + 8 > jj + 1 && h < w) {  // uniform: some column c of the 8 has jj < c < w
+        T uh[8];
 #pragma unroll
-        for (int c = 0; c < 16; ++c) uh[c] = urow[h + c];
+        for (int c = 0; c < 8; ++c) uh[c] = urow[h + c];
         if (below) {
-          T* row = L + r * LD + h;
-          T x[16];
 #pragma unroll
-          for (int c = 0; c < 16; ++c) x[c] = row[c];
-#pragma unroll
-          for (int c = 0; c < 16; ++c)
-            if (h + c > jj && h + c < w) row[c] = x[c] - l * uh[c];
+          for (int c = 0; c < 8; ++c)
+            if (h + c > jj && h + c < w) row[h + c] = row[h + c] - l * uh[c];
         }
       }
     }
-    // this thread's candidate for the next column (its own row, just written); the
-    // next argmax_256 synchronises before any other thread reads the row
     v = -1.0;
     id = -1;
     if (below && jj + 1 < w) {
-      v = fabs((double)L[r * LD + jj + 1]);
+      T x = T(0);
+#pragma unroll
+      for (int c = 0; c < W; ++c) x = c == jj + 1 ? row[c] : x;
+      v = fabs((double)x);
       id = r0 + r;
     }
   }
@@ -824,10 +875,12 @@ __global__ __launch_bounds__(FFP_THREADS) void ffp_kernel(T* __restrict__ A, int
       for (int jj = jfail + tid; jj < w; jj += FFP_THREADS) ipiv[c0 + jj] = c0 + jj;
     return;
   }
-  for (int e = tid; e < nr * w; e += FFP_THREADS) {
-    const int r = e % nr, c = e / nr;
-    A[(int64_t)(r0 + r) + (int64_t)(c0 + c) * lda] = L[r * LD + c];
+  if (own) {
+#pragma unroll
+    for (int c = 0; c < W; ++c)
+      if (c < w) A[(int64_t)(r0 + r) + (int64_t)(c0 + c) * lda] = row[c];
   }
+  __syncthreads();  // s_piv of the last column
   // the interchanges on the panel's other columns: the affected rows are rows
   // c0 .. c1-1 and the pivot rows (2 w entries, duplicates harmless: they carry the
   // same value); entry k's content comes from the row found by undoing the
@@ -887,10 +940,9 @@ size_t ffp_bytes() { return (size_t)PANEL_BASE * FfpSlots<double, PANEL_BASE>::C
 // rows per workgroup, or false when it does not fit (the per-column kernels run).
 template <typename T>
 bool ffp_shape(int m, int ncu, int& grid, int& rpw) {
-  // one row per thread: a second row for a few threads would double the latency of
-  // every column step for the whole workgroup
-  const int rpw_max = std::min(FFP_THREADS, FFP_ROW_BYTES / (int)((PANEL_BASE + 1) * sizeof(T)));
-  grid = (int)cdiv(m, rpw_max);
+  // one row per thread, held in registers: a second row for a few threads would
+  // double the latency of every column step for the whole workgroup
+  grid = (int)cdiv(m, FFP_THREADS);
   if (m < 1 || grid > std::min(ncu, FFP_MAX_GRID)) return false;
   rpw = (int)cdiv(m, grid);
   grid = (int)cdiv(m, rpw);
@@ -904,12 +956,6 @@ int ffp_launch(T* A, int64_t lda, int n, int c0, int c1, int k0, int k1, int* ip
   int grid, rpw;
   if (!ffp_shape<T>(n - c0, w.ncu, grid, rpw) || rpw > FFP_THREADS || c1 - c0 > PANEL_BASE || c1 > n)
     return KAD_ERR_ARG;
-  static bool lds_set = false;  // one attribute call per element type (dynamic LDS above 64 KB)
-  if (!lds_set) {
-    KAD_HIP_CHECK(hipFuncSetAttribute((const void*)ffp_kernel<T, PANEL_BASE>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, FFP_ROW_BYTES));
-    lds_set = true;
-  }
   // flags: PANEL_BASE per launch, never one the slots still hold (the buffer starts
   // zeroed and the sequence at 1; before the 32-bit flags wrap it is cleared again)
   if (++w.ffp_seq >= (1u << 26)) {
@@ -917,8 +963,7 @@ int ffp_launch(T* A, int64_t lda, int n, int c0, int c1, int k0, int k1, int* ip
     w.ffp_seq = 1;
   }
   const unsigned flag0 = w.ffp_seq * (unsigned)PANEL_BASE;
-  const size_t lds = (size_t)rpw * (PANEL_BASE + 1) * sizeof(T);
-  ffp_kernel<T, PANEL_BASE><<<grid, FFP_THREADS, lds, w.stream>>>(
+  ffp_kernel<T, PANEL_BASE><<<grid, FFP_THREADS, 0, w.stream>>>(
       A, lda, n, c0, c1, k0, k1, rpw, (unsigned long long*)w.ffp, flag0, ipiv, w.info, w.ffp_sync, w.ffp_spin);
   KAD_HIP_CHECK(hipGetLastError());
   return KAD_OK;
