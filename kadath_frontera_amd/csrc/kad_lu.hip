@@ -667,6 +667,7 @@ __global__ __launch_bounds__(FFP_THREADS) void ffp_kernel(T* __restrict__ A, int
                                                           int* __restrict__ sync, int spin_limit) {
   static_assert(FFP_MAX_GRID <= 64, "one candidate per lane of a wave");
   static_assert(FFP_THREADS == 256, "256-slot strips");
+  static_assert((W & (W - 1)) == 0, "rotation mask");
   using S = FfpSlots<T, W>;
   constexpr int NU = S::NU;
   // issue priority over the GEMM waves that share the SIMDs: every column waits for
@@ -689,8 +690,9 @@ __global__ __launch_bounds__(FFP_THREADS) void ffp_kernel(T* __restrict__ A, int
   // this thread's row of the block, in registers for the whole launch (one row per
   // thread: the host launches at most FFP_THREADS rows per workgroup); columns past
   // the panel's end hold zeros
-  // (every index into it is a compile-time constant after unrolling, so it stays in
-  // registers; a uniform column index is reached by value selects)
+  // (rotated by one position per column: at column jj, row[q] holds column
+  // (q + jj) mod W, so the active column is row[0] and every index into the array is
+  // a compile-time constant)
   const int r = tid;
   const bool own = r < nr;
   T row[W];
@@ -716,11 +718,11 @@ __global__ __launch_bounds__(FFP_THREADS) void ffp_kernel(T* __restrict__ A, int
     const bool rowj_wave = j >= r0 && j < r0 + nr && (tid >> 6) == ((j - r0) >> 6);
     if (cand_wave && tid == id - r0) {
 #pragma unroll
-      for (int c = 0; c < W; ++c) stage[0][c] = row[c];
+      for (int q = 0; q < W; ++q) stage[0][(q + jj) & (W - 1)] = row[q];
     }
     if (rowj_wave && tid == j - r0) {
 #pragma unroll
-      for (int c = 0; c < W; ++c) stage[1][c] = row[c];
+      for (int q = 0; q < W; ++q) stage[1][(q + jj) & (W - 1)] = row[q];
     }
     if (cand_wave || rowj_wave) {
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -824,48 +826,38 @@ __global__ __launch_bounds__(FFP_THREADS) void ffp_kernel(T* __restrict__ A, int
     if (p != j) {
       if (j >= r0 && j < r0 + nr && tid == j - r0) {
 #pragma unroll
-        for (int c = 0; c < W; ++c) row[c] = urow[c];
+        for (int q = 0; q < W; ++q) row[q] = urow[(q + jj) & (W - 1)];
       } else if (p >= r0 && p < r0 + nr && tid == p - r0) {
 #pragma unroll
-        for (int c = 0; c < W; ++c) row[c] = jrow[c];
+        for (int q = 0; q < W; ++q) row[q] = jrow[(q + jj) & (W - 1)];
       }
     }
-    // 5. rows below j: the multiplier (column jj; jj is uniform, so the register
-    //    index is too), the rank-1 update of columns jj < c < w with the pivot row,
-    //    and the candidate of the next column
+    // 5. rows below j: the multiplier (column jj: row[0]), the rank-1 update of
+    //    columns jj < c < W (row[q], q = c - jj: no wrap; columns past the panel's end
+    //    are never written back) with the pivot row, and the candidate of the next
+    //    column (row[1]); then every row rotates by one position
     const bool below = own && r0 + r > j;
-    const T piv = urow[jj];
-    T l = T(0);
-    if (below) {
-      T a = T(0);
-#pragma unroll
-      for (int c = 0; c < W; ++c) a = c == jj ? row[c] : a;
-      l = a;
-      if (piv != T(0)) l = l / piv;
-#pragma unroll
-      for (int c = 0; c < W; ++c) row[c] = c == jj ? l : row[c];
-    }
-#pragma unroll
-    for (int h = 0; h < W; h += 8) {
-      if (h + 8 > jj + 1 && h < w) {  // uniform: some column c of the 8 has jj < c < w
-        T uh[8];
-#pragma unroll
-        for (int c = 0; c < 8; ++c) uh[c] = urow[h + c];
-        if (below) {
-#pragma unroll
-          for (int c = 0; c < 8; ++c)
-            if (h + c > jj && h + c < w) row[h + c] = row[h + c] - l * uh[c];
-        }
-      }
-    }
+    const T* const uj = urow + jj;  // uj[q]: the pivot row's column jj + q
+    const T piv = uj[0];
     v = -1.0;
     id = -1;
-    if (below && jj + 1 < w) {
-      T x = T(0);
+    if (below) {
+      T l = row[0];
+      if (piv != T(0)) l = l / piv;
+      row[0] = l;
 #pragma unroll
-      for (int c = 0; c < W; ++c) x = c == jj + 1 ? row[c] : x;
-      v = fabs((double)x);
-      id = r0 + r;
+      for (int q = 1; q < W; ++q)
+        if (q < W - jj) row[q] = row[q] - l * uj[q];
+      if (jj + 1 < w) {
+        v = fabs((double)row[1]);
+        id = r0 + r;
+      }
+    }
+    {
+      const T t = row[0];
+#pragma unroll
+      for (int q = 0; q < W - 1; ++q) row[q] = row[q + 1];
+      row[W - 1] = t;
     }
   }
   if (!live) {
@@ -875,10 +867,12 @@ __global__ __launch_bounds__(FFP_THREADS) void ffp_kernel(T* __restrict__ A, int
       for (int jj = jfail + tid; jj < w; jj += FFP_THREADS) ipiv[c0 + jj] = c0 + jj;
     return;
   }
-  if (own) {
+  if (own) {  // after w columns, row[q] holds column (q + w) mod W
 #pragma unroll
-    for (int c = 0; c < W; ++c)
-      if (c < w) A[(int64_t)(r0 + r) + (int64_t)(c0 + c) * lda] = row[c];
+    for (int q = 0; q < W; ++q) {
+      const int c = (q + w) & (W - 1);
+      if (c < w) A[(int64_t)(r0 + r) + (int64_t)(c0 + c) * lda] = row[q];
+    }
   }
   __syncthreads();  // s_piv of the last column
   // the interchanges on the panel's other columns: the affected rows are rows
