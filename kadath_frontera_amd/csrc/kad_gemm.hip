@@ -87,6 +87,9 @@ __global__ __launch_bounds__(GNT) void gemm_k4_kernel(int m, int n, int k,
                                                       typename GemmCfg<P>::T* __restrict__ C, int64_t ldc,
                                                       TileSel sel) {
   using T = typename GemmCfg<P>::T;
+  // the small products (panel recursion, triangular-solve updates) lie on the
+  // factorisation's critical path: issue priority over the bulk update's waves
+  if constexpr (BM == 64) __builtin_amdgcn_s_setprio(2);
   constexpr int BK = GemmCfg<P>::BK;
   constexpr int PADA = (sizeof(T) == 8) ? 4 : 8;
   constexpr int LDA_S = BM + PADA, LDB_S = BM + PADA;

@@ -247,6 +247,7 @@ template <typename T>
 __global__ __launch_bounds__(64, 4) void trsm_leaf_kernel(const T* __restrict__ L, int64_t lda, T* __restrict__ B,
                                                        int64_t ldb, int ncols, const int* __restrict__ cmask) {
   constexpr int R = TRSM_FUSED;
+  __builtin_amdgcn_s_setprio(2);  // a serial chain on the critical path, beside the bulk GEMM
   __shared__ T Ls[R * R + R];  // column-major, Ls[i + R k] = L(i, k); R entries of padding read past the end
   const int c0 = blockIdx.x * 64;
   if (cmask && !cmask[c0 / 128]) return;  // uniform: the block lies in one 128-column tile

@@ -309,7 +309,12 @@ int kad_gemm(kad_ctx* ctx, int prec, int op, int m, int n, int k, const void* A,
 int kad_lu_factor(kad_ctx* ctx, int prec, int n, void* A, int64_t lda, int* ipiv, int* perm, int nb) {
   if (!ctx || !A || !ipiv || n <= 0 || lda < n || prec < 0 || prec > 2) return KAD_ERR_ARG;
   Ctx* c = reinterpret_cast<Ctx*>(ctx);
-  if (nb <= 0) nb = (prec == kad::BF16) ? 512 : 256;
+  if (nb <= 0) {
+    // block size: KAD_LU_NB (a multiple of the 128-row tiles), else 256 (512 for BF16)
+    nb = (prec == kad::BF16) ? 512 : 256;
+    const char* e = std::getenv("KAD_LU_NB");
+    if (e && std::atoi(e) > 0 && std::atoi(e) % 128 == 0) nb = std::atoi(e);
+  }
   if (nb % 16 != 0) return KAD_ERR_ARG;
   int rc = c->lu_work(n, nb);
   if (rc) return rc;
