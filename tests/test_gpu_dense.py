@@ -155,8 +155,10 @@ def test_persistent_panel_ties_match_per_column(ctx, prec, monkeypatch):
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 def test_persistent_panel_large_grid_matches_per_column(ctx, prec, monkeypatch):
-    """Dense n = 40000: the first panels span 40000 rows, 173 workgroups in the grid
-    barrier.  Bitwise agreement with the per-column kernels."""
+    """Dense n = 40000: the first panels are taller than the persistent panel takes
+    (64 workgroups of 256 rows) and run on the per-column kernels; from 16384 rows down
+    the persistent panel runs with up to 64 workgroups exchanging candidates.  Bitwise
+    agreement with the per-column kernels throughout."""
     print(f"persistent panel n=40000 {prec}", flush=True)
     n = 40000
     dt = torch.float64 if prec == "f64" else torch.float32
@@ -177,9 +179,10 @@ def test_persistent_panel_large_grid_matches_per_column(ctx, prec, monkeypatch):
 
 def test_persistent_panel_abandoned_barrier_reports_error(ctx, monkeypatch):
     """The persistent panel's waits are bounded: with a spin limit of 0 (set from the
-    host, KAD_LU_SPIN) its grid barriers are abandoned at once, every workgroup exits,
-    and kad_lu_factor reports an error instead of waiting.  A context with the default
-    limit then factors the same matrix with LAPACK's pivots."""
+    host, KAD_LU_SPIN) its waits for the other workgroups' candidates are abandoned at
+    once, every workgroup exits, and kad_lu_factor reports an error instead of waiting.
+    A context with the default limit then factors the same matrix with LAPACK's
+    pivots."""
     from kadath_frontera_amd.device import Context
     from kadath_frontera_amd._lib import KadError
     print("persistent panel abandoned barrier", flush=True)
@@ -199,6 +202,41 @@ def test_persistent_panel_abandoned_barrier_reports_error(ctx, monkeypatch):
     assert info == 0
     _, piv_ref = scipy.linalg.lu_factor(A)
     assert np.array_equal(ipiv.cpu().numpy(), piv_ref)
+
+
+def test_persistent_panel_back_to_back_factorisations(ctx):
+    """The persistent panel's exchange slots are reused by every factorisation, each
+    word flagged with a launch sequence that must never repeat while a stale word
+    with that flag is still there.  Different matrices factored back to back on one
+    context (and the first again) each give LAPACK's pivots and a backward-stable
+    factorisation."""
+    print("persistent panel back-to-back factorisations", flush=True)
+    n = 2600  # 11 workgroups for the first panels, a partial last panel
+    mats = [np.random.default_rng(40 + q).standard_normal((n, n)) for q in range(3)]
+    for q in (0, 1, 2, 0):
+        A = mats[q]
+        dA = colmajor(A, torch.float64)
+        ipiv, perm, info = ctx.lu_factor("f64", dA)
+        assert info == 0
+        lu_ref, piv_ref = scipy.linalg.lu_factor(A)
+        assert np.array_equal(ipiv.cpu().numpy(), piv_ref), q
+        np.testing.assert_allclose(dA.cpu().numpy(), lu_ref, rtol=0, atol=1e-9 * np.abs(lu_ref).max())
+
+
+def test_lu_block_size_override(ctx, monkeypatch):
+    """KAD_LU_NB (read when the block size is left at 0) changes the blocking, not the
+    pivoting: LAPACK's pivots at NB = 128 and 512."""
+    print("lu block size override", flush=True)
+    n = 1500
+    A = np.random.default_rng(77).standard_normal((n, n))
+    _, piv_ref = scipy.linalg.lu_factor(A)
+    for nb in ("128", "512"):
+        monkeypatch.setenv("KAD_LU_NB", nb)
+        dA = colmajor(A, torch.float64)
+        ipiv, _, info = ctx.lu_factor("f64", dA)
+        assert info == 0
+        assert np.array_equal(ipiv.cpu().numpy(), piv_ref), nb
+    monkeypatch.delenv("KAD_LU_NB")
 
 
 @pytest.mark.parametrize("trans", [0, 1])
