@@ -65,7 +65,10 @@ int kad_gemm(kad_ctx* ctx, int prec, int op, int m, int n, int k, const void* A,
 /* In-place LU with partial pivoting of the n x n column-major A (fp64 for
  * KAD_PREC_F64, fp32 otherwise).  ipiv: n ints, 0-based interchanges (LAPACK
  * dgetrf semantics shifted by one); perm (may be NULL): n ints, composed row
- * order used by kad_lu_solve.  nb: outer panel width (0 = default). */
+ * order used by kad_lu_solve.  nb: outer panel width (0 = default: KAD_LU_NB, else
+ * 256, 512 for BF16).  Returns info (> 0: first exact zero pivot, 1-based), or
+ * KAD_ERR_HIP when a wait of the persistent panel reached its spin limit (the
+ * factors are then not valid). */
 int kad_lu_factor(kad_ctx* ctx, int prec, int n, void* A, int64_t lda, int* ipiv, int* perm, int nb);
 
 /* Solve op(A) x = b in place with the factors (b fp64).  trans = 1: A^T x = b
