@@ -56,8 +56,8 @@ struct LuWork {
   void* tinv_m = nullptr;
   void* tmp_m = nullptr;
   size_t tmp_m_bytes = 0;
-  // persistent base panel: candidate / row exchange buffers, grid-barrier words
-  // (arrivals, exits, abort) and the CU count that bounds its grid
+  // persistent base panel: the flagged exchange slots, the abort word (ffp_sync[2])
+  // and the CU count that bounds its grid
   void* ffp = nullptr;
   int* ffp_sync = nullptr;
   int ncu = 0;
