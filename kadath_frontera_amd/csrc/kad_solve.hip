@@ -332,9 +332,9 @@ int kad_lu_factor(kad_ctx* ctx, int prec, int n, void* A, int64_t lda, int* ipiv
   KAD_HIP_CHECK(hipStreamSynchronize(c->stream));
   const int info = words[0];
   if (words[6] != 0) {
-    // a grid barrier of the persistent panel reached its spin limit: the factors are
-    // not valid; clear the barrier words for the next call
-    std::fprintf(stderr, "kad_lu_factor: persistent panel barrier timed out\n");
+    // a wait of the persistent panel reached its spin limit: the factors are not
+    // valid; clear the abort word for the next call
+    std::fprintf(stderr, "kad_lu_factor: persistent panel wait timed out\n");
     KAD_HIP_CHECK(hipMemsetAsync(c->lu.ffp_sync, 0, 4 * sizeof(int), c->stream));
     KAD_HIP_CHECK(hipStreamSynchronize(c->stream));
     return KAD_ERR_HIP;

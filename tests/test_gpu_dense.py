@@ -134,9 +134,9 @@ def _per_column_factor(monkeypatch, prec, A):
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 def test_persistent_panel_ties_match_per_column(ctx, prec, monkeypatch):
-    """The persistent base panel (one launch per 32 columns, one grid barrier per
-    column, rows held in LDS by 13 workgroups here) performs the per-column kernels'
-    operations in the same order: pivots and factors agree bit for bit.  Integer
+    """The persistent base panel (one launch per 32 columns, one candidate exchange
+    per column, rows held in registers by 12 workgroups here) performs the per-column
+    kernels' operations in the same order: pivots and factors agree bit for bit.  Integer
     entries make exact ties in |a|, where the choice must be the first index (the
     first column's pivot is checked against argmax directly)."""
     print(f"persistent panel ties {prec}", flush=True)
