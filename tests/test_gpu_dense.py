@@ -223,6 +223,28 @@ def test_persistent_panel_back_to_back_factorisations(ctx):
         np.testing.assert_allclose(dA.cpu().numpy(), lu_ref, rtol=0, atol=1e-9 * np.abs(lu_ref).max())
 
 
+def test_persistent_panel_interleaved_precisions(ctx):
+    """FP64 and FP32 factorisations share one context's exchange slots and launch
+    sequence.  Interleaved, the FP64 ones keep LAPACK's pivots and the two FP32
+    factorisations of the same matrix agree bit for bit."""
+    print("persistent panel interleaved precisions", flush=True)
+    n = 2100
+    A64 = np.random.default_rng(90).standard_normal((n, n))
+    A32 = np.random.default_rng(91).standard_normal((n, n))
+    _, piv_ref = scipy.linalg.lu_factor(A64)
+    out32 = []
+    for q in range(2):
+        d64 = colmajor(A64, torch.float64)
+        ipiv, _, info = ctx.lu_factor("f64", d64)
+        assert info == 0 and np.array_equal(ipiv.cpu().numpy(), piv_ref), q
+        d32 = colmajor(A32, torch.float32)
+        ipiv32, _, info32 = ctx.lu_factor("f32", d32)
+        assert info32 == 0
+        out32.append((ipiv32.cpu().clone(), d32.cpu().clone()))
+    assert torch.equal(out32[0][0], out32[1][0])
+    assert torch.equal(out32[0][1], out32[1][1])
+
+
 def test_lu_block_size_override(ctx, monkeypatch):
     """KAD_LU_NB (read when the block size is left at 0) changes the blocking, not the
     pivoting: LAPACK's pivots at NB = 128 and 512."""
