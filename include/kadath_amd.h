@@ -95,15 +95,19 @@ int kad_cast_f64_to_f32(kad_ctx* ctx, int n, const double* A, int64_t lda, float
 
 /* Mixed-precision solve: fp64 residuals with A64, corrections from the
  * low-precision factors (prec_lo, LUlo, perm).  mode 0 = classical IR,
- * mode 1 = GMRES-IR.  Stops at ||b - op(A) x||_inf <= tol ||b||_inf, or when an
- * outer step gains less than a factor 2 (a step that increases the residual is
- * undone).  iters[0] = outer steps, iters[1] = inner GMRES steps.  Returns
- * KAD_ERR_NOCONV if tol is not reached (x holds the best iterate). */
+ * mode 1 = GMRES-IR (flexible GMRES right-preconditioned by the factors, so the
+ * attainable accuracy is that of the fp64 products).  Stops at
+ * ||b - op(A) x||_inf <= tol ||b||_inf, or when two outer steps in a row gain less
+ * than a factor 2 (a step that increases the residual is undone).  Products with A
+ * skip its exactly-zero 256 x 1024 tiles, found by one scan per call.
+ * iters[0] = outer steps, iters[1] = inner GMRES steps.  Returns KAD_ERR_NOCONV if
+ * tol is not reached (x holds the best iterate; x = 0 if no step helped). */
 int kad_ir_solve(kad_ctx* ctx, int n, const double* A64, int64_t lda, int prec_lo, const void* LUlo, int64_t ldlu,
                  const int* perm, const double* b, double* x, double tol, int max_outer, int mode, int trans,
                  int* iters, double* resid);
 
-/* GMRES-IR restart length of the context (default 60, at most 4096). */
+/* GMRES-IR Krylov dimension per restart of the context (default 100, at most 4096;
+ * the basis takes (2 m + 1) n doubles). */
 int kad_ir_set_restart(kad_ctx* ctx, int m);
 /* Residual history of the last refinement (HOST out): kind 0 = inner GMRES
  * residual 2-norms, kind 1 = ||b||_inf then ||b - op(A) x||_inf after each outer
@@ -194,7 +198,11 @@ int kad_xcts_do_newton(kad_system* s, double precision, double* error, int solve
 /* seeded perturbation amp 2^-(i+j+k) U(-1,1) of every unknown (splitmix64, drawn in
  * the reference unknown order whatever the system's order), HOST out */
 int kad_xcts_perturbation(kad_system* s, uint64_t seed, double amp, double* out);
-/* refinement statistics of the last low-precision solve */
+/* Statistics of the last do_newton step's solve.  iters[3] (HOST): outer refinement
+ * steps, inner GMRES steps, and 1 if the low-precision refinement did not reach a
+ * relative residual of 1e-6 and the step was solved with an fp64 factorisation of
+ * the same J instead (0 otherwise, and for solver 0); resid: the refinement's final
+ * relative residual ||F - J X||_inf / ||F||_inf. */
 int kad_xcts_last_solve(kad_system* s, int* iters, double* resid);
 /* host-only tracing of the same system (no device): info[6] = n_unknowns, n_rows,
  * value instr, tangent instr, value slots, tangent programs; returns #layout entries */

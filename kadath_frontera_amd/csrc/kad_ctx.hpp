@@ -86,6 +86,27 @@ struct GemmTimer {
 extern GemmTimer* g_gemm_timer;   // GEMM launches (work = flops)
 extern GemmTimer* g_apply_timer;  // apply3d launches (work = HBM bytes: arrays in + out)
 
+// Nonzero 256 x 1024 tiles of an n x n fp64 matrix, for products that skip the
+// structural zeros (kad_struct.hip).  Valid for the contents A had at build().
+struct GemvMap {
+  const double* A = nullptr;
+  int64_t lda = 0;
+  int n = 0, nrb = 0, nchunk = 0, nact = 0;
+  int* map = nullptr;   // nchunk x nrb flags
+  int* list = nullptr;  // (row tile, chunk) pairs of the nonzero tiles
+  size_t cap = 0;
+  int build(const double* A, int64_t lda, int n, hipStream_t s);
+  int apply(const double* x, const double* b, double* y, int trans, double* part, hipStream_t s) const;
+  ~GemvMap();
+};
+
+// Row extents of the panel updates of the two triangular sweeps, per 128-column
+// block (kad_trsv.hip): the forward update ends at row fhi[b], the backward one
+// starts at row blo[b]; the factor is exactly zero outside.
+struct TrsvExtents {
+  std::vector<int> fhi, blo;
+};
+
 // Device copy of a chain of 1-D passes (matrix pool + per-line tables).
 struct DevPlan {
   Passes P;
@@ -109,7 +130,11 @@ struct Ctx {
   size_t cap_krylov = 0;
   double* trs_tab = nullptr;    // triangular-solve tables of the current factors
   size_t cap_trs_tab = 0;
-  int gmres_m = 60;                 // GMRES-IR restart length
+  int* trs_flag = nullptr;      // nonzero flags of the factor tiles (set-up scratch)
+  size_t cap_trs_flag = 0;
+  TrsvExtents trs_ext;          // row extents of the current factors' sweeps
+  GemvMap jmap;                 // nonzero tiles of the refinement's fp64 matrix
+  int gmres_m = 100;                // GMRES-IR Krylov dimension per restart
   std::vector<double> hist_inner;   // last refinement: inner GMRES residual 2-norms
   std::vector<double> hist_outer;   // last refinement: true residual inf-norms
 

@@ -19,11 +19,13 @@ int gemm_launch(int prec, int op, int m, int n, int k, const void* A, int64_t ld
                 int64_t ldc, hipStream_t s);
 template <typename T> int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w);
 size_t trsv_table_doubles(int n);
+size_t trsv_flag_ints(int n);
 size_t ffp_bytes();
-template <typename T> int trsv_prepare(const T* LU, int64_t lda, int n, int trans, double* W, hipStream_t s);
 template <typename T>
-int getrs_fast(const T* LU, int64_t lda, int n, const int* perm, const double* W, double* b, int trans,
-               double* work, hipStream_t s);
+int trsv_prepare(const T* LU, int64_t lda, int n, int trans, double* W, int* flag, TrsvExtents& ext, hipStream_t s);
+template <typename T>
+int getrs_fast(const T* LU, int64_t lda, int n, const int* perm, const double* W, const TrsvExtents& ext, double* b,
+               int trans, double* work, hipStream_t s);
 bool ipiv_to_perm(const int* ipiv, int n, int* perm);
 int gemv64(const double* A, int64_t lda, int n, const double* x, const double* b, double* y, int trans, double* part,
            hipStream_t s);
@@ -141,20 +143,21 @@ int Ctx::vec_work(int n) {
   if ((rc = ensure((void**)&gemv_part, &cap_part, std::max<size_t>(part, RED_BLOCKS * sizeof(double))))) return rc;
   if ((rc = ensure((void**)&vwork, &cap_vwork, (size_t)8 * n * sizeof(double)))) return rc;
   if ((rc = ensure((void**)&trs_tab, &cap_trs_tab, trsv_table_doubles(n) * sizeof(double)))) return rc;
+  if ((rc = ensure((void**)&trs_flag, &cap_trs_flag, trsv_flag_ints(n) * sizeof(int)))) return rc;
   return KAD_OK;
 }
 
 // Triangular-solve tables of one set of factors (for op(A), trans as in solve),
 // then b <- op(A)^{-1} b with them.
 int Ctx::solve_prepare(int prec, int n, const void* LU, int64_t lda, int trans) {
-  if (prec == F64) return trsv_prepare<double>((const double*)LU, lda, n, trans, trs_tab, stream);
-  return trsv_prepare<float>((const float*)LU, lda, n, trans, trs_tab, stream);
+  if (prec == F64) return trsv_prepare<double>((const double*)LU, lda, n, trans, trs_tab, trs_flag, trs_ext, stream);
+  return trsv_prepare<float>((const float*)LU, lda, n, trans, trs_tab, trs_flag, trs_ext, stream);
 }
 
 int Ctx::solve(int prec, int n, const void* LU, int64_t lda, const int* perm, double* b, int trans) {
   double* w = vwork + (size_t)4 * n;  // 3 n doubles; vwork[0:4n] belongs to the refinement loop
-  if (prec == F64) return getrs_fast<double>((const double*)LU, lda, n, perm, trs_tab, b, trans, w, stream);
-  return getrs_fast<float>((const float*)LU, lda, n, perm, trs_tab, b, trans, w, stream);
+  if (prec == F64) return getrs_fast<double>((const double*)LU, lda, n, perm, trs_tab, trs_ext, b, trans, w, stream);
+  return getrs_fast<float>((const float*)LU, lda, n, perm, trs_tab, trs_ext, b, trans, w, stream);
 }
 
 double Ctx::absmax(const double* x, int n) {
@@ -190,8 +193,8 @@ int Ctx::upload(const void* host, size_t bytes, void** dev) {
 Ctx::~Ctx() {
   for (auto& kv : plans) delete kv.second;
   for (void* p : owned) (void)hipFree(p);
-  void* bufs[] = {lu.red_val, lu.red_idx,   lu.counter, lu.tinv,   lu.tmp,    lu.tmp2, lu.map,  lu.mask_r,
-                  lu.swap_map, lu.ext,      lu.ffp,     gemv_part, vwork,     krylov,  trs_tab};
+  void* bufs[] = {lu.red_val, lu.red_idx, lu.counter, lu.tinv,    lu.tmp,    lu.tmp2,  lu.map,  lu.mask_r,
+                  lu.swap_map, lu.ext,     lu.ffp,     gemv_part, vwork,     krylov,   trs_tab, trs_flag};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (int q = 0; q < 2; ++q) {
@@ -435,10 +438,19 @@ int kad_ir_solve(kad_ctx* ctx, int n, const double* A64, int64_t lda, int prec_l
 // As kad_ir_solve, with the low-precision factors being those of diag(Rs) A diag(Cs):
 // the preconditioner applies v -> Cs .* LU^{-1} (Rs .* v) (Rs <-> Cs for the transpose),
 // while residuals and the stopping test use the unscaled A.
+//
+// GMRES-IR is the flexible variant: the preconditioned vectors z_j = M^{-1} v_j are
+// kept, and the correction is Z y.  The Arnoldi relation A Z = V H then holds to
+// fp64 GEMV accuracy whatever the rounding of the low-precision triangular solves
+// (which is not a linear map), so the attainable residual is set by the fp64
+// products and not by the preconditioner.  Products with A skip its exactly-zero
+// tiles (scanned once per call).
 int kad_ir_solve_scaled(kad_ctx* ctx, int n, const double* A64, int64_t lda, int prec_lo, const void* LUlo,
                         int64_t ldlu, const int* perm, const double* Rs, const double* Cs, const double* b, double* x,
                         double tol, int max_outer, int mode, int trans, int* iters, double* resid) {
-  if (!ctx || n <= 0 || !A64 || !LUlo || !perm || !b || !x) return KAD_ERR_ARG;
+  if (!ctx || n <= 0 || lda < n || ldlu < n || !A64 || !LUlo || !perm || !b || !x || max_outer < 1 ||
+      (mode != 0 && mode != 1) || (trans != 0 && trans != 1) || prec_lo < 0 || prec_lo > 2)
+    return KAD_ERR_ARG;
   Ctx* c = reinterpret_cast<Ctx*>(ctx);
   int rc = c->vec_work(n);
   if (rc) return rc;
@@ -447,6 +459,8 @@ int kad_ir_solve_scaled(kad_ctx* ctx, int n, const double* A64, int64_t lda, int
   double* r = c->vwork + (size_t)2 * n;
   double* d = c->vwork + (size_t)3 * n;
   if ((rc = c->solve_prepare(prec_lo, n, LUlo, ldlu, trans))) return rc;
+  if ((rc = c->jmap.build(A64, lda, n, s))) return rc;
+  const kad::GemvMap& Am = c->jmap;
   const double* pre = trans ? Cs : Rs;   // applied before the factor solve
   const double* post = trans ? Rs : Cs;  // applied after
   auto solve_lo = [&](double* v) {
@@ -460,18 +474,23 @@ int kad_ir_solve_scaled(kad_ctx* ctx, int n, const double* A64, int64_t lda, int
   c->hist_outer.assign(1, bnorm);
   KAD_HIP_CHECK(hipMemsetAsync(x, 0, n * sizeof(double), s));
   KAD_HIP_CHECK(hipMemcpyAsync(r, b, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-  int outer = 0, inner_total = 0;
+  int outer = 0, inner_total = 0, slow = 0;
   double rel = (bnorm == 0.0) ? 0.0 : 1.0;
+  if (bnorm != bnorm) rel = bnorm;  // NaN right-hand side: never reported as solved
   const int m = std::max(2, c->gmres_m);  // Krylov dimension per outer step
-  const double inner_tol = 1e-12;
+  const double inner_tol = 1e-14;
   std::vector<double> H, cs, sn, gvec, y;
   double* V = nullptr;
+  double* Z = nullptr;
   double* hbuf = nullptr;
   if (mode == 1) {
-    rc = c->ensure((void**)&c->krylov, &c->cap_krylov, (size_t)(m + 1) * n * sizeof(double) + (m + 8) * sizeof(double));
+    // V: m + 1 basis vectors, Z: m preconditioned vectors, then the small h / y buffer
+    rc = c->ensure((void**)&c->krylov, &c->cap_krylov,
+                   (size_t)(2 * m + 1) * n * sizeof(double) + (m + 8) * sizeof(double));
     if (rc) return rc;
     V = c->krylov;
-    hbuf = c->krylov + (size_t)(m + 1) * n;
+    Z = c->krylov + (size_t)(m + 1) * n;
+    hbuf = c->krylov + (size_t)(2 * m + 1) * n;
   }
   double* corr = d;  // the correction added by the current outer step
   while (rel > tol && outer < max_outer) {
@@ -481,9 +500,8 @@ int kad_ir_solve_scaled(kad_ctx* ctx, int n, const double* A64, int64_t lda, int
       kad::axpy_kernel<<<g, 256, 0, s>>>(n, 1.0, d, x);
       corr = d;
     } else {
-      // Right-preconditioned GMRES on A M^{-1} u = r (M = low-precision LU), CGS2
-      // with batched projections; the Givens residual is the true inner residual.
-      double* w = V + (size_t)m * n;       // work vector (slot m of the basis buffer is spare)
+      // Flexible right-preconditioned GMRES on A M^{-1} u = r (M = low-precision LU),
+      // CGS2 with batched projections; the Givens residual is the inner residual.
       const double beta = std::sqrt(c->dot(r, r, n));
       if (!(beta > 0.0)) break;
       KAD_HIP_CHECK(hipMemcpyAsync(V, r, n * sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -495,11 +513,12 @@ int kad_ir_solve_scaled(kad_ctx* ctx, int n, const double* A64, int64_t lda, int
       gvec[0] = beta;
       int kdim = 0;
       std::vector<double> hh(m + 1);
-      for (int j = 0; j < m - 1; ++j) {
+      for (int j = 0; j < m; ++j) {
         double* vj1 = V + (size_t)(j + 1) * n;
-        KAD_HIP_CHECK(hipMemcpyAsync(d, V + (size_t)j * n, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-        if ((rc = solve_lo(d))) return rc;  // z = M^{-1} v_j
-        if ((rc = kad::gemv64(A64, lda, n, d, nullptr, vj1, trans, c->gemv_part, s))) return rc;
+        double* zj = Z + (size_t)j * n;
+        KAD_HIP_CHECK(hipMemcpyAsync(zj, V + (size_t)j * n, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        if ((rc = solve_lo(zj))) return rc;  // z_j = M^{-1} v_j (kept)
+        if ((rc = Am.apply(zj, nullptr, vj1, trans, c->gemv_part, s))) return rc;
         for (int pass = 0; pass < 2; ++pass) {
           kad::multidot_kernel<<<j + 1, 256, 0, s>>>(V, n, j + 1, vj1, hbuf);
           kad::multiaxpy_kernel<<<g, 256, 0, s>>>(V, n, j + 1, hbuf, vj1, 1);
@@ -526,36 +545,37 @@ int kad_ir_solve_scaled(kad_ctx* ctx, int n, const double* A64, int64_t lda, int
         kdim = j + 1;
         ++inner_total;
         c->hist_inner.push_back(std::fabs(gvec[j + 1]));
-        // |g_{j+1}| is ||r - A M^{-1} V y||_2 >= the inf-norm the outer test uses
+        // |g_{j+1}| is ||r - A Z y||_2 >= the inf-norm the outer test uses
         if (std::fabs(gvec[j + 1]) <= std::max(inner_tol * beta, 0.5 * tol * bnorm) || hn == 0.0) break;
       }
       y.assign(kdim, 0.0);
       for (int i = kdim - 1; i >= 0; --i) {
         double sacc = gvec[i];
         for (int l = i + 1; l < kdim; ++l) sacc -= H[(size_t)i * m + l] * y[l];
-        y[i] = sacc / H[(size_t)i * m + i];
+        y[i] = (H[(size_t)i * m + i] != 0.0) ? sacc / H[(size_t)i * m + i] : 0.0;
       }
       KAD_HIP_CHECK(hipMemcpyAsync(hbuf, y.data(), kdim * sizeof(double), hipMemcpyHostToDevice, s));
-      kad::multiaxpy_kernel<<<g, 256, 0, s>>>(V, n, kdim, hbuf, w, 0);  // u = V y
-      if ((rc = solve_lo(w))) return rc;                                // x += M^{-1} u
-      kad::axpy_kernel<<<g, 256, 0, s>>>(n, 1.0, w, x);
-      corr = w;
+      kad::multiaxpy_kernel<<<g, 256, 0, s>>>(Z, n, kdim, hbuf, d, 0);  // correction Z y
+      kad::axpy_kernel<<<g, 256, 0, s>>>(n, 1.0, d, x);
+      corr = d;
     }
     // r = b - op(A) x in fp64
-    if ((rc = kad::gemv64(A64, lda, n, x, b, r, trans, c->gemv_part, s))) return rc;
+    if ((rc = Am.apply(x, b, r, trans, c->gemv_part, s))) return rc;
     const double rn = c->absmax(r, n);
     const double rel_new = (bnorm == 0.0) ? 0.0 : rn / bnorm;
     c->hist_outer.push_back(rn);
     ++outer;
-    if (rel_new > rel) {
-      // the step made the residual worse (attainable accuracy reached): undo it
+    if (!(rel_new <= rel)) {
+      // the step made the residual worse (attainable accuracy reached) or produced a
+      // NaN: undo it
       kad::axpy_kernel<<<g, 256, 0, s>>>(n, -1.0, corr, x);
-      if ((rc = kad::gemv64(A64, lda, n, x, b, r, trans, c->gemv_part, s))) return rc;
+      if ((rc = Am.apply(x, b, r, trans, c->gemv_part, s))) return rc;
       break;
     }
-    const bool stalled = outer > 1 && rel_new > 0.5 * rel;
+    // two outer steps in a row that gain less than a halving: attainable accuracy
+    slow = (rel_new > 0.5 * rel) ? slow + 1 : 0;
     rel = rel_new;
-    if (stalled) break;  // less than a halving per step: stop at the attainable accuracy
+    if (slow >= 2) break;
   }
   if (iters) {
     iters[0] = outer;

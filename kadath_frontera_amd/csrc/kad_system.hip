@@ -124,14 +124,23 @@ __global__ void csr_update_kernel(double* __restrict__ cf, const int* __restrict
   cf[q] += scale * s;
 }
 
+// max |x| over one workgroup.  A NaN propagates (fmax would drop it): a residual
+// holding a NaN must read as NaN, so that do_newton never reports it as converged
+// (the reference's max starts from data[0] and keeps a NaN there too).
 __global__ void absmax_kernel(const double* x, int n, double* out) {
   __shared__ double red[256];
   double m = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) m = fmax(m, fabs(x[i]));
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const double a = fabs(x[i]);
+    m = (a > m || a != a) ? a : m;
+  }
   red[threadIdx.x] = m;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    if (threadIdx.x < s) {
+      const double o = red[threadIdx.x + s];
+      if (o > red[threadIdx.x] || o != o) red[threadIdx.x] = o;
+    }
     __syncthreads();
   }
   if (threadIdx.x == 0) *out = red[0];
@@ -238,6 +247,7 @@ struct kad_system_impl {
   double last_times[4] = {0, 0, 0, 0};
   int last_iters[2] = {0, 0};
   double last_resid = 0;
+  int last_fallback = 0;  // the last low-precision step fell back to an fp64 factorisation
 
   ~kad_system_impl() {
     for (void* p : allocs) (void)hipFree(p);
@@ -820,7 +830,10 @@ int kad_xcts_do_newton(kad_system* s, double precision, double* error, int solve
   *error = err;
   double t1 = now_s();
   if (times) times[0] = t1 - t0;
-  if (err < precision) return 1;
+  S->last_iters[0] = S->last_iters[1] = 0;
+  S->last_resid = 0.0;
+  S->last_fallback = 0;
+  if (err < precision) return 1;  // false for a NaN error: a NaN residual is never converged
   if (!S->J) {
     if ((rc = S->dmalloc((void**)&S->J, (size_t)n * n * sizeof(double)))) return rc;
     if ((rc = S->dmalloc((void**)&S->ipiv, (size_t)n * sizeof(int)))) return rc;
@@ -879,6 +892,25 @@ int kad_xcts_do_newton(kad_system* s, double precision, double* error, int solve
                                S->F, S->X, eta, solver == 1 ? 30 : 8, solver == 1 ? 0 : 1, 0, S->last_iters,
                                &S->last_resid);
       if (rc < 0 && rc != KAD_ERR_NOCONV) return rc;
+      // The refinement stops at its attainable accuracy.  A step whose relative linear
+      // residual is above kAcceptResid is not a usable Newton direction (a refinement
+      // that made no progress leaves X = 0): the step is then solved with an fp64
+      // factorisation of the same (unscaled, untouched) J, as the reference's pdgesv_
+      // would, and the fallback is reported through kad_xcts_last_solve.
+      constexpr double kAcceptResid = 1e-6;
+      if (rc == KAD_ERR_NOCONV && !(S->last_resid <= kAcceptResid)) {
+        S->last_fallback = 1;
+        const double tg0 = now_s();
+        rc = kad_lu_factor(reinterpret_cast<kad_ctx*>(S->ctx), KAD_PREC_F64, n, S->J, n, S->ipiv, S->perm, 0);
+        if (rc < 0) return rc;
+        tf1 += now_s() - tg0;  // both factorisations count as factor time
+        info = rc;
+        if (info == 0) {
+          KAD_HIP_CHECK(hipMemcpyAsync(S->X, S->F, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, st));
+          if ((rc = kad_lu_solve(reinterpret_cast<kad_ctx*>(S->ctx), KAD_PREC_F64, n, S->J, n, S->perm, S->X, 0)))
+            return rc;
+        }
+      }
     }
   }
   if (times) times[4] = tf1 - tf0;
@@ -933,6 +965,7 @@ int kad_xcts_last_solve(kad_system* s, int* iters, double* resid) {
   if (iters) {
     iters[0] = S->last_iters[0];
     iters[1] = S->last_iters[1];
+    iters[2] = S->last_fallback;
   }
   if (resid) *resid = S->last_resid;
   return KAD_OK;

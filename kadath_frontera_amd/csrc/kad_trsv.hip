@@ -13,6 +13,9 @@
 // coalesced GEMV, the only dependent work), the other workgroups subtract the
 // columns of block p from the rows not yet reached.
 #include "kad_common.hpp"
+#include "kad_ctx.hpp"
+#include <vector>
+#include <algorithm>
 
 namespace kad {
 
@@ -126,7 +129,7 @@ __global__ __launch_bounds__(256) void couple_kernel(const T* __restrict__ F, in
 // and workgroups >= CHAIN_WG subtract T(rows, cur) sol_cur from the rows beyond next.
 template <typename T, bool TR, bool FWD>
 __global__ __launch_bounds__(256) void sweep_step_kernel(const T* __restrict__ F, int64_t lda, int n, int nblk,
-                                                         const double* __restrict__ W, int cur,
+                                                         const double* __restrict__ W, int cur, int rlo, int rhi,
                                                          double* __restrict__ rhs, double* __restrict__ sol) {
   __shared__ double vin[WCOLS];
   __shared__ double part[4][PANEL_ROWS];
@@ -171,15 +174,10 @@ __global__ __launch_bounds__(256) void sweep_step_kernel(const T* __restrict__ F
   __syncthreads();
   const int i = t % PANEL_ROWS;
   const int g = t / PANEL_ROWS;  // 4 groups of 32 columns
+  // rows [rlo, rhi): beyond the next block, within the nonzero extent of block cur
   const int wb = blockIdx.x - CHAIN_WG;
-  int r, rend;
-  if (FWD) {
-    r = (next + 1) * TBS + wb * PANEL_ROWS + i;
-    rend = n;
-  } else {
-    r = wb * PANEL_ROWS + i;
-    rend = next * TBS;
-  }
+  const int r = rlo + wb * PANEL_ROWS + i;
+  const int rend = rhi;
   double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
   if (r < rend) {
     const int cb = g * 32;
@@ -218,15 +216,79 @@ __global__ void copy_vec_kernel(const double* __restrict__ src, double* __restri
   if (i < n) dst[i] = src[i];
 }
 
+// Nonzero flags of the off-diagonal 128 x 128 tiles of a sweep's triangle:
+// flag[b * nblk + rc] = 1 if T(row block rc, column block b) has a nonzero, for rc
+// beyond b in the sweep's direction (0 elsewhere).  T(r, c) is the stored factor's
+// (r, c) entry, or its (c, r) entry for the transposed sweeps.
+template <typename T, bool TR, bool FWD>
+__global__ __launch_bounds__(256) void tile_flag_kernel(const T* __restrict__ F, int64_t lda, int n, int nblk,
+                                                        int* __restrict__ flag) {
+  const int b = blockIdx.x, rc = blockIdx.y;
+  int nz = 0;
+  if (FWD ? rc > b : rc < b) {
+    // stored tile: rows of block (TR ? b : rc), columns of block (TR ? rc : b)
+    const int r0 = (TR ? b : rc) * TBS, c0 = (TR ? rc : b) * TBS;
+    const int nr = min(TBS, n - r0), nc = min(TBS, n - c0);
+    const int i = threadIdx.x % TBS;
+    for (int c = threadIdx.x / TBS; c < nc && !nz; c += 2)
+      if (i < nr) nz |= (F[(r0 + i) + (int64_t)(c0 + c) * lda] != (T)0);
+  }
+  nz = __syncthreads_or(nz);
+  if (threadIdx.x == 0) flag[(int64_t)b * nblk + rc] = nz ? 1 : 0;
+}
+
 // Doubles of the W tables of one factorisation (forward then backward sweep).
 size_t trsv_table_doubles(int n) { return (size_t)2 * cdiv(n, TBS) * TBS * WCOLS; }
+// Ints of the tile flags of one sweep.
+size_t trsv_flag_ints(int n) { return (size_t)cdiv(n, TBS) * cdiv(n, TBS); }
 
-// Build the W tables of the sweeps of op(A) (trans as in getrs_fast).
+// Row extents of the panel updates of both sweeps from the tile flags: for column
+// block b, the forward sweep's update ends at row fhi[b] and the backward sweep's
+// starts at row blo[b] (rows outside are exactly zero in that column block).
+template <typename T, bool TR>
+static int sweep_extents(const T* LU, int64_t lda, int n, int* flag, TrsvExtents& ext, hipStream_t s) {
+  const int nblk = (int)cdiv(n, TBS);
+  const size_t cells = (size_t)nblk * nblk;
+  std::vector<int> h(cells);
+  ext.fhi.assign(nblk, 0);
+  ext.blo.assign(nblk, 0);
+  tile_flag_kernel<T, TR, true><<<dim3(nblk, nblk), 256, 0, s>>>(LU, lda, n, nblk, flag);
+  KAD_HIP_CHECK(hipGetLastError());
+  KAD_HIP_CHECK(hipMemcpyAsync(h.data(), flag, cells * sizeof(int), hipMemcpyDeviceToHost, s));
+  KAD_HIP_CHECK(hipStreamSynchronize(s));
+  for (int b = 0; b < nblk; ++b) {
+    int last = -1;
+    for (int rc = b + 1; rc < nblk; ++rc)
+      if (h[(size_t)b * nblk + rc]) last = rc;
+    ext.fhi[b] = last < 0 ? 0 : std::min(n, (last + 1) * TBS);
+  }
+  tile_flag_kernel<T, TR, false><<<dim3(nblk, nblk), 256, 0, s>>>(LU, lda, n, nblk, flag);
+  KAD_HIP_CHECK(hipGetLastError());
+  KAD_HIP_CHECK(hipMemcpyAsync(h.data(), flag, cells * sizeof(int), hipMemcpyDeviceToHost, s));
+  KAD_HIP_CHECK(hipStreamSynchronize(s));
+  for (int b = 0; b < nblk; ++b) {
+    int first = b;
+    for (int rc = 0; rc < b; ++rc)
+      if (h[(size_t)b * nblk + rc]) {
+        first = rc;
+        break;
+      }
+    ext.blo[b] = first * TBS;
+  }
+  return KAD_OK;
+}
+
+// Build the W tables and the row extents of the sweeps of op(A) (trans as in getrs_fast).
 template <typename T>
-int trsv_prepare(const T* LU, int64_t lda, int n, int trans, double* W, hipStream_t s) {
+int trsv_prepare(const T* LU, int64_t lda, int n, int trans, double* W, int* flag, TrsvExtents& ext,
+                 hipStream_t s) {
   const int nblk = (int)cdiv(n, TBS);
   double* Wf = W;
   double* Wb = W + (size_t)nblk * TBS * WCOLS;
+  // forward triangle L (trans 0) or U^T (trans 1): stored lower part, or upper part transposed;
+  // backward triangle U or L^T: the same tiles as the forward ones of the other factor
+  int rc = trans ? sweep_extents<T, true>(LU, lda, n, flag, ext, s) : sweep_extents<T, false>(LU, lda, n, flag, ext, s);
+  if (rc) return rc;
   // forward triangle: L (trans 0) or U^T (trans 1); backward: U or L^T
   diag_inverse_kernel<T><<<nblk, 256, 0, s>>>(LU, lda, n, trans ? 1 : 0, trans, Wf);
   diag_inverse_kernel<T><<<nblk, 256, 0, s>>>(LU, lda, n, trans ? 0 : 1, trans, Wb);
@@ -244,27 +306,33 @@ int trsv_prepare(const T* LU, int64_t lda, int n, int trans, double* W, hipStrea
 }
 
 template <typename T, bool TR, bool FWD>
-static void sweep(const T* LU, int64_t lda, int n, int nblk, const double* W, double* rhs, double* sol,
-                  hipStream_t s) {
+static void sweep(const T* LU, int64_t lda, int n, int nblk, const double* W, const TrsvExtents& ext, double* rhs,
+                  double* sol, hipStream_t s) {
   if (FWD) {
     for (int cur = -1; cur < nblk - 1; ++cur) {
-      const int rows = n - (cur + 2) * TBS;
+      // panel rows: beyond block cur + 1, up to the column block's last nonzero row tile
+      const int rlo = (cur + 2) * TBS;
+      const int rhi = cur >= 0 ? std::min(n, ext.fhi[cur]) : 0;
+      const int rows = rhi - rlo;
       const unsigned grid = CHAIN_WG + ((cur >= 0 && rows > 0) ? (unsigned)cdiv(rows, PANEL_ROWS) : 0u);
-      sweep_step_kernel<T, TR, true><<<grid, 256, 0, s>>>(LU, lda, n, nblk, W, cur, rhs, sol);
+      sweep_step_kernel<T, TR, true><<<grid, 256, 0, s>>>(LU, lda, n, nblk, W, cur, rlo, rhi, rhs, sol);
     }
   } else {
     for (int cur = nblk; cur >= 1; --cur) {
-      const int rows = (cur - 1) * TBS;
+      // panel rows: from the column block's first nonzero row tile, before block cur - 1
+      const int rhi = (cur - 1) * TBS;
+      const int rlo = cur < nblk ? ext.blo[cur] : rhi;
+      const int rows = rhi - rlo;
       const unsigned grid = CHAIN_WG + ((cur < nblk && rows > 0) ? (unsigned)cdiv(rows, PANEL_ROWS) : 0u);
-      sweep_step_kernel<T, TR, false><<<grid, 256, 0, s>>>(LU, lda, n, nblk, W, cur, rhs, sol);
+      sweep_step_kernel<T, TR, false><<<grid, 256, 0, s>>>(LU, lda, n, nblk, W, cur, rlo, rhi, rhs, sol);
     }
   }
 }
 
 // b <- op(A)^{-1} b.  work: 3 n doubles.  W from trsv_prepare with the same trans.
 template <typename T>
-int getrs_fast(const T* LU, int64_t lda, int n, const int* perm, const double* W, double* b, int trans,
-               double* work, hipStream_t s) {
+int getrs_fast(const T* LU, int64_t lda, int n, const int* perm, const double* W, const TrsvExtents& ext, double* b,
+               int trans, double* work, hipStream_t s) {
   const int nblk = (int)cdiv(n, TBS);
   const double* Wf = W;
   const double* Wb = W + (size_t)nblk * TBS * WCOLS;
@@ -273,24 +341,24 @@ int getrs_fast(const T* LU, int64_t lda, int n, const int* perm, const double* W
   const unsigned g = (unsigned)cdiv(n, 256);
   if (!trans) {
     gather_perm_kernel<<<g, 256, 0, s>>>(b, perm, x, n);
-    sweep<T, false, true>(LU, lda, n, nblk, Wf, x, y, s);   // L y = P b
-    sweep<T, false, false>(LU, lda, n, nblk, Wb, y, b, s);  // U x = y
+    sweep<T, false, true>(LU, lda, n, nblk, Wf, ext, x, y, s);   // L y = P b
+    sweep<T, false, false>(LU, lda, n, nblk, Wb, ext, y, b, s);  // U x = y
   } else {
     double* z = work + 2 * (size_t)n;
     copy_vec_kernel<<<g, 256, 0, s>>>(b, x, n);
-    sweep<T, true, true>(LU, lda, n, nblk, Wf, x, y, s);   // U^T z = b
-    sweep<T, true, false>(LU, lda, n, nblk, Wb, y, z, s);  // L^T w = z
+    sweep<T, true, true>(LU, lda, n, nblk, Wf, ext, x, y, s);   // U^T z = b
+    sweep<T, true, false>(LU, lda, n, nblk, Wb, ext, y, z, s);  // L^T w = z
     scatter_perm_kernel<<<g, 256, 0, s>>>(z, perm, b, n);  // x = P^T w
   }
   KAD_HIP_CHECK(hipGetLastError());
   return KAD_OK;
 }
 
-template int trsv_prepare<double>(const double*, int64_t, int, int, double*, hipStream_t);
-template int trsv_prepare<float>(const float*, int64_t, int, int, double*, hipStream_t);
-template int getrs_fast<double>(const double*, int64_t, int, const int*, const double*, double*, int, double*,
-                                hipStream_t);
-template int getrs_fast<float>(const float*, int64_t, int, const int*, const double*, double*, int, double*,
-                               hipStream_t);
+template int trsv_prepare<double>(const double*, int64_t, int, int, double*, int*, TrsvExtents&, hipStream_t);
+template int trsv_prepare<float>(const float*, int64_t, int, int, double*, int*, TrsvExtents&, hipStream_t);
+template int getrs_fast<double>(const double*, int64_t, int, const int*, const double*, const TrsvExtents&, double*,
+                                int, double*, hipStream_t);
+template int getrs_fast<float>(const float*, int64_t, int, const int*, const double*, const TrsvExtents&, double*,
+                               int, double*, hipStream_t);
 
 }  // namespace kad

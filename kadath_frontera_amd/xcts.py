@@ -145,7 +145,8 @@ class XctsSystem:
         if rc == -5:
             raise RuntimeError("Jacobian is singular! Stopping.")
         check(rc, "kad_xcts_do_newton")
-        it = (ctypes.c_int * 2)()
+        it = (ctypes.c_int * 3)()
         res = ctypes.c_double()
         lib.kad_xcts_last_solve(self.h, it, ctypes.byref(res))
-        return rc == 1, err.value, times, (it[0], it[1], res.value)
+        # (outer refinement steps, inner GMRES steps, relative residual, fp64 fallback)
+        return rc == 1, err.value, times, (it[0], it[1], res.value, it[2])
