@@ -9,9 +9,10 @@
  * The reference has no C ABI: its Newton step is the C++ member
  *   bool System_of_eqs::do_newton(double precision, double& error, SOLVER)
  *   (reference include/system_of_eqs.hpp:871, src_par/do_newton.cpp:65-430)
- * swapped at link time (build_release/CMakeLists.txt:24-31).  The C++ host
- * library of this package keeps that member; the functions below are the thin
- * layer it calls, one per reference subsystem that moves to the GPU.
+ * swapped at link time (build_release/CMakeLists.txt:24-31).  The functions below
+ * are the thin layer under that member, one per reference subsystem that moves to
+ * the GPU; include/kadath_amd.hpp restates the member itself (a C++
+ * System_of_eqs / Space_spheric / Scalar / Vector over this layer).
  */
 #ifndef KADATH_AMD_H
 #define KADATH_AMD_H
@@ -139,7 +140,7 @@ int kad_coef_i_1d(kad_ctx* ctx, int base, int nconf, int64_t nlines, const doubl
 int kad_host_transform_matrix(int kind, int base, int nconf, double* out, int cap);
 /* Matrix of a coefficient-space 1-D operator (op: 0 der_1d, 1 mult_sin_1d,
  * 2 mult_cos_1d, 3 div_sin_1d, 4 mult_x_1d, 5 mult_xm1_1d, 6 div_xm1_1d;
- * src/Ope_1d/*.cpp); *out_base receives the output basis. */
+ * src/Ope_1d/<op>_1d.cpp); *out_base receives the output basis. */
 int kad_host_op1d_matrix(int op, int base, int n, double* out, int cap, int* out_base);
 
 /* ---- the Newton system (replaces System_of_eqs::do_newton and what it calls:
@@ -215,6 +216,93 @@ int kad_xcts_host_info(int nr, int nt, int np, const double* bounds, int nbounds
 int kad_ir_solve_scaled(kad_ctx* ctx, int n, const double* A64, int64_t lda, int prec_lo, const void* LUlo,
                         int64_t ldlu, const int* perm, const double* Rs, const double* Cs, const double* b, double* x,
                         double tol, int max_outer, int mode, int trans, int* iters, double* resid);
+
+/* ---- systems given as equation strings (the reference's System_of_eqs front end:
+ *      add_var / add_cst / add_def / add_eq_*, src/System_of_eqs/system_of_eqs.cpp,
+ *      add_eq.cpp, the parser give_ope.cpp:559-1135; Space_spheric::add_eq /
+ *      add_inner_bc / add_outer_bc, src/Space/Spheric/spheric_add_eq.cpp:26-52) ---
+ *
+ * A declaration (kad_sysdef, host only) collects unknowns, constants, definitions and
+ * equations; kad_system_build traces it into device programs.  Expressions follow the
+ * reference's grammar: split at the last top-level '+', else '-', else a leading '-',
+ * else the last '*', else the last '/'; then Lap / lap / dn / dr / multr / divr /
+ * multsint / divsint (x), D_i x / D^i x / partial_i x (flat metric, Cartesian basis),
+ * x^n (integer n > 0), numbers, and names with indices (A^ij, bet_i).  A repeated
+ * index is summed.  Unknowns and constant fields are scalars (valence 0) or Cartesian
+ * vectors (valence 1).  Operators without a device form (sqrt, exp, integ, ...) are
+ * rejected with KAD_ERR_ARG and a message on stderr.  bound: 0 INNER_BC, 1 OUTER_BC. */
+typedef struct kad_sysdef kad_sysdef;
+
+/* Space_spheric(CHEB_TYPE, center 0, (nr, nt, np), bounds[0..nbounds)) with the
+ * System_of_eqs(space, dmin, dmax) constructor's domain range (dmax = nbounds, the
+ * compactified domain). */
+int kad_sysdef_create(int nr, int nt, int np, const double* bounds, int nbounds, int dmin, int dmax,
+                      kad_sysdef** out);
+int kad_sysdef_destroy(kad_sysdef* def);
+int kad_sysdef_add_var(kad_sysdef* def, const char* name, int valence);   /* add_var */
+int kad_sysdef_add_cst(kad_sysdef* def, const char* name, double value);  /* add_cst(name, double) */
+/* a double constant that carries a tangent: a parameter p of dF/dp and of the adjoint
+ * sensitivities; returns its parameter index (>= 0) */
+int kad_sysdef_add_param(kad_sysdef* def, const char* name, double value);
+/* add_cst(name, Tensor): conf values, component-major, then domains dmin..dmax, each
+ * nr*nt*np values in the reference's Array order (r fastest); count = the total */
+int kad_sysdef_add_cst_field(kad_sysdef* def, const char* name, int valence, const double* conf, int64_t count);
+/* Metric_flat::set_system(syst, name): name^ij = name_ij = delta in the Cartesian basis */
+int kad_sysdef_set_flat_metric(kad_sysdef* def, const char* name);
+int kad_sysdef_add_def(kad_sysdef* def, const char* def_text);                       /* add_def */
+int kad_sysdef_add_eq_inside(kad_sysdef* def, int dom, const char* eq, int order);  /* add_eq_inside / add_eq_order */
+int kad_sysdef_add_eq_bc(kad_sysdef* def, int dom, int bound, const char* eq);      /* add_eq_bc */
+int kad_sysdef_add_eq_matching(kad_sysdef* def, int dom, int bound, const char* expr); /* add_eq_matching */
+int kad_sysdef_space_add_eq(kad_sysdef* def, const char* eq, const char* m1, const char* m2);
+int kad_sysdef_space_add_bc(kad_sysdef* def, int bound, const char* eq);  /* add_inner_bc / add_outer_bc */
+/* collocation r, theta, phi of domain d's conf grid (HOST out, nr*nt*np each; r = +inf
+ * at infinity); any pointer may be NULL */
+int kad_sysdef_coloc(kad_sysdef* def, int d, double* r, double* theta, double* phi);
+/* Host-only trace of the declaration (no device work), for checks without a GPU:
+ * info[6] = n_unknowns, n_rows, value instructions, tangent instructions, value slots,
+ * tangent programs; row_layout (may be NULL): 5 ints per (equation, component) as
+ * kad_xcts_row_layout.  Returns the entry count.  The declaration can then only be
+ * destroyed. */
+int kad_sysdef_host_trace(kad_sysdef* def, int order, int* info, int* row_layout, int cap);
+/* Trace and upload (order 0: reference unknown / row order, 1: domain-major; batch:
+ * Jacobian columns per tangent launch chain).  Consumes def (freed, also on error). */
+int kad_system_build(kad_ctx* ctx, kad_sysdef* def, int order, int batch, kad_system** out);
+
+/* ---- entry points of every system (XCTS or built from strings) ---------- */
+int kad_system_destroy(kad_system* s);
+int kad_system_sizes(kad_system* s, int* n_unknowns, int* n_rows, int* nfields);
+int kad_system_layout(kad_system* s, int* out, int cap);      /* as kad_xcts_layout */
+int kad_system_row_layout(kad_system* s, int* out, int cap);  /* as kad_xcts_row_layout */
+/* field component f (declaration order; vectors x, y, z) of domain d from / to its
+ * values at the collocation points (HOST, nr*nt*np) */
+int kad_system_set_field(kad_system* s, int f, int d, const double* conf);
+int kad_system_get_field(kad_system* s, int f, int d, double* conf);
+int kad_system_get_coef(kad_system* s, int f, int d, double* out);
+int kad_system_set_coef(kad_system* s, int f, int d, const double* in);
+/* fields += affecte_tau(X0), X0 HOST (n_unknowns) */
+int kad_system_add_unknowns(kad_system* s, const double* X0);
+int kad_system_update(kad_system* s, const double* X);  /* fields -= affecte_tau(X), X device */
+int kad_system_perturbation(kad_system* s, uint64_t seed, double amp, double* out);
+int kad_system_set_param(kad_system* s, int which, double value);
+/* as kad_xcts_do_newton / kad_xcts_last_solve */
+int kad_system_do_newton(kad_system* s, double precision, double* error, int solver, double* times);
+int kad_system_last_solve(kad_system* s, int* iters, double* resid);
+
+/* F(u) (device, n_rows): System_of_eqs::sec_member (solver.cpp:49-85) */
+int kad_residual(kad_system* s, double* F);
+/* per (equation, component) max |F| (HOST out, row-layout order):
+ * System_of_eqs::check_equations (solver.cpp:24-47); returns the entry count */
+int kad_check_equations(kad_system* s, double* errors, int cap);
+/* Columns [col_begin, col_begin + col_count) of J into J_tile (device, column-major,
+ * ld >= n_rows), dtype 0 fp64 / 1 fp32: the do_col_J loop of do_newton.cpp:132-174
+ * handing a column range to the GPU (System_of_eqs::do_col_J, solver.cpp:120-214). */
+int kad_jvp_fill(kad_system* s, int col_begin, int col_count, void* J_tile, int64_t ld, int dtype);
+/* out = J^T w (device; w n_rows, out n_unknowns), matrix-free: the tangent programs with
+ * their row exports transposed; J is never formed (no reference counterpart: the
+ * reference forms J^T only inside Matrice::solve with 'T', matrice.cpp:221-250). */
+int kad_vjp(kad_system* s, const double* w, double* out);
+/* dF/dp of parameter `which` at the current fields (device out, n_rows) */
+int kad_param_tangent(kad_system* s, int which, double* out);
 
 #ifdef __cplusplus
 }

@@ -22,6 +22,7 @@ c_dbl = ctypes.c_double
 c_vp = ctypes.c_void_p
 c_ip = ctypes.POINTER(ctypes.c_int)
 c_dp = ctypes.POINTER(ctypes.c_double)
+c_char_p = ctypes.c_char_p
 
 # name -> (restype, argtypes)
 SIGNATURES = {
@@ -71,6 +72,42 @@ SIGNATURES = {
     "kad_kernel_timing_read": (c_int, [c_vp, c_int, c_dp, c_dp, c_ip]),
     "kad_xcts_host_info": (c_int, [c_int, c_int, c_int, c_dp, c_int, c_int, c_int, c_dbl, c_dbl, c_dbl, c_int, c_ip,
                                    c_ip, c_int]),
+    # systems given as equation strings
+    "kad_sysdef_create": (c_int, [c_int, c_int, c_int, c_dp, c_int, c_int, c_int, ctypes.POINTER(c_vp)]),
+    "kad_sysdef_destroy": (c_int, [c_vp]),
+    "kad_sysdef_add_var": (c_int, [c_vp, c_char_p, c_int]),
+    "kad_sysdef_add_cst": (c_int, [c_vp, c_char_p, c_dbl]),
+    "kad_sysdef_add_param": (c_int, [c_vp, c_char_p, c_dbl]),
+    "kad_sysdef_add_cst_field": (c_int, [c_vp, c_char_p, c_int, c_dp, c_i64]),
+    "kad_sysdef_set_flat_metric": (c_int, [c_vp, c_char_p]),
+    "kad_sysdef_add_def": (c_int, [c_vp, c_char_p]),
+    "kad_sysdef_add_eq_inside": (c_int, [c_vp, c_int, c_char_p, c_int]),
+    "kad_sysdef_add_eq_bc": (c_int, [c_vp, c_int, c_int, c_char_p]),
+    "kad_sysdef_add_eq_matching": (c_int, [c_vp, c_int, c_int, c_char_p]),
+    "kad_sysdef_space_add_eq": (c_int, [c_vp, c_char_p, c_char_p, c_char_p]),
+    "kad_sysdef_space_add_bc": (c_int, [c_vp, c_int, c_char_p]),
+    "kad_sysdef_coloc": (c_int, [c_vp, c_int, c_dp, c_dp, c_dp]),
+    "kad_sysdef_host_trace": (c_int, [c_vp, c_int, c_ip, c_ip, c_int]),
+    "kad_system_build": (c_int, [c_vp, c_vp, c_int, c_int, ctypes.POINTER(c_vp)]),
+    "kad_system_destroy": (c_int, [c_vp]),
+    "kad_system_sizes": (c_int, [c_vp, c_ip, c_ip, c_ip]),
+    "kad_system_layout": (c_int, [c_vp, c_ip, c_int]),
+    "kad_system_row_layout": (c_int, [c_vp, c_ip, c_int]),
+    "kad_system_set_field": (c_int, [c_vp, c_int, c_int, c_dp]),
+    "kad_system_get_field": (c_int, [c_vp, c_int, c_int, c_dp]),
+    "kad_system_get_coef": (c_int, [c_vp, c_int, c_int, c_dp]),
+    "kad_system_set_coef": (c_int, [c_vp, c_int, c_int, c_dp]),
+    "kad_system_add_unknowns": (c_int, [c_vp, c_dp]),
+    "kad_system_update": (c_int, [c_vp, c_vp]),
+    "kad_system_perturbation": (c_int, [c_vp, ctypes.c_uint64, c_dbl, c_dp]),
+    "kad_system_set_param": (c_int, [c_vp, c_int, c_dbl]),
+    "kad_system_do_newton": (c_int, [c_vp, c_dbl, c_dp, c_int, c_dp]),
+    "kad_system_last_solve": (c_int, [c_vp, c_ip, c_dp]),
+    "kad_residual": (c_int, [c_vp, c_vp]),
+    "kad_check_equations": (c_int, [c_vp, c_dp, c_int]),
+    "kad_jvp_fill": (c_int, [c_vp, c_int, c_int, c_vp, c_i64, c_int]),
+    "kad_vjp": (c_int, [c_vp, c_vp, c_vp]),
+    "kad_param_tangent": (c_int, [c_vp, c_int, c_vp]),
 }
 
 for _name, (_res, _args) in SIGNATURES.items():

@@ -7,10 +7,15 @@
 
 namespace kadhost {
 
-XctsSystem::XctsSystem(const XctsParams& p) : prm(p), tr(&doms) {
+XctsSystem::XctsSystem(const XctsParams& p) : prm(p) {
   const int ndom = (int)p.bounds.size() + 1;
   if (p.dmin < 1 || p.dmax != ndom - 1 || p.dmin > p.dmax)
     throw std::runtime_error("XctsSystem: solve from the first shell to the compactified domain");
+  dmin = p.dmin;
+  dmax = p.dmax;
+  nfields = NF;
+  field_names = {"P", "N", "bet^x", "bet^y", "bet^z"};
+  var_fields = {{0}, {1}, {2, 3, 4}};
   doms.resize(ndom);
   // Space_spheric (space_spheric.cpp:25-47); the nucleus is excised and not built.
   for (int i = 1; i < ndom - 1; ++i) doms[i] = Domain(i, false, p.nr, p.nt, p.np, p.bounds[i - 1], p.bounds[i]);
@@ -474,11 +479,12 @@ void XctsSystem::trace_ome_tangent() {
   st.pos = {0, 0, 0};
   st.val = {1.0, 0.0, 0.0};
   st.cnt = {1};
-  ome_tab = (int)seeds.size();
+  par_names = {"ome"};
+  par_tab = {(int)seeds.size()};
   seeds.push_back(st);
-  ome_prog = Program();
-  tr.begin_tangent(&ome_prog);
-  ome_seed = tr.seed_node(d, doms[d].std_base(), ome_tab);
+  par_prog.assign(1, Program());
+  tr.begin_tangent(&par_prog[0]);
+  ome_seed = tr.seed_node(d, doms[d].std_base(), par_tab[0]);
   auto zd = zero_ders();
   Defs D = defs(d, zd[d]);
   for (auto& e : eqs) {

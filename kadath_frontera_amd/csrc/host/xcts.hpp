@@ -16,7 +16,7 @@
 #include <vector>
 #include <string>
 #include <array>
-#include "tape.hpp"
+#include "system_base.hpp"
 
 namespace kadhost {
 
@@ -32,69 +32,21 @@ struct XctsParams {
   int order = 0;
 };
 
-struct EqRec {
-  int kind;  // 0 inside, 1 match, 2 outer bc, 3 inner bc
-  int d;
-  std::string which;
-  int ncomp;
-  int row0;
-  std::vector<int> nrows;  // per component
-};
-
-struct SeedTab {
-  std::vector<int> pos;     // 3 per unknown
-  std::vector<double> val;  // 3 per unknown
-  std::vector<int> cnt;
-};
-
-class XctsSystem {
+class XctsSystem : public SystemBase {
  public:
   static constexpr int NF = 5;  // P, N, bet^x, bet^y, bet^z
   XctsParams prm;
-  std::vector<Domain> doms;
-  Tracer tr;
-  std::vector<EqRec> eqs;
-  int n_unknowns = 0, n_rows = 0;
-  // unknown layout: (field, d) -> offset, count
-  std::vector<std::array<int, 4>> layout;  // field, d, off, cnt
-  // value-space input slots of the fields' coefficients, [field][d]
-  std::vector<std::vector<int>> field_slot;
-  // tangent programs, one per (field, d), and their seed tables
-  std::vector<Program> tprog;
   std::vector<std::array<int, 2>> tprog_fd;
-  std::vector<SeedTab> seeds;
-  // affecte_tau maps per (field, d): CSR coefficient <- unknowns
-  struct Csr {
-    std::vector<int> rowptr, col;
-    std::vector<double> w;
-  };
-  std::vector<Csr> tau_maps;  // indexed like layout
-  // global quantities at the outer boundary: value-space program producing, per
-  // quantity, the coefficients of r r sin(theta) so; result = fac * integ weights
-  struct ObsRec {
-    int slot = -1;  // value slot of the coefficients (-1: identically zero)
-    int bt0 = 0;    // theta basis of phi coefficient 0
-    double fac = 0;
-  };
-  Program obs_prog;
-  std::vector<ObsRec> obs;  // M_ADM, M_Komar, J_ADM
-  // d(observables)/d(unknowns): one tangent program per field of the outer domain,
-  // seeded like the Jacobian programs, exporting the 3 integrals per column
-  std::vector<Program> obs_tprog;
-  std::vector<std::array<int, 4>> obs_tprog_lay;  // field, domain, first unknown, count
-  std::vector<int> obs_tprog_tab;                 // seed table
-  // dF/dOme: tangent program of the horizon rotation (a constant field seeded with 1)
-  Program ome_prog;
-  int ome_tab = -1;
 
   explicit XctsSystem(const XctsParams& p);
-  Base field_base(int f, int d) const { return f == 4 ? doms[d].anti_base() : doms[d].std_base(); }
+  Base field_base(int f, int d) const override { return f == 4 ? doms[d].anti_base() : doms[d].std_base(); }
   // value slot of the configuration values of a cached constant field (-1 if absent)
   int const_slot(const std::string& name, int d) const {
     auto it = consts.find(name + ":" + std::to_string(d));
     return it == consts.end() ? -1 : it->second.val->c;
   }
-  void build();
+  int param_slot(const std::string& name, int d) const override { return const_slot(name, d); }
+  void build() override;
 
  private:
   struct Defs {

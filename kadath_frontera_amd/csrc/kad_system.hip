@@ -11,6 +11,7 @@
 #include "kad_ctx.hpp"
 #include "kad_spectral.hpp"
 #include "host/xcts.hpp"
+#include "host/parsed_system.hpp"
 #include "../../include/kadath_amd.h"
 #include <vector>
 #include <chrono>
@@ -61,8 +62,9 @@ __global__ void ew_kernel(int kind, double* __restrict__ dst, int64_t dstride, c
 }
 
 // Inside rows: out[(row0 + r) + (col0 + b) ldo] (+)= sign (src[pos] - gw src[gpos])
+template <typename T>
 __global__ void export_in_kernel(const DevRow* __restrict__ rows, int nrows, const double* __restrict__ src,
-                                 int64_t sstride, double* __restrict__ out, int row0, int64_t col0, int64_t ldo,
+                                 int64_t sstride, T* __restrict__ out, int row0, int64_t col0, int64_t ldo,
                                  double sign, int acc, int nbatch) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (int64_t)nrows * nbatch) return;
@@ -72,13 +74,14 @@ __global__ void export_in_kernel(const DevRow* __restrict__ rows, int nrows, con
   const double* x = src + bb * sstride;
   double v = x[R.pos];
   if (R.gpos >= 0) v -= R.gw * x[R.gpos];
-  double* o = out + (row0 + r) + (col0 + bb) * ldo;
-  *o = acc ? (*o + sign * v) : sign * v;
+  T* o = out + (row0 + r) + (col0 + bb) * ldo;
+  *o = acc ? (T)((double)*o + sign * v) : (T)(sign * v);
 }
 
 // Boundary rows: X(line) = sum_i bsign^i cf(i, line)  (val_boundary, domain_shell_systems.cpp:43-74)
+template <typename T>
 __global__ void export_bd_kernel(const DevRow* __restrict__ rows, int nrows, const double* __restrict__ src,
-                                 int64_t sstride, int nr, int bsign, double* __restrict__ out, int row0,
+                                 int64_t sstride, int nr, int bsign, T* __restrict__ out, int row0,
                                  int64_t col0, int64_t ldo, double sign, int acc, int nbatch) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (int64_t)nrows * nbatch) return;
@@ -93,8 +96,47 @@ __global__ void export_bd_kernel(const DevRow* __restrict__ rows, int nrows, con
     sg *= bsign;
   }
   if (R.gpos >= 0) v -= R.gw * g;
-  double* o = out + (row0 + r) + (col0 + bb) * ldo;
-  *o = acc ? (*o + sign * v) : sign * v;
+  T* o = out + (row0 + r) + (col0 + bb) * ldo;
+  *o = acc ? (T)((double)*o + sign * v) : (T)(sign * v);
+}
+
+// Transposed exports (the VJP of the row export): for every column b of the batch,
+// out[col0 + b] += sign * sum_r w[row0 + r] row_r(b), where row_r is the inside
+// (bd = 0) or boundary (bd = 1) row recipe above.  One workgroup per column, a
+// fixed-order reduction, and instructions run in stream order: deterministic.
+__global__ __launch_bounds__(256) void export_dot_kernel(const DevRow* __restrict__ rows, int nrows, int bd,
+                                                         const double* __restrict__ src, int64_t sstride, int nr,
+                                                         int bsign, const double* __restrict__ w, int row0,
+                                                         double* __restrict__ out, int64_t col0, double sign) {
+  __shared__ double red[256];
+  const int bb = blockIdx.x;
+  const double* x = src + (int64_t)bb * sstride;
+  double acc = 0.0;
+  for (int r = threadIdx.x; r < nrows; r += 256) {
+    const DevRow R = rows[r];
+    double v;
+    if (!bd) {
+      v = x[R.pos];
+      if (R.gpos >= 0) v -= R.gw * x[R.gpos];
+    } else {
+      double g = 0.0, sg = 1.0;
+      v = 0.0;
+      for (int i = 0; i < nr; ++i) {
+        v += sg * x[i + (int64_t)nr * R.pos];
+        if (R.gpos >= 0) g += sg * x[i + (int64_t)nr * R.gpos];
+        sg *= bsign;
+      }
+      if (R.gpos >= 0) v -= R.gw * g;
+    }
+    acc += w[row0 + r] * v;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[col0 + bb] += sign * red[0];
 }
 
 // Unit seeds of unknowns u0 .. u0 + nbatch - 1 (affecte_tau_one_coef), zero elsewhere.
@@ -214,7 +256,8 @@ double now_s() {
 
 struct kad_system_impl {
   kad::Ctx* ctx = nullptr;
-  XctsSystem* sys = nullptr;
+  SystemBase* sys = nullptr;
+  XctsSystem* xsys = nullptr;  // the same object when the system is the hand-traced XCTS one
   int B = 256;
   int64_t stride = 0;          // doubles per slot per batch element
   double* val_arena = nullptr;  // nval_slots * stride
@@ -230,9 +273,9 @@ struct kad_system_impl {
   std::vector<double*> csr_w;
   std::vector<DevProgram> tprogs;
   std::vector<DevProgram> obs_tprogs;  // d(observables)/d(unknowns)
-  DevProgram ome_prog;                 // dF/dOme
+  std::vector<DevProgram> par_progs;   // dF/dp of each parameter
   double* dQ = nullptr;                // 3 x n, row q contiguous
-  double* dFdp = nullptr;              // n
+  double* dFdp = nullptr;              // n per parameter
   int nphys_max = 0;
   // dense linear algebra buffers (lazily allocated)
   double* J = nullptr;
@@ -270,7 +313,13 @@ struct kad_system_impl {
   }
   int64_t slot_stride(const Opnd& o) const { return o.space == SP_VAL ? 0 : stride; }
   int setup();
-  int run(const std::vector<Instr>& code, int nb, int u0, int seed_tab, double* out, int64_t col0, int64_t ldo);
+  // out: fp64 (dtype 0) or fp32 (dtype 1) column-major rows x columns with leading
+  // dimension ldo, columns from col0; with w != nullptr the exports are transposed
+  // instead (out: fp64 per column, += w . row)
+  int run(const std::vector<Instr>& code, int nb, int u0, int seed_tab, void* out, int64_t col0, int64_t ldo,
+          int dtype = 0, const double* w = nullptr);
+  bool vals_current = false;  // the value arena holds the value pass of the current fields
+  int ensure_values() { return vals_current ? KAD_OK : value_pass(); }
   int value_pass();
   int jacobian(double* Jd, int64_t ldj);
   int update(const double* Xd, double scale);
@@ -331,7 +380,7 @@ int kad_system_impl::setup() {
   Tracer& tr = sys->tr;
   // slot stride: the largest domain array (coef layout), rounded to 8 doubles
   int maxsz = 0;
-  for (int d = sys->prm.dmin; d <= sys->prm.dmax; ++d) maxsz = std::max(maxsz, sys->doms[d].coef_size());
+  for (int d = sys->dmin; d <= sys->dmax; ++d) maxsz = std::max(maxsz, sys->doms[d].coef_size());
   stride = (maxsz + 7) / 8 * 8;
   int rc = dmalloc((void**)&val_arena, (size_t)std::max(1, tr.nval_slots) * stride * sizeof(double));
   if (rc) return rc;
@@ -407,19 +456,23 @@ int kad_system_impl::setup() {
     nphys_max = std::max(nphys_max, dp.nphys);
     obs_tprogs.push_back(std::move(dp));
   }
-  ome_prog.code = sys->ome_prog.code;
-  ome_prog.nphys = allocate_slots(ome_prog.code, sys->ome_prog.nslots);
-  ome_prog.seed_tab = sys->ome_tab;
-  ome_prog.cnt = 1;
-  nphys_max = std::max(nphys_max, ome_prog.nphys);
+  for (size_t p = 0; p < sys->par_prog.size(); ++p) {
+    DevProgram dp;
+    dp.code = sys->par_prog[p].code;
+    dp.nphys = allocate_slots(dp.code, sys->par_prog[p].nslots);
+    dp.seed_tab = sys->par_tab[p];
+    dp.cnt = 1;
+    nphys_max = std::max(nphys_max, dp.nphys);
+    par_progs.push_back(std::move(dp));
+  }
   if ((rc = dmalloc((void**)&tan_arena, (size_t)std::max(1, nphys_max) * B * stride * sizeof(double)))) return rc;
   if ((rc = dmalloc((void**)&F, (size_t)sys->n_rows * sizeof(double)))) return rc;
   if ((rc = dmalloc((void**)&scratch, 64 * sizeof(double)))) return rc;
   return KAD_OK;
 }
 
-int kad_system_impl::run(const std::vector<Instr>& code, int nb, int u0, int seed_tab, double* out, int64_t col0,
-                         int64_t ldo) {
+int kad_system_impl::run(const std::vector<Instr>& code, int nb, int u0, int seed_tab, void* out, int64_t col0,
+                         int64_t ldo, int dtype, const double* w) {
   hipStream_t s = ctx->stream;
   for (const Instr& in : code) {
     const Domain& dm = sys->doms[in.dom];
@@ -444,21 +497,37 @@ int kad_system_impl::run(const std::vector<Instr>& code, int nb, int u0, int see
       case I_EXPORT_BD: {
         const int batch = (in.a.space == SP_VAL) ? 1 : nb;
         const int64_t tot = (int64_t)in.nrows * batch;
-        if (in.kind == I_EXPORT_IN)
-          export_in_kernel<<<(unsigned)kad::cdiv(tot, 256), 256, 0, s>>>(row_tabs[in.rows], in.nrows, slot_ptr(in.a),
-                                                                          slot_stride(in.a), out, in.row0, col0, ldo,
-                                                                          in.sign, in.accumulate, batch);
-        else
-          export_bd_kernel<<<(unsigned)kad::cdiv(tot, 256), 256, 0, s>>>(
-              row_tabs[in.rows], in.nrows, slot_ptr(in.a), slot_stride(in.a), dm.nr, in.bsign, out, in.row0, col0,
-              ldo, in.sign, in.accumulate, batch);
+        const unsigned g = (unsigned)kad::cdiv(tot, 256);
+        if (w) {
+          export_dot_kernel<<<(unsigned)batch, 256, 0, s>>>(row_tabs[in.rows], in.nrows, in.kind == I_EXPORT_BD,
+                                                            slot_ptr(in.a), slot_stride(in.a), dm.nr, in.bsign, w,
+                                                            in.row0, (double*)out, col0, in.sign);
+        } else if (in.kind == I_EXPORT_IN) {
+          if (dtype == 0)
+            export_in_kernel<double><<<g, 256, 0, s>>>(row_tabs[in.rows], in.nrows, slot_ptr(in.a), slot_stride(in.a),
+                                                       (double*)out, in.row0, col0, ldo, in.sign, in.accumulate,
+                                                       batch);
+          else
+            export_in_kernel<float><<<g, 256, 0, s>>>(row_tabs[in.rows], in.nrows, slot_ptr(in.a), slot_stride(in.a),
+                                                      (float*)out, in.row0, col0, ldo, in.sign, in.accumulate, batch);
+        } else {
+          if (dtype == 0)
+            export_bd_kernel<double><<<g, 256, 0, s>>>(row_tabs[in.rows], in.nrows, slot_ptr(in.a), slot_stride(in.a),
+                                                       dm.nr, in.bsign, (double*)out, in.row0, col0, ldo, in.sign,
+                                                       in.accumulate, batch);
+          else
+            export_bd_kernel<float><<<g, 256, 0, s>>>(row_tabs[in.rows], in.nrows, slot_ptr(in.a), slot_stride(in.a),
+                                                      dm.nr, in.bsign, (float*)out, in.row0, col0, ldo, in.sign,
+                                                      in.accumulate, batch);
+        }
         break;
       }
       case I_EXPORT_INT: {
+        if (w || dtype != 0) return KAD_ERR_ARG;
         const int batch = (in.a.space == SP_VAL) ? 1 : nb;
         export_int_kernel<<<(unsigned)kad::cdiv(batch, 64), 64, 0, s>>>(slot_ptr(in.a), slot_stride(in.a), dm.nr,
-                                                                         dm.nt, in.aux, in.s, out, in.row0, col0,
-                                                                         ldo, batch);
+                                                                         dm.nt, in.aux, in.s, (double*)out, in.row0,
+                                                                         col0, ldo, batch);
         break;
       }
       case I_SEED: {
@@ -477,7 +546,11 @@ int kad_system_impl::run(const std::vector<Instr>& code, int nb, int u0, int see
   return KAD_OK;
 }
 
-int kad_system_impl::value_pass() { return run(sys->tr.val.code, 1, 0, -1, F, 0, 0); }
+int kad_system_impl::value_pass() {
+  int rc = run(sys->tr.val.code, 1, 0, -1, F, 0, 0);
+  vals_current = (rc == KAD_OK);
+  return rc;
+}
 
 int kad_system_impl::jacobian(double* Jd, int64_t ldj) {
   int rc = value_pass();
@@ -494,6 +567,7 @@ int kad_system_impl::jacobian(double* Jd, int64_t ldj) {
 }
 
 int kad_system_impl::update(const double* Xd, double scale) {
+  vals_current = false;
   for (size_t q = 0; q < sys->layout.size(); ++q) {
     const int f = sys->layout[q][0], d = sys->layout[q][1], off = sys->layout[q][2];
     const int ncf = sys->doms[d].coef_size();
@@ -525,7 +599,8 @@ int kad_xcts_create(kad_ctx* ctx, int nr, int nt, int np, const double* bounds, 
   p.ome = ome;
   p.order = order;
   try {
-    S->sys = new XctsSystem(p);
+    S->xsys = new XctsSystem(p);
+    S->sys = S->xsys;
     S->sys->build();
   } catch (const std::exception& e) {
     fprintf(stderr, "[kadath_amd] kad_xcts_create: %s\n", e.what());
@@ -596,14 +671,53 @@ int kad_xcts_row_layout(kad_system* s, int* out, int cap) {
   return q;
 }
 
+// Coefficients of configuration values on the host, with the device's tables
+// (Base_spectral::coef, coef.cpp:96-104: phi, then theta, then r).
+static std::vector<double> host_coef(const Domain& dm, const Base& b, const std::vector<double>& conf) {
+  Mat Mp = coef_matrix(b.bp, dm.np);
+  std::vector<double> t1((size_t)dm.nr * dm.nt * dm.npc, 0.0), t2(t1.size(), 0.0), t3(t1.size(), 0.0);
+  for (int kk = 0; kk < dm.npc; ++kk)
+    for (int k = 0; k < dm.np; ++k)
+      for (int j = 0; j < dm.nt; ++j)
+        for (int i = 0; i < dm.nr; ++i)
+          t1[i + dm.nr * (j + (size_t)dm.nt * kk)] += Mp(kk, k) * conf[i + dm.nr * (j + (size_t)dm.nt * k)];
+  for (int k = 0; k < dm.npc; ++k) {
+    Mat Mt = coef_matrix(b.bt[k], dm.nt);
+    for (int jj = 0; jj < dm.nt; ++jj)
+      for (int j = 0; j < dm.nt; ++j)
+        for (int i = 0; i < dm.nr; ++i)
+          t2[i + dm.nr * (jj + (size_t)dm.nt * k)] += Mt(jj, j) * t1[i + dm.nr * (j + (size_t)dm.nt * k)];
+  }
+  Mat Mr = coef_matrix(CHEB, dm.nr);
+  for (int k = 0; k < dm.npc; ++k)
+    for (int j = 0; j < dm.nt; ++j)
+      for (int ii = 0; ii < dm.nr; ++ii)
+        for (int i = 0; i < dm.nr; ++i)
+          t3[ii + dm.nr * (j + (size_t)dm.nt * k)] += Mr(ii, i) * t2[i + dm.nr * (j + (size_t)dm.nt * k)];
+  return t3;
+}
+
+// fields += affecte_tau(X0), X0 HOST (n_unknowns)
+static int add_unknowns_host(kad_system_impl* S, const double* X0) {
+  hipStream_t st = S->ctx->stream;
+  double* dX;
+  KAD_HIP_CHECK(hipMalloc(&dX, (size_t)S->sys->n_unknowns * sizeof(double)));
+  KAD_HIP_CHECK(hipMemcpyAsync(dX, X0, (size_t)S->sys->n_unknowns * sizeof(double), hipMemcpyHostToDevice, st));
+  int rc = S->update(dX, 1.0);
+  KAD_HIP_CHECK(hipStreamSynchronize(st));
+  (void)hipFree(dX);
+  return rc;
+}
+
 // Fields := flat guess (P = 1 + a/r, N = 1, bet = 0; codes/Par_version/Kerr/kerr.cpp:36-52)
 // + affecte_tau(X0).  X0: HOST, n_unknowns doubles (may be NULL for the flat guess).
 int kad_xcts_set_guess(kad_system* s, const double* X0) {
   if (!s) return KAD_ERR_ARG;
   auto* S = reinterpret_cast<kad_system_impl*>(s);
-  XctsSystem& sy = *S->sys;
+  if (!S->xsys) return KAD_ERR_ARG;
+  XctsSystem& sy = *S->xsys;
   hipStream_t st = S->ctx->stream;
-  for (int d = sy.prm.dmin; d <= sy.prm.dmax; ++d) {
+  for (int d = sy.dmin; d <= sy.dmax; ++d) {
     const Domain& dm = sy.doms[d];
     for (int f = 0; f < XctsSystem::NF; ++f) {
       std::vector<double> conf(dm.conf_size(), 0.0);
@@ -615,42 +729,14 @@ int kad_xcts_set_guess(kad_system* s, const double* X0) {
           conf[q] = 1.0;
         }
       }
-      // forward transform on the host with the same tables as the device
-      const Base b = sy.field_base(f, d);
-      Mat Mp = coef_matrix(b.bp, dm.np);
-      std::vector<double> t1((size_t)dm.nr * dm.nt * dm.npc, 0.0), t2(t1.size(), 0.0), t3(t1.size(), 0.0);
-      for (int kk = 0; kk < dm.npc; ++kk)
-        for (int k = 0; k < dm.np; ++k)
-          for (int j = 0; j < dm.nt; ++j)
-            for (int i = 0; i < dm.nr; ++i)
-              t1[i + dm.nr * (j + (size_t)dm.nt * kk)] += Mp(kk, k) * conf[i + dm.nr * (j + (size_t)dm.nt * k)];
-      for (int k = 0; k < dm.npc; ++k) {
-        Mat Mt = coef_matrix(b.bt[k], dm.nt);
-        for (int jj = 0; jj < dm.nt; ++jj)
-          for (int j = 0; j < dm.nt; ++j)
-            for (int i = 0; i < dm.nr; ++i)
-              t2[i + dm.nr * (jj + (size_t)dm.nt * k)] += Mt(jj, j) * t1[i + dm.nr * (j + (size_t)dm.nt * k)];
-      }
-      Mat Mr = coef_matrix(CHEB, dm.nr);
-      for (int k = 0; k < dm.npc; ++k)
-        for (int j = 0; j < dm.nt; ++j)
-          for (int ii = 0; ii < dm.nr; ++ii)
-            for (int i = 0; i < dm.nr; ++i)
-              t3[ii + dm.nr * (j + (size_t)dm.nt * k)] += Mr(ii, i) * t2[i + dm.nr * (j + (size_t)dm.nt * k)];
-      KAD_HIP_CHECK(hipMemcpyAsync(S->val_arena + (int64_t)sy.field_slot[f][d] * S->stride, t3.data(),
-                                   t3.size() * sizeof(double), hipMemcpyHostToDevice, st));
+      const std::vector<double> cf = host_coef(dm, sy.field_base(f, d), conf);
+      S->vals_current = false;
+      KAD_HIP_CHECK(hipMemcpyAsync(S->val_arena + (int64_t)sy.field_slot[f][d] * S->stride, cf.data(),
+                                   cf.size() * sizeof(double), hipMemcpyHostToDevice, st));
     }
   }
-  if (X0) {
-    double* dX;
-    KAD_HIP_CHECK(hipMalloc(&dX, (size_t)sy.n_unknowns * sizeof(double)));
-    KAD_HIP_CHECK(hipMemcpyAsync(dX, X0, (size_t)sy.n_unknowns * sizeof(double), hipMemcpyHostToDevice, st));
-    int rc = S->update(dX, 1.0);
-    KAD_HIP_CHECK(hipStreamSynchronize(st));
-    (void)hipFree(dX);
-    if (rc) return rc;
-  }
   KAD_HIP_CHECK(hipStreamSynchronize(st));
+  if (X0) return add_unknowns_host(S, X0);
   return KAD_OK;
 }
 
@@ -691,8 +777,8 @@ int kad_xcts_update(kad_system* s, const double* Xd) {
 int kad_xcts_get_coef(kad_system* s, int f, int d, double* out) {
   if (!s || !out) return KAD_ERR_ARG;
   auto* S = reinterpret_cast<kad_system_impl*>(s);
-  XctsSystem& sy = *S->sys;
-  if (f < 0 || f >= XctsSystem::NF || d < sy.prm.dmin || d > sy.prm.dmax) return KAD_ERR_ARG;
+  SystemBase& sy = *S->sys;
+  if (f < 0 || f >= sy.nfields || d < sy.dmin || d > sy.dmax) return KAD_ERR_ARG;
   KAD_HIP_CHECK(hipMemcpyAsync(out, S->val_arena + (int64_t)sy.field_slot[f][d] * S->stride,
                                (size_t)sy.doms[d].coef_size() * sizeof(double), hipMemcpyDeviceToHost,
                                S->ctx->stream));
@@ -705,8 +791,9 @@ int kad_xcts_get_coef(kad_system* s, int f, int d, double* out) {
 int kad_xcts_set_coef(kad_system* s, int f, int d, const double* in) {
   if (!s || !in) return KAD_ERR_ARG;
   auto* S = reinterpret_cast<kad_system_impl*>(s);
-  XctsSystem& sy = *S->sys;
-  if (f < 0 || f >= XctsSystem::NF || d < sy.prm.dmin || d > sy.prm.dmax) return KAD_ERR_ARG;
+  SystemBase& sy = *S->sys;
+  if (f < 0 || f >= sy.nfields || d < sy.dmin || d > sy.dmax) return KAD_ERR_ARG;
+  S->vals_current = false;
   KAD_HIP_CHECK(hipMemcpyAsync(S->val_arena + (int64_t)sy.field_slot[f][d] * S->stride, in,
                                (size_t)sy.doms[d].coef_size() * sizeof(double), hipMemcpyHostToDevice,
                                S->ctx->stream));
@@ -720,13 +807,15 @@ int kad_xcts_set_coef(kad_system* s, int f, int d, const double* in) {
 int kad_xcts_observables(kad_system* s, double* out) {
   if (!s || !out) return KAD_ERR_ARG;
   auto* S = reinterpret_cast<kad_system_impl*>(s);
-  XctsSystem& sy = *S->sys;
+  SystemBase& sy = *S->sys;
+  if (sy.obs.empty() || sy.obs.size() > 32) return KAD_ERR_ARG;
   hipStream_t st = S->ctx->stream;
   int rc = S->value_pass();
   if (rc) return rc;
   if ((rc = S->run(sy.obs_prog.code, 1, 0, -1, S->F, 0, 0))) return rc;
-  const Domain& dm = sy.doms[sy.prm.dmax];
-  for (int q = 0; q < 3; ++q) {
+  const Domain& dm = sy.doms[sy.dmax];
+  const int nq = (int)sy.obs.size();
+  for (int q = 0; q < nq; ++q) {
     const auto& r = sy.obs[q];
     if (r.slot < 0) {
       KAD_HIP_CHECK(hipMemsetAsync(S->scratch + q, 0, sizeof(double), st));
@@ -736,41 +825,49 @@ int kad_xcts_observables(kad_system* s, double* out) {
                                               r.fac, S->scratch + q);
     KAD_HIP_CHECK(hipGetLastError());
   }
-  KAD_HIP_CHECK(hipMemcpyAsync(out, S->scratch, 3 * sizeof(double), hipMemcpyDeviceToHost, st));
+  KAD_HIP_CHECK(hipMemcpyAsync(out, S->scratch, nq * sizeof(double), hipMemcpyDeviceToHost, st));
   KAD_HIP_CHECK(hipStreamSynchronize(st));
   return KAD_OK;
 }
 
-// Set a parameter of the system: which 0 = Ome (horizon rotation, kerr.cpp:91-95).
-int kad_xcts_set_param(kad_system* s, int which, double value) {
-  if (!s || which != 0) return KAD_ERR_ARG;
-  auto* S = reinterpret_cast<kad_system_impl*>(s);
-  XctsSystem& sy = *S->sys;
-  for (int d = sy.prm.dmin; d <= sy.prm.dmax; ++d) {
-    const int slot = sy.const_slot("ome", d);
+// Set a parameter of the system: which indexes its parameters (XCTS: 0 = Ome, the
+// horizon rotation of kerr.cpp:91-95).  The constant field is rewritten in every
+// domain that holds it.
+static int set_param_impl(kad_system_impl* S, int which, double value) {
+  SystemBase& sy = *S->sys;
+  if (which < 0 || which >= (int)sy.par_names.size()) return KAD_ERR_ARG;
+  for (int d = sy.dmin; d <= sy.dmax; ++d) {
+    const int slot = sy.param_slot(sy.par_names[which], d);
     if (slot < 0) continue;
     std::vector<double> h(sy.doms[d].conf_size(), value);
+    S->vals_current = false;
     KAD_HIP_CHECK(hipMemcpyAsync(S->val_arena + (int64_t)slot * S->stride, h.data(), h.size() * sizeof(double),
                                  hipMemcpyHostToDevice, S->ctx->stream));
     KAD_HIP_CHECK(hipStreamSynchronize(S->ctx->stream));
   }
-  sy.prm.ome = value;
+  if (S->xsys && sy.par_names[which] == "ome") S->xsys->prm.ome = value;
   return KAD_OK;
 }
 
-// Sensitivities of the global quantities Q = (M_ADM, M_Komar, J_ADM) at a solution
-// F(u, Ome) = 0 with respect to Ome, by the adjoint method:
+int kad_xcts_set_param(kad_system* s, int which, double value) {
+  if (!s || which != 0) return KAD_ERR_ARG;
+  return set_param_impl(reinterpret_cast<kad_system_impl*>(s), which, value);
+}
+
+// Sensitivities of the global quantities Q_q at a solution F(u, p) = 0 with respect
+// to the parameter p = par, by the adjoint method:
 //   J^T lambda_q = dQ_q/du   (the transposed solve reuses the LU factors of J),
-//   dQ_q/dOme = dQ_q/dOme|_u - lambda_q . dF/dOme   (the first term is 0 here).
-// dQ/du comes from tangent programs of the observables, dF/dOme from the tangent
-// program of Ome; J is refilled and factored at the current fields.  HOST out[3];
-// dQdu (HOST, 3 x n, may be NULL) receives dQ/du.
-int kad_xcts_sensitivity(kad_system* s, double* out, double* dQdu) {
-  if (!s || !out) return KAD_ERR_ARG;
-  auto* S = reinterpret_cast<kad_system_impl*>(s);
-  XctsSystem& sy = *S->sys;
+//   dQ_q/dp = dQ_q/dp|_u - lambda_q . dF/dp   (the first term is 0: the quantities
+//   read no parameter).
+// dQ/du comes from tangent programs of the quantities, dF/dp from the tangent
+// program of the parameter; J is refilled and factored at the current fields.
+// out: HOST, one per quantity; dQdu (HOST, nq x n, may be NULL) receives dQ/du.
+static int sensitivity_impl(kad_system_impl* S, int par, double* out, double* dQdu) {
+  SystemBase& sy = *S->sys;
   hipStream_t st = S->ctx->stream;
   const int n = sy.n_unknowns;
+  const int nq = (int)sy.obs.size();
+  if (nq == 0 || par < 0 || par >= (int)S->par_progs.size()) return KAD_ERR_ARG;
   int rc;
   if (!S->J) {
     if ((rc = S->dmalloc((void**)&S->J, (size_t)n * n * sizeof(double)))) return rc;
@@ -779,26 +876,27 @@ int kad_xcts_sensitivity(kad_system* s, double* out, double* dQdu) {
     if ((rc = S->dmalloc((void**)&S->X, (size_t)n * sizeof(double)))) return rc;
   }
   if (!S->dQ) {
-    if ((rc = S->dmalloc((void**)&S->dQ, (size_t)3 * n * sizeof(double)))) return rc;
+    if ((rc = S->dmalloc((void**)&S->dQ, (size_t)nq * n * sizeof(double)))) return rc;
     if ((rc = S->dmalloc((void**)&S->dFdp, (size_t)n * sizeof(double)))) return rc;
   }
   if ((rc = S->jacobian(S->J, n))) return rc;  // includes the value pass at the current fields
   if ((rc = S->run(sy.obs_prog.code, 1, 0, -1, S->F, 0, 0))) return rc;
-  KAD_HIP_CHECK(hipMemsetAsync(S->dQ, 0, (size_t)3 * n * sizeof(double), st));
+  KAD_HIP_CHECK(hipMemsetAsync(S->dQ, 0, (size_t)nq * n * sizeof(double), st));
   KAD_HIP_CHECK(hipMemsetAsync(S->dFdp, 0, (size_t)n * sizeof(double), st));
   for (auto& p : S->obs_tprogs)
     for (int u0 = 0; u0 < p.cnt; u0 += S->B) {
       const int nb = std::min(S->B, p.cnt - u0);
       if ((rc = S->run(p.code, nb, u0, p.seed_tab, S->dQ, (int64_t)p.off + u0, n))) return rc;
     }
-  if ((rc = S->run(S->ome_prog.code, 1, 0, S->ome_prog.seed_tab, S->dFdp, 0, n))) return rc;
+  const DevProgram& pp = S->par_progs[par];
+  if ((rc = S->run(pp.code, 1, 0, pp.seed_tab, S->dFdp, 0, n))) return rc;
   if (dQdu) {
-    KAD_HIP_CHECK(hipMemcpyAsync(dQdu, S->dQ, (size_t)3 * n * sizeof(double), hipMemcpyDeviceToHost, st));
+    KAD_HIP_CHECK(hipMemcpyAsync(dQdu, S->dQ, (size_t)nq * n * sizeof(double), hipMemcpyDeviceToHost, st));
   }
   rc = kad_lu_factor(reinterpret_cast<kad_ctx*>(S->ctx), KAD_PREC_F64, n, S->J, n, S->ipiv, S->perm, 0);
   if (rc < 0) return rc;
   if (rc > 0) return KAD_ERR_SINGULAR;
-  for (int q = 0; q < 3; ++q) {
+  for (int q = 0; q < nq; ++q) {
     KAD_HIP_CHECK(hipMemcpyAsync(S->X, S->dQ + (size_t)q * n, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice,
                                  st));
     if ((rc = kad_lu_solve(reinterpret_cast<kad_ctx*>(S->ctx), KAD_PREC_F64, n, S->J, n, S->perm, S->X, 1)))
@@ -809,6 +907,12 @@ int kad_xcts_sensitivity(kad_system* s, double* out, double* dQdu) {
   return KAD_OK;
 }
 
+// XCTS: out[3] = d(M_ADM, M_Komar, J_ADM)/dOme.
+int kad_xcts_sensitivity(kad_system* s, double* out, double* dQdu) {
+  if (!s || !out) return KAD_ERR_ARG;
+  return sensitivity_impl(reinterpret_cast<kad_system_impl*>(s), 0, out, dQdu);
+}
+
 // One Newton-Raphson iteration (System_of_eqs::do_newton, src_par/do_newton.cpp:65-430).
 // solver: 0 fp64 LU; 1 fp32 LU + fp64 IR; 2 fp32 LU + fp64 GMRES-IR; 3 bf16-update LU + fp64 GMRES-IR.
 // Returns 1 when ||F||_inf < precision at entry (converged; no step taken), 0 after a step,
@@ -817,7 +921,7 @@ int kad_xcts_sensitivity(kad_system* s, double* out, double* dQdu) {
 int kad_xcts_do_newton(kad_system* s, double precision, double* error, int solver, double* times) {
   if (!s || !error || solver < 0 || solver > 3) return KAD_ERR_ARG;
   auto* S = reinterpret_cast<kad_system_impl*>(s);
-  XctsSystem& sy = *S->sys;
+  SystemBase& sy = *S->sys;
   hipStream_t st = S->ctx->stream;
   const int n = sy.n_unknowns;
   int rc;
@@ -932,7 +1036,7 @@ int kad_xcts_do_newton(kad_system* s, double precision, double* error, int solve
 int kad_xcts_perturbation(kad_system* s, uint64_t seed, double amp, double* out) {
   if (!s || !out) return KAD_ERR_ARG;
   auto* S = reinterpret_cast<kad_system_impl*>(s);
-  XctsSystem& sy = *S->sys;
+  SystemBase& sy = *S->sys;
   uint64_t x = seed;
   auto next = [&]() {
     uint64_t z = (x += 0x9E3779B97F4A7C15ull);
@@ -941,11 +1045,10 @@ int kad_xcts_perturbation(kad_system* s, uint64_t seed, double amp, double* out)
     z ^= (z >> 31);
     return (double)(z >> 11) * (1.0 / 9007199254740992.0);
   };
-  // draws in the reference unknown order (variables by domain), stored at this
-  // system's layout offsets, so both orderings start from the same guess
-  const std::vector<std::vector<int>> vars = {{0}, {1}, {2, 3, 4}};
-  for (auto& var : vars)
-    for (int d = sy.prm.dmin; d <= sy.prm.dmax; ++d)
+  // draws in the reference unknown order (variables, then domains, then components),
+  // stored at this system's layout offsets, so both orderings start from the same guess
+  for (auto& var : sy.var_fields)
+    for (int d = sy.dmin; d <= sy.dmax; ++d)
       for (int f : var) {
         int u = -1;
         for (auto& L : sy.layout)
@@ -1014,3 +1117,368 @@ int kad_xcts_host_info(int nr, int nt, int np, const double* bounds, int nbounds
   }
 }
 }
+
+// ---------------------------------------------------------------------------
+// Systems given as equation strings, and the entry points common to every system
+// ---------------------------------------------------------------------------
+struct kad_sysdef_impl {
+  ParsedSystem* ps = nullptr;
+  std::string err;
+  ~kad_sysdef_impl() { delete ps; }
+};
+
+template <typename F>
+static int guarded(kad_sysdef* def, F f) {
+  if (!def) return KAD_ERR_ARG;
+  auto* D = reinterpret_cast<kad_sysdef_impl*>(def);
+  if (!D->ps) return KAD_ERR_ARG;
+  try {
+    f(*D->ps);
+  } catch (const std::exception& e) {
+    D->err = e.what();
+    fprintf(stderr, "[kadath_amd] %s\n", e.what());
+    return KAD_ERR_ARG;
+  }
+  return KAD_OK;
+}
+
+extern "C" {
+
+int kad_sysdef_create(int nr, int nt, int np, const double* bounds, int nbounds, int dmin, int dmax,
+                      kad_sysdef** out) {
+  if (!bounds || nbounds < 1 || !out) return KAD_ERR_ARG;
+  auto* D = new kad_sysdef_impl();
+  try {
+    D->ps = new ParsedSystem(nr, nt, np, std::vector<double>(bounds, bounds + nbounds), dmin, dmax, 0);
+  } catch (const std::exception& e) {
+    fprintf(stderr, "[kadath_amd] kad_sysdef_create: %s\n", e.what());
+    delete D;
+    return KAD_ERR_ARG;
+  }
+  *out = reinterpret_cast<kad_sysdef*>(D);
+  return KAD_OK;
+}
+
+int kad_sysdef_destroy(kad_sysdef* def) {
+  if (!def) return KAD_ERR_ARG;
+  delete reinterpret_cast<kad_sysdef_impl*>(def);
+  return KAD_OK;
+}
+
+int kad_sysdef_add_var(kad_sysdef* def, const char* name, int valence) {
+  if (!name) return KAD_ERR_ARG;
+  return guarded(def, [&](ParsedSystem& p) { p.add_var(name, valence); });
+}
+int kad_sysdef_add_cst(kad_sysdef* def, const char* name, double value) {
+  if (!name) return KAD_ERR_ARG;
+  return guarded(def, [&](ParsedSystem& p) { p.add_cst(name, value); });
+}
+int kad_sysdef_add_param(kad_sysdef* def, const char* name, double value) {
+  if (!name) return KAD_ERR_ARG;
+  int idx = -1;
+  int rc = guarded(def, [&](ParsedSystem& p) { idx = p.add_param(name, value); });
+  return rc ? rc : idx;
+}
+int kad_sysdef_add_cst_field(kad_sysdef* def, const char* name, int valence, const double* conf, int64_t count) {
+  if (!name || !conf || count <= 0) return KAD_ERR_ARG;
+  return guarded(def, [&](ParsedSystem& p) { p.add_cst_field(name, valence, std::vector<double>(conf, conf + count)); });
+}
+int kad_sysdef_set_flat_metric(kad_sysdef* def, const char* name) {
+  if (!name) return KAD_ERR_ARG;
+  return guarded(def, [&](ParsedSystem& p) { p.set_flat_metric(name); });
+}
+int kad_sysdef_add_def(kad_sysdef* def, const char* text) {
+  if (!text) return KAD_ERR_ARG;
+  return guarded(def, [&](ParsedSystem& p) { p.add_def(text); });
+}
+int kad_sysdef_add_eq_inside(kad_sysdef* def, int dom, const char* eq, int order) {
+  if (!eq) return KAD_ERR_ARG;
+  return guarded(def, [&](ParsedSystem& p) { p.add_eq_inside(dom, eq, order); });
+}
+int kad_sysdef_add_eq_bc(kad_sysdef* def, int dom, int bound, const char* eq) {
+  if (!eq) return KAD_ERR_ARG;
+  return guarded(def, [&](ParsedSystem& p) { p.add_eq_bc(dom, bound, eq); });
+}
+int kad_sysdef_add_eq_matching(kad_sysdef* def, int dom, int bound, const char* expr) {
+  if (!expr) return KAD_ERR_ARG;
+  return guarded(def, [&](ParsedSystem& p) { p.add_eq_matching(dom, bound, expr); });
+}
+int kad_sysdef_space_add_eq(kad_sysdef* def, const char* eq, const char* m1, const char* m2) {
+  if (!eq || !m1 || !m2) return KAD_ERR_ARG;
+  return guarded(def, [&](ParsedSystem& p) { p.space_add_eq(eq, m1, m2); });
+}
+int kad_sysdef_space_add_bc(kad_sysdef* def, int bound, const char* eq) {
+  if (!eq) return KAD_ERR_ARG;
+  return guarded(def, [&](ParsedSystem& p) {
+    if (bound == INNER_BC) p.space_add_inner_bc(eq);
+    else p.space_add_outer_bc(eq);
+  });
+}
+
+// Collocation points of the conf grid of domain d (r fastest, then theta, then phi):
+// r (compactified domain: infinity at x = 1 is reported as +inf), theta, phi.  HOST out.
+int kad_sysdef_coloc(kad_sysdef* def, int d, double* r, double* theta, double* phi) {
+  return guarded(def, [&](ParsedSystem& p) {
+    if (d < p.dmin || d > p.dmax) throw std::runtime_error("kad_sysdef_coloc: domain out of range");
+    const Domain& dm = p.doms[d];
+    for (int k = 0; k < dm.np; ++k)
+      for (int j = 0; j < dm.nt; ++j)
+        for (int i = 0; i < dm.nr; ++i) {
+          const size_t q = (size_t)i + dm.nr * (j + (size_t)dm.nt * k);
+          if (r) r[q] = std::isfinite(dm.R[q]) ? dm.R[q] : HUGE_VAL;
+          if (theta) theta[q] = dm.th[j];
+          if (phi) phi[q] = dm.ph[k];
+        }
+  });
+}
+
+// Host-only trace (no device): sizes and row layout of the declared system.
+int kad_sysdef_host_trace(kad_sysdef* def, int order, int* info, int* row_layout, int cap) {
+  if (!info || (order != 0 && order != 1)) return KAD_ERR_ARG;
+  int q = 0;
+  int rc = guarded(def, [&](ParsedSystem& p) {
+    p.order_mode = order;
+    p.build();
+    size_t tcode = 0;
+    for (auto& pr : p.tprog) tcode += pr.code.size();
+    info[0] = p.n_unknowns;
+    info[1] = p.n_rows;
+    info[2] = (int)p.tr.val.code.size();
+    info[3] = (int)tcode;
+    info[4] = p.tr.nval_slots;
+    info[5] = (int)p.tprog.size();
+    for (const auto& e : p.eqs) {
+      int row = e.row0;
+      for (int c = 0; c < e.ncomp; ++c) {
+        if (row_layout) {
+          if (5 * (q + 1) > cap) throw std::runtime_error("kad_sysdef_host_trace: row layout capacity");
+          const int v[5] = {e.kind, e.d, c, row, e.nrows[c]};
+          for (int t = 0; t < 5; ++t) row_layout[5 * q + t] = v[t];
+        }
+        row += e.nrows[c];
+        ++q;
+      }
+    }
+  });
+  if (rc) return rc;
+  // a traced declaration is not built again
+  auto* D = reinterpret_cast<kad_sysdef_impl*>(def);
+  delete D->ps;
+  D->ps = nullptr;
+  return q;
+}
+
+// Trace the declared system into device programs and upload them.  The declaration
+// is consumed (def is freed, also on failure).
+int kad_system_build(kad_ctx* ctx, kad_sysdef* def, int order, int batch, kad_system** out) {
+  if (!def) return KAD_ERR_ARG;
+  auto* D = reinterpret_cast<kad_sysdef_impl*>(def);
+  if (!ctx || !out || batch <= 0 || (order != 0 && order != 1) || !D->ps) {
+    delete D;
+    return KAD_ERR_ARG;
+  }
+  auto* S = new kad_system_impl();
+  S->ctx = reinterpret_cast<kad::Ctx*>(ctx);
+  S->B = batch;
+  S->sys = D->ps;
+  D->ps = nullptr;
+  delete D;
+  try {
+    static_cast<ParsedSystem*>(S->sys)->order_mode = order;
+    S->sys->build();
+  } catch (const std::exception& e) {
+    fprintf(stderr, "[kadath_amd] kad_system_build: %s\n", e.what());
+    delete S;
+    return KAD_ERR_ARG;
+  }
+  int rc = S->setup();
+  if (rc) {
+    delete S;
+    return rc;
+  }
+  *out = reinterpret_cast<kad_system*>(S);
+  return KAD_OK;
+}
+
+int kad_system_destroy(kad_system* s) { return kad_xcts_destroy(s); }
+
+// n_unknowns, n_rows, scalar field components
+int kad_system_sizes(kad_system* s, int* n_unknowns, int* n_rows, int* nfields) {
+  if (!s) return KAD_ERR_ARG;
+  auto* S = reinterpret_cast<kad_system_impl*>(s);
+  if (n_unknowns) *n_unknowns = S->sys->n_unknowns;
+  if (n_rows) *n_rows = S->sys->n_rows;
+  if (nfields) *nfields = S->sys->nfields;
+  return KAD_OK;
+}
+
+int kad_system_layout(kad_system* s, int* out, int cap) { return kad_xcts_layout(s, out, cap); }
+int kad_system_row_layout(kad_system* s, int* out, int cap) { return kad_xcts_row_layout(s, out, cap); }
+
+// Values of field component f in domain d at the collocation points (HOST in, conf
+// order): the field's coefficients become their transform (Scalar::set_domain then
+// std_base / coef, tensor.cpp:609-650).
+int kad_system_set_field(kad_system* s, int f, int d, const double* conf) {
+  if (!s || !conf) return KAD_ERR_ARG;
+  auto* S = reinterpret_cast<kad_system_impl*>(s);
+  SystemBase& sy = *S->sys;
+  if (f < 0 || f >= sy.nfields || d < sy.dmin || d > sy.dmax) return KAD_ERR_ARG;
+  const Domain& dm = sy.doms[d];
+  std::vector<double> c(conf, conf + dm.conf_size());
+  for (double& x : c)
+    if (!std::isfinite(x)) return KAD_ERR_ARG;
+  const std::vector<double> cf = host_coef(dm, sy.field_base(f, d), c);
+  S->vals_current = false;
+  KAD_HIP_CHECK(hipMemcpyAsync(S->val_arena + (int64_t)sy.field_slot[f][d] * S->stride, cf.data(),
+                               cf.size() * sizeof(double), hipMemcpyHostToDevice, S->ctx->stream));
+  KAD_HIP_CHECK(hipStreamSynchronize(S->ctx->stream));
+  return KAD_OK;
+}
+
+// Values of field component f in domain d at the collocation points (HOST out):
+// the synthesis of its coefficients (Val_domain::coef_i).
+int kad_system_get_field(kad_system* s, int f, int d, double* conf) {
+  if (!s || !conf) return KAD_ERR_ARG;
+  auto* S = reinterpret_cast<kad_system_impl*>(s);
+  SystemBase& sy = *S->sys;
+  if (f < 0 || f >= sy.nfields || d < sy.dmin || d > sy.dmax) return KAD_ERR_ARG;
+  const Domain& dm = sy.doms[d];
+  std::vector<double> cf(dm.coef_size());
+  int rc = kad_xcts_get_coef(s, f, d, cf.data());
+  if (rc) return rc;
+  const Base b = sy.field_base(f, d);
+  // coef_i: r, then theta, then phi (coef_i.cpp:96-104)
+  Mat Mr = coef_i_matrix(CHEB, dm.nr);
+  std::vector<double> t1(cf.size(), 0.0), t2(cf.size(), 0.0);
+  for (int k = 0; k < dm.npc; ++k)
+    for (int j = 0; j < dm.nt; ++j)
+      for (int i = 0; i < dm.nr; ++i)
+        for (int ii = 0; ii < dm.nr; ++ii)
+          t1[i + dm.nr * (j + (size_t)dm.nt * k)] += Mr(i, ii) * cf[ii + dm.nr * (j + (size_t)dm.nt * k)];
+  for (int k = 0; k < dm.npc; ++k) {
+    Mat Mt = coef_i_matrix(b.bt[k], dm.nt);
+    for (int j = 0; j < dm.nt; ++j)
+      for (int jj = 0; jj < dm.nt; ++jj)
+        for (int i = 0; i < dm.nr; ++i)
+          t2[i + dm.nr * (j + (size_t)dm.nt * k)] += Mt(j, jj) * t1[i + dm.nr * (jj + (size_t)dm.nt * k)];
+  }
+  Mat Mp = coef_i_matrix(b.bp, dm.np);
+  for (int k = 0; k < dm.np; ++k)
+    for (int j = 0; j < dm.nt; ++j)
+      for (int i = 0; i < dm.nr; ++i) {
+        double v = 0.0;
+        for (int kk = 0; kk < dm.npc; ++kk) v += Mp(k, kk) * t2[i + dm.nr * (j + (size_t)dm.nt * kk)];
+        conf[i + dm.nr * (j + (size_t)dm.nt * k)] = v;
+      }
+  return KAD_OK;
+}
+
+int kad_system_get_coef(kad_system* s, int f, int d, double* out) { return kad_xcts_get_coef(s, f, d, out); }
+int kad_system_set_coef(kad_system* s, int f, int d, const double* in) { return kad_xcts_set_coef(s, f, d, in); }
+int kad_system_update(kad_system* s, const double* X) { return kad_xcts_update(s, X); }
+int kad_system_perturbation(kad_system* s, uint64_t seed, double amp, double* out) {
+  return kad_xcts_perturbation(s, seed, amp, out);
+}
+int kad_system_add_unknowns(kad_system* s, const double* X0) {
+  if (!s || !X0) return KAD_ERR_ARG;
+  return add_unknowns_host(reinterpret_cast<kad_system_impl*>(s), X0);
+}
+int kad_system_set_param(kad_system* s, int which, double value) {
+  if (!s) return KAD_ERR_ARG;
+  return set_param_impl(reinterpret_cast<kad_system_impl*>(s), which, value);
+}
+int kad_system_do_newton(kad_system* s, double precision, double* error, int solver, double* times) {
+  return kad_xcts_do_newton(s, precision, error, solver, times);
+}
+int kad_system_last_solve(kad_system* s, int* iters, double* resid) { return kad_xcts_last_solve(s, iters, resid); }
+
+// F(u) (device, n_rows): System_of_eqs::sec_member.
+int kad_residual(kad_system* s, double* F) { return kad_xcts_residual(s, F); }
+
+// Per (equation, component) max |F| over its rows (HOST out, cap entries), in row-layout
+// order: System_of_eqs::check_equations (solver.cpp:24-47).  Returns the entry count.
+int kad_check_equations(kad_system* s, double* errors, int cap) {
+  if (!s || !errors) return KAD_ERR_ARG;
+  auto* S = reinterpret_cast<kad_system_impl*>(s);
+  int rc = S->value_pass();
+  if (rc) return rc;
+  std::vector<double> h(S->sys->n_rows);
+  KAD_HIP_CHECK(hipMemcpyAsync(h.data(), S->F, h.size() * sizeof(double), hipMemcpyDeviceToHost, S->ctx->stream));
+  KAD_HIP_CHECK(hipStreamSynchronize(S->ctx->stream));
+  int q = 0;
+  for (const auto& e : S->sys->eqs) {
+    int row = e.row0;
+    for (int c = 0; c < e.ncomp; ++c) {
+      if (q >= cap) return KAD_ERR_ARG;
+      double m = 0.0;
+      for (int r = row; r < row + e.nrows[c]; ++r) {
+        const double a = std::fabs(h[r]);
+        if (a > m || a != a) m = a;
+      }
+      errors[q++] = m;
+      row += e.nrows[c];
+    }
+  }
+  return q;
+}
+
+// Columns [col_begin, col_begin + col_count) of J (every do_col_J of that range,
+// solver.cpp:120-214) into the device tile J_tile (column-major, ld >= n_rows), fp64
+// (dtype 0) or fp32 (dtype 1).  The tile's columns are cleared first; the values are
+// those of the current fields (the value pass runs when they changed).
+int kad_jvp_fill(kad_system* s, int col_begin, int col_count, void* J_tile, int64_t ld, int dtype) {
+  if (!s || !J_tile || col_begin < 0 || col_count <= 0 || (dtype != 0 && dtype != 1)) return KAD_ERR_ARG;
+  auto* S = reinterpret_cast<kad_system_impl*>(s);
+  const int n = S->sys->n_unknowns, m = S->sys->n_rows;
+  if (ld < m || col_begin + (int64_t)col_count > n) return KAD_ERR_ARG;
+  hipStream_t st = S->ctx->stream;
+  int rc = S->ensure_values();
+  if (rc) return rc;
+  const size_t es = dtype == 0 ? sizeof(double) : sizeof(float);
+  KAD_HIP_CHECK(hipMemset2DAsync(J_tile, (size_t)ld * es, 0, (size_t)m * es, (size_t)col_count, st));
+  const int c1 = col_begin + col_count;
+  for (auto& p : S->tprogs) {
+    const int lo = std::max(col_begin, p.off), hi = std::min(c1, p.off + p.cnt);
+    for (int u = lo; u < hi; u += S->B) {
+      const int nb = std::min(S->B, hi - u);
+      if ((rc = S->run(p.code, nb, u - p.off, p.seed_tab, J_tile, (int64_t)u - col_begin, ld, dtype))) return rc;
+    }
+  }
+  KAD_HIP_CHECK(hipStreamSynchronize(st));
+  return KAD_OK;
+}
+
+// out = J^T w (device, n_unknowns; w device, n_rows), matrix-free: every tangent
+// program runs as for the fill, and its row exports are replaced by their transposes
+// (per column, the dot product of w with the exported rows), so J is never formed.
+int kad_vjp(kad_system* s, const double* w, double* out) {
+  if (!s || !w || !out) return KAD_ERR_ARG;
+  auto* S = reinterpret_cast<kad_system_impl*>(s);
+  hipStream_t st = S->ctx->stream;
+  int rc = S->ensure_values();
+  if (rc) return rc;
+  KAD_HIP_CHECK(hipMemsetAsync(out, 0, (size_t)S->sys->n_unknowns * sizeof(double), st));
+  for (auto& p : S->tprogs)
+    for (int u0 = 0; u0 < p.cnt; u0 += S->B) {
+      const int nb = std::min(S->B, p.cnt - u0);
+      if ((rc = S->run(p.code, nb, u0, p.seed_tab, out, (int64_t)p.off + u0, 0, 0, w))) return rc;
+    }
+  KAD_HIP_CHECK(hipStreamSynchronize(st));
+  return KAD_OK;
+}
+
+// dF/dp of parameter `which` at the current fields (device out, n_rows).
+int kad_param_tangent(kad_system* s, int which, double* out) {
+  if (!s || !out) return KAD_ERR_ARG;
+  auto* S = reinterpret_cast<kad_system_impl*>(s);
+  if (which < 0 || which >= (int)S->par_progs.size()) return KAD_ERR_ARG;
+  int rc = S->ensure_values();
+  if (rc) return rc;
+  KAD_HIP_CHECK(hipMemsetAsync(out, 0, (size_t)S->sys->n_rows * sizeof(double), S->ctx->stream));
+  const DevProgram& pp = S->par_progs[which];
+  if ((rc = S->run(pp.code, 1, 0, pp.seed_tab, out, 0, S->sys->n_rows))) return rc;
+  KAD_HIP_CHECK(hipStreamSynchronize(S->ctx->stream));
+  return KAD_OK;
+}
+
+}  // extern "C"

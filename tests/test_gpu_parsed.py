@@ -1,0 +1,142 @@
+"""GPU: systems given as equation strings (kad_sysdef_* / kad_system_*).
+
+- tuto5 (tutorials/tuto5.cpp): one Newton step reaches the reference's known answer
+  1 + a/r at every collocation point, and check_equations is then below 1e-10.
+- The Kerr program of kerr.cpp:69-99 as strings gives the same F and J as the
+  hand-traced XCTS system (itself pinned against the oracle) at the seeded guess, in
+  both unknown orders, and Newton from that guess converges to the same fields.
+- kad_jvp_fill column tiles (fp64 and fp32) equal the matching columns of the full J.
+- kad_vjp(w) = J^T w without forming J, and the torch.autograd.Function's backward is
+  that product; dF/dome from the parameter program equals central differences.
+"""
+import numpy as np
+import pytest
+import torch
+
+from kadath_frontera_amd import programs
+from kadath_frontera_amd.system import Residual
+
+pytestmark = pytest.mark.gpu
+
+RES, BOUNDS = (9, 7, 4), [1.0, 2.0, 4.0]
+
+
+def _flat_guess(s, sd_coloc, a=1.0):
+    for d in range(1, len(BOUNDS) + 1):
+        r, t, p = sd_coloc[d]
+        P = np.where(np.isfinite(r), 1.0 + a / np.where(np.isfinite(r), r, 1.0), 1.0)
+        s.set_field("P", d, P)
+        s.set_field("N", d, np.ones_like(P))
+        for c in "xyz":
+            s.set_field("bet^" + c, d, np.zeros_like(P))
+
+
+def _kerr(ctx, order, ome=0.05):
+    sd = programs.kerr(ctx, RES, BOUNDS, n0=0.5, a=1.0, ome=ome)
+    co = {d: sd.coloc(d) for d in range(1, len(BOUNDS) + 1)}
+    s = sd.build(order=order, batch=256)
+    return s, co
+
+
+def test_tuto5_known_answer(ctx):
+    sd, rad = programs.tuto5(ctx)
+    co = {d: sd.coloc(d) for d in range(1, 4)}
+    s = sd.build(order=0, batch=256)
+    for d in range(1, 4):
+        s.set_field("F", d, np.ones_like(co[d][0]))
+    print("tuto5: Newton", flush=True)
+    conv, err, _, _ = s.do_newton(1e-8)
+    assert not conv and err > 1e-3
+    conv, err, _, _ = s.do_newton(1e-8)
+    assert conv and err < 1e-10, err  # linear problem: one step
+    assert s.check_equations().max() < 1e-10
+    for d in range(1, 4):
+        r = co[d][0]
+        exact = np.where(np.isfinite(r), 1.0 + rad / np.where(np.isfinite(r), r, 1.0), 1.0)
+        assert np.abs(s.get_field("F", d) - exact).max() < 1e-9, d
+    s.close()
+
+
+@pytest.mark.parametrize("order", [0, 1])
+def test_kerr_strings_match_hand_traced(ctx, order):
+    from kadath_frontera_amd.xcts import XctsSystem
+    s, co = _kerr(ctx, order)
+    x = XctsSystem(ctx, RES, BOUNDS, n0=0.5, a=1.0, ome=0.05, batch=256, order=order)
+    assert (s.n, s.m) == (x.n, x.m)
+    assert [L[2:] for L in s.layout] == [L[2:] for L in x.layout]
+    X0 = x.perturbation(20261202)
+    assert np.array_equal(X0, s.perturbation(20261202))
+    x.set_guess(X0)
+    _flat_guess(s, co)
+    s.add_unknowns(X0)
+    print(f"kerr strings order {order}: F, J", flush=True)
+    F1, F2 = s.residual().cpu().numpy(), x.residual().cpu().numpy()
+    assert np.abs(F1 - F2).max() < 1e-12 * max(1.0, np.abs(F2).max())
+    J1, J2 = s.jacobian().cpu().numpy(), x.jacobian().cpu().numpy()
+    scale = np.maximum(1.0, np.abs(J2).max(axis=0))
+    assert (np.abs(J1 - J2) / scale).max() < 1e-11
+    for sysd in (s, x):
+        for _ in range(10):
+            conv, err, _, _ = sysd.do_newton(1e-11, "f64")
+            if conv:
+                break
+        assert conv
+    for f in range(5):
+        for d in range(1, len(BOUNDS) + 1):
+            a, b = s.coef(f, d), x.coef(f, d)
+            assert np.abs(a - b).max() <= 1e-10 * max(1.0, np.abs(b).max()), (f, d)
+    s.close()
+    x.close()
+
+
+def test_jvp_fill_tiles_and_vjp(ctx):
+    s, co = _kerr(ctx, 1)
+    _flat_guess(s, co)
+    s.add_unknowns(s.perturbation(7))
+    J = s.jacobian()
+    for (c0, cnt) in [(0, 1), (5, 300), (s.n - 77, 77), (1000, 2048)]:
+        T64 = s.jvp_fill(c0, cnt)
+        assert torch.equal(T64, J[:, c0:c0 + cnt])
+        T32 = s.jvp_fill(c0, cnt, dtype=torch.float32, ld=s.m + 3)
+        assert torch.equal(T32, J[:, c0:c0 + cnt].to(torch.float32))
+    g = torch.Generator(device="cpu").manual_seed(3)
+    w = torch.randn(s.m, generator=g, dtype=torch.float64).cuda()
+    v = s.vjp(w)
+    ref = J.T @ w
+    assert ((v - ref).abs().max() / ref.abs().max()).item() < 1e-13
+    # autograd: backward of F(u) is J^T grad_F
+    u = torch.zeros(s.n, dtype=torch.float64, device="cuda", requires_grad=True)
+    F = Residual.apply(u, s)
+    F.backward(w)
+    assert ((u.grad - ref).abs().max() / ref.abs().max()).item() < 1e-13
+    # at a shifted point the backward linearises there: compare with central differences
+    h = 1e-6
+    rng = np.random.default_rng(5)
+    cols = rng.choice(s.n, 6, replace=False)
+    u1 = torch.zeros(s.n, dtype=torch.float64, device="cuda")
+    u1[int(cols[0])] = 1e-4
+    u1.requires_grad_(True)
+    Residual.apply(u1, s).backward(w)
+    for c in cols:
+        e = torch.zeros(s.n, dtype=torch.float64, device="cuda")
+        e[int(c)] = h
+        fd = (Residual.apply(u1.detach() + e, s) - Residual.apply(u1.detach() - e, s)) @ w / (2 * h)
+        assert abs(u1.grad[int(c)].item() - fd.item()) <= 1e-6 * max(1.0, abs(fd.item())), c
+    s.close()
+
+
+def test_param_tangent_matches_central_difference(ctx):
+    s, co = _kerr(ctx, 0)
+    _flat_guess(s, co)
+    s.add_unknowns(s.perturbation(11))
+    dF = s.param_tangent("ome").cpu().numpy()
+    h = 1e-6
+    s.set_param("ome", 0.05 + h)
+    Fp = s.residual().cpu().numpy()
+    s.set_param("ome", 0.05 - h)
+    Fm = s.residual().cpu().numpy()
+    s.set_param("ome", 0.05)
+    fd = (Fp - Fm) / (2 * h)
+    assert np.abs(dF).max() > 0
+    assert np.abs(dF - fd).max() < 1e-7 * max(1.0, np.abs(fd).max())
+    s.close()
