@@ -42,6 +42,8 @@ typedef struct kad_ctx kad_ctx;
  * Replaces the process-global state of the reference (FFTW plan maps
  * src/Coef/coef_1d.cpp:34-44, transform scratch include/memory.hpp:55-69). */
 int kad_ctx_create(kad_ctx** ctx, int device);
+/* Systems built on a context (kad_xcts_create, kad_system_build) must be destroyed
+ * before it. */
 int kad_ctx_destroy(kad_ctx* ctx);
 void* kad_ctx_stream(kad_ctx* ctx);
 int kad_sync(kad_ctx* ctx);

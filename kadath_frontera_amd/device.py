@@ -5,6 +5,7 @@ exposed to torch as an ExternalStream so callers can time it with torch.cuda.Eve
 or order torch work after it.
 """
 import ctypes
+import weakref
 
 import numpy as np
 import torch
@@ -32,9 +33,17 @@ class Context:
         check(lib.kad_ctx_create(ctypes.byref(h), device), "kad_ctx_create")
         self.h = h
         self.stream = torch.cuda.ExternalStream(lib.kad_ctx_stream(h), device=torch.device("cuda", device))
+        # systems built on this context: a system's device state belongs to its context,
+        # so they are destroyed before it (kad_system_destroy uses the context's stream)
+        self._systems = weakref.WeakSet()
+
+    def adopt(self, system):
+        self._systems.add(system)
 
     def close(self):
         if self.h:
+            for sysd in list(self._systems):
+                sysd.close()
             lib.kad_ctx_destroy(self.h)
             self.h = None
 

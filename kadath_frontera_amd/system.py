@@ -131,6 +131,7 @@ class DeviceSystem:
     def __init__(self, ctx, h, res, dmin, dmax, fields, params):
         self.ctx, self.h, self.res, self.dmin, self.dmax = ctx, h, res, dmin, dmax
         self.fields, self.params = fields, params
+        ctx.adopt(self)
         n, m, nf = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         check(lib.kad_system_sizes(h, ctypes.byref(n), ctypes.byref(m), ctypes.byref(nf)), "kad_system_sizes")
         self.n, self.m = n.value, m.value

@@ -42,6 +42,7 @@ class XctsSystem:
         check(lib.kad_xcts_create(ctx.h, res[0], res[1], res[2], _dp(self.bounds), len(self.bounds), 1,
                                   len(self.bounds), n0, a, ome, order, batch, ctypes.byref(h)), "kad_xcts_create")
         self.h = h
+        ctx.adopt(self)
         self.order = order
         n, m = ctypes.c_int(), ctypes.c_int()
         st = (ctypes.c_int * 4)()

@@ -3,7 +3,7 @@
 // solution 1 + a/r) through System_of_eqs::do_newton on the device.
 //
 // Exit status: 0 = converged and within 1e-8 of the exact solution at every
-// collocation point; 1 = wrong answer; 3 = no device (the declaration and its
+// collocation point (the radial truncation of 13 points is 4e-9); 1 = wrong answer; 3 = no device (the declaration and its
 // argument checks ran, the first device call failed cleanly).
 #include <cmath>
 #include <cstdio>
