@@ -201,11 +201,12 @@ int kad_xcts_do_newton(kad_system* s, double precision, double* error, int solve
 /* seeded perturbation amp 2^-(i+j+k) U(-1,1) of every unknown (splitmix64, drawn in
  * the reference unknown order whatever the system's order), HOST out */
 int kad_xcts_perturbation(kad_system* s, uint64_t seed, double amp, double* out);
-/* Statistics of the last do_newton step's solve.  iters[3] (HOST): outer refinement
- * steps, inner GMRES steps, and 1 if the low-precision refinement did not reach a
- * relative residual of 1e-6 and the step was solved with an fp64 factorisation of
- * the same J instead (0 otherwise, and for solver 0); resid: the refinement's final
- * relative residual ||F - J X||_inf / ||F||_inf. */
+/* Statistics of the last do_newton step's solve.  iters[4] (HOST): outer refinement
+ * steps; inner GMRES steps; 1 if the low-precision refinement did not reach a
+ * relative residual of max(1e-6, its target) and the step was solved with an fp64
+ * factorisation of the same J instead (0 otherwise, and for solver 0); 1 if the step
+ * factored J (0 when it reused earlier low-precision factors, kad_system_set_option).
+ * resid: the refinement's final relative residual ||F - J X||_inf / ||F||_inf. */
 int kad_xcts_last_solve(kad_system* s, int* iters, double* resid);
 /* host-only tracing of the same system (no device): info[6] = n_unknowns, n_rows,
  * value instr, tangent instr, value slots, tangent programs; returns #layout entries */
@@ -286,6 +287,13 @@ int kad_system_add_unknowns(kad_system* s, const double* X0);
 int kad_system_update(kad_system* s, const double* X);  /* fields -= affecte_tau(X), X device */
 int kad_system_perturbation(kad_system* s, uint64_t seed, double amp, double* out);
 int kad_system_set_param(kad_system* s, int which, double value);
+/* Options of the low-precision solvers (1-3) of do_newton: which 0 = forcing factor
+ * c >= 0, the refinement's target becomes max(1e-13, c min(1, ||F||_inf)) relative
+ * (0, the default: 1e-13, i.e. its attainable accuracy); which 1 = reuse the
+ * low-precision factors of an earlier step while the previous refinement took at most
+ * `value` inner steps (0, the default: factor every step).  Reused factors that stop
+ * converging are replaced by a factorisation of the current J within the step. */
+int kad_system_set_option(kad_system* s, int which, double value);
 /* as kad_xcts_do_newton / kad_xcts_last_solve */
 int kad_system_do_newton(kad_system* s, double precision, double* error, int solver, double* times);
 int kad_system_last_solve(kad_system* s, int* iters, double* resid);

@@ -103,6 +103,7 @@ SIGNATURES = {
     "kad_system_set_param": (c_int, [c_vp, c_int, c_dbl]),
     "kad_system_do_newton": (c_int, [c_vp, c_dbl, c_dp, c_int, c_dp]),
     "kad_system_last_solve": (c_int, [c_vp, c_ip, c_dp]),
+    "kad_system_set_option": (c_int, [c_vp, c_int, c_dbl]),
     "kad_residual": (c_int, [c_vp, c_vp]),
     "kad_check_equations": (c_int, [c_vp, c_dp, c_int]),
     "kad_jvp_fill": (c_int, [c_vp, c_int, c_int, c_vp, c_i64, c_int]),

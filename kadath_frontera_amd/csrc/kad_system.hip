@@ -291,6 +291,14 @@ struct kad_system_impl {
   int last_iters[2] = {0, 0};
   double last_resid = 0;
   int last_fallback = 0;  // the last low-precision step fell back to an fp64 factorisation
+  int last_factored = 0;  // the last step factored J (0: it reused earlier low-precision factors)
+  // low-precision solve options (kad_system_set_option)
+  double forcing = 0.0;   // refinement target max(1e-13, forcing * min(1, ||F||_inf)); 0: 1e-13
+  int reuse_inner = 0;    // reuse the low-precision factors while the previous refinement took
+                          // at most this many inner steps (0: factor every step)
+  bool lo_valid = false;  // Jlo / perm / Rs / Cs hold usable low-precision factors
+  int lo_prec = -1;
+  int prev_inner = 0;     // inner steps of the previous refinement
 
   ~kad_system_impl() {
     for (void* p : allocs) (void)hipFree(p);
@@ -937,6 +945,7 @@ int kad_xcts_do_newton(kad_system* s, double precision, double* error, int solve
   S->last_iters[0] = S->last_iters[1] = 0;
   S->last_resid = 0.0;
   S->last_fallback = 0;
+  S->last_factored = 0;
   if (err < precision) return 1;  // false for a NaN error: a NaN residual is never converged
   if (!S->J) {
     if ((rc = S->dmalloc((void**)&S->J, (size_t)n * n * sizeof(double)))) return rc;
@@ -960,6 +969,8 @@ int kad_xcts_do_newton(kad_system* s, double precision, double* error, int solve
   int info = 0;
   double tf0 = 0, tf1 = 0;
   if (solver == 0) {
+    S->lo_valid = false;  // the fp64 factorisation takes over ipiv / perm
+    S->last_factored = 1;
     tf0 = now_s();
     if ((rc = kad_lu_factor(reinterpret_cast<kad_ctx*>(S->ctx), KAD_PREC_F64, n, S->J, n, S->ipiv, S->perm, 0)) < 0)
       return rc;
@@ -977,33 +988,53 @@ int kad_xcts_do_newton(kad_system* s, double precision, double* error, int solve
       if ((rc = S->dmalloc((void**)&S->Rs, (size_t)n * sizeof(double)))) return rc;
       if ((rc = S->dmalloc((void**)&S->Cs, (size_t)n * sizeof(double)))) return rc;
     }
-    if ((rc = kad_cast_equilibrated(reinterpret_cast<kad_ctx*>(S->ctx), n, S->J, n, S->Jlo, n, S->Rs, S->Cs)))
-      return rc;
     const int prec = (solver == 3) ? KAD_PREC_BF16 : KAD_PREC_F32;
-    KAD_HIP_CHECK(hipStreamSynchronize(st));
-    tf0 = now_s();
-    if ((rc = kad_lu_factor(reinterpret_cast<kad_ctx*>(S->ctx), prec, n, S->Jlo, n, S->ipiv, S->perm, 0)) < 0)
-      return rc;
-    tf1 = now_s();
-    info = rc;
-    if (info == 0) {
-      // Tight target: the tau system has near-null modes (high radial orders in the
-      // compactified domain) that a residual test barely sees; a loose inexact-Newton
-      // forcing term leaves O(eta) errors there and the next Jacobians degrade.  The
-      // refinement therefore runs to its attainable accuracy (stagnation stop).
-      const double eta = 1e-13;
+    // The factors of an earlier J precondition the refinement on the current one as
+    // long as it keeps converging quickly; J itself (the residuals) is always current.
+    bool reuse = S->reuse_inner > 0 && S->lo_valid && S->lo_prec == prec && S->prev_inner <= S->reuse_inner;
+    // Target of the refinement.  The default runs it to its attainable accuracy (the tau
+    // system's near-null modes, high radial orders in the compactified domain, are
+    // barely seen by a residual test).  With a forcing factor c the target follows
+    // ||F||: c min(1, ||F||) relative, which keeps Newton quadratic while early
+    // iterations need only a few inner steps.
+    const double eta = S->forcing > 0.0 ? std::max(1e-13, S->forcing * std::min(1.0, err)) : 1e-13;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+      if (!reuse) {
+        if ((rc = kad_cast_equilibrated(reinterpret_cast<kad_ctx*>(S->ctx), n, S->J, n, S->Jlo, n, S->Rs, S->Cs)))
+          return rc;
+        KAD_HIP_CHECK(hipStreamSynchronize(st));
+        const double tg0 = now_s();
+        if ((rc = kad_lu_factor(reinterpret_cast<kad_ctx*>(S->ctx), prec, n, S->Jlo, n, S->ipiv, S->perm, 0)) < 0)
+          return rc;
+        tf1 += now_s() - tg0;
+        info = rc;
+        S->lo_valid = (info == 0);
+        S->lo_prec = prec;
+        S->last_factored = 1;
+        if (info != 0) break;
+      }
       rc = kad_ir_solve_scaled(reinterpret_cast<kad_ctx*>(S->ctx), n, S->J, n, prec, S->Jlo, n, S->perm, S->Rs, S->Cs,
                                S->F, S->X, eta, solver == 1 ? 30 : 8, solver == 1 ? 0 : 1, 0, S->last_iters,
                                &S->last_resid);
       if (rc < 0 && rc != KAD_ERR_NOCONV) return rc;
+      S->prev_inner = S->last_iters[1];
+      // reused factors that no longer precondition well: factor the current J and retry
+      if (reuse && rc == KAD_ERR_NOCONV && !(S->last_resid <= std::max(eta, 1e-6))) {
+        reuse = false;
+        continue;
+      }
+      break;
+    }
+    if (info == 0) {
       // The refinement stops at its attainable accuracy.  A step whose relative linear
       // residual is above kAcceptResid is not a usable Newton direction (a refinement
       // that made no progress leaves X = 0): the step is then solved with an fp64
       // factorisation of the same (unscaled, untouched) J, as the reference's pdgesv_
       // would, and the fallback is reported through kad_xcts_last_solve.
       constexpr double kAcceptResid = 1e-6;
-      if (rc == KAD_ERR_NOCONV && !(S->last_resid <= kAcceptResid)) {
+      if (rc == KAD_ERR_NOCONV && !(S->last_resid <= std::max(eta, kAcceptResid))) {
         S->last_fallback = 1;
+        S->lo_valid = false;  // the fp64 factorisation below takes over ipiv / perm
         const double tg0 = now_s();
         rc = kad_lu_factor(reinterpret_cast<kad_ctx*>(S->ctx), KAD_PREC_F64, n, S->J, n, S->ipiv, S->perm, 0);
         if (rc < 0) return rc;
@@ -1017,7 +1048,7 @@ int kad_xcts_do_newton(kad_system* s, double precision, double* error, int solve
       }
     }
   }
-  if (times) times[4] = tf1 - tf0;
+  if (times) times[4] = solver == 0 ? tf1 - tf0 : tf1;
   if (info != 0) {
     fprintf(stderr, "Jacobian is singular! Stopping.\n");
     return -5;
@@ -1069,6 +1100,7 @@ int kad_xcts_last_solve(kad_system* s, int* iters, double* resid) {
     iters[0] = S->last_iters[0];
     iters[1] = S->last_iters[1];
     iters[2] = S->last_fallback;
+    iters[3] = S->last_factored;
   }
   if (resid) *resid = S->last_resid;
   return KAD_OK;
@@ -1481,4 +1513,19 @@ int kad_param_tangent(kad_system* s, int which, double* out) {
   return KAD_OK;
 }
 
+}  // extern "C"
+
+extern "C" {
+// Options of the low-precision solvers: which 0 = forcing factor c (refinement target
+// max(1e-13, c min(1, ||F||_inf)); 0, the default, runs it to its attainable
+// accuracy), which 1 = reuse the low-precision factors while the previous refinement
+// took at most `value` inner steps (0, the default: factor every step).
+int kad_system_set_option(kad_system* s, int which, double value) {
+  if (!s || !(value >= 0.0)) return KAD_ERR_ARG;
+  auto* S = reinterpret_cast<kad_system_impl*>(s);
+  if (which == 0) S->forcing = value;
+  else if (which == 1) S->reuse_inner = (int)value;
+  else return KAD_ERR_ARG;
+  return KAD_OK;
+}
 }  // extern "C"

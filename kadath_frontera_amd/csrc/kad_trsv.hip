@@ -286,9 +286,12 @@ int trsv_prepare(const T* LU, int64_t lda, int n, int trans, double* W, int* fla
   double* Wf = W;
   double* Wb = W + (size_t)nblk * TBS * WCOLS;
   // forward triangle L (trans 0) or U^T (trans 1): stored lower part, or upper part transposed;
-  // backward triangle U or L^T: the same tiles as the forward ones of the other factor
-  int rc = trans ? sweep_extents<T, true>(LU, lda, n, flag, ext, s) : sweep_extents<T, false>(LU, lda, n, flag, ext, s);
-  if (rc) return rc;
+  // backward triangle U or L^T: the same tiles as the forward ones of the other factor.
+  // Without a flag buffer the extents stay empty and the sweeps cover every row.
+  if (flag) {
+    int rc = trans ? sweep_extents<T, true>(LU, lda, n, flag, ext, s) : sweep_extents<T, false>(LU, lda, n, flag, ext, s);
+    if (rc) return rc;
+  }
   // forward triangle: L (trans 0) or U^T (trans 1); backward: U or L^T
   diag_inverse_kernel<T><<<nblk, 256, 0, s>>>(LU, lda, n, trans ? 1 : 0, trans, Wf);
   diag_inverse_kernel<T><<<nblk, 256, 0, s>>>(LU, lda, n, trans ? 0 : 1, trans, Wb);
@@ -312,7 +315,7 @@ static void sweep(const T* LU, int64_t lda, int n, int nblk, const double* W, co
     for (int cur = -1; cur < nblk - 1; ++cur) {
       // panel rows: beyond block cur + 1, up to the column block's last nonzero row tile
       const int rlo = (cur + 2) * TBS;
-      const int rhi = cur >= 0 ? std::min(n, ext.fhi[cur]) : 0;
+      const int rhi = cur < 0 ? 0 : ext.fhi.empty() ? n : std::min(n, ext.fhi[cur]);
       const int rows = rhi - rlo;
       const unsigned grid = CHAIN_WG + ((cur >= 0 && rows > 0) ? (unsigned)cdiv(rows, PANEL_ROWS) : 0u);
       sweep_step_kernel<T, TR, true><<<grid, 256, 0, s>>>(LU, lda, n, nblk, W, cur, rlo, rhi, rhs, sol);
@@ -321,7 +324,7 @@ static void sweep(const T* LU, int64_t lda, int n, int nblk, const double* W, co
     for (int cur = nblk; cur >= 1; --cur) {
       // panel rows: from the column block's first nonzero row tile, before block cur - 1
       const int rhi = (cur - 1) * TBS;
-      const int rlo = cur < nblk ? ext.blo[cur] : rhi;
+      const int rlo = cur >= nblk ? rhi : ext.blo.empty() ? 0 : ext.blo[cur];
       const int rows = rhi - rlo;
       const unsigned grid = CHAIN_WG + ((cur < nblk && rows > 0) ? (unsigned)cdiv(rows, PANEL_ROWS) : 0u);
       sweep_step_kernel<T, TR, false><<<grid, 256, 0, s>>>(LU, lda, n, nblk, W, cur, rlo, rhi, rhs, sol);

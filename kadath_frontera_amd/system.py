@@ -229,6 +229,14 @@ class DeviceSystem:
         check(lib.kad_param_tangent(self.h, self.params[name], ctypes.c_void_p(out.data_ptr())), "kad_param_tangent")
         return out
 
+    def set_option(self, forcing=None, reuse_inner=None):
+        """Low-precision solver options: forcing factor of the refinement target and the
+        inner-step threshold for reusing earlier low-precision factors."""
+        if forcing is not None:
+            check(lib.kad_system_set_option(self.h, 0, float(forcing)), "kad_system_set_option")
+        if reuse_inner is not None:
+            check(lib.kad_system_set_option(self.h, 1, float(reuse_inner)), "kad_system_set_option")
+
     def do_newton(self, prec, solver="f64"):
         err = ctypes.c_double()
         times = np.zeros(5)
@@ -237,10 +245,10 @@ class DeviceSystem:
         if rc == -5:
             raise RuntimeError("Jacobian is singular! Stopping.")
         check(rc, "kad_system_do_newton")
-        it = (ctypes.c_int * 3)()
+        it = (ctypes.c_int * 4)()
         res = ctypes.c_double()
         lib.kad_system_last_solve(self.h, it, ctypes.byref(res))
-        return rc == 1, err.value, times, (it[0], it[1], res.value, it[2])
+        return rc == 1, err.value, times, (it[0], it[1], res.value, it[2], it[3])
 
     def solve(self, precision=1e-10, solver="f64", max_iter=20):
         for it in range(max_iter + 1):

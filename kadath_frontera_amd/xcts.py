@@ -138,6 +138,14 @@ class XctsSystem:
         check(lib.kad_xcts_get_coef(self.h, f, d, _dp(out)), "kad_xcts_get_coef")
         return out.reshape(npt + 2, nt, nr)
 
+    def set_option(self, forcing=None, reuse_inner=None):
+        """Low-precision solver options: forcing factor of the refinement target and the
+        inner-step threshold for reusing earlier low-precision factors."""
+        if forcing is not None:
+            check(lib.kad_system_set_option(self.h, 0, float(forcing)), "kad_system_set_option")
+        if reuse_inner is not None:
+            check(lib.kad_system_set_option(self.h, 1, float(reuse_inner)), "kad_system_set_option")
+
     def do_newton(self, prec, solver="f64"):
         err = ctypes.c_double()
         times = np.zeros(5)
@@ -146,8 +154,8 @@ class XctsSystem:
         if rc == -5:
             raise RuntimeError("Jacobian is singular! Stopping.")
         check(rc, "kad_xcts_do_newton")
-        it = (ctypes.c_int * 3)()
+        it = (ctypes.c_int * 4)()
         res = ctypes.c_double()
         lib.kad_xcts_last_solve(self.h, it, ctypes.byref(res))
-        # (outer refinement steps, inner GMRES steps, relative residual, fp64 fallback)
-        return rc == 1, err.value, times, (it[0], it[1], res.value, it[2])
+        # (outer refinement steps, inner GMRES steps, relative residual, fp64 fallback, factored)
+        return rc == 1, err.value, times, (it[0], it[1], res.value, it[2], it[3])
