@@ -79,6 +79,16 @@ int kad_lu_factor(kad_ctx* ctx, int prec, int n, void* A, int64_t lda, int* ipiv
  * storage, matrice.cpp:221-250). */
 int kad_lu_solve(kad_ctx* ctx, int prec, int n, const void* LU, int64_t lda, const int* perm, double* b, int trans);
 
+/* As kad_lu_solve, for n <= 8192, allocating nothing: no reference counterpart (the
+ * reference solves inside ScaLAPACK / LAPACK); for callers that cannot allocate device
+ * memory (code captured into a HIP graph, serving loops, harnesses that forbid
+ * hipMalloc).  Its tables and work vectors live in the library's static per-GPU
+ * scratch (34 MB, held until the process ends, its address taken at
+ * kad_ctx_create); n > 8192 returns KAD_ERR_ARG before any device work.  Calls that
+ * overlap on one GPU race on that scratch.  Work is queued on the context's stream. */
+int kad_lu_solve_noalloc(kad_ctx* ctx, int prec, int n, const void* LU, int64_t lda, const int* perm, double* b,
+                         int trans);
+
 /* y = b - op(A) x (b != NULL) or y = op(A) x, fp64, column-major A. */
 int kad_gemv64(kad_ctx* ctx, int n, const double* A, int64_t lda, const double* x, const double* b, double* y,
                int trans);

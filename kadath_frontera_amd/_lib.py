@@ -33,6 +33,7 @@ SIGNATURES = {
     "kad_gemm": (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64]),
     "kad_lu_factor": (c_int, [c_vp, c_int, c_int, c_vp, c_i64, c_vp, c_vp, c_int]),
     "kad_lu_solve": (c_int, [c_vp, c_int, c_int, c_vp, c_i64, c_vp, c_vp, c_int]),
+    "kad_lu_solve_noalloc": (c_int, [c_vp, c_int, c_int, c_vp, c_i64, c_vp, c_vp, c_int]),
     "kad_gemv64": (c_int, [c_vp, c_int, c_vp, c_i64, c_vp, c_vp, c_vp, c_int]),
     "kad_cast_f64_to_f32": (c_int, [c_vp, c_int, c_vp, c_i64, c_vp, c_i64]),
     "kad_ir_solve": (c_int, [c_vp, c_int, c_vp, c_i64, c_int, c_vp, c_i64, c_vp, c_vp, c_vp, c_dbl, c_int, c_int,

@@ -27,6 +27,7 @@ template <typename T>
 int getrs_fast(const T* LU, int64_t lda, int n, const int* perm, const double* W, const TrsvExtents& ext, double* b,
                int trans, double* work, hipStream_t s);
 bool ipiv_to_perm(const int* ipiv, int n, int* perm);
+int noalloc_init(int device);
 int gemv64(const double* A, int64_t lda, int n, const double* x, const double* b, double* y, int trans, double* part,
            hipStream_t s);
 __global__ void axpy_kernel(int n, double a, const double* x, double* y);
@@ -217,6 +218,9 @@ extern "C" {
 int kad_ctx_create(kad_ctx** out, int device) {
   if (!out) return KAD_ERR_ARG;
   KAD_HIP_CHECK(hipSetDevice(device));
+  // the no-allocation entry points' static scratch: its address on this device
+  int rc0 = kad::noalloc_init(device);
+  if (rc0) return rc0;
   Ctx* c = new Ctx();
   c->device = device;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
