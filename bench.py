@@ -27,6 +27,7 @@ sys.path.insert(0, ROOT)
 
 RES = (17, 13, 12)
 BOUNDS = [1.0, 2.0, 4.0, 8.0, 16.0]
+CFG1_RES, CFG1_BOUNDS = (9, 7, 4), [1.0, 2.0, 4.0]  # the CPU-runnable configuration
 N0, A, OME = 0.5, 1.0, 0.05
 SEED = 20261202
 PEAK = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0}  # MI355X dense matrix TFLOP/s (vendor figures)
@@ -41,19 +42,56 @@ PEAK = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0}  # MI355X dense matrix TFLOP/
 #     measured on an earlier build of the LU (same GEMM kernel; 7580 launches per
 #     iteration against 8256 now).
 APPLY3D_TRAFFIC_RATIO = (2.0 * 83618276.7 + 166635597.5) * 1024.0 / 340514582928.0
-GEMM_TRAFFIC_B = (2.0 * 18463.96 + 23028.09) * 1024.0
-GEMM_TRAFFIC_NOTE = ("rocprofv3 --pmc of the same GEMM kernel on an earlier build of the LU (7580 launches per "
-                     "iteration, mean over all); the pass over the current build crashed inside the profiler's "
-                     "host code and was not repeated")
+# gemm_k4: the counter pass over the current LU ended in a segfault of the profiler's host
+# counter path and was not repeated on that build, so the bench reports no
+# measured traffic for it; traffic_model_bytes is the kernel's operand traffic per launch
+# (each executed 128 x 128 C tile reads its 128 x k A and k x 128 B panels, reads and writes
+# C), an upper bound on HBM bytes when nothing is reused through L2.
+GEMM_TRAFFIC_NOTE = ("traffic is not measured for gemm_k4 on this build (the --pmc pass crashed in the "
+                     "profiler's host code); traffic_model_bytes = operand bytes per launch of the executed "
+                     "tiles (A and B panels + C read and write, fp64), with no L2 reuse")
 
 
-def cpu_baseline(budget_s=25.0):
-    """Oracle (numpy/scipy restatement of the reference's FP64 Newton step) timed on
-    the host on a bounded sample, extrapolated to one full step of the same workload."""
+def GEMM_TRAFFIC_MODEL(flops, launches):
+    # per executed tile pair (128 x 128 x k): 2*128*128*k flop; operands 8*(2*128*k + 2*128*128) bytes;
+    # the launches' mean k is 256 in the trailing update (the panel width), so bytes/flop =
+    # 8*(2*128*256 + 2*128*128) / (2*128*128*256)
+    if launches == 0:
+        return None
+    per_flop = 8.0 * (2 * 128 * 256 + 2 * 128 * 128) / (2.0 * 128 * 128 * 256)
+    return per_flop * flops / launches
+
+
+def _host_cores():
+    """(threads available to this process, physical cores from lscpu or None)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    threads = int(os.environ.get("OMP_NUM_THREADS", avail))
+    phys = None
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True, timeout=10).stdout
+        cores = {tuple(l.split(",")) for l in out.splitlines() if l and not l.startswith("#")}
+        phys = len(cores) or None
+    except Exception:
+        pass
+    return min(threads, avail), phys
+
+
+def cpu_baseline():
+    """The oracle (numpy/scipy restatement of the reference's FP64 Newton step) on the
+    host.  value: one Newton step of this workload extrapolated from a bounded sample
+    (one residual, 16 AD columns per (field, domain) block, LAPACK dgetrf at n = 3000
+    scaled as n^3).  cfg1_full_step: one complete Newton step (residual, every AD
+    column, LU, update) of the 9x7x4 three-domain system, measured, not extrapolated,
+    with all host threads and with one."""
     import numpy as np
     import scipy.linalg
+    from threadpoolctl import threadpool_limits
     from oracle import xcts as OX
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads, phys = _host_cores()
     sp = OX.Space(RES, BOUNDS)
     sy = OX.XCTSSystem(sp, 1, len(BOUNDS), n0=N0, a=A, ome=OME)
     X0 = OX.initial_perturbation(sy, SEED)
@@ -79,9 +117,22 @@ def cpu_baseline(budget_s=25.0):
     del lu
     t_lu = t_lu_s * (n / ns) ** 3
     step = t_res + t_fill + t_lu
+    # the small configuration in full (do_newton_seq: residual, every column, LU, update)
+    full = {}
+    for label, nthreads in (("all_threads", threads), ("one_thread", 1)):
+        with threadpool_limits(limits=nthreads):
+            s1 = OX.XCTSSystem(OX.Space(CFG1_RES, CFG1_BOUNDS), 1, len(CFG1_BOUNDS), n0=N0, a=A, ome=0.0)
+            x1 = OX.initial_perturbation(s1, SEED)
+            s1.set_fields(OX.initial_fields(s1, A, x1))
+            t0 = time.perf_counter()
+            OX.newton_step(s1)
+            full[label] = time.perf_counter() - t0
     return {"value": step, "unit": "s", "cores": threads, "kind": "port",
-            "sample": f"oracle FP64: residual + {len(cols)} AD Jacobian columns (scaled to n={n}) + "
-                      f"LAPACK dgetrf at n={ns} (scaled as n^3); fill {t_fill:.1f} s, LU {t_lu:.1f} s"}
+            "physical_cores": phys,
+            "sample": f"oracle FP64 on {threads} host threads: residual + {len(cols)} AD Jacobian columns "
+                      f"(scaled to n={n}) + LAPACK dgetrf at n={ns} (scaled as n^3); fill {t_fill:.1f} s, "
+                      f"LU {t_lu:.1f} s",
+            "cfg1_full_step_s": {"n": s1.unknown_layout()[1], "threads": threads, **full}}
 
 
 def main():
@@ -90,8 +141,8 @@ def main():
     ap.add_argument("--steps", type=int, default=6, help="timed Newton iterations")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--solver", default="f64", choices=["f64", "f32-ir", "f32-gmres", "bf16-gmres"])
-    ap.add_argument("--compare", default="f32-gmres", choices=["none", "f64", "f32-ir", "f32-gmres", "bf16-gmres"],
-                    help="solver run to ||F|| < 1e-10 beside the headline one")
+    ap.add_argument("--compare", default="f32-gmres,bf16-gmres",
+                    help="comma-separated solvers run to ||F|| < 1e-10 beside the headline one (or 'none')")
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--order", type=int, default=1, choices=[0, 1],
                     help="unknown/row order: 0 reference, 1 domain-major (block-tridiagonal J)")
@@ -152,9 +203,13 @@ def main():
         torch.cuda.synchronize()
         return conv, err, time.perf_counter() - t0, times, ir
 
-    def solve(solver, timed_steps=None):
-        """Newton from the perturbed guess until ||F||_inf < 1e-10.  Returns the per-iteration
-        records, the time to solution and the converged coefficients."""
+    def fields():
+        return np.concatenate([sysd.coef(f, d).ravel() for f in range(5) for d in range(1, len(BOUNDS) + 1)])
+
+    def solve(solver):
+        """Newton from the perturbed guess until ||F||_inf < 1e-10 (the time to solution),
+        then the fields and global quantities there, and again after one more step of the
+        same solver ("polished": ||F|| at the floor of fp64 residual evaluation)."""
         sysd.set_guess(X0)
         recs, t_total, conv = [], 0.0, False
         for it in range(args.max_newton + 1):
@@ -164,8 +219,18 @@ def main():
             if conv:
                 break
             recs.append((err, dt, list(times), ir))
-        fields = np.concatenate([sysd.coef(f, d).ravel() for f in range(5) for d in range(1, len(BOUNDS) + 1)])
-        return recs, t_total, conv, err, fields
+        f_stop, q_stop = fields(), sysd.observables()
+        _, err_p, _, _, _ = newton_iter_prec(solver, 0.0)
+        f_pol, q_pol = fields(), sysd.observables()
+        return dict(recs=recs, t=t_total, conv=conv, err=err, f_stop=f_stop, q_stop=q_stop, f_pol=f_pol,
+                    q_pol=q_pol)
+
+    def newton_iter_prec(solver, p):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        conv, err, times, ir = sysd.do_newton(p, solver)
+        torch.cuda.synchronize()
+        return conv, err, time.perf_counter() - t0, times, ir
 
     # warmup: first iteration(s) from the guess (allocations, kernel loads)
     for _ in range(args.warmup):
@@ -190,7 +255,6 @@ def main():
     ms_step = 1e3 * float(np.mean(step_times))
     phases = np.mean(step_phases, axis=0)
     t_fact = float(np.mean([p[4] for p in step_phases]))
-    lu_tflops = (2.0 / 3.0) * n ** 3 / t_fact / 1e12
     log(f"timed {len(step_times)} iterations: {ms_step:.1f} ms/iteration (region {t_region:.2f} s)")
 
     # live roofline of the two kernel classes (HIP events on the launch stream) over one
@@ -208,9 +272,13 @@ def main():
         meas[kind] = (w.value, kms.value, nl.value)
     gw, gms, gn = meas[0]
     aw, ams, an = meas[1]
+    # LU TFLOP/s: the flops its GEMMs executed (structurally zero tile pairs are skipped
+    # and not counted) over the factorisation's wall time
+    lu_tflops = gw / t_fact / 1e12 if t_fact > 0 else None
     roof_gemm = {"bound": "mfma", "achieved": gw / (gms / 1e3) / 1e12, "peak": PEAK[prec_of[args.solver]],
-                 "unit": "TFLOP/s", "traffic": GEMM_TRAFFIC_B if args.solver == "f64" else None,
+                 "unit": "TFLOP/s", "traffic": None, "traffic_model_bytes": GEMM_TRAFFIC_MODEL(gw, gn),
                  "traffic_note": GEMM_TRAFFIC_NOTE,
+                 "work_per_launch_flop": gw / max(gn, 1),
                  "kernel": "gemm_k4_kernel: LU trailing update + blocked-TRSM updates, executed tile pairs "
                            "(%d launches, %.3e flop, %.1f ms per iteration)" % (gn, gw, gms)}
     roof_gemm["frac"] = roof_gemm["achieved"] / roof_gemm["peak"]
@@ -221,23 +289,44 @@ def main():
     roof_apply["frac"] = roof_apply["achieved"] / roof_apply["peak"]
     dominant, other = (roof_apply, roof_gemm) if ams > gms else (roof_gemm, roof_apply)
 
-    # time to solution: headline solver and the comparison solver beside it
-    runs = {args.solver: solve(args.solver)}
-    if args.compare not in ("none", args.solver):
-        runs[args.compare] = solve(args.compare)
-    conv_info = {k: {"newton_iters": len(v[0]), "time_to_solution_s": v[1], "converged": v[2], "final_Finf": v[3],
-                     "ms_per_iteration": 1e3 * float(np.mean([r[1] for r in v[0]])) if v[0] else None,
-                     "factor_s_per_iteration": float(np.mean([r[2][4] for r in v[0]])) if v[0] else None,
-                     "gmres_iters": [r[3][1] for r in v[0]] if k != "f64" else None}
-                 for k, v in runs.items()}
-    diff = None
-    if len(runs) == 2:
-        ref_key = "f64" if "f64" in runs else args.solver
-        oth_key = [k for k in runs if k != ref_key][0]
-        a, b = runs[oth_key][4], runs[ref_key][4]
-        diff = {"solver": oth_key, "vs": ref_key,
-                "max_rel": float(np.abs(a - b).max() / max(1.0, np.abs(b).max()))}
-    cmp_info = conv_info.get(args.compare) if args.compare in conv_info and args.compare != args.solver else None
+    # time to solution: headline solver and the comparison solvers beside it
+    legs = [args.solver] + [c for c in args.compare.split(",") if c and c != "none" and c != args.solver]
+    runs = {k: solve(k) for k in legs}
+    ref_key = "f64" if "f64" in runs else args.solver
+    ref = runs[ref_key]
+    sc = max(1.0, float(np.abs(ref["f_stop"]).max()))
+    conv_info = {}
+    for k, v in runs.items():
+        recs = v["recs"]
+        info = {"newton_iters": len(recs), "time_to_solution_s": v["t"], "converged": v["conv"], "final_Finf": v["err"],
+                "ms_per_iteration": 1e3 * float(np.mean([r[1] for r in recs])) if recs else None,
+                "factor_s_per_iteration": float(np.mean([r[2][4] for r in recs])) if recs else None}
+        if k != "f64":
+            info["gmres_inner_steps"] = [r[3][1] for r in recs]
+            info["refinement_rel_resid"] = [r[3][2] for r in recs]
+            info["fp64_fallback_steps"] = int(sum(r[3][3] for r in recs))
+        if k != ref_key:
+            info["vs_" + ref_key] = {
+                "fields_max_rel_at_stop": float(np.abs(v["f_stop"] - ref["f_stop"]).max() / sc),
+                "fields_max_rel_polished": float(np.abs(v["f_pol"] - ref["f_pol"]).max() / sc),
+                "M_adm_abs_at_stop": abs(v["q_stop"]["M_adm"] - ref["q_stop"]["M_adm"]),
+                "J_adm_abs_at_stop": abs(v["q_stop"]["J_adm"] - ref["q_stop"]["J_adm"]),
+                "M_adm_abs_polished": abs(v["q_pol"]["M_adm"] - ref["q_pol"]["M_adm"]),
+                "J_adm_abs_polished": abs(v["q_pol"]["J_adm"] - ref["q_pol"]["J_adm"])}
+        conv_info[k] = info
+    conv_info[ref_key]["observables_at_stop"] = ref["q_stop"]
+    # the CPU-runnable configuration on the GPU (one full Newton step), beside the CPU's
+    s1 = XctsSystem(ctx, CFG1_RES, CFG1_BOUNDS, n0=N0, a=A, ome=0.0, batch=args.batch, order=args.order)
+    s1.set_guess(s1.perturbation(SEED))
+    s1.do_newton(prec, "f64")  # warm-up (allocations)
+    s1.set_guess(s1.perturbation(SEED))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s1.do_newton(prec, "f64")
+    torch.cuda.synchronize()
+    gpu_cfg1 = time.perf_counter() - t0
+    n_cfg1 = s1.n
+    s1.close()
     cpu = None if args.no_cpu_baseline else cpu_baseline()
     out = {
         "metric": "Newton-step s + LU TFLOP/s: FP32/BF16+IR vs FP64 at ||F||inf<1e-10, 1 GPU",
@@ -256,11 +345,13 @@ def main():
         "config": {"workload": "spinning-BH XCTS Newton iteration, 5 spherical domains 17x13x12, n=%d" % n,
                    "n_unknowns": n, "solver": args.solver, "jvp_batch": args.batch,
                    "order": ["reference", "domain-major"][args.order]},
-        "lu_tflops_dense_equivalent": lu_tflops,
+        "lu_tflops": lu_tflops,
+        "lu_tflops_note": "GEMM flops executed in one FP64 factorisation (structurally zero tile pairs skipped, "
+                          "not counted) / the factorisation's wall time",
         "phases_s": {"residual": phases[0], "jacobian_fill": phases[1], "factor_solve": phases[2],
                      "update": phases[3], "factor": phases[4]},
-        "compare": cmp_info,
-        "convergence": {"solvers": conv_info, "fields_rel_diff": diff},
+        "convergence": conv_info,
+        "gpu_cfg1_newton_step_s": {"config": "9x7x4, 3 domains, n=%d, f64" % n_cfg1, "value": gpu_cfg1},
         "roofline": dominant,
         "roofline_other": other,
         "cpu_baseline": cpu,

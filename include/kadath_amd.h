@@ -254,6 +254,15 @@ int kad_sysdef_create(int nr, int nt, int np, const double* bounds, int nbounds,
 int kad_sysdef_destroy(kad_sysdef* def);
 int kad_sysdef_add_var(kad_sysdef* def, const char* name, int valence);   /* add_var */
 int kad_sysdef_add_cst(kad_sysdef* def, const char* name, double value);  /* add_cst(name, double) */
+/* add_var(name, double&) (system_of_eqs.cpp:382-400): a global unknown; double unknowns
+ * are unknowns 0 .. ndouble-1, before every field, as in the reference; returns its
+ * index.  It is used in equations like a constant (its Jacobian column is its dF/dp). */
+int kad_sysdef_add_var_double(kad_sysdef* def, const char* name, double value);
+/* Integral equation in domain dom at boundary bound, e.g. "integ(intS) - chi * CM * CM = 0":
+ * integ(x) is Domain::integ of x there (domain_shell.cpp:862-897); its terms combine
+ * linearly with numbers (Eq_int, eq_int.cpp:38-51; Space_adapted_bh::add_eq_int_bh,
+ * adapted_bh_add_eq.cpp:88-119).  One row, before every other equation's rows. */
+int kad_sysdef_add_eq_int(kad_sysdef* def, int dom, int bound, const char* eq);
 /* a double constant that carries a tangent: a parameter p of dF/dp and of the adjoint
  * sensitivities; returns its parameter index (>= 0) */
 int kad_sysdef_add_param(kad_sysdef* def, const char* name, double value);
@@ -297,6 +306,9 @@ int kad_system_add_unknowns(kad_system* s, const double* X0);
 int kad_system_update(kad_system* s, const double* X);  /* fields -= affecte_tau(X), X device */
 int kad_system_perturbation(kad_system* s, uint64_t seed, double amp, double* out);
 int kad_system_set_param(kad_system* s, int which, double value);
+/* number of double unknowns; the current value of double unknown k (HOST out) */
+int kad_system_ndouble(kad_system* s);
+int kad_system_get_double(kad_system* s, int k, double* value);
 /* Options of the low-precision solvers (1-3) of do_newton: which 0 = forcing factor
  * c >= 0, the refinement's target becomes max(1e-13, c min(1, ||F||_inf)) relative
  * (0, the default: 1e-13, i.e. its attainable accuracy); which 1 = reuse the

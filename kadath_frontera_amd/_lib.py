@@ -79,6 +79,10 @@ SIGNATURES = {
     "kad_sysdef_add_var": (c_int, [c_vp, c_char_p, c_int]),
     "kad_sysdef_add_cst": (c_int, [c_vp, c_char_p, c_dbl]),
     "kad_sysdef_add_param": (c_int, [c_vp, c_char_p, c_dbl]),
+    "kad_sysdef_add_var_double": (c_int, [c_vp, c_char_p, c_dbl]),
+    "kad_sysdef_add_eq_int": (c_int, [c_vp, c_int, c_int, c_char_p]),
+    "kad_system_ndouble": (c_int, [c_vp]),
+    "kad_system_get_double": (c_int, [c_vp, c_int, c_dp]),
     "kad_sysdef_add_cst_field": (c_int, [c_vp, c_char_p, c_int, c_dp, c_i64]),
     "kad_sysdef_set_flat_metric": (c_int, [c_vp, c_char_p]),
     "kad_sysdef_add_def": (c_int, [c_vp, c_char_p]),

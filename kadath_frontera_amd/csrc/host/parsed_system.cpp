@@ -137,6 +137,32 @@ int ParsedSystem::add_param(const std::string& name, double value) {
   return c.par;
 }
 
+int ParsedSystem::add_var_double(const std::string& name, double value) {
+  for (auto& c : csts)
+    if (c.name == name) throw std::runtime_error("add_var: '" + name + "' already declared");
+  Cst c;
+  c.name = name;
+  c.value = value;
+  c.param = true;
+  c.par = (int)par_names.size();
+  c.dvar = ndouble++;
+  par_names.push_back(name);
+  dvar_par.push_back(c.par);
+  csts.push_back(c);
+  return c.dvar;
+}
+
+void ParsedSystem::add_eq_int(int dom, int bound, const std::string& eq) {
+  if (dom < dmin || dom > dmax || (bound != INNER_BC && bound != OUTER_BC))
+    throw std::runtime_error("add_eq_int: bad domain or boundary");
+  EqRec e{4, dom, eq, 1, 0, {}};
+  e.bound = bound;
+  // integral equations keep their order among themselves and come first
+  auto it = eqs.begin();
+  while (it != eqs.end() && it->kind == 4) ++it;
+  eqs.insert(it, e);
+}
+
 void ParsedSystem::add_cst_field(const std::string& name, int valence, const std::vector<double>& conf) {
   if (valence != 0 && valence != 1) throw std::runtime_error("add_cst: scalar or Cartesian vector fields only");
   size_t need = 0;
@@ -233,6 +259,20 @@ TE ParsedSystem::as_field(const Sc& a, Ctx& cx) {
 
 ParsedSystem::Sc ParsedSystem::sc_add(const Sc& a, const Sc& b, bool sub, Ctx& cx) {
   Sc r;
+  if (a.glob || b.glob) {
+    // sums of integrals and numbers (Term_eq doubles, term_eq_math.cpp)
+    if ((!a.glob && !a.num) || (!b.glob && !b.num))
+      throw std::runtime_error("an integral added to a field");
+    r.glob = true;
+    r.x = sub ? a.x - b.x : a.x + b.x;
+    r.gl = a.glob ? a.gl : std::vector<GlTerm>();
+    if (b.glob)
+      for (GlTerm t : b.gl) {
+        if (sub) t.fac = -t.fac;
+        r.gl.push_back(t);
+      }
+    return r;
+  }
   if (a.num && b.num) {
     r.num = true;
     r.x = sub ? a.x - b.x : a.x + b.x;
@@ -256,6 +296,14 @@ ParsedSystem::Sc ParsedSystem::sc_add(const Sc& a, const Sc& b, bool sub, Ctx& c
 
 ParsedSystem::Sc ParsedSystem::sc_mul(const Sc& a, const Sc& b, Ctx& cx) {
   Sc r;
+  if (a.glob || b.glob) {
+    if (!(a.num || b.num)) throw std::runtime_error("integrals combine linearly with numbers only");
+    r = a.glob ? a : b;
+    const double x = a.glob ? b.x : a.x;
+    r.x *= x;
+    for (auto& t : r.gl) t.fac *= x;
+    return r;
+  }
   if (a.num && b.num) {
     r.num = true;
     r.x = a.x * b.x;
@@ -281,6 +329,13 @@ ParsedSystem::Sc ParsedSystem::sc_mul(const Sc& a, const Sc& b, Ctx& cx) {
 
 ParsedSystem::Sc ParsedSystem::sc_div(const Sc& a, const Sc& b, Ctx& cx) {
   Sc r;
+  if (a.glob || b.glob) {
+    if (!a.glob || !b.num || b.x == 0.0) throw std::runtime_error("integrals divide by non-zero numbers only");
+    r = a;
+    r.x /= b.x;
+    for (auto& t : r.gl) t.fac /= b.x;
+    return r;
+  }
   if (b.num) {
     if (b.x == 0.0) throw std::runtime_error("division by the number 0");
     if (a.num) {
@@ -387,7 +442,10 @@ ParsedSystem::Tv ParsedSystem::tv_div(const Tv& a, const Tv& b, Ctx& cx) {
 ParsedSystem::Tv ParsedSystem::tv_neg(const Tv& a, Ctx& cx) {
   Tv r = a;
   for (auto& x : r.c) {
-    if (x.num) x.x = -x.x;
+    if (x.glob) {
+      x.x = -x.x;
+      for (auto& t : x.gl) t.fac = -t.fac;
+    } else if (x.num) x.x = -x.x;
     else x.te = TE{tr.neg(x.te.val), tr.neg(x.te.der)};
   }
   (void)cx;
@@ -431,27 +489,18 @@ ParsedSystem::Tv ParsedSystem::tv_grad(const Tv& a, char name, Ctx& cx) {
 // Component-wise linear operators of one domain.
 ParsedSystem::Tv ParsedSystem::tv_map(const Tv& a, const std::string& op, Ctx& cx) {
   Tv r = a;
-  const Domain& dm = doms[cx.d];
   auto lin = [&](const VDp& v) -> VDp {
     if (v->zero) return v;
     if (op == "dn") return tr.der_normal(v, cx.bound);
     if (op == "dr") return tr.der_normal(v, OUTER_BC);
     if (op == "multsint") return tr.mult_sin_theta(v);
     if (op == "divsint") return tr.div_sin_theta(v);
-    if (op == "divr") {
-      // 1/r: shell (x alpha + beta)^-1 through the radius field; compactified: alpha (x - 1)
-      if (dm.compact) return tr.scal(tr.mult_xm1(v), dm.alpha);
-      VDp rr = tr.radius(cx.d);
-      return tr.div(v, rr);
-    }
-    if (op == "multr") {
-      if (dm.compact) return tr.scal(tr.div_xm1(v), 1.0 / dm.alpha);
-      VDp rr = tr.radius(cx.d);
-      return tr.mul(v, rr);
-    }
+    if (op == "divr") return tr.div_r(v);
+    if (op == "multr") return tr.mult_r(v);
     throw std::runtime_error("unknown operator " + op);
   };
   for (auto& x : r.c) {
+    if (x.glob) throw std::runtime_error("operator " + op + " applied to an integral");
     if (x.num && (op == "dn" || op == "dr")) {
       x.x = 0.0;  // derivative of a number
       continue;
@@ -485,7 +534,21 @@ ParsedSystem::Tv ParsedSystem::eval(const std::string& s0, Ctx& cx) {
   }
   for (const char* op : {"dn", "dr", "multr", "divr", "multsint", "divsint"})
     if (unary(s, op, arg)) return tv_map(eval(arg, cx), op, cx);
-  for (const char* fn : {"sqrt", "exp", "log", "cos", "sin", "cosh", "sinh", "atanh", "integ", "integvolume"})
+  if (unary(s, "integ", arg)) {
+    // surface integral at the equation's boundary (Ope_int -> Domain::integ,
+    // domain_shell.cpp:862-897, domain_compact.cpp:89-121): r r sin(theta) so, then the
+    // theta weights of the boundary values
+    Tv a = eval(arg, cx);
+    if (!a.idx.empty()) throw std::runtime_error("integ of a tensor");
+    const TE t = as_field(a.c[0], cx);
+    auto rr = [&](const VDp& v) { return tr.mult_r(tr.mult_r(tr.mult_sin_theta(v))); };
+    Tv r;
+    r.c.resize(1);
+    r.c[0].glob = true;
+    r.c[0].gl.push_back(GlTerm{rr(t.val), rr(t.der), cx.d, cx.bound, 1.0});
+    return r;
+  }
+  for (const char* fn : {"sqrt", "exp", "log", "cos", "sin", "cosh", "sinh", "atanh", "integvolume"})
     if (unary(s, fn, arg)) throw std::runtime_error(std::string("operator ") + fn + " is not supported on the device");
   // D_i X, D^i X, partial_i X
   for (const char* pre : {"D_", "D^", "partial_"}) {
@@ -620,7 +683,8 @@ ParsedSystem::Tv ParsedSystem::residual(const std::string& eq, Ctx& cx) {
 // The expression of equation e evaluated in its domain (part 0) or, for matching,
 // in the outer neighbour at its inner boundary (part 1); components in storage order.
 std::vector<ParsedSystem::Sc> ParsedSystem::eq_parts(const EqRec& e, int part, Ctx& cx) {
-  cx.bound = (e.kind == 1) ? (part == 0 ? OUTER_BC : INNER_BC) : (e.kind == 3 ? INNER_BC : OUTER_BC);
+  cx.bound = (e.kind == 1) ? (part == 0 ? OUTER_BC : INNER_BC)
+                            : (e.kind == 4 ? e.bound : (e.kind == 3 ? INNER_BC : OUTER_BC));
   Tv t = (e.kind == 1) ? eval(e.which, cx) : residual(e.which, cx);
   return t.c;
 }
@@ -641,8 +705,19 @@ void ParsedSystem::trace_value() {
     std::vector<Sc> p0 = eq_parts(e, 0, cxs[e.d]), p1;
     if (e.kind == 1) p1 = eq_parts(e, 1, cxs[e.d + 1]);
     e.ncomp = (int)p0.size();
+    if (e.kind == 4) {
+      if (e.ncomp != 1 || !p0[0].glob) throw std::runtime_error("integral equation '" + e.which + "' is not an integral");
+      const Sc& g = p0[0];
+      if (g.gl.empty()) tr.export_global(tr.zero(e.d, SP_VAL), e.d, e.bound, row, 0.0, g.x, 0);
+      for (size_t k = 0; k < g.gl.size(); ++k)
+        tr.export_global(g.gl[k].val, g.gl[k].dom, g.gl[k].bound, row, g.gl[k].fac, k == 0 ? g.x : 0.0, k > 0);
+      e.nrows.push_back(1);
+      row += 1;
+      continue;
+    }
     for (int c = 0; c < e.ncomp; ++c) {
       if (p0[c].num) throw std::runtime_error("equation '" + e.which + "' has a constant component");
+      if (p0[c].glob) throw std::runtime_error("equation '" + e.which + "' mixes an integral with fields");
       int nr = 0;
       if (e.kind == 0) {
         if (tr.export_inside(p0[c].te.val, e.order, row, 1.0, 0, &nr) < 0)
@@ -691,6 +766,10 @@ void ParsedSystem::trace_tangent(int f, int d) {
     if (e.kind == 1 && e.d + 1 == d) part = 1;
     if (part < 0) continue;
     std::vector<Sc> p = eq_parts(e, part, cx);
+    if (e.kind == 4) {
+      for (const GlTerm& t : p[0].gl) tr.export_global(t.der, t.dom, t.bound, e.row0, t.fac, 0.0, 1);
+      continue;
+    }
     int row = e.row0;
     for (int c = 0; c < e.ncomp; ++c) {
       int nr = 0;
@@ -725,6 +804,10 @@ void ParsedSystem::trace_param(int p) {
       if (e.kind == 1 && e.d + 1 == d) part = 1;
       if (part < 0) continue;
       std::vector<Sc> v = eq_parts(e, part, cx);
+      if (e.kind == 4) {
+        for (const GlTerm& t : v[0].gl) tr.export_global(t.der, t.dom, t.bound, e.row0, t.fac, 0.0, 1);
+        continue;
+      }
       int row = e.row0;
       for (int c = 0; c < e.ncomp; ++c) {
         int nr = 0;
@@ -795,7 +878,7 @@ void ParsedSystem::build() {
       for (int d = dmin; d <= dmax; ++d)
         for (int f : vf) order_fd.push_back({f, d});
   }
-  int off = 0;
+  int off = ndouble;  // the double unknowns come first (System_of_eqs::xx_to_vars)
   for (auto& fd : order_fd) {
     const int f = fd.first, d = fd.second;
     const auto bt = field_base(f, d).bt;

@@ -35,6 +35,9 @@ class ParsedSystem : public SystemBase {
   void add_var(const std::string& name, int valence);
   // double constant (System_of_eqs::add_cst(const char*, double))
   void add_cst(const std::string& name, double value);
+  // double unknown (System_of_eqs::add_var(const char*, double&), system_of_eqs.cpp:382-400):
+  // a global unknown, before every field in the unknown order; returns its index
+  int add_var_double(const std::string& name, double value);
   // double constant that carries a tangent: a parameter of the adjoint sensitivities
   // (its value can be changed after build by set_param); index = its parameter number
   int add_param(const std::string& name, double value);
@@ -48,6 +51,10 @@ class ParsedSystem : public SystemBase {
   void add_eq_inside(int dom, const std::string& eq, int order = 2);
   void add_eq_bc(int dom, int bound, const std::string& eq);
   void add_eq_matching(int dom, int bound, const std::string& expr);
+  // integral equation evaluated in domain dom at a boundary, e.g. "integ(intS) - chi = 0"
+  // (Eq_int, eq_int.cpp:38-51; Space_adapted_bh::add_eq_int_bh, adapted_bh_add_eq.cpp:88-119);
+  // its row comes before every other equation's (sec_member, solver.cpp:49-85)
+  void add_eq_int(int dom, int bound, const std::string& eq);
   // Space_spheric helpers (spheric_add_eq.cpp:26-52)
   void space_add_eq(const std::string& eq, const std::string& m1, const std::string& m2);
   void space_add_inner_bc(const std::string& eq) { add_eq_bc(dmin, 0, eq); }
@@ -62,11 +69,21 @@ class ParsedSystem : public SystemBase {
   int order_mode = 0;
 
  private:
-  // A scalar component in one domain: a plain number or a Term_eq.
+  // One integral of a sum of integrals: r r sin(theta) so of the value and of the
+  // tangent, the boundary it is taken at, and its factor.
+  struct GlTerm {
+    VDp val, der;
+    int dom, bound;
+    double fac;
+  };
+  // A scalar component in one domain: a plain number, a Term_eq, or (glob) a global
+  // double sum_k fac_k integ(.)_k + x, as Ope_int returns (ope_int.cpp).
   struct Sc {
     bool num = false;
     double x = 0.0;
     TE te;
+    bool glob = false;
+    std::vector<GlTerm> gl;
   };
   // A Cartesian tensor with named indices: 3^valence components, the first index
   // slowest.
@@ -85,6 +102,7 @@ class ParsedSystem : public SystemBase {
     bool field = false;      // a constant field (else a double)
     bool param = false;      // carries a tangent in its parameter program
     int par = -1;
+    int dvar = -1;           // >= 0: a double unknown (its index)
     double value = 0.0;
     std::vector<double> conf;  // field: component-major, then domain, conf values
   };

@@ -17,7 +17,7 @@ namespace kadhost {
 
 // One equation (or equation block) and its rows.
 struct EqRec {
-  int kind;  // 0 inside, 1 matching d|d+1, 2 outer bc, 3 inner bc
+  int kind;  // 0 inside, 1 matching d|d+1, 2 outer bc, 3 inner bc, 4 integral (one row)
   int d;
   std::string which;
   int ncomp;
@@ -73,6 +73,10 @@ class SystemBase {
   std::vector<std::string> par_names;
   std::vector<Program> par_prog;
   std::vector<int> par_tab;
+  // double unknowns (add_var of a double): unknowns 0 .. ndouble-1, each a parameter
+  // whose dF/dp program is its Jacobian column; the fields' unknowns follow
+  int ndouble = 0;
+  std::vector<int> dvar_par;  // parameter index of each double unknown
 
   SystemBase() : tr(&doms) {}
   SystemBase(const SystemBase&) = delete;

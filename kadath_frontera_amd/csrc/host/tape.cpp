@@ -532,6 +532,22 @@ VDp Tracer::div_sin_theta(const VDp& v) { return lin_theta(v, div_sin_1d, "dst")
 VDp Tracer::mult_xm1(const VDp& v) { return lin_r(v, mult_xm1_1d, "mxm1"); }
 VDp Tracer::div_xm1(const VDp& v) { return lin_r(v, div_xm1_1d, "dxm1"); }
 
+VDp Tracer::mult_r(const VDp& v) {
+  if (v->zero) return v;
+  Domain& d = D(v->dom);
+  if (d.compact) return scal_div(div_xm1(v), d.alpha);
+  return add(scal(lin_r(v, mult_x_1d, "mx"), d.alpha), scal(v, d.beta));
+}
+
+VDp Tracer::div_r(const VDp& v) {
+  if (v->zero) return v;
+  Domain& d = D(v->dom);
+  if (d.compact) return scal(mult_xm1(v), d.alpha);
+  // conf-space division by the radius; the result keeps the operand's base
+  VDp r = div(v, radius(v->dom));
+  return clone_with_base(r, v->base);
+}
+
 // do_der_abs_from_der_var: shell domain_shell.cpp:665-683, compact domain_compact.cpp:674-693
 void Tracer::der_abs(const VDp& v, VDp out[3]) {
   if (v->zero) {
@@ -633,6 +649,27 @@ void Tracer::export_integral(const VDp& v, int q, double fac) {
   in.a = {v->cf, v->space};
   in.row0 = q;
   in.s = fac;
+  in.aux = v->base.bt[0];
+  prog(v->space).code.push_back(in);
+}
+
+void Tracer::export_global(const VDp& v, int dom, int bound, int row, double fac, double c0, int accumulate) {
+  Instr in;
+  in.kind = I_EXPORT_GL;
+  in.dom = dom;
+  in.row0 = row;
+  in.s = fac;
+  in.c0 = c0;
+  in.accumulate = accumulate;
+  in.bsign = bound == INNER_BC ? -1 : 1;
+  if (v->zero) {
+    if (v->space != SP_VAL) return;  // a zero tangent adds nothing
+    in.a = {-1, SP_VAL};
+    val.code.push_back(in);
+    return;
+  }
+  coef(v);
+  in.a = {v->cf, v->space};
   in.aux = v->base.bt[0];
   prog(v->space).code.push_back(in);
 }

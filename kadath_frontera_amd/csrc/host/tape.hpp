@@ -36,7 +36,10 @@ struct Node {
 typedef std::shared_ptr<Node> VDp;
 
 // Instruction kinds.
-enum InstrKind { I_LIN = 0, I_EW = 1, I_EXPORT_IN = 2, I_EXPORT_BD = 3, I_SEED = 4, I_ZERO = 5, I_EXPORT_INT = 6 };
+enum InstrKind {
+  I_LIN = 0, I_EW = 1, I_EXPORT_IN = 2, I_EXPORT_BD = 3, I_SEED = 4, I_ZERO = 5, I_EXPORT_INT = 6,
+  I_EXPORT_GL = 7  // one row: fac * surface integral of a (r r sin(theta) so) at a boundary (+ c0)
+};
 enum EwKind { EW_ADD = 0, EW_SUB = 1, EW_MUL = 2, EW_DIV = 3, EW_SCAL = 4, EW_ADDS = 5, EW_COPY = 6, EW_SDIV = 7 };
 
 struct Opnd {
@@ -67,6 +70,8 @@ struct Instr {
   int seed_tab = -1;
   // EXPORT_INT: theta basis of phi coefficient 0 (row0 = quantity index, s = factor)
   int aux = 0;
+  // EXPORT_GL: constant added to the row (value programs); a.slot < 0: the constant alone
+  double c0 = 0.0;
 };
 
 // A chain of 1-D passes with host matrices (uploaded once per plan).
@@ -138,6 +143,12 @@ class Tracer {
   VDp div_sin_theta(const VDp& v);
   VDp mult_xm1(const VDp& v);
   VDp div_xm1(const VDp& v);
+  // multiplication / division by r (Domain_shell: mult_x alpha + beta in coefficient
+  // space and a conf-space division keeping the base, domain_shell_ope.cpp:155-164,
+  // 215-219; Domain_compact: div_xm1 / alpha and mult_xm1 alpha,
+  // domain_compact_ope.cpp:173-185)
+  VDp mult_r(const VDp& v);
+  VDp div_r(const VDp& v);
   void der_abs(const VDp& v, VDp out[3]);
   VDp der_normal(const VDp& v, int bound);
 
@@ -148,6 +159,10 @@ class Tracer {
   // surface integral at the outer boundary of the coefficients of v (Domain_compact::integ
   // weights, times fac) into quantity q of the column
   void export_integral(const VDp& v, int q, double fac);
+  // integral-equation row: row (+)= fac * integ(v) at the boundary (v already holds
+  // r r sin(theta) so, as Domain_shell::integ forms it) + c0; a zero v in the value
+  // program still writes c0
+  void export_global(const VDp& v, int dom, int bound, int row, double fac, double c0, int accumulate);
   int nrows_inside(const Base& b, int dom, int order) { return (int)D(dom).tau_rows_inside(b.bt, order).size(); }
   int nrows_boundary(const Base& b, int dom) { return (int)D(dom).tau_rows_boundary(b.bt).size(); }
 

@@ -236,6 +236,48 @@ __global__ void export_int_kernel(const double* __restrict__ src, int64_t sstrid
   out[col0 + b + (int64_t)q * ldo] = tot * 2.0 * M_PI * fac;
 }
 
+// Integral-equation row (Domain::integ at x = bsign of the coefficients of r r sin(theta)
+// so, theta weights of phi coefficient 0 (pi for COS_EVEN j = 0, 2/(2j+1) for SIN_ODD),
+// times 2 pi and fac): value programs write out[row] = v + c0 (or add v); tangent
+// programs write J(row, col0 + b) (column-major, ld ldo, always added: J is cleared);
+// with w, the transposed export adds w[row] v to out[col0 + b].  src == nullptr: the
+// constant alone.
+template <typename T>
+__global__ void export_gl_kernel(const double* __restrict__ src, int64_t sstride, int nr, int nt, int bt0,
+                                 int bsign, double fac, double c0, int acc, T* __restrict__ out, int row,
+                                 int64_t col0, int64_t ldo, const double* __restrict__ w, int nbatch) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbatch) return;
+  double tot = 0.0;
+  if (src && (bt0 == COS_EVEN || bt0 == SIN_ODD)) {
+    const double* x = src + (int64_t)b * sstride;
+    for (int j = 0; j < nt; ++j) {
+      const double wj = bt0 == COS_EVEN ? (j == 0 ? M_PI : 0.0) : 2.0 / (2.0 * j + 1.0);
+      if (wj == 0.0) continue;
+      double v = 0.0, sg = 1.0;
+      for (int i = 0; i < nr; ++i) {
+        v += sg * x[i + (int64_t)nr * j];
+        sg *= bsign;
+      }
+      tot += wj * v;
+    }
+  }
+  const double val = tot * 2.0 * M_PI * fac;
+  if (w) {
+    ((double*)out)[col0 + b] += w[row] * val;
+    return;
+  }
+  T* o = out + row + (col0 + b) * ldo;
+  *o = acc ? (T)((double)*o + val) : (T)(val + c0);
+}
+
+// Double unknowns' fields (constant fields of the value arena): v[e] += scale * X[k].
+__global__ void double_update_kernel(double* __restrict__ v, int size, const double* __restrict__ X, int k,
+                                     double scale) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < size) v[e] += scale * X[k];
+}
+
 struct DevPlanX {
   kad::Passes P;
   double* pool = nullptr;
@@ -331,6 +373,16 @@ struct kad_system_impl {
   int value_pass();
   int jacobian(double* Jd, int64_t ldj);
   int update(const double* Xd, double scale);
+  // columns [c0, c1) of the double unknowns (their parameter programs) into J, whose
+  // column c0 sits at Jout + (c0 - col_base) ld
+  int double_columns(void* Jout, int64_t ld, int c0, int c1, int col_base, int dtype = 0) {
+    for (int k = c0; k < c1 && k < sys->ndouble; ++k) {
+      const DevProgram& pp = par_progs[sys->dvar_par[k]];
+      int rc = run(pp.code, 1, 0, pp.seed_tab, Jout, (int64_t)k - col_base, ld, dtype);
+      if (rc) return rc;
+    }
+    return KAD_OK;
+  }
 };
 
 // Liveness-based reuse of tangent slots within one program.
@@ -538,6 +590,20 @@ int kad_system_impl::run(const std::vector<Instr>& code, int nb, int u0, int see
                                                                          col0, ldo, batch);
         break;
       }
+      case I_EXPORT_GL: {
+        const bool val = in.a.slot < 0 || in.a.space == SP_VAL;
+        const int batch = val ? 1 : nb;
+        const double* src = in.a.slot >= 0 ? slot_ptr(in.a) : nullptr;
+        const int64_t ss = in.a.slot >= 0 ? slot_stride(in.a) : 0;
+        const unsigned g = (unsigned)kad::cdiv(batch, 64);
+        if (w || dtype == 0)
+          export_gl_kernel<double><<<g, 64, 0, s>>>(src, ss, dm.nr, dm.nt, in.aux, in.bsign, in.s, in.c0,
+                                                    in.accumulate, (double*)out, in.row0, col0, ldo, w, batch);
+        else
+          export_gl_kernel<float><<<g, 64, 0, s>>>(src, ss, dm.nr, dm.nt, in.aux, in.bsign, in.s, in.c0,
+                                                   in.accumulate, (float*)out, in.row0, col0, ldo, nullptr, batch);
+        break;
+      }
       case I_SEED: {
         const int64_t tot = (int64_t)in.size * nb;
         const int nu = (int)sys->seeds[seed_tab].cnt.size();
@@ -564,6 +630,7 @@ int kad_system_impl::jacobian(double* Jd, int64_t ldj) {
   int rc = value_pass();
   if (rc) return rc;
   KAD_HIP_CHECK(hipMemsetAsync(Jd, 0, (size_t)ldj * sys->n_unknowns * sizeof(double), ctx->stream));
+  if ((rc = double_columns(Jd, ldj, 0, sys->ndouble, 0))) return rc;
   for (auto& p : tprogs) {
     for (int u0 = 0; u0 < p.cnt; u0 += B) {
       const int nb = std::min(B, p.cnt - u0);
@@ -576,6 +643,16 @@ int kad_system_impl::jacobian(double* Jd, int64_t ldj) {
 
 int kad_system_impl::update(const double* Xd, double scale) {
   vals_current = false;
+  for (int k = 0; k < sys->ndouble; ++k) {
+    const std::string& name = sys->par_names[sys->dvar_par[k]];
+    for (int d = sys->dmin; d <= sys->dmax; ++d) {
+      const int slot = sys->param_slot(name, d);
+      if (slot < 0) continue;
+      const int size = sys->doms[d].conf_size();
+      double_update_kernel<<<(unsigned)kad::cdiv(size, 256), 256, 0, ctx->stream>>>(val_arena + (int64_t)slot * stride,
+                                                                                   size, Xd, k, scale);
+    }
+  }
   for (size_t q = 0; q < sys->layout.size(); ++q) {
     const int f = sys->layout[q][0], d = sys->layout[q][1], off = sys->layout[q][2];
     const int ncf = sys->doms[d].coef_size();
@@ -957,6 +1034,7 @@ int kad_xcts_do_newton(kad_system* s, double precision, double* error, int solve
     if ((rc = S->dmalloc((void**)&S->Jlo, (size_t)n * n * sizeof(float)))) return rc;
   // Jacobian (the value pass above already holds the intermediates)
   KAD_HIP_CHECK(hipMemsetAsync(S->J, 0, (size_t)n * n * sizeof(double), st));
+  if ((rc = S->double_columns(S->J, n, 0, sy.ndouble, 0))) return rc;
   for (auto& p : S->tprogs)
     for (int u0 = 0; u0 < p.cnt; u0 += S->B) {
       const int nb = std::min(S->B, p.cnt - u0);
@@ -1077,7 +1155,9 @@ int kad_xcts_perturbation(kad_system* s, uint64_t seed, double amp, double* out)
     return (double)(z >> 11) * (1.0 / 9007199254740992.0);
   };
   // draws in the reference unknown order (variables, then domains, then components),
-  // stored at this system's layout offsets, so both orderings start from the same guess
+  // stored at this system's layout offsets, so both orderings start from the same guess;
+  // the double unknowns are not perturbed
+  for (int k = 0; k < sy.ndouble; ++k) out[k] = 0.0;
   for (auto& var : sy.var_fields)
     for (int d = sy.dmin; d <= sy.dmax; ++d)
       for (int f : var) {
@@ -1204,6 +1284,16 @@ int kad_sysdef_add_var(kad_sysdef* def, const char* name, int valence) {
 int kad_sysdef_add_cst(kad_sysdef* def, const char* name, double value) {
   if (!name) return KAD_ERR_ARG;
   return guarded(def, [&](ParsedSystem& p) { p.add_cst(name, value); });
+}
+int kad_sysdef_add_var_double(kad_sysdef* def, const char* name, double value) {
+  if (!name) return KAD_ERR_ARG;
+  int idx = -1;
+  int rc = guarded(def, [&](ParsedSystem& p) { idx = p.add_var_double(name, value); });
+  return rc ? rc : idx;
+}
+int kad_sysdef_add_eq_int(kad_sysdef* def, int dom, int bound, const char* eq) {
+  if (!eq) return KAD_ERR_ARG;
+  return guarded(def, [&](ParsedSystem& p) { p.add_eq_int(dom, bound, eq); });
 }
 int kad_sysdef_add_param(kad_sysdef* def, const char* name, double value) {
   if (!name) return KAD_ERR_ARG;
@@ -1415,6 +1505,28 @@ int kad_system_add_unknowns(kad_system* s, const double* X0) {
   if (!s || !X0) return KAD_ERR_ARG;
   return add_unknowns_host(reinterpret_cast<kad_system_impl*>(s), X0);
 }
+// Number of double unknowns (unknowns 0 .. count-1).
+int kad_system_ndouble(kad_system* s) {
+  if (!s) return KAD_ERR_ARG;
+  return reinterpret_cast<kad_system_impl*>(s)->sys->ndouble;
+}
+// Current value of double unknown k (HOST out).
+int kad_system_get_double(kad_system* s, int k, double* value) {
+  if (!s || !value) return KAD_ERR_ARG;
+  auto* S = reinterpret_cast<kad_system_impl*>(s);
+  SystemBase& sy = *S->sys;
+  if (k < 0 || k >= sy.ndouble) return KAD_ERR_ARG;
+  const std::string& name = sy.par_names[sy.dvar_par[k]];
+  for (int d = sy.dmin; d <= sy.dmax; ++d) {
+    const int slot = sy.param_slot(name, d);
+    if (slot < 0) continue;
+    KAD_HIP_CHECK(hipMemcpyAsync(value, S->val_arena + (int64_t)slot * S->stride, sizeof(double),
+                                 hipMemcpyDeviceToHost, S->ctx->stream));
+    KAD_HIP_CHECK(hipStreamSynchronize(S->ctx->stream));
+    return KAD_OK;
+  }
+  return KAD_ERR_ARG;  // the unknown appears in no equation
+}
 int kad_system_set_param(kad_system* s, int which, double value) {
   if (!s) return KAD_ERR_ARG;
   return set_param_impl(reinterpret_cast<kad_system_impl*>(s), which, value);
@@ -1469,6 +1581,7 @@ int kad_jvp_fill(kad_system* s, int col_begin, int col_count, void* J_tile, int6
   const size_t es = dtype == 0 ? sizeof(double) : sizeof(float);
   KAD_HIP_CHECK(hipMemset2DAsync(J_tile, (size_t)ld * es, 0, (size_t)m * es, (size_t)col_count, st));
   const int c1 = col_begin + col_count;
+  if ((rc = S->double_columns(J_tile, ld, col_begin, c1, col_begin, dtype))) return rc;
   for (auto& p : S->tprogs) {
     const int lo = std::max(col_begin, p.off), hi = std::min(c1, p.off + p.cnt);
     for (int u = lo; u < hi; u += S->B) {
@@ -1490,6 +1603,10 @@ int kad_vjp(kad_system* s, const double* w, double* out) {
   int rc = S->ensure_values();
   if (rc) return rc;
   KAD_HIP_CHECK(hipMemsetAsync(out, 0, (size_t)S->sys->n_unknowns * sizeof(double), st));
+  for (int k = 0; k < S->sys->ndouble; ++k) {
+    const DevProgram& pp = S->par_progs[S->sys->dvar_par[k]];
+    if ((rc = S->run(pp.code, 1, 0, pp.seed_tab, out, k, 0, 0, w))) return rc;
+  }
   for (auto& p : S->tprogs)
     for (int u0 = 0; u0 < p.cnt; u0 += S->B) {
       const int nb = std::min(S->B, p.cnt - u0);

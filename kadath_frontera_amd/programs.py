@@ -7,6 +7,9 @@ kerr(): codes/Par_version/Kerr/kerr.cpp:69-99 in the Cartesian tensor basis of t
   FUKA BH solver (bh_3d_xcts_solver_imp.cpp:119-159): the XCTS black hole with lapse
   n0 on the excision sphere r = a and horizon rotation ome (a parameter, so dF/dome
   is available), the same system the hand-traced XCTS path builds.
+kerr(spin=chi): the horizon rotation becomes a double unknown fixed by the spin
+  integral, integ(A_ij m^i s^j / 8 pi) - chi CM^2 = 0 at the excision sphere, as the
+  FUKA BH solver fixes it (bh_3d_xcts_stages.cpp:66,90).
 """
 import numpy as np
 
@@ -44,7 +47,7 @@ def kerr_constants(sd):
     return np.array([np.concatenate(c) for c in s_c]), np.array([np.concatenate(c) for c in m_c])
 
 
-def kerr(ctx, res, bounds, n0=0.5, a=1.0, ome=0.0):
+def kerr(ctx, res, bounds, n0=0.5, a=1.0, ome=0.0, spin=None, cm=1.0):
     sd = SystemDef(ctx, res, bounds)
     sd.add_var("P")
     sd.add_var("N")
@@ -54,9 +57,18 @@ def kerr(ctx, res, bounds, n0=0.5, a=1.0, ome=0.0):
     sd.add_cst_field("s", s_c, 1)
     sd.add_cst_field("m", m_c, 1)
     sd.add_cst("n0", n0)
-    sd.add_param("ome", ome)
+    if spin is None:
+        sd.add_param("ome", ome)
+    else:
+        sd.add_var_double("ome", ome)
+        sd.add_cst("chi", spin)
+        sd.add_cst("CM", cm)
+        sd.add_cst("PI", np.pi)
     sd.set_flat_metric("f")
     sd.add_def("A^ij = (D^i bet^j + D^j bet^i - 2. / 3.* D_k bet^k * f^ij) /2. / N")
+    if spin is not None:
+        sd.add_def("intS = A_ij * m^i * s^j / 8. / PI")
+        sd.add_eq_int(1, INNER_BC, "integ(intS) - chi * CM * CM = 0")
     sd.space_add_inner_bc("N=n0")
     sd.space_add_inner_bc("bet^i = n0 / P^2 * s^i + ome * a * m^i")
     sd.space_add_inner_bc("dn(P) + 0.5 * P / a + P^3 * A_ij * s^i * s^j /4.= 0")

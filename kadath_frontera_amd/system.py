@@ -46,6 +46,8 @@ class SystemDef:
         self.h = h
         self.fields = []   # scalar field components in declaration order
         self.params = {}
+        self.doubles = {}
+        self._npar = 0
 
     def close(self):
         if self.h:
@@ -68,7 +70,19 @@ class SystemDef:
     def add_param(self, name, value):
         idx = check(lib.kad_sysdef_add_param(self.h, _b(name), float(value)), f"add_param({name})")
         self.params[name] = idx
+        self._npar += 1
         return idx
+
+    def add_var_double(self, name, value):
+        """A global (double) unknown; returns its index among the unknowns."""
+        idx = check(lib.kad_sysdef_add_var_double(self.h, _b(name), float(value)), f"add_var({name})")
+        self.doubles[name] = idx
+        self.params[name] = self._npar  # it is also a parameter (set_param, param_tangent)
+        self._npar += 1
+        return idx
+
+    def add_eq_int(self, dom, bound, eq):
+        check(lib.kad_sysdef_add_eq_int(self.h, dom, bound, _b(eq)), f"add_eq_int({eq})")
 
     def add_cst_field(self, name, conf, valence=0):
         """conf: array (components, domains dmin..dmax, nr*nt*np) of collocation values."""
@@ -124,13 +138,14 @@ class SystemDef:
         h = c_vp()
         d, self.h = self.h, None
         check(lib.kad_system_build(self.ctx.h, d, order, batch, ctypes.byref(h)), "kad_system_build")
-        return DeviceSystem(self.ctx, h, self.res, self.dmin, self.dmax, list(self.fields), dict(self.params))
+        return DeviceSystem(self.ctx, h, self.res, self.dmin, self.dmax, list(self.fields), dict(self.params),
+                            dict(self.doubles))
 
 
 class DeviceSystem:
-    def __init__(self, ctx, h, res, dmin, dmax, fields, params):
+    def __init__(self, ctx, h, res, dmin, dmax, fields, params, doubles=None):
         self.ctx, self.h, self.res, self.dmin, self.dmax = ctx, h, res, dmin, dmax
-        self.fields, self.params = fields, params
+        self.fields, self.params, self.doubles = fields, params, doubles or {}
         ctx.adopt(self)
         n, m, nf = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         check(lib.kad_system_sizes(h, ctypes.byref(n), ctypes.byref(m), ctypes.byref(nf)), "kad_system_sizes")
@@ -188,6 +203,11 @@ class DeviceSystem:
         out = np.zeros(self.n)
         check(lib.kad_system_perturbation(self.h, seed, amp, _dp(out)), "kad_system_perturbation")
         return out
+
+    def get_double(self, name):
+        v = ctypes.c_double()
+        check(lib.kad_system_get_double(self.h, self.doubles[name], ctypes.byref(v)), "kad_system_get_double")
+        return v.value
 
     def set_param(self, name, value):
         check(lib.kad_system_set_param(self.h, self.params[name], float(value)), "kad_system_set_param")

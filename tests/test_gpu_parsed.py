@@ -153,3 +153,56 @@ def test_param_tangent_matches_central_difference(ctx):
     assert np.abs(dF).max() > 0
     assert np.abs(dF - fd).max() < 1e-7 * max(1.0, np.abs(fd).max())
     s.close()
+
+
+def test_spin_system_matches_oracle(ctx):
+    """Double unknown + integral equation: the spin-fixed hole of bh_3d_xcts_stages.cpp
+    (Ome add_var'd, integ(A_ij m^i s^j / 8 pi) - chi CM^2 = 0 at the excision sphere)
+    as strings, against the oracle's spin mode: F and J (the Ome column and the integral
+    row included) at the fixed-Ome solution, then Newton to the same Ome and fields."""
+    from oracle import xcts as OX
+    res, bounds = (5, 5, 4), [1.0, 2.0, 4.0]
+    sp = OX.Space(res, bounds)
+    sy = OX.XCTSSystem(sp, 1, 3, n0=0.5, a=1.0, ome=0.05)
+    sy.set_fields(OX.initial_fields(sy, 1.0, OX.initial_perturbation(sy)))
+    for _ in range(12):
+        if OX.newton_step(sy)[0] < 1e-11:
+            break
+    probe = OX.XCTSSystem(sp, 1, 3, n0=0.5, a=1.0, ome=0.05, spin=0.0)
+    probe.set_fields(sy.fields)
+    chi = 1.05 * probe.residual()[0]
+    orc = OX.XCTSSystem(sp, 1, 3, n0=0.5, a=1.0, ome=0.05, spin=chi)
+    orc.set_fields(sy.fields)
+    F_o, J_o = orc.residual(), orc.jacobian()
+    sd = programs.kerr(ctx, res, bounds, n0=0.5, a=1.0, ome=0.05, spin=chi, cm=1.0)
+    s = sd.build(order=0, batch=256)
+    names = {"P": 0, "N": 1, "bx": 2, "by": 3, "bz": 4}
+    for name, f in names.items():
+        for d in range(1, 4):
+            v = sy.fields[name][d]
+            v.coef()
+            s.set_coef(f, d, v.cf)
+    assert s.n == J_o.shape[1] and abs(s.get_double("ome") - 0.05) == 0.0
+    print("spin system: F, J", flush=True)
+    F = s.residual().cpu().numpy()
+    assert np.abs(F - F_o).max() < 1e-12 * max(1.0, np.abs(F_o).max())
+    J = s.jacobian().cpu().numpy()
+    scale = np.maximum(1.0, np.abs(J_o).max(axis=0))
+    assert (np.abs(J - J_o) / scale).max() < 1e-11
+    assert np.abs(J[0]).max() > 0 and np.abs(J[:, 0]).max() > 0  # integral row, Ome column
+    for it in range(10):
+        conv, err, _, _ = s.do_newton(1e-11, "f64")
+        print(f"spin device it {it} |F| {err:.3e} ome {s.get_double('ome'):.12f}", flush=True)
+        if conv:
+            break
+    assert conv
+    for _ in range(10):
+        if OX.newton_step(orc)[0] < 1e-11:
+            break
+    assert abs(s.get_double("ome") - orc.ome) < 1e-10
+    for name, f in names.items():
+        for d in range(1, 4):
+            v = orc.fields[name][d]
+            v.coef()
+            assert np.abs(s.coef(f, d) - v.cf).max() <= 1e-10 * max(1.0, np.abs(v.cf).max()), (name, d)
+    s.close()
