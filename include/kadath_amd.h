@@ -263,8 +263,13 @@ int kad_sysdef_add_var_double(kad_sysdef* def, const char* name, double value);
  * linearly with numbers (Eq_int, eq_int.cpp:38-51; Space_adapted_bh::add_eq_int_bh,
  * adapted_bh_add_eq.cpp:88-119).  One row, before every other equation's rows. */
 int kad_sysdef_add_eq_int(kad_sysdef* def, int dom, int bound, const char* eq);
+/* Global quantity: an integral expression (as in kad_sysdef_add_eq_int, without '=')
+ * evaluated in domain dom at boundary bound, e.g. M_ADM = "-integ(dn(P)) / 2. / PI" at
+ * the outer boundary (codes/Par_version/Kerr/reader.cpp:114-125); returns its index. */
+int kad_sysdef_add_global(kad_sysdef* def, const char* name, int dom, int bound, const char* expr);
 /* a double constant that carries a tangent: a parameter p of dF/dp and of the adjoint
- * sensitivities; returns its parameter index (>= 0) */
+ * sensitivities; returns its parameter index (>= 0; parameters and double unknowns are
+ * numbered together in declaration order) */
 int kad_sysdef_add_param(kad_sysdef* def, const char* name, double value);
 /* add_cst(name, Tensor): conf values, component-major, then domains dmin..dmax, each
  * nr*nt*np values in the reference's Array order (r fastest); count = the total */
@@ -306,6 +311,13 @@ int kad_system_add_unknowns(kad_system* s, const double* X0);
 int kad_system_update(kad_system* s, const double* X);  /* fields -= affecte_tau(X), X device */
 int kad_system_perturbation(kad_system* s, uint64_t seed, double amp, double* out);
 int kad_system_set_param(kad_system* s, int which, double value);
+/* Global quantities at the current fields (HOST out, one per kad_sysdef_add_global). */
+int kad_system_globals(kad_system* s, double* out);
+/* Adjoint sensitivities dQ_q/dp of the global quantities with respect to parameter
+ * `which` at the current (converged) fields: J^T lambda_q = dQ_q/du with the LU factors
+ * of J refilled there, dQ_q/dp = dQ_q/dp|_u - lambda_q . dF/dp.  HOST out (one per
+ * quantity); dQdu (HOST, nq x n_unknowns, may be NULL) receives dQ/du. */
+int kad_system_sensitivity(kad_system* s, int which, double* out, double* dQdu);
 /* number of double unknowns; the current value of double unknown k (HOST out) */
 int kad_system_ndouble(kad_system* s);
 int kad_system_get_double(kad_system* s, int k, double* value);

@@ -81,6 +81,9 @@ SIGNATURES = {
     "kad_sysdef_add_param": (c_int, [c_vp, c_char_p, c_dbl]),
     "kad_sysdef_add_var_double": (c_int, [c_vp, c_char_p, c_dbl]),
     "kad_sysdef_add_eq_int": (c_int, [c_vp, c_int, c_int, c_char_p]),
+    "kad_sysdef_add_global": (c_int, [c_vp, c_char_p, c_int, c_int, c_char_p]),
+    "kad_system_globals": (c_int, [c_vp, c_dp]),
+    "kad_system_sensitivity": (c_int, [c_vp, c_int, c_dp, c_dp]),
     "kad_system_ndouble": (c_int, [c_vp]),
     "kad_system_get_double": (c_int, [c_vp, c_int, c_dp]),
     "kad_sysdef_add_cst_field": (c_int, [c_vp, c_char_p, c_int, c_dp, c_i64]),

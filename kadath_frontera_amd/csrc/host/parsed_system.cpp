@@ -163,6 +163,14 @@ void ParsedSystem::add_eq_int(int dom, int bound, const std::string& eq) {
   eqs.insert(it, e);
 }
 
+int ParsedSystem::add_global(const std::string& name, int dom, int bound, const std::string& expr) {
+  if (dom < dmin || dom > dmax || (bound != INNER_BC && bound != OUTER_BC))
+    throw std::runtime_error("add_global: bad domain or boundary");
+  globs.push_back(Glob{name, expr, dom, bound});
+  glob_names.push_back(name);
+  return (int)globs.size() - 1;
+}
+
 void ParsedSystem::add_cst_field(const std::string& name, int valence, const std::vector<double>& conf) {
   if (valence != 0 && valence != 1) throw std::runtime_error("add_cst: scalar or Cartesian vector fields only");
   size_t need = 0;
@@ -538,8 +546,12 @@ ParsedSystem::Tv ParsedSystem::eval(const std::string& s0, Ctx& cx) {
     // surface integral at the equation's boundary (Ope_int -> Domain::integ,
     // domain_shell.cpp:862-897, domain_compact.cpp:89-121): r r sin(theta) so, then the
     // theta weights of the boundary values
+    const bool saved = cx.glob_ctx;
+    cx.glob_ctx = false;
     Tv a = eval(arg, cx);
+    cx.glob_ctx = saved;
     if (!a.idx.empty()) throw std::runtime_error("integ of a tensor");
+    if (a.c[0].glob) throw std::runtime_error("integ of an integral");
     const TE t = as_field(a.c[0], cx);
     auto rr = [&](const VDp& v) { return tr.mult_r(tr.mult_r(tr.mult_sin_theta(v))); };
     Tv r;
@@ -636,6 +648,15 @@ ParsedSystem::Tv ParsedSystem::eval_name(const std::string& s, Ctx& cx) {
       } else {
         sc.te.der = tr.zero(d, SP_TAN);
       }
+      if (cx.glob_ctx) {
+        // in an integral expression a parameter is a scalar: its value read from the field
+        // at run time (so set_param reaches it) and its tangent from the seed
+        Sc g;
+        g.glob = true;
+        g.gl.push_back(GlTerm{sc.te.val, sc.te.der, d, cx.bound, 1.0, 1});
+        r.c.push_back(g);
+        return r;
+      }
       r.c.push_back(sc);
       return r;
     }
@@ -685,7 +706,9 @@ ParsedSystem::Tv ParsedSystem::residual(const std::string& eq, Ctx& cx) {
 std::vector<ParsedSystem::Sc> ParsedSystem::eq_parts(const EqRec& e, int part, Ctx& cx) {
   cx.bound = (e.kind == 1) ? (part == 0 ? OUTER_BC : INNER_BC)
                             : (e.kind == 4 ? e.bound : (e.kind == 3 ? INNER_BC : OUTER_BC));
+  cx.glob_ctx = (e.kind == 4);
   Tv t = (e.kind == 1) ? eval(e.which, cx) : residual(e.which, cx);
+  cx.glob_ctx = false;
   return t.c;
 }
 
@@ -707,10 +730,7 @@ void ParsedSystem::trace_value() {
     e.ncomp = (int)p0.size();
     if (e.kind == 4) {
       if (e.ncomp != 1 || !p0[0].glob) throw std::runtime_error("integral equation '" + e.which + "' is not an integral");
-      const Sc& g = p0[0];
-      if (g.gl.empty()) tr.export_global(tr.zero(e.d, SP_VAL), e.d, e.bound, row, 0.0, g.x, 0);
-      for (size_t k = 0; k < g.gl.size(); ++k)
-        tr.export_global(g.gl[k].val, g.gl[k].dom, g.gl[k].bound, row, g.gl[k].fac, k == 0 ? g.x : 0.0, k > 0);
+      export_glob_terms(p0[0], row, false, 0);
       e.nrows.push_back(1);
       row += 1;
       continue;
@@ -767,7 +787,7 @@ void ParsedSystem::trace_tangent(int f, int d) {
     if (part < 0) continue;
     std::vector<Sc> p = eq_parts(e, part, cx);
     if (e.kind == 4) {
-      for (const GlTerm& t : p[0].gl) tr.export_global(t.der, t.dom, t.bound, e.row0, t.fac, 0.0, 1);
+      export_glob_terms(p[0], e.row0, true, 0);
       continue;
     }
     int row = e.row0;
@@ -805,7 +825,7 @@ void ParsedSystem::trace_param(int p) {
       if (part < 0) continue;
       std::vector<Sc> v = eq_parts(e, part, cx);
       if (e.kind == 4) {
-        for (const GlTerm& t : v[0].gl) tr.export_global(t.der, t.dom, t.bound, e.row0, t.fac, 0.0, 1);
+        export_glob_terms(v[0], e.row0, true, 0);
         continue;
       }
       int row = e.row0;
@@ -826,6 +846,77 @@ void ParsedSystem::trace_param(int p) {
     }
   }
   tr.tan = nullptr;
+}
+
+// Rows of a global scalar: value programs write the row (the first term sets it, with
+// the constant), tangent programs add every term's tangent (their output is cleared).
+void ParsedSystem::export_glob_terms(const Sc& g, int row, bool der, int rowstride_q) {
+  (void)rowstride_q;
+  if (!g.glob) throw std::runtime_error("not an integral expression");
+  if (!der) {
+    if (g.gl.empty()) tr.export_global(tr.zero(dmin, SP_VAL), dmin, OUTER_BC, row, 0.0, g.x, 0);
+    for (size_t k = 0; k < g.gl.size(); ++k)
+      tr.export_global(g.gl[k].val, g.gl[k].dom, g.gl[k].bound, row, g.gl[k].fac, k == 0 ? g.x : 0.0, k > 0,
+                       g.gl[k].point);
+    return;
+  }
+  for (const GlTerm& t : g.gl) tr.export_global(t.der, t.dom, t.bound, row, t.fac, 0.0, 1, t.point);
+}
+
+// Global quantities: the value program (Q[q]), then per (field, domain) the tangent
+// program of the quantities evaluated there (dQ_q/du at q * n + column), then per
+// parameter the quantities' own dependence on it at fixed fields.
+void ParsedSystem::trace_globals() {
+  if (globs.empty()) return;
+  Program saved = std::move(tr.val);
+  tr.val = Program();
+  Program dummy;
+  tr.tan = &dummy;
+  for (size_t q = 0; q < globs.size(); ++q) {
+    Ctx cx;
+    cx.d = globs[q].dom;
+    cx.bound = globs[q].bound;
+    cx.glob_ctx = true;
+    Tv t = eval(globs[q].expr, cx);
+    if (!t.idx.empty() || !t.c[0].glob) throw std::runtime_error("global '" + globs[q].name + "' is not an integral");
+    export_glob_terms(t.c[0], (int)q, false, 0);
+  }
+  glob_prog = std::move(tr.val);
+  tr.val = std::move(saved);
+  tr.tan = nullptr;
+  glob_tprog.assign(layout.size(), Program());
+  for (size_t L = 0; L < layout.size(); ++L) {
+    const int f = layout[L][0], d = layout[L][1];
+    tr.begin_tangent(&glob_tprog[L]);
+    Ctx cx;
+    cx.d = d;
+    cx.seed_f = f;
+    cx.seed = tr.seed_node(d, field_base(f, d), seed_tab_of.at({f, d}));
+    for (size_t q = 0; q < globs.size(); ++q) {
+      if (globs[q].dom != d) continue;
+      cx.bound = globs[q].bound;
+      cx.glob_ctx = true;
+      Tv t = eval(globs[q].expr, cx);
+      // dQ_q / du lands at q * n + column: row q * n_unknowns with a unit column stride
+      export_glob_terms(t.c[0], (int)q * n_unknowns, true, 0);
+    }
+    tr.tan = nullptr;
+  }
+  glob_par_prog.assign(par_names.size(), Program());
+  for (size_t p = 0; p < par_names.size(); ++p) {
+    tr.begin_tangent(&glob_par_prog[p]);
+    for (size_t q = 0; q < globs.size(); ++q) {
+      Ctx cx;
+      cx.d = globs[q].dom;
+      cx.bound = globs[q].bound;
+      cx.seed_par = (int)p;
+      cx.par_tab = par_tab[p];
+      cx.glob_ctx = true;
+      Tv t = eval(globs[q].expr, cx);
+      export_glob_terms(t.c[0], (int)q, true, 0);
+    }
+    tr.tan = nullptr;
+  }
 }
 
 // Domain-major rows: equation blocks keep their components together and are ordered
@@ -908,6 +999,7 @@ void ParsedSystem::build() {
     seeds.push_back(st);
   }
   for (size_t p = 0; p < par_names.size(); ++p) trace_param((int)p);
+  trace_globals();
 }
 
 }  // namespace kadhost

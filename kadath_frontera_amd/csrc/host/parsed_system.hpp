@@ -55,6 +55,9 @@ class ParsedSystem : public SystemBase {
   // (Eq_int, eq_int.cpp:38-51; Space_adapted_bh::add_eq_int_bh, adapted_bh_add_eq.cpp:88-119);
   // its row comes before every other equation's (sec_member, solver.cpp:49-85)
   void add_eq_int(int dom, int bound, const std::string& eq);
+  // global quantity: an integral expression evaluated in dom at bound (M_ADM, J_ADM, ...),
+  // reported by the value program and differentiated for the adjoint sensitivities
+  int add_global(const std::string& name, int dom, int bound, const std::string& expr);
   // Space_spheric helpers (spheric_add_eq.cpp:26-52)
   void space_add_eq(const std::string& eq, const std::string& m1, const std::string& m2);
   void space_add_inner_bc(const std::string& eq) { add_eq_bc(dmin, 0, eq); }
@@ -75,6 +78,7 @@ class ParsedSystem : public SystemBase {
     VDp val, der;
     int dom, bound;
     double fac;
+    int point = 0;  // 1: the value of a parameter (a constant field), not an integral
   };
   // A scalar component in one domain: a plain number, a Term_eq, or (glob) a global
   // double sum_k fac_k integ(.)_k + x, as Ope_int returns (ope_int.cpp).
@@ -117,6 +121,7 @@ class ParsedSystem : public SystemBase {
     int seed_par = -1;     // tangent-seeded parameter (-1: none)
     int par_tab = -1;
     VDp par_seed;          // its seed node in this domain (created on first use)
+    bool glob_ctx = false; // evaluating an integral expression: parameters are scalars
     std::map<std::string, Tv> defs;  // definitions evaluated in this context
   };
 
@@ -124,6 +129,13 @@ class ParsedSystem : public SystemBase {
   std::vector<int> field_var, field_comp;
   std::vector<Cst> csts;
   std::vector<Def> defs;
+  struct Glob {
+    std::string name, expr;
+    int dom, bound;
+  };
+  std::vector<Glob> globs;
+  void trace_globals();
+  void export_glob_terms(const Sc& g, int row, bool der, int rowstride_q);
   std::string metric;
   std::vector<std::vector<VDp>> field_nodes;  // [domain][field] value nodes
   std::map<std::string, VDp> const_nodes;     // "name:comp:domain" value nodes of constant fields

@@ -73,6 +73,14 @@ class SystemBase {
   std::vector<std::string> par_names;
   std::vector<Program> par_prog;
   std::vector<int> par_tab;
+  // global quantities given as integral expressions (the parsed systems' counterpart of
+  // the observables above): a value program writing Q[q], one tangent program per
+  // (field, domain) writing dQ_q/du at q * n + column, and per parameter a program
+  // writing dQ_q/dp at fixed fields
+  std::vector<std::string> glob_names;
+  Program glob_prog;
+  std::vector<Program> glob_tprog;                 // indexed like layout
+  std::vector<Program> glob_par_prog;              // indexed like par_names
   // double unknowns (add_var of a double): unknowns 0 .. ndouble-1, each a parameter
   // whose dF/dp program is its Jacobian column; the fields' unknowns follow
   int ndouble = 0;

@@ -653,9 +653,11 @@ void Tracer::export_integral(const VDp& v, int q, double fac) {
   prog(v->space).code.push_back(in);
 }
 
-void Tracer::export_global(const VDp& v, int dom, int bound, int row, double fac, double c0, int accumulate) {
+void Tracer::export_global(const VDp& v, int dom, int bound, int row, double fac, double c0, int accumulate,
+                           int point) {
   Instr in;
   in.kind = I_EXPORT_GL;
+  in.pt = point;
   in.dom = dom;
   in.row0 = row;
   in.s = fac;
@@ -666,6 +668,13 @@ void Tracer::export_global(const VDp& v, int dom, int bound, int row, double fac
     if (v->space != SP_VAL) return;  // a zero tangent adds nothing
     in.a = {-1, SP_VAL};
     val.code.push_back(in);
+    return;
+  }
+  if (point) {
+    // a constant field: its value is conf[0] (conf slot) or coefficient 0 (coef slot, the
+    // T_0 cos(0) term of the std base)
+    in.a = {v->c >= 0 ? v->c : v->cf, v->space};
+    prog(v->space).code.push_back(in);
     return;
   }
   coef(v);

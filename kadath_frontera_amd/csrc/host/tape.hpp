@@ -70,8 +70,10 @@ struct Instr {
   int seed_tab = -1;
   // EXPORT_INT: theta basis of phi coefficient 0 (row0 = quantity index, s = factor)
   int aux = 0;
-  // EXPORT_GL: constant added to the row (value programs); a.slot < 0: the constant alone
+  // EXPORT_GL: constant added to the row (value programs); a.slot < 0: the constant alone;
+  // pt = 1: the point value a[0] of a constant field instead of an integral
   double c0 = 0.0;
+  int pt = 0;
 };
 
 // A chain of 1-D passes with host matrices (uploaded once per plan).
@@ -162,7 +164,8 @@ class Tracer {
   // integral-equation row: row (+)= fac * integ(v) at the boundary (v already holds
   // r r sin(theta) so, as Domain_shell::integ forms it) + c0; a zero v in the value
   // program still writes c0
-  void export_global(const VDp& v, int dom, int bound, int row, double fac, double c0, int accumulate);
+  void export_global(const VDp& v, int dom, int bound, int row, double fac, double c0, int accumulate,
+                     int point = 0);
   int nrows_inside(const Base& b, int dom, int order) { return (int)D(dom).tau_rows_inside(b.bt, order).size(); }
   int nrows_boundary(const Base& b, int dom) { return (int)D(dom).tau_rows_boundary(b.bt).size(); }
 

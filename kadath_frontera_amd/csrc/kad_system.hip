@@ -244,11 +244,22 @@ __global__ void export_int_kernel(const double* __restrict__ src, int64_t sstrid
 // constant alone.
 template <typename T>
 __global__ void export_gl_kernel(const double* __restrict__ src, int64_t sstride, int nr, int nt, int bt0,
-                                 int bsign, double fac, double c0, int acc, T* __restrict__ out, int row,
+                                 int bsign, double fac, double c0, int acc, int pt, T* __restrict__ out, int row,
                                  int64_t col0, int64_t ldo, const double* __restrict__ w, int nbatch) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nbatch) return;
   double tot = 0.0;
+  if (src && pt) {
+    // point value of a constant field (a parameter): entry 0 of its array, times fac
+    const double val = src[(int64_t)b * sstride] * fac;
+    if (w) {
+      ((double*)out)[col0 + b] += w[row] * val;
+      return;
+    }
+    T* o = out + row + (col0 + b) * ldo;
+    *o = acc ? (T)((double)*o + val + c0) : (T)(val + c0);
+    return;
+  }
   if (src && (bt0 == COS_EVEN || bt0 == SIN_ODD)) {
     const double* x = src + (int64_t)b * sstride;
     for (int j = 0; j < nt; ++j) {
@@ -268,7 +279,7 @@ __global__ void export_gl_kernel(const double* __restrict__ src, int64_t sstride
     return;
   }
   T* o = out + row + (col0 + b) * ldo;
-  *o = acc ? (T)((double)*o + val) : (T)(val + c0);
+  *o = acc ? (T)((double)*o + val + c0) : (T)(val + c0);
 }
 
 // Double unknowns' fields (constant fields of the value arena): v[e] += scale * X[k].
@@ -316,6 +327,8 @@ struct kad_system_impl {
   std::vector<DevProgram> tprogs;
   std::vector<DevProgram> obs_tprogs;  // d(observables)/d(unknowns)
   std::vector<DevProgram> par_progs;   // dF/dp of each parameter
+  std::vector<DevProgram> glob_tprogs;      // dQ/du of the global quantities, per layout entry
+  std::vector<DevProgram> glob_par_progs;   // dQ/dp at fixed fields, per parameter
   double* dQ = nullptr;                // 3 x n, row q contiguous
   double* dFdp = nullptr;              // n per parameter
   int nphys_max = 0;
@@ -525,6 +538,27 @@ int kad_system_impl::setup() {
     nphys_max = std::max(nphys_max, dp.nphys);
     par_progs.push_back(std::move(dp));
   }
+  for (size_t p = 0; p < sys->glob_tprog.size(); ++p) {
+    DevProgram dp;
+    dp.code = sys->glob_tprog[p].code;
+    dp.nphys = allocate_slots(dp.code, sys->glob_tprog[p].nslots);
+    dp.field = sys->layout[p][0];
+    dp.dom = sys->layout[p][1];
+    dp.off = sys->layout[p][2];
+    dp.cnt = sys->layout[p][3];
+    dp.seed_tab = tprogs[p].seed_tab;
+    nphys_max = std::max(nphys_max, dp.nphys);
+    glob_tprogs.push_back(std::move(dp));
+  }
+  for (size_t p = 0; p < sys->glob_par_prog.size(); ++p) {
+    DevProgram dp;
+    dp.code = sys->glob_par_prog[p].code;
+    dp.nphys = allocate_slots(dp.code, sys->glob_par_prog[p].nslots);
+    dp.seed_tab = sys->par_tab[p];
+    dp.cnt = 1;
+    nphys_max = std::max(nphys_max, dp.nphys);
+    glob_par_progs.push_back(std::move(dp));
+  }
   if ((rc = dmalloc((void**)&tan_arena, (size_t)std::max(1, nphys_max) * B * stride * sizeof(double)))) return rc;
   if ((rc = dmalloc((void**)&F, (size_t)sys->n_rows * sizeof(double)))) return rc;
   if ((rc = dmalloc((void**)&scratch, 64 * sizeof(double)))) return rc;
@@ -598,10 +632,12 @@ int kad_system_impl::run(const std::vector<Instr>& code, int nb, int u0, int see
         const unsigned g = (unsigned)kad::cdiv(batch, 64);
         if (w || dtype == 0)
           export_gl_kernel<double><<<g, 64, 0, s>>>(src, ss, dm.nr, dm.nt, in.aux, in.bsign, in.s, in.c0,
-                                                    in.accumulate, (double*)out, in.row0, col0, ldo, w, batch);
+                                                    in.accumulate, in.pt, (double*)out, in.row0, col0, ldo, w,
+                                                    batch);
         else
           export_gl_kernel<float><<<g, 64, 0, s>>>(src, ss, dm.nr, dm.nt, in.aux, in.bsign, in.s, in.c0,
-                                                   in.accumulate, (float*)out, in.row0, col0, ldo, nullptr, batch);
+                                                   in.accumulate, in.pt, (float*)out, in.row0, col0, ldo, nullptr,
+                                                   batch);
         break;
       }
       case I_SEED: {
@@ -992,6 +1028,82 @@ static int sensitivity_impl(kad_system_impl* S, int par, double* out, double* dQ
   return KAD_OK;
 }
 
+// dst[q n + k] = src[q] for every quantity q (a double unknown's column of dQ/du)
+__global__ void scatter_q_kernel(const double* __restrict__ src, int nq, double* __restrict__ dst, int64_t n, int k) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < nq) dst[(int64_t)q * n + k] = src[q];
+}
+
+// Global quantities of a parsed system at the current fields: out[q] (HOST).
+static int globals_impl(kad_system_impl* S, double* out) {
+  SystemBase& sy = *S->sys;
+  const int nq = (int)sy.glob_names.size();
+  if (nq == 0 || nq > 32) return KAD_ERR_ARG;
+  int rc = S->ensure_values();
+  if (rc) return rc;
+  if ((rc = S->run(sy.glob_prog.code, 1, 0, -1, S->scratch, 0, 0))) return rc;
+  KAD_HIP_CHECK(hipMemcpyAsync(out, S->scratch, nq * sizeof(double), hipMemcpyDeviceToHost, S->ctx->stream));
+  KAD_HIP_CHECK(hipStreamSynchronize(S->ctx->stream));
+  return KAD_OK;
+}
+
+// Adjoint sensitivities of the parsed system's global quantities:
+// dQ_q/dp = dQ_q/dp|_u - lambda_q . dF/dp with J^T lambda_q = dQ_q/du, J refilled and
+// factored at the current fields.  out[q] HOST; dQdu (HOST nq x n, may be NULL).
+static int glob_sensitivity_impl(kad_system_impl* S, int par, double* out, double* dQdu) {
+  SystemBase& sy = *S->sys;
+  hipStream_t st = S->ctx->stream;
+  const int n = sy.n_unknowns;
+  const int nq = (int)sy.glob_names.size();
+  if (nq == 0 || nq > 16 || par < 0 || par >= (int)S->par_progs.size()) return KAD_ERR_ARG;
+  int rc;
+  if (!S->J) {
+    if ((rc = S->dmalloc((void**)&S->J, (size_t)n * n * sizeof(double)))) return rc;
+    if ((rc = S->dmalloc((void**)&S->ipiv, (size_t)n * sizeof(int)))) return rc;
+    if ((rc = S->dmalloc((void**)&S->perm, (size_t)n * sizeof(int)))) return rc;
+    if ((rc = S->dmalloc((void**)&S->X, (size_t)n * sizeof(double)))) return rc;
+  }
+  if (!S->dQ) {
+    if ((rc = S->dmalloc((void**)&S->dQ, (size_t)std::max(3, nq) * n * sizeof(double)))) return rc;
+    if ((rc = S->dmalloc((void**)&S->dFdp, (size_t)n * sizeof(double)))) return rc;
+  }
+  if ((rc = S->jacobian(S->J, n))) return rc;  // value pass + every column, doubles included
+  double* tmp = S->scratch + 32;   // nq: a double unknown's dQ column, then dQ/dp at fixed fields
+  KAD_HIP_CHECK(hipMemsetAsync(S->dQ, 0, (size_t)nq * n * sizeof(double), st));
+  KAD_HIP_CHECK(hipMemsetAsync(S->dFdp, 0, (size_t)n * sizeof(double), st));
+  for (int k = 0; k < sy.ndouble; ++k) {
+    const DevProgram& gp = S->glob_par_progs[sy.dvar_par[k]];
+    KAD_HIP_CHECK(hipMemsetAsync(tmp, 0, nq * sizeof(double), st));
+    if ((rc = S->run(gp.code, 1, 0, gp.seed_tab, tmp, 0, 1))) return rc;
+    scatter_q_kernel<<<1, 64, 0, st>>>(tmp, nq, S->dQ, n, k);
+  }
+  for (auto& p : S->glob_tprogs)
+    for (int u0 = 0; u0 < p.cnt; u0 += S->B) {
+      const int nb = std::min(S->B, p.cnt - u0);
+      if ((rc = S->run(p.code, nb, u0, p.seed_tab, S->dQ, (int64_t)p.off + u0, 1))) return rc;
+    }
+  const DevProgram& pp = S->par_progs[par];
+  if ((rc = S->run(pp.code, 1, 0, pp.seed_tab, S->dFdp, 0, n))) return rc;
+  KAD_HIP_CHECK(hipMemsetAsync(tmp, 0, nq * sizeof(double), st));
+  if ((rc = S->run(S->glob_par_progs[par].code, 1, 0, S->glob_par_progs[par].seed_tab, tmp, 0, 1))) return rc;
+  double direct[16];
+  KAD_HIP_CHECK(hipMemcpyAsync(direct, tmp, nq * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (dQdu) KAD_HIP_CHECK(hipMemcpyAsync(dQdu, S->dQ, (size_t)nq * n * sizeof(double), hipMemcpyDeviceToHost, st));
+  rc = kad_lu_factor(reinterpret_cast<kad_ctx*>(S->ctx), KAD_PREC_F64, n, S->J, n, S->ipiv, S->perm, 0);
+  if (rc < 0) return rc;
+  if (rc > 0) return KAD_ERR_SINGULAR;
+  KAD_HIP_CHECK(hipStreamSynchronize(st));
+  for (int q = 0; q < nq; ++q) {
+    KAD_HIP_CHECK(hipMemcpyAsync(S->X, S->dQ + (size_t)q * n, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice,
+                                 st));
+    if ((rc = kad_lu_solve(reinterpret_cast<kad_ctx*>(S->ctx), KAD_PREC_F64, n, S->J, n, S->perm, S->X, 1)))
+      return rc;
+    out[q] = direct[q] - S->ctx->dot(S->X, S->dFdp, n);
+  }
+  KAD_HIP_CHECK(hipStreamSynchronize(st));
+  return KAD_OK;
+}
+
 // XCTS: out[3] = d(M_ADM, M_Komar, J_ADM)/dOme.
 int kad_xcts_sensitivity(kad_system* s, double* out, double* dQdu) {
   if (!s || !out) return KAD_ERR_ARG;
@@ -1291,6 +1403,12 @@ int kad_sysdef_add_var_double(kad_sysdef* def, const char* name, double value) {
   int rc = guarded(def, [&](ParsedSystem& p) { idx = p.add_var_double(name, value); });
   return rc ? rc : idx;
 }
+int kad_sysdef_add_global(kad_sysdef* def, const char* name, int dom, int bound, const char* expr) {
+  if (!name || !expr) return KAD_ERR_ARG;
+  int idx = -1;
+  int rc = guarded(def, [&](ParsedSystem& p) { idx = p.add_global(name, dom, bound, expr); });
+  return rc ? rc : idx;
+}
 int kad_sysdef_add_eq_int(kad_sysdef* def, int dom, int bound, const char* eq) {
   if (!eq) return KAD_ERR_ARG;
   return guarded(def, [&](ParsedSystem& p) { p.add_eq_int(dom, bound, eq); });
@@ -1526,6 +1644,17 @@ int kad_system_get_double(kad_system* s, int k, double* value) {
     return KAD_OK;
   }
   return KAD_ERR_ARG;  // the unknown appears in no equation
+}
+// Global quantities (kad_sysdef_add_global) at the current fields, HOST out.
+int kad_system_globals(kad_system* s, double* out) {
+  if (!s || !out) return KAD_ERR_ARG;
+  return globals_impl(reinterpret_cast<kad_system_impl*>(s), out);
+}
+// dQ_q/dp of the global quantities with respect to parameter `which`, by the adjoint
+// method at the current (converged) fields; HOST out[nq], dQdu (HOST nq x n) may be NULL.
+int kad_system_sensitivity(kad_system* s, int which, double* out, double* dQdu) {
+  if (!s || !out) return KAD_ERR_ARG;
+  return glob_sensitivity_impl(reinterpret_cast<kad_system_impl*>(s), which, out, dQdu);
 }
 int kad_system_set_param(kad_system* s, int which, double value) {
   if (!s) return KAD_ERR_ARG;

@@ -33,9 +33,10 @@ def tuto5(ctx, res=(13, 5, 4), bounds=(1.0, 2.0, 4.0)):
 
 
 def kerr_constants(sd):
-    """Radial unit vector s^i and rotation generator m^i = (-y, x, 0) at the collocation
-    points of the solved domains (component-major, then domain)."""
-    s_c, m_c = [[], [], []], [[], [], []]
+    """Radial unit vector s^i, rotation generator m^i = (-y, x, 0) and mh^i = m^i / r
+    (bounded at infinity) at the collocation points of the solved domains
+    (component-major, then domain)."""
+    s_c, m_c, h_c = [[], [], []], [[], [], []], [[], [], []]
     for d in range(sd.dmin, sd.dmax + 1):
         r, t, p = sd.coloc(d)
         st, ct, cp, sp = np.sin(t), np.cos(t), np.cos(p), np.sin(p)
@@ -44,7 +45,10 @@ def kerr_constants(sd):
         rr = np.where(np.isfinite(r), r, 0.0)
         for i, v in enumerate((-rr * st * sp, rr * st * cp, np.zeros_like(rr))):
             m_c[i].append(v.ravel())
-    return np.array([np.concatenate(c) for c in s_c]), np.array([np.concatenate(c) for c in m_c])
+        for i, v in enumerate((-st * sp, st * cp, np.zeros_like(rr))):
+            h_c[i].append(v.ravel())
+    cat = lambda c: np.array([np.concatenate(x) for x in c])  # noqa: E731
+    return cat(s_c), cat(m_c), cat(h_c)
 
 
 def kerr(ctx, res, bounds, n0=0.5, a=1.0, ome=0.0, spin=None, cm=1.0):
@@ -53,17 +57,18 @@ def kerr(ctx, res, bounds, n0=0.5, a=1.0, ome=0.0, spin=None, cm=1.0):
     sd.add_var("N")
     sd.add_var("bet", 1)
     sd.add_cst("a", a)
-    s_c, m_c = kerr_constants(sd)
+    s_c, m_c, h_c = kerr_constants(sd)
     sd.add_cst_field("s", s_c, 1)
     sd.add_cst_field("m", m_c, 1)
+    sd.add_cst_field("mh", h_c, 1)
     sd.add_cst("n0", n0)
+    sd.add_cst("PI", np.pi)
     if spin is None:
         sd.add_param("ome", ome)
     else:
         sd.add_var_double("ome", ome)
-        sd.add_cst("chi", spin)
+        sd.add_param("chi", spin)
         sd.add_cst("CM", cm)
-        sd.add_cst("PI", np.pi)
     sd.set_flat_metric("f")
     sd.add_def("A^ij = (D^i bet^j + D^j bet^i - 2. / 3.* D_k bet^k * f^ij) /2. / N")
     if spin is not None:
@@ -78,4 +83,10 @@ def kerr(ctx, res, bounds, n0=0.5, a=1.0, ome=0.0, spin=None, cm=1.0):
     sd.space_add_outer_bc("N=1")
     sd.space_add_outer_bc("P=1")
     sd.space_add_outer_bc("bet^i=0")
+    # global quantities at infinity (codes/Par_version/Kerr/reader.cpp:114-125 in the
+    # Cartesian basis): ADM mass, Komar mass, ADM angular momentum about z
+    last = len(bounds)
+    sd.add_global("M_adm", last, OUTER_BC, "-integ(dn(P)) / 2. / PI")
+    sd.add_global("M_komar", last, OUTER_BC, "integ(dn(N)) / 4. / PI")
+    sd.add_global("J_adm", last, OUTER_BC, "-integ(multr(A_ij * s^i * mh^j)) / 8. / PI")
     return sd

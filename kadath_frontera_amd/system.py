@@ -47,6 +47,7 @@ class SystemDef:
         self.fields = []   # scalar field components in declaration order
         self.params = {}
         self.doubles = {}
+        self.globals = []
         self._npar = 0
 
     def close(self):
@@ -80,6 +81,11 @@ class SystemDef:
         self.params[name] = self._npar  # it is also a parameter (set_param, param_tangent)
         self._npar += 1
         return idx
+
+    def add_global(self, name, dom, bound, expr):
+        """A global quantity (integral expression), reported by DeviceSystem.globals()."""
+        check(lib.kad_sysdef_add_global(self.h, _b(name), dom, bound, _b(expr)), f"add_global({name})")
+        self.globals.append(name)
 
     def add_eq_int(self, dom, bound, eq):
         check(lib.kad_sysdef_add_eq_int(self.h, dom, bound, _b(eq)), f"add_eq_int({eq})")
@@ -139,13 +145,14 @@ class SystemDef:
         d, self.h = self.h, None
         check(lib.kad_system_build(self.ctx.h, d, order, batch, ctypes.byref(h)), "kad_system_build")
         return DeviceSystem(self.ctx, h, self.res, self.dmin, self.dmax, list(self.fields), dict(self.params),
-                            dict(self.doubles))
+                            dict(self.doubles), list(self.globals))
 
 
 class DeviceSystem:
-    def __init__(self, ctx, h, res, dmin, dmax, fields, params, doubles=None):
+    def __init__(self, ctx, h, res, dmin, dmax, fields, params, doubles=None, globals_=None):
         self.ctx, self.h, self.res, self.dmin, self.dmax = ctx, h, res, dmin, dmax
         self.fields, self.params, self.doubles = fields, params, doubles or {}
+        self.global_names = globals_ or []
         ctx.adopt(self)
         n, m, nf = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         check(lib.kad_system_sizes(h, ctypes.byref(n), ctypes.byref(m), ctypes.byref(nf)), "kad_system_sizes")
@@ -203,6 +210,22 @@ class DeviceSystem:
         out = np.zeros(self.n)
         check(lib.kad_system_perturbation(self.h, seed, amp, _dp(out)), "kad_system_perturbation")
         return out
+
+    def globals(self):
+        out = np.zeros(max(1, len(self.global_names)))
+        check(lib.kad_system_globals(self.h, _dp(out)), "kad_system_globals")
+        return dict(zip(self.global_names, out))
+
+    def sensitivity(self, param, with_dqdu=False):
+        """Adjoint dQ/dparam of the global quantities at the current (converged) fields."""
+        nq = len(self.global_names)
+        out = np.zeros(max(1, nq))
+        dq = np.zeros((nq, self.n)) if with_dqdu else None
+        self.ctx._enter()
+        check(lib.kad_system_sensitivity(self.h, self.params[param], _dp(out), _dp(dq) if dq is not None else None),
+              "kad_system_sensitivity")
+        res = dict(zip(self.global_names, out))
+        return (res, dq) if with_dqdu else res
 
     def get_double(self, name):
         v = ctypes.c_double()
