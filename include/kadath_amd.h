@@ -203,7 +203,9 @@ int kad_xcts_set_param(kad_system* s, int which, double value);
  * dQdu (HOST 3 x n, may be NULL) receives dQ/du. */
 int kad_xcts_sensitivity(kad_system* s, double* out, double* dQdu);
 /* One Newton iteration.  solver: 0 fp64 LU, 1 fp32 LU + fp64 IR, 2 fp32 LU +
- * GMRES-IR, 3 bf16-update LU + GMRES-IR.  Returns 1 if ||F||_inf < precision at
+ * GMRES-IR, 3 bf16-update LU + GMRES-IR, 4 Gauss-Newton (J^T J X = J^T F in fp64,
+ * the reference's SOLVER GAUSS_NEWTON, do_newton.cpp:264-311; it holds J^T and J^T J
+ * beside J, 3 n^2 doubles).  Returns 1 if ||F||_inf < precision at
  * entry (no step), 0 after a step, KAD_ERR_SINGULAR on an exactly singular J.
  * times[5]: residual, Jacobian fill, factor + solve (incl. refinement), update,
  * factorisation alone (seconds). */
@@ -330,6 +332,19 @@ int kad_system_get_double(kad_system* s, int k, double* value);
 int kad_system_set_option(kad_system* s, int which, double value);
 /* as kad_xcts_do_newton / kad_xcts_last_solve */
 int kad_system_do_newton(kad_system* s, double precision, double* error, int solver, double* times);
+/* Newton iteration with backtracking line search:
+ * System_of_eqs::do_newton_with_linesearch(precision, error, ntrymax, stepmax)
+ * (src_par/do_newton.cpp:454-517; reference defaults ntrymax 10, stepmax 1.0).  The
+ * step p solves J p = F with `solver` (0-4 as kad_system_do_newton) and is scaled by
+ * stepmax / max(p) when its signed maximum exceeds stepmax; u = u_old - lambda p with
+ * lambda from the quadratic, then cubic, model until sum F^2 <= f_old + 1e-4 lambda
+ * slope (slope = -2 f_old), sum F^2 <= 5e-13, lambda <= 1e-10, or ntrymax tries.
+ * Returns 1 if ||F||_inf < precision at entry (no step), 0 after a step,
+ * KAD_ERR_NOCONV where the reference aborts with "roundoff error in
+ * do_newton_with_linesearch".  lambda, ntry (HOST, may be NULL): the lambda applied,
+ * the residual evaluations of the search. */
+int kad_system_do_newton_linesearch(kad_system* s, double precision, double* error, int ntrymax, double stepmax,
+                                    int solver, double* lambda, int* ntry);
 int kad_system_last_solve(kad_system* s, int* iters, double* resid);
 
 /* F(u) (device, n_rows): System_of_eqs::sec_member (solver.cpp:49-85) */

@@ -40,9 +40,10 @@ namespace kadath_amd {
 
 enum { INNER_BC = 0, OUTER_BC = 1 };
 enum { CHEB_TYPE = 1 };
-// do_newton's solver argument: the reference's SOLVER (NEWTON_RAPHSON) plus the
-// mixed-precision solvers of this build
-enum SOLVER { NEWTON_RAPHSON = 0, NEWTON_F32_IR = 1, NEWTON_F32_GMRES = 2, NEWTON_BF16_GMRES = 3 };
+// do_newton's solver argument: the reference's SOLVER (NEWTON_RAPHSON, GAUSS_NEWTON,
+// system_of_eqs.hpp:134) plus the mixed-precision solvers of this build; the values are
+// the C-ABI's solver codes
+enum SOLVER { NEWTON_RAPHSON = 0, NEWTON_F32_IR = 1, NEWTON_F32_GMRES = 2, NEWTON_BF16_GMRES = 3, GAUSS_NEWTON = 4 };
 
 namespace detail {
 inline void check(int rc, const char* what) {
@@ -213,6 +214,23 @@ class System_of_eqs {
       std::abort();
     }
     detail::check(rc, "do_newton");
+    if (rc == 1) return true;
+    download();
+    return false;
+  }
+  // Newton iteration with backtracking line search (do_newton.cpp:454-517); true when
+  // ||F||_inf < precision at entry.  The reference aborts on a roundoff failure of the
+  // search, and so does this call.
+  bool do_newton_with_linesearch(double precision, double& error, int ntrymax = 10, double stepmax = 1.0) {
+    ready();
+    const int rc = kad_system_do_newton_linesearch(sys_, precision, &error, ntrymax, stepmax, NEWTON_RAPHSON,
+                                                   nullptr, nullptr);
+    if (rc == KAD_ERR_SINGULAR) {
+      std::fprintf(stderr, "Jacobian is singular! Stopping.\n");
+      std::abort();
+    }
+    if (rc == KAD_ERR_NOCONV) std::abort();
+    detail::check(rc, "do_newton_with_linesearch");
     if (rc == 1) return true;
     download();
     return false;

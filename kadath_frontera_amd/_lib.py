@@ -110,6 +110,7 @@ SIGNATURES = {
     "kad_system_perturbation": (c_int, [c_vp, ctypes.c_uint64, c_dbl, c_dp]),
     "kad_system_set_param": (c_int, [c_vp, c_int, c_dbl]),
     "kad_system_do_newton": (c_int, [c_vp, c_dbl, c_dp, c_int, c_dp]),
+    "kad_system_do_newton_linesearch": (c_int, [c_vp, c_dbl, c_dp, c_int, c_dbl, c_int, c_dp, c_ip]),
     "kad_system_last_solve": (c_int, [c_vp, c_ip, c_dp]),
     "kad_system_set_option": (c_int, [c_vp, c_int, c_dbl]),
     "kad_residual": (c_int, [c_vp, c_vp]),

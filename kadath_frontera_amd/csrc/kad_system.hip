@@ -188,6 +188,53 @@ __global__ void absmax_kernel(const double* x, int n, double* out) {
   if (threadIdx.x == 0) *out = red[0];
 }
 
+// sum of squares (compute_f of the line search, do_newton.cpp:594-599) and signed maximum
+// (Array max, array_math.cpp:308-314) of x[0..n), one workgroup; out[0] = sum, out[1] = max.
+// A NaN reaches both results.
+__global__ void sumsq_max_kernel(const double* x, int n, double* out) {
+  __shared__ double rs[256], rm[256];
+  double s = 0.0, m = n > 0 ? x[0] : 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const double a = x[i];
+    s += a * a;
+    m = (a > m || a != a) ? a : m;
+  }
+  rs[threadIdx.x] = s;
+  rm[threadIdx.x] = m;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) {
+      rs[threadIdx.x] += rs[threadIdx.x + k];
+      const double o = rm[threadIdx.x + k];
+      if (o > rm[threadIdx.x] || o != o) rm[threadIdx.x] = o;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = rs[0];
+    out[1] = rm[0];
+  }
+}
+
+// B = A^T for an n x n column-major matrix: 64 x 64 tiles through LDS (padded row), both
+// sides coalesced.  Grid (cdiv(n, 64), cdiv(n, 64)), 256 threads.
+__global__ __launch_bounds__(256) void transpose_kernel(const double* __restrict__ A, int64_t lda,
+                                                        double* __restrict__ B, int64_t ldb, int n) {
+  __shared__ double t[64][65];
+  const int r0 = blockIdx.x * 64, c0 = blockIdx.y * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int c = ty; c < 64; c += 4) {
+    const int gr = r0 + tx, gc = c0 + c;
+    t[c][tx] = (gr < n && gc < n) ? A[gr + (int64_t)gc * lda] : 0.0;
+  }
+  __syncthreads();
+  // B(c0 + tx, r0 + r) = A(r0 + r, c0 + tx)
+  for (int r = ty; r < 64; r += 4) {
+    const int gr = c0 + tx, gc = r0 + r;
+    if (gr < n && gc < n) B[gr + (int64_t)gc * ldb] = t[tx][r];
+  }
+}
+
 // Surface integral of one quantity (Domain_compact::integ, domain_compact.cpp:89-121):
 // cf holds the coefficients of r r sin(theta) so; value on the boundary x = bsign of
 // phi coefficient 0, theta weights pi (COS_EVEN, j = 0) or 2/(2j+1) (SIN_ODD), times
@@ -339,6 +386,10 @@ struct kad_system_impl {
   int* perm = nullptr;
   double* X = nullptr;
   double* Fsave = nullptr;
+  double* Jt = nullptr;   // Gauss-Newton: J^T and J^T J
+  double* JtJ = nullptr;
+  double* held = nullptr;   // line search: the unknowns' storage at the start of the search
+  size_t held_cap = 0;
   double* Rs = nullptr;  // equilibration scales
   double* Cs = nullptr;
   std::vector<void*> allocs;
@@ -1110,41 +1161,43 @@ int kad_xcts_sensitivity(kad_system* s, double* out, double* dQdu) {
   return sensitivity_impl(reinterpret_cast<kad_system_impl*>(s), 0, out, dQdu);
 }
 
-// One Newton-Raphson iteration (System_of_eqs::do_newton, src_par/do_newton.cpp:65-430).
-// solver: 0 fp64 LU; 1 fp32 LU + fp64 IR; 2 fp32 LU + fp64 GMRES-IR; 3 bf16-update LU + fp64 GMRES-IR.
-// Returns 1 when ||F||_inf < precision at entry (converged; no step taken), 0 after a step,
-// < 0 on error, KAD_ERR_SINGULAR (-5) when the factorisation meets an exact zero pivot.
-// times[5] (may be NULL): residual, Jacobian fill, factor+solve, update, factor alone (seconds).
-int kad_xcts_do_newton(kad_system* s, double precision, double* error, int solver, double* times) {
-  if (!s || !error || solver < 0 || solver > 3) return KAD_ERR_ARG;
-  auto* S = reinterpret_cast<kad_system_impl*>(s);
+// ||F||_inf at the current fields (the value pass leaves F on the device).
+static int newton_residual(kad_system_impl* S, double* err) {
+  hipStream_t st = S->ctx->stream;
+  int rc;
+  if ((rc = S->value_pass())) return rc;
+  absmax_kernel<<<1, 256, 0, st>>>(S->F, S->sys->n_rows, S->scratch);
+  KAD_HIP_CHECK(hipMemcpyAsync(err, S->scratch, sizeof(double), hipMemcpyDeviceToHost, st));
+  KAD_HIP_CHECK(hipStreamSynchronize(st));
+  return KAD_OK;
+}
+
+// The Newton direction X (S->X) with J X = F: Jacobian fill, then the solver's
+// factorisation and solve.  err = ||F||_inf (the forcing term of the low-precision
+// refinement).  Gauss-Newton (solver 4) solves J^T J X = J^T F instead, as
+// do_newton.cpp:264-311 (pdgemv_ 'T', pdgemm_ 'T','N', pdgesv_).  times as for
+// kad_xcts_do_newton from index 1; t1 = wall clock at entry.
+static int newton_direction(kad_system_impl* S, int solver, double err, double* times, double t1) {
   SystemBase& sy = *S->sys;
   hipStream_t st = S->ctx->stream;
+  kad_ctx* cx = reinterpret_cast<kad_ctx*>(S->ctx);
   const int n = sy.n_unknowns;
   int rc;
-  double t0 = now_s();
-  if ((rc = S->value_pass())) return rc;
-  absmax_kernel<<<1, 256, 0, st>>>(S->F, sy.n_rows, S->scratch);
-  double err = 0;
-  KAD_HIP_CHECK(hipMemcpyAsync(&err, S->scratch, sizeof(double), hipMemcpyDeviceToHost, st));
-  KAD_HIP_CHECK(hipStreamSynchronize(st));
-  *error = err;
-  double t1 = now_s();
-  if (times) times[0] = t1 - t0;
-  S->last_iters[0] = S->last_iters[1] = 0;
-  S->last_resid = 0.0;
-  S->last_fallback = 0;
-  S->last_factored = 0;
-  if (err < precision) return 1;  // false for a NaN error: a NaN residual is never converged
   if (!S->J) {
     if ((rc = S->dmalloc((void**)&S->J, (size_t)n * n * sizeof(double)))) return rc;
     if ((rc = S->dmalloc((void**)&S->ipiv, (size_t)n * sizeof(int)))) return rc;
     if ((rc = S->dmalloc((void**)&S->perm, (size_t)n * sizeof(int)))) return rc;
     if ((rc = S->dmalloc((void**)&S->X, (size_t)n * sizeof(double)))) return rc;
   }
-  if (solver > 0 && !S->Jlo)
+  if ((solver == 1 || solver == 2 || solver == 3) && !S->Jlo)
     if ((rc = S->dmalloc((void**)&S->Jlo, (size_t)n * n * sizeof(float)))) return rc;
-  // Jacobian (the value pass above already holds the intermediates)
+  if (solver == 4 && !S->Jt) {
+    if ((rc = S->dmalloc((void**)&S->Jt, (size_t)n * n * sizeof(double)))) return rc;
+    if ((rc = S->dmalloc((void**)&S->JtJ, (size_t)n * n * sizeof(double)))) return rc;
+  }
+  // Jacobian (the value pass of the residual already holds the intermediates)
+  if (!S->vals_current)
+    if ((rc = S->value_pass())) return rc;
   KAD_HIP_CHECK(hipMemsetAsync(S->J, 0, (size_t)n * n * sizeof(double), st));
   if ((rc = S->double_columns(S->J, n, 0, sy.ndouble, 0))) return rc;
   for (auto& p : S->tprogs)
@@ -1155,21 +1208,31 @@ int kad_xcts_do_newton(kad_system* s, double precision, double* error, int solve
   KAD_HIP_CHECK(hipStreamSynchronize(st));
   double t2 = now_s();
   if (times) times[1] = t2 - t1;
-  // solve J X = F
   int info = 0;
   double tf0 = 0, tf1 = 0;
-  if (solver == 0) {
+  if (solver == 0 || solver == 4) {
     S->lo_valid = false;  // the fp64 factorisation takes over ipiv / perm
     S->last_factored = 1;
+    double* M = S->J;
+    if (solver == 4) {
+      // J^T F, then J^T J = (J^T) J with an explicit transpose and the MFMA GEMM; J
+      // itself stays unfactored
+      if ((rc = kad_gemv64(cx, n, S->J, n, S->F, nullptr, S->X, 1))) return rc;
+      dim3 g((unsigned)kad::cdiv(n, 64), (unsigned)kad::cdiv(n, 64));
+      transpose_kernel<<<g, 256, 0, st>>>(S->J, n, S->Jt, n, n);
+      KAD_HIP_CHECK(hipGetLastError());
+      if ((rc = kad_gemm(cx, KAD_PREC_F64, 1, n, n, n, S->Jt, n, S->J, n, S->JtJ, n))) return rc;
+      M = S->JtJ;
+    }
+    KAD_HIP_CHECK(hipStreamSynchronize(st));
     tf0 = now_s();
-    if ((rc = kad_lu_factor(reinterpret_cast<kad_ctx*>(S->ctx), KAD_PREC_F64, n, S->J, n, S->ipiv, S->perm, 0)) < 0)
-      return rc;
+    if ((rc = kad_lu_factor(cx, KAD_PREC_F64, n, M, n, S->ipiv, S->perm, 0)) < 0) return rc;
     tf1 = now_s();
     info = rc;
     if (info == 0) {
-      KAD_HIP_CHECK(hipMemcpyAsync(S->X, S->F, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, st));
-      if ((rc = kad_lu_solve(reinterpret_cast<kad_ctx*>(S->ctx), KAD_PREC_F64, n, S->J, n, S->perm, S->X, 0)))
-        return rc;
+      if (solver == 0)
+        KAD_HIP_CHECK(hipMemcpyAsync(S->X, S->F, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, st));
+      if ((rc = kad_lu_solve(cx, KAD_PREC_F64, n, M, n, S->perm, S->X, 0))) return rc;
     }
   } else {
     // low-precision factors of the equilibrated R J C (exact powers of two); the
@@ -1190,12 +1253,10 @@ int kad_xcts_do_newton(kad_system* s, double precision, double* error, int solve
     const double eta = S->forcing > 0.0 ? std::max(1e-13, S->forcing * std::min(1.0, err)) : 1e-13;
     for (int attempt = 0; attempt < 2; ++attempt) {
       if (!reuse) {
-        if ((rc = kad_cast_equilibrated(reinterpret_cast<kad_ctx*>(S->ctx), n, S->J, n, S->Jlo, n, S->Rs, S->Cs)))
-          return rc;
+        if ((rc = kad_cast_equilibrated(cx, n, S->J, n, S->Jlo, n, S->Rs, S->Cs))) return rc;
         KAD_HIP_CHECK(hipStreamSynchronize(st));
         const double tg0 = now_s();
-        if ((rc = kad_lu_factor(reinterpret_cast<kad_ctx*>(S->ctx), prec, n, S->Jlo, n, S->ipiv, S->perm, 0)) < 0)
-          return rc;
+        if ((rc = kad_lu_factor(cx, prec, n, S->Jlo, n, S->ipiv, S->perm, 0)) < 0) return rc;
         tf1 += now_s() - tg0;
         info = rc;
         S->lo_valid = (info == 0);
@@ -1203,9 +1264,8 @@ int kad_xcts_do_newton(kad_system* s, double precision, double* error, int solve
         S->last_factored = 1;
         if (info != 0) break;
       }
-      rc = kad_ir_solve_scaled(reinterpret_cast<kad_ctx*>(S->ctx), n, S->J, n, prec, S->Jlo, n, S->perm, S->Rs, S->Cs,
-                               S->F, S->X, eta, solver == 1 ? 30 : 8, solver == 1 ? 0 : 1, 0, S->last_iters,
-                               &S->last_resid);
+      rc = kad_ir_solve_scaled(cx, n, S->J, n, prec, S->Jlo, n, S->perm, S->Rs, S->Cs, S->F, S->X, eta,
+                               solver == 1 ? 30 : 8, solver == 1 ? 0 : 1, 0, S->last_iters, &S->last_resid);
       if (rc < 0 && rc != KAD_ERR_NOCONV) return rc;
       S->prev_inner = S->last_iters[1];
       // reused factors that no longer precondition well: factor the current J and retry
@@ -1226,29 +1286,175 @@ int kad_xcts_do_newton(kad_system* s, double precision, double* error, int solve
         S->last_fallback = 1;
         S->lo_valid = false;  // the fp64 factorisation below takes over ipiv / perm
         const double tg0 = now_s();
-        rc = kad_lu_factor(reinterpret_cast<kad_ctx*>(S->ctx), KAD_PREC_F64, n, S->J, n, S->ipiv, S->perm, 0);
+        rc = kad_lu_factor(cx, KAD_PREC_F64, n, S->J, n, S->ipiv, S->perm, 0);
         if (rc < 0) return rc;
         tf1 += now_s() - tg0;  // both factorisations count as factor time
         info = rc;
         if (info == 0) {
           KAD_HIP_CHECK(hipMemcpyAsync(S->X, S->F, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, st));
-          if ((rc = kad_lu_solve(reinterpret_cast<kad_ctx*>(S->ctx), KAD_PREC_F64, n, S->J, n, S->perm, S->X, 0)))
-            return rc;
+          if ((rc = kad_lu_solve(cx, KAD_PREC_F64, n, S->J, n, S->perm, S->X, 0))) return rc;
         }
       }
     }
   }
-  if (times) times[4] = solver == 0 ? tf1 - tf0 : tf1;
+  if (times) times[4] = (solver == 0 || solver == 4) ? tf1 - tf0 : tf1;
   if (info != 0) {
     fprintf(stderr, "Jacobian is singular! Stopping.\n");
-    return -5;
+    return KAD_ERR_SINGULAR;
   }
   KAD_HIP_CHECK(hipStreamSynchronize(st));
+  if (times) times[2] = now_s() - t2;
+  return KAD_OK;
+}
+
+static void reset_solve_record(kad_system_impl* S) {
+  S->last_iters[0] = S->last_iters[1] = 0;
+  S->last_resid = 0.0;
+  S->last_fallback = 0;
+  S->last_factored = 0;
+}
+
+// One Newton-Raphson iteration (System_of_eqs::do_newton, src_par/do_newton.cpp:65-430).
+// solver: 0 fp64 LU; 1 fp32 LU + fp64 IR; 2 fp32 LU + fp64 GMRES-IR; 3 bf16-update LU +
+// fp64 GMRES-IR; 4 Gauss-Newton (fp64 J^T J X = J^T F, the reference's GAUSS_NEWTON).
+// Returns 1 when ||F||_inf < precision at entry (converged; no step taken), 0 after a step,
+// < 0 on error, KAD_ERR_SINGULAR (-5) when the factorisation meets an exact zero pivot.
+// times[5] (may be NULL): residual, Jacobian fill, factor+solve, update, factor alone (seconds).
+int kad_xcts_do_newton(kad_system* s, double precision, double* error, int solver, double* times) {
+  if (!s || !error || solver < 0 || solver > 4) return KAD_ERR_ARG;
+  auto* S = reinterpret_cast<kad_system_impl*>(s);
+  hipStream_t st = S->ctx->stream;
+  int rc;
+  double t0 = now_s();
+  double err = 0;
+  if ((rc = newton_residual(S, &err))) return rc;
+  *error = err;
+  double t1 = now_s();
+  if (times) times[0] = t1 - t0;
+  reset_solve_record(S);
+  if (err < precision) return 1;  // false for a NaN error: a NaN residual is never converged
+  if ((rc = newton_direction(S, solver, err, times, t1))) return rc;
   double t3 = now_s();
-  if (times) times[2] = t3 - t2;
   if ((rc = S->update(S->X, -1.0))) return rc;
   KAD_HIP_CHECK(hipStreamSynchronize(st));
   if (times) times[3] = now_s() - t3;
+  return 0;
+}
+
+// Newton iteration with a backtracking line search
+// (System_of_eqs::do_newton_with_linesearch, src_par/do_newton.cpp:454-517, after
+// Numerical Recipes 2007 section 9.7.1).  f = sum F^2 (compute_f, :594-599), slope =
+// -2 f_old; the Newton step p (J p = F, by `solver`) is scaled by stepmax / max(p) when
+// its (signed) maximum exceeds stepmax; u(lambda) = u_old - lambda p.  lambda starts at
+// 1 and is taken from the quadratic model, then the cubic one, kept in [0.1, 0.5] of the
+// previous lambda, until f <= 5e-13, f <= f_old + 1e-4 lambda slope, lambda <= 1e-10, or
+// ntrymax tries (the last computed lambda is then applied).
+// Returns 1 when ||F||_inf < precision at entry, 0 after a step; KAD_ERR_NOCONV where the
+// reference aborts on roundoff ("roundoff error in do_newton_with_linesearch": a
+// non-negative slope or a negative cubic discriminant).  lambda_out, ntry_out (may be
+// NULL): the lambda applied and the number of residual evaluations of the search.
+int kad_system_do_newton_linesearch(kad_system* s, double precision, double* error, int ntrymax, double stepmax,
+                                    int solver, double* lambda_out, int* ntry_out) {
+  if (!s || !error || ntrymax < 1 || !(stepmax > 0.0) || solver < 0 || solver > 4) return KAD_ERR_ARG;
+  auto* S = reinterpret_cast<kad_system_impl*>(s);
+  SystemBase& sy = *S->sys;
+  hipStream_t st = S->ctx->stream;
+  int rc;
+  double err = 0;
+  if ((rc = newton_residual(S, &err))) return rc;
+  *error = err;
+  reset_solve_record(S);
+  if (lambda_out) *lambda_out = 0.0;
+  if (ntry_out) *ntry_out = 0;
+  if (err < precision) return 1;
+  auto sumsq_max = [&](const double* x, int n, double out[2]) -> int {
+    sumsq_max_kernel<<<1, 256, 0, st>>>(x, n, S->scratch + 8);
+    KAD_HIP_CHECK(hipMemcpyAsync(out, S->scratch + 8, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    KAD_HIP_CHECK(hipStreamSynchronize(st));
+    return KAD_OK;
+  };
+  double r[2];
+  if ((rc = sumsq_max(S->F, sy.n_rows, r))) return rc;
+  const double fold = r[0], slope = -2.0 * fold;
+  if (!(slope < 0.0)) {
+    fprintf(stderr, "roundoff error in do_newton_with_linesearch\n");
+    return KAD_ERR_NOCONV;
+  }
+  if ((rc = newton_direction(S, solver, err, nullptr, now_s()))) return rc;
+  if ((rc = sumsq_max(S->X, sy.n_unknowns, r))) return rc;
+  const double pmax = r[1];
+  const double pscale = (pmax > stepmax) ? stepmax / pmax : 1.0;
+  // snapshot of the unknowns' storage: field coefficients and double-unknown constants
+  std::vector<std::pair<double*, int>> held;
+  for (auto& L : sy.layout)
+    held.push_back({S->val_arena + (int64_t)sy.field_slot[L[0]][L[1]] * S->stride, sy.doms[L[1]].coef_size()});
+  for (int k = 0; k < sy.ndouble; ++k)
+    for (int d = sy.dmin; d <= sy.dmax; ++d) {
+      const int slot = sy.param_slot(sy.par_names[sy.dvar_par[k]], d);
+      if (slot >= 0) held.push_back({S->val_arena + (int64_t)slot * S->stride, sy.doms[d].conf_size()});
+    }
+  size_t total = 0;
+  for (auto& h : held) total += (size_t)h.second;
+  if (S->held_cap < total) {
+    if ((rc = S->dmalloc((void**)&S->held, total * sizeof(double)))) return rc;
+    S->held_cap = total;
+  }
+  auto copy_held = [&](bool save) -> int {
+    size_t o = 0;
+    for (auto& h : held) {
+      double* a = S->held + o;
+      KAD_HIP_CHECK(hipMemcpyAsync(save ? a : h.first, save ? h.first : a, (size_t)h.second * sizeof(double),
+                                   hipMemcpyDeviceToDevice, st));
+      o += (size_t)h.second;
+    }
+    return KAD_OK;
+  };
+  auto apply = [&](double lam) -> int {
+    int q = copy_held(false);
+    if (q) return q;
+    return S->update(S->X, -lam * pscale);
+  };
+  if ((rc = copy_held(true))) return rc;
+  const double alpha = 1.0e-4, lambdamin = 1.0e-10;
+  double lambda = 1.0, lambda2 = 0.0, f2 = 0.0;
+  int tries = 0;
+  for (int itry = 1; itry <= ntrymax; ++itry) {
+    if ((rc = apply(lambda))) return rc;
+    if ((rc = S->value_pass())) return rc;
+    if ((rc = sumsq_max(S->F, sy.n_rows, r))) return rc;
+    const double f = r[0];
+    tries = itry;
+    if (f <= 5.0e-13) break;
+    if (f <= fold + alpha * lambda * slope || lambda <= lambdamin) break;
+    double lt;
+    if (lambda == 1.0) {
+      lt = -0.5 * slope / (f - fold - slope);
+    } else {
+      const double rhs1 = f - fold - lambda * slope, rhs2 = f2 - fold - lambda2 * slope;
+      const double a = (rhs1 / (lambda * lambda) - rhs2 / (lambda2 * lambda2)) / (lambda - lambda2);
+      const double b = (-lambda2 * rhs1 / (lambda * lambda) + lambda * rhs2 / (lambda2 * lambda2)) / (lambda - lambda2);
+      if (a == 0.0) {
+        lt = -0.5 * slope / b;
+      } else {
+        const double delta = b * b - 3.0 * a * slope;
+        if (delta < 0.0) {
+          fprintf(stderr, "roundoff error in do_newton_with_linesearch\n");
+          return KAD_ERR_NOCONV;
+        }
+        lt = (-b + std::sqrt(delta)) / (3.0 * a);
+      }
+      if (lt > 0.5 * lambda) lt = 0.5 * lambda;
+    }
+    lambda2 = lambda;
+    f2 = f;
+    lambda = std::max(lt, 0.1 * lambda);
+    if (itry == ntrymax)
+      if ((rc = apply(lambda))) return rc;
+  }
+  S->vals_current = false;
+  KAD_HIP_CHECK(hipStreamSynchronize(st));
+  if (lambda_out) *lambda_out = lambda;
+  if (ntry_out) *ntry_out = tries;
   return 0;
 }
 

@@ -293,6 +293,23 @@ class DeviceSystem:
         lib.kad_system_last_solve(self.h, it, ctypes.byref(res))
         return rc == 1, err.value, times, (it[0], it[1], res.value, it[2], it[3])
 
+    def do_newton_with_linesearch(self, prec, ntrymax=10, stepmax=1.0, solver="f64"):
+        """System_of_eqs::do_newton_with_linesearch (src_par/do_newton.cpp:454-517):
+        returns (converged at entry, ||F||_inf at entry, lambda applied, residual
+        evaluations of the search)."""
+        err = ctypes.c_double()
+        lam = ctypes.c_double()
+        ntry = ctypes.c_int()
+        self.ctx._enter()
+        rc = lib.kad_system_do_newton_linesearch(self.h, prec, ctypes.byref(err), int(ntrymax), float(stepmax),
+                                                 SOLVERS[solver], ctypes.byref(lam), ctypes.byref(ntry))
+        if rc == -5:
+            raise RuntimeError("Jacobian is singular! Stopping.")
+        if rc == -4:
+            raise RuntimeError("roundoff error in do_newton_with_linesearch")
+        check(rc, "kad_system_do_newton_linesearch")
+        return rc == 1, err.value, lam.value, ntry.value
+
     def solve(self, precision=1e-10, solver="f64", max_iter=20):
         for it in range(max_iter + 1):
             conv, err, _, _ = self.do_newton(precision, solver)
