@@ -284,6 +284,12 @@ int kad_sysdef_add_eq_bc(kad_sysdef* def, int dom, int bound, const char* eq);  
 int kad_sysdef_add_eq_matching(kad_sysdef* def, int dom, int bound, const char* expr); /* add_eq_matching */
 int kad_sysdef_space_add_eq(kad_sysdef* def, const char* eq, const char* m1, const char* m2);
 int kad_sysdef_space_add_bc(kad_sysdef* def, int bound, const char* eq);  /* add_inner_bc / add_outer_bc */
+/* Domain d is a shell of an adapted space whose boundary spheres are round (the
+ * homothetic shells of Space_adapted_bh, space_adapted_bh.cpp:59-75): its surface
+ * integrals use Domain_shell_inner_adapted::integ's configuration-space weight
+ * (domain_shell_inner_adapted.cpp:1134-1208) instead of Domain_shell::integ's
+ * coefficient-space mult_r (domain_shell.cpp:862-897).  Call before building. */
+int kad_sysdef_set_adapted(kad_sysdef* def, int d);
 /* collocation r, theta, phi of domain d's conf grid (HOST out, nr*nt*np each; r = +inf
  * at infinity); any pointer may be NULL */
 int kad_sysdef_coloc(kad_sysdef* def, int d, double* r, double* theta, double* phi);

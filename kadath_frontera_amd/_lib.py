@@ -94,6 +94,7 @@ SIGNATURES = {
     "kad_sysdef_add_eq_matching": (c_int, [c_vp, c_int, c_int, c_char_p]),
     "kad_sysdef_space_add_eq": (c_int, [c_vp, c_char_p, c_char_p, c_char_p]),
     "kad_sysdef_space_add_bc": (c_int, [c_vp, c_int, c_char_p]),
+    "kad_sysdef_set_adapted": (c_int, [c_vp, c_int]),
     "kad_sysdef_coloc": (c_int, [c_vp, c_int, c_dp, c_dp, c_dp]),
     "kad_sysdef_host_trace": (c_int, [c_vp, c_int, c_ip, c_ip, c_int]),
     "kad_system_build": (c_int, [c_vp, c_vp, c_int, c_int, ctypes.POINTER(c_vp)]),

@@ -38,6 +38,10 @@ class Domain {
  public:
   int num = 0;
   bool compact = false;
+  // a shell of an adapted space (Domain_shell_inner_adapted / _outer_adapted, e.g. the
+  // homothetic shells of Space_adapted_bh) whose boundary spheres are round: its
+  // surface integrals weight in configuration space (domain_shell_inner_adapted.cpp:1134-1208)
+  bool adapted = false;
   int nr = 0, nt = 0, np = 0, npc = 0;
   double alpha = 0, beta = 0;
   std::vector<double> xr, th, ph;  // collocation points

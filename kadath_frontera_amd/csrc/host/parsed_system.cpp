@@ -554,6 +554,27 @@ ParsedSystem::Tv ParsedSystem::eval(const std::string& s0, Ctx& cx) {
     if (a.c[0].glob) throw std::runtime_error("integ of an integral");
     const TE t = as_field(a.c[0], cx);
     auto rr = [&](const VDp& v) { return tr.mult_r(tr.mult_r(tr.mult_sin_theta(v))); };
+    if (doms[cx.d].adapted) {
+      // adapted shell: so times sqrt(det q) = r^2 sin(theta) (round boundary), formed in
+      // configuration space and given the base of sin(theta) (one.mult_sin_theta()),
+      // then the same theta weights (domain_shell_inner_adapted.cpp:1134-1208)
+      const Domain& dm = doms[cx.d];
+      Base wb = dm.std_base();
+      for (int k = 0; k < dm.npc; ++k) (void)op_matrix(mult_sin_1d, wb.bt[k], dm.nt, &wb.bt[k]);
+      std::vector<double> w(dm.conf_size());
+      for (int k = 0; k < dm.np; ++k)
+        for (int j = 0; j < dm.nt; ++j)
+          for (int i = 0; i < dm.nr; ++i) {
+            const size_t q = (size_t)i + dm.nr * (j + (size_t)dm.nt * k);
+            w[q] = dm.R[q] * dm.R[q] * std::sin(dm.th[j]);
+          }
+      VDp wn = const_field("integ-weight:" + std::to_string(cx.d), cx.d, w, wb);
+      Tv r;
+      r.c.resize(1);
+      r.c[0].glob = true;
+      r.c[0].gl.push_back(GlTerm{tr.mul(wn, t.val), tr.mul(wn, t.der), cx.d, cx.bound, 1.0});
+      return r;
+    }
     Tv r;
     r.c.resize(1);
     r.c[0].glob = true;
@@ -982,10 +1003,9 @@ void ParsedSystem::build() {
     tau_maps.push_back(m);
     off += cnt;
   }
+  // A system need not be square to be evaluated (sec_member, check_equations, the
+  // Jacobian's columns); the solvers check it (do_newton.cpp:113-118).
   n_unknowns = off;
-  if (n_unknowns != n_rows)
-    throw std::runtime_error("Number of unknowns (" + std::to_string(n_unknowns) +
-                             ") is different from the number of equations (" + std::to_string(n_rows) + ")");
   for (auto& L : layout) trace_tangent(L[0], L[1]);
   // parameter programs
   par_prog.assign(par_names.size(), Program());

@@ -1002,6 +1002,15 @@ int kad_xcts_observables(kad_system* s, double* out) {
   return KAD_OK;
 }
 
+// The solvers need as many equations as unknowns (System_of_eqs::check_size_VS_unknowns,
+// do_newton.cpp:113-118 and :600-609, which abort).
+static int check_square(const SystemBase& sy) {
+  if (sy.n_rows == sy.n_unknowns) return KAD_OK;
+  fprintf(stderr, "Number of unknowns is different from the number of equations\nnbr equations = %d\n"
+                  "nbr unknowns  = %d\n", sy.n_rows, sy.n_unknowns);
+  return KAD_ERR_ARG;
+}
+
 // Set a parameter of the system: which indexes its parameters (XCTS: 0 = Ome, the
 // horizon rotation of kerr.cpp:91-95).  The constant field is rewritten in every
 // domain that holds it.
@@ -1038,6 +1047,7 @@ static int sensitivity_impl(kad_system_impl* S, int par, double* out, double* dQ
   SystemBase& sy = *S->sys;
   hipStream_t st = S->ctx->stream;
   const int n = sy.n_unknowns;
+  if (check_square(sy)) return KAD_ERR_ARG;
   const int nq = (int)sy.obs.size();
   if (nq == 0 || par < 0 || par >= (int)S->par_progs.size()) return KAD_ERR_ARG;
   int rc;
@@ -1105,6 +1115,7 @@ static int glob_sensitivity_impl(kad_system_impl* S, int par, double* out, doubl
   SystemBase& sy = *S->sys;
   hipStream_t st = S->ctx->stream;
   const int n = sy.n_unknowns;
+  if (check_square(sy)) return KAD_ERR_ARG;
   const int nq = (int)sy.glob_names.size();
   if (nq == 0 || nq > 16 || par < 0 || par >= (int)S->par_progs.size()) return KAD_ERR_ARG;
   int rc;
@@ -1183,6 +1194,7 @@ static int newton_direction(kad_system_impl* S, int solver, double err, double* 
   kad_ctx* cx = reinterpret_cast<kad_ctx*>(S->ctx);
   const int n = sy.n_unknowns;
   int rc;
+  if ((rc = check_square(sy))) return rc;
   if (!S->J) {
     if ((rc = S->dmalloc((void**)&S->J, (size_t)n * n * sizeof(double)))) return rc;
     if ((rc = S->dmalloc((void**)&S->ipiv, (size_t)n * sizeof(int)))) return rc;
@@ -1663,6 +1675,17 @@ int kad_sysdef_space_add_bc(kad_sysdef* def, int bound, const char* eq) {
 
 // Collocation points of the conf grid of domain d (r fastest, then theta, then phi):
 // r (compactified domain: infinity at x = 1 is reported as +inf), theta, phi.  HOST out.
+// Mark domain d as a shell of an adapted space with round boundaries (the homothetic
+// shells of Space_adapted_bh): its surface integrals weight in configuration space
+// (Domain_shell_inner_adapted::integ, domain_shell_inner_adapted.cpp:1134-1208).
+int kad_sysdef_set_adapted(kad_sysdef* def, int d) {
+  return guarded(def, [&](ParsedSystem& p) {
+    if (d < p.dmin || d > p.dmax) throw std::runtime_error("kad_sysdef_set_adapted: domain out of range");
+    if (p.doms[d].compact) throw std::runtime_error("kad_sysdef_set_adapted: the compactified domain is not adapted");
+    p.doms[d].adapted = true;
+  });
+}
+
 int kad_sysdef_coloc(kad_sysdef* def, int d, double* r, double* theta, double* phi) {
   return guarded(def, [&](ParsedSystem& p) {
     if (d < p.dmin || d > p.dmax) throw std::runtime_error("kad_sysdef_coloc: domain out of range");

@@ -119,6 +119,11 @@ class SystemDef:
     def space_add_outer_bc(self, eq):
         check(lib.kad_sysdef_space_add_bc(self.h, OUTER_BC, _b(eq)), f"add_outer_bc({eq})")
 
+    def set_adapted(self, d):
+        """Domain d is a round shell of an adapted space (Space_adapted_bh's homothetic
+        shells): its surface integrals weight in configuration space."""
+        check(lib.kad_sysdef_set_adapted(self.h, int(d)), f"set_adapted({d})")
+
     def coloc(self, d):
         """(r, theta, phi) at the collocation points of domain d, each (np, nt, nr)."""
         n = self.res[0] * self.res[1] * self.res[2]
