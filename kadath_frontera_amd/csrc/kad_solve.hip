@@ -424,9 +424,10 @@ int kad_cast_f64_to_f32(kad_ctx* ctx, int n, const double* A, int64_t lda, float
 // Mixed-precision refinement: x = A^{-1} b with A given in fp64 (A64, for the
 // residual) and factored in prec_lo (LUlo, perm).  mode 0: classical IR;
 // mode 1: GMRES-IR (right preconditioned by the low-precision factors, CGS2,
-// restart m).  Stops when ||b - A x||_inf <= tol * ||b||_inf, or at the attainable
-// accuracy: an outer step that gains less than a factor 2 ends the loop, and one
-// that increases the residual is undone.  trans = 1 solves A^T x = b.
+// restart m = kad_ir_set_restart, default 100).  Stops when
+// ||b - A x||_inf <= tol * ||b||_inf, or at the attainable accuracy: an outer step
+// that gains less than a factor 2 ends the loop, and one that increases the residual
+// is undone.  trans = 1 solves A^T x = b.
 // iters[0] = outer steps, iters[1] = total inner GMRES steps; resid = final relative residual.
 int kad_ir_solve_scaled(kad_ctx* ctx, int n, const double* A64, int64_t lda, int prec_lo, const void* LUlo,
                         int64_t ldlu, const int* perm, const double* Rs, const double* Cs, const double* b, double* x,
@@ -590,7 +591,7 @@ int kad_ir_solve_scaled(kad_ctx* ctx, int n, const double* A64, int64_t lda, int
   return (rel <= tol) ? KAD_OK : KAD_ERR_NOCONV;
 }
 
-// GMRES-IR restart length of the context (default 60).
+// GMRES-IR Krylov dimension per restart of the context (default 100).
 int kad_ir_set_restart(kad_ctx* ctx, int m) {
   if (!ctx || m < 2 || m > 4096) return KAD_ERR_ARG;
   reinterpret_cast<Ctx*>(ctx)->gmres_m = m;
