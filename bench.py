@@ -2,13 +2,16 @@
 
 Workload (the spinning-BH configuration on one GPU): XCTS black-hole system on a
 Space_spheric with 5 solved domains (4 shells + compactified) at 17x13x12
-collocation points, n = 58905 unknowns, dense Jacobian 27.8 GB in FP64; seeded
-1e-3 perturbation of the flat guess; horizon rotation Ome = 0.05.
+collocation points, n = 58905 unknowns, dense Jacobian 27.8 GB in FP64.  The horizon
+rotation Ome = 0.1421 makes a hole of dimensionless spin chi = J_ADM / M_ADM^2 = 0.600
+at the solution (the bench reports the value it reaches).  Guess: the converged hole
+at Ome = 0.10 (reached from the flat guess through Ome = 0.05, before any timing)
+plus the seeded 1e-3 perturbation of every coefficient.
 
 One "step" = one System_of_eqs::do_newton iteration: residual F(u) + batched-JVP
 Jacobian fill + factorisation + solve (+ fp64 refinement) + field update, all
 device-resident.  The K timed steps are the iterations of the Newton solve from
-the perturbed guess (restarting from the guess once ||F||_inf < 1e-10).  The
+the guess (restarting from the guess once ||F||_inf < 1e-10).  The
 system is assembled in domain-major order (block-tridiagonal J) and the FP64 LU
 skips the structurally zero tile pairs.  The headline solver is FP64 (the one
 that reaches 1e-10 here); FP32-LU + FP64 GMRES-IR runs beside it to the same
@@ -28,7 +31,8 @@ sys.path.insert(0, ROOT)
 RES = (17, 13, 12)
 BOUNDS = [1.0, 2.0, 4.0, 8.0, 16.0]
 CFG1_RES, CFG1_BOUNDS = (9, 7, 4), [1.0, 2.0, 4.0]  # the CPU-runnable configuration
-N0, A, OME = 0.5, 1.0, 0.05
+N0, A, OME = 0.5, 1.0, 0.1421  # chi = J_ADM / M_ADM^2 = 0.600 at the solution
+OME_PATH = (0.05, 0.10)         # continuation from the flat guess to the guess's hole
 SEED = 20261202
 PEAK = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0}  # MI355X dense matrix TFLOP/s (vendor figures)
 # HBM bytes per launch, from two rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE),
@@ -43,8 +47,8 @@ PEAK = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0}  # MI355X dense matrix TFLOP/
 #     iteration against 8256 now).
 APPLY3D_TRAFFIC_RATIO = (2.0 * 83618276.7 + 166635597.5) * 1024.0 / 340514582928.0
 # gemm_k4: the counter pass over the current LU ended in a segfault of the profiler's host
-# counter path and was not repeated on that build, so the bench reports no
-# measured traffic for it; traffic_model_bytes is the kernel's operand traffic per launch
+# counter path and was not repeated on that build, so the bench reports no measured traffic
+# for it; traffic_model_bytes is the kernel's operand traffic per launch
 # (each executed 128 x 128 C tile reads its 128 x k A and k x 128 B panels, reads and writes
 # C), an upper bound on HBM bytes when nothing is reused through L2.
 GEMM_TRAFFIC_NOTE = ("traffic is not measured for gemm_k4 on this build (the --pmc pass crashed in the "
@@ -182,8 +186,30 @@ def main():
     def log(msg):
         print("[bench] " + msg, file=sys.stderr, flush=True)
 
+    # the guess's hole: Newton continuation in Ome from the flat guess (untimed)
+    sysd.set_guess(None)
+    for om in OME_PATH:
+        sysd.set_ome(om)
+        for it in range(30):
+            conv, err, _, _ = sysd.do_newton(prec, "f64")
+            if conv:
+                break
+        if not conv:
+            raise RuntimeError(f"continuation at Ome = {om} did not converge (|F| = {err:.3e})")
+        log(f"guess hole at Ome = {om}: {it} iterations")
+    guess = {(f, d): sysd.coef(f, d).copy() for f in range(5) for d in range(1, len(BOUNDS) + 1)}
+    sysd.set_ome(OME)
+    X0c = np.ascontiguousarray(X0, dtype=np.float64)
+
+    def set_guess():
+        for (f, d), cf in guess.items():
+            sysd.set_coef(f, d, cf)
+        rc = lib.kad_system_add_unknowns(sysd.h, X0c.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        if rc:
+            raise RuntimeError(f"kad_system_add_unknowns failed with status {rc}")
+
     if args.fill_only:
-        sysd.set_guess(X0)
+        set_guess()
         lib.kad_kernel_timing(ctx.h, 1, 1)
         sysd.residual()
         sysd.jacobian()
@@ -210,7 +236,7 @@ def main():
         """Newton from the perturbed guess until ||F||_inf < 1e-10 (the time to solution),
         then the fields and global quantities there, and again after one more step of the
         same solver ("polished": ||F|| at the floor of fp64 residual evaluation)."""
-        sysd.set_guess(X0)
+        set_guess()
         recs, t_total, conv = [], 0.0, False
         for it in range(args.max_newton + 1):
             conv, err, dt, times, ir = newton_iter(solver)
@@ -234,18 +260,18 @@ def main():
 
     # warmup: first iteration(s) from the guess (allocations, kernel loads)
     for _ in range(args.warmup):
-        sysd.set_guess(X0)
+        set_guess()
         newton_iter(args.solver)
     # timed: K Newton iterations along the converging sequence (restarting from the
     # guess when it has converged); every timed call is a full iteration
-    sysd.set_guess(X0)
+    set_guess()
     step_times, step_phases, step_ir = [], [], []
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     while len(step_times) < args.steps:
         conv, err, dt, times, ir = newton_iter(args.solver)
         if conv:
-            sysd.set_guess(X0)
+            set_guess()
             continue
         step_times.append(dt)
         step_phases.append(list(times))
@@ -260,7 +286,7 @@ def main():
     # live roofline of the two kernel classes (HIP events on the launch stream) over one
     # iteration from the guess: kind 0 MFMA GEMMs (flops), kind 1 fused 3-D spectral
     # passes of the residual / Jacobian programs (algorithmic HBM bytes)
-    sysd.set_guess(X0)
+    set_guess()
     lib.kad_kernel_timing(ctx.h, 0, 1)
     lib.kad_kernel_timing(ctx.h, 1, 1)
     newton_iter(args.solver)
@@ -315,6 +341,7 @@ def main():
                 "J_adm_abs_polished": abs(v["q_pol"]["J_adm"] - ref["q_pol"]["J_adm"])}
         conv_info[k] = info
     conv_info[ref_key]["observables_at_stop"] = ref["q_stop"]
+    chi = ref["q_stop"]["J_adm"] / ref["q_stop"]["M_adm"] ** 2
     # the CPU-runnable configuration on the GPU (one full Newton step), beside the CPU's
     s1 = XctsSystem(ctx, CFG1_RES, CFG1_BOUNDS, n0=N0, a=A, ome=0.0, batch=args.batch, order=args.order)
     s1.set_guess(s1.perturbation(SEED))
@@ -341,10 +368,11 @@ def main():
         "vs_baseline": None,
         "dtype": {"f64": "f64", "f32-ir": "f32-lu+f64-ir", "f32-gmres": "f32-lu+f64-gmres-ir",
                   "bf16-gmres": "bf16-lu+f64-gmres-ir"}[args.solver],
-        "data": "synthetic (seeded 1e-3 perturbation of the flat guess)",
+        "data": "synthetic (the converged Ome = 0.10 hole plus a seeded 1e-3 perturbation)",
         "config": {"workload": "spinning-BH XCTS Newton iteration, 5 spherical domains 17x13x12, n=%d" % n,
                    "n_unknowns": n, "solver": args.solver, "jvp_batch": args.batch,
-                   "order": ["reference", "domain-major"][args.order]},
+                   "order": ["reference", "domain-major"][args.order],
+                   "horizon_rotation": OME, "chi_at_solution": chi},
         "lu_tflops": lu_tflops,
         "lu_tflops_note": "GEMM flops executed in one FP64 factorisation (structurally zero tile pairs skipped, "
                           "not counted) / the factorisation's wall time",
