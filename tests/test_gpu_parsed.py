@@ -306,3 +306,18 @@ def test_spin_sensitivity_matches_central_differences(ctx):
         assert abs(adj[k] - rich) <= 1e-6 * max(1.0, abs(rich)), k
     assert abs(adj["J_adm"]) > 1.0  # the ADM angular momentum follows the horizon spin
     s.close()
+
+
+def test_nan_residual_never_reads_as_converged(ctx):
+    """A NaN in the fields makes ||F||_inf NaN, which must not pass the convergence test
+    (do_newton.cpp:71-107 returns true only for error < precision; the device norm
+    propagates NaN instead of dropping it as fmax would)."""
+    sd, rad = programs.tuto5(ctx)
+    s = sd.build(order=0, batch=256)
+    cf = s.coef(0, 2)
+    cf.flat[3] = np.nan
+    s.set_coef(0, 2, cf)
+    print("nan residual: tuto5 with one NaN coefficient", flush=True)
+    conv, err, _, _ = s.do_newton(1e300, "f64")
+    assert not conv and np.isnan(err)
+    s.close()
