@@ -13,9 +13,10 @@ Jacobian fill + factorisation + solve (+ fp64 refinement) + field update, all
 device-resident.  The K timed steps are the iterations of the Newton solve from
 the guess (restarting from the guess once ||F||_inf < 1e-10).  The
 system is assembled in domain-major order (block-tridiagonal J) and the FP64 LU
-skips the structurally zero tile pairs.  The headline solver is FP64 (the one
-that reaches 1e-10 here); FP32-LU + FP64 GMRES-IR runs beside it to the same
-test, with time to solution and the converged fields compared.
+skips the structurally zero tile pairs.  The headline solver is FP64 with a factorisation every step (the reference's
+algorithm); FP32-LU and BF16-LU + FP64 GMRES-IR, and FP64 factors of an earlier
+Jacobian + FP64 GMRES ("f64-reuse"), run beside it to the same test, with time to
+solution and the converged fields compared.
 
 Prints ONE JSON line (rank 0).
 """
@@ -144,8 +145,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=6, help="timed Newton iterations")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--solver", default="f64", choices=["f64", "f32-ir", "f32-gmres", "bf16-gmres"])
-    ap.add_argument("--compare", default="f32-gmres,bf16-gmres",
+    ap.add_argument("--solver", default="f64", choices=["f64", "f32-ir", "f32-gmres", "bf16-gmres", "f64-reuse"])
+    ap.add_argument("--compare", default="f32-gmres,bf16-gmres,f64-reuse",
                     help="comma-separated solvers run to ||F|| < 1e-10 beside the headline one (or 'none')")
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--order", type=int, default=1, choices=[0, 1],
@@ -180,7 +181,7 @@ def main():
     sysd = XctsSystem(ctx, RES, BOUNDS, n0=N0, a=A, ome=OME, batch=args.batch, order=args.order)
     n = sysd.n
     X0 = sysd.perturbation(SEED)
-    prec_of = {"f64": "f64", "f32-ir": "f32", "f32-gmres": "f32", "bf16-gmres": "bf16"}
+    prec_of = {"f64": "f64", "f32-ir": "f32", "f32-gmres": "f32", "bf16-gmres": "bf16", "f64-reuse": "f64"}
     prec = 1e-10
 
     def log(msg):
@@ -331,6 +332,7 @@ def main():
             info["gmres_inner_steps"] = [r[3][1] for r in recs]
             info["refinement_rel_resid"] = [r[3][2] for r in recs]
             info["fp64_fallback_steps"] = int(sum(r[3][3] for r in recs))
+            info["factorizations"] = int(sum(r[3][4] for r in recs))
         if k != ref_key:
             info["vs_" + ref_key] = {
                 "fields_max_rel_at_stop": float(np.abs(v["f_stop"] - ref["f_stop"]).max() / sc),
@@ -367,7 +369,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"f64": "f64", "f32-ir": "f32-lu+f64-ir", "f32-gmres": "f32-lu+f64-gmres-ir",
-                  "bf16-gmres": "bf16-lu+f64-gmres-ir"}[args.solver],
+                  "bf16-gmres": "bf16-lu+f64-gmres-ir", "f64-reuse": "f64"}[args.solver],
         "data": "synthetic (the converged Ome = 0.10 hole plus a seeded 1e-3 perturbation)",
         "config": {"workload": "spinning-BH XCTS Newton iteration, 5 spherical domains 17x13x12, n=%d" % n,
                    "n_unknowns": n, "solver": args.solver, "jvp_batch": args.batch,

@@ -205,7 +205,10 @@ int kad_xcts_sensitivity(kad_system* s, double* out, double* dQdu);
 /* One Newton iteration.  solver: 0 fp64 LU, 1 fp32 LU + fp64 IR, 2 fp32 LU +
  * GMRES-IR, 3 bf16-update LU + GMRES-IR, 4 Gauss-Newton (J^T J X = J^T F in fp64,
  * the reference's SOLVER GAUSS_NEWTON, do_newton.cpp:264-311; it holds J^T and J^T J
- * beside J, 3 n^2 doubles).  Returns 1 if ||F||_inf < precision at
+ * beside J, 3 n^2 doubles), 5 fp64 LU factors of an earlier J as the preconditioner of
+ * fp64 flexible GMRES on the current J, refactored (and solved directly) when the
+ * previous refinement took more than the reuse limit (kad_system_set_option 1, default
+ * 30) inner steps or did not converge; it holds two n^2 fp64 buffers.  Returns 1 if ||F||_inf < precision at
  * entry (no step), 0 after a step, KAD_ERR_SINGULAR on an exactly singular J.
  * times[5]: residual, Jacobian fill, factor + solve (incl. refinement), update,
  * factorisation alone (seconds). */
@@ -329,7 +332,7 @@ int kad_system_sensitivity(kad_system* s, int which, double* out, double* dQdu);
 /* number of double unknowns; the current value of double unknown k (HOST out) */
 int kad_system_ndouble(kad_system* s);
 int kad_system_get_double(kad_system* s, int k, double* value);
-/* Options of the low-precision solvers (1-3) of do_newton: which 0 = forcing factor
+/* Options of the iterative solvers (1-3, 5) of do_newton: which 0 = forcing factor
  * c >= 0, the refinement's target becomes max(1e-13, c min(1, ||F||_inf)) relative
  * (0, the default: 1e-13, i.e. its attainable accuracy); which 1 = reuse the
  * low-precision factors of an earlier step while the previous refinement took at most

@@ -43,7 +43,14 @@ enum { CHEB_TYPE = 1 };
 // do_newton's solver argument: the reference's SOLVER (NEWTON_RAPHSON, GAUSS_NEWTON,
 // system_of_eqs.hpp:134) plus the mixed-precision solvers of this build; the values are
 // the C-ABI's solver codes
-enum SOLVER { NEWTON_RAPHSON = 0, NEWTON_F32_IR = 1, NEWTON_F32_GMRES = 2, NEWTON_BF16_GMRES = 3, GAUSS_NEWTON = 4 };
+enum SOLVER {
+  NEWTON_RAPHSON = 0,
+  NEWTON_F32_IR = 1,
+  NEWTON_F32_GMRES = 2,
+  NEWTON_BF16_GMRES = 3,
+  GAUSS_NEWTON = 4,
+  NEWTON_F64_REUSE = 5  // fp64 factors of an earlier Jacobian + fp64 flexible GMRES
+};
 
 namespace detail {
 inline void check(int rc, const char* what) {

@@ -405,6 +405,8 @@ struct kad_system_impl {
   bool lo_valid = false;  // Jlo / perm / Rs / Cs hold usable low-precision factors
   int lo_prec = -1;
   int prev_inner = 0;     // inner steps of the previous refinement
+  double* Jfac = nullptr;  // solver 5: fp64 factors of an earlier Jacobian (J and Jfac alternate)
+  bool fac_valid = false;  // Jfac / perm hold usable fp64 factors
 
   ~kad_system_impl() {
     for (void* p : allocs) (void)hipFree(p);
@@ -1203,6 +1205,8 @@ static int newton_direction(kad_system_impl* S, int solver, double err, double* 
   }
   if ((solver == 1 || solver == 2 || solver == 3) && !S->Jlo)
     if ((rc = S->dmalloc((void**)&S->Jlo, (size_t)n * n * sizeof(float)))) return rc;
+  if (solver == 5 && !S->Jfac)
+    if ((rc = S->dmalloc((void**)&S->Jfac, (size_t)n * n * sizeof(double)))) return rc;
   if (solver == 4 && !S->Jt) {
     if ((rc = S->dmalloc((void**)&S->Jt, (size_t)n * n * sizeof(double)))) return rc;
     if ((rc = S->dmalloc((void**)&S->JtJ, (size_t)n * n * sizeof(double)))) return rc;
@@ -1222,8 +1226,48 @@ static int newton_direction(kad_system_impl* S, int solver, double err, double* 
   if (times) times[1] = t2 - t1;
   int info = 0;
   double tf0 = 0, tf1 = 0;
-  if (solver == 0 || solver == 4) {
+  // The refinement target of the iterative solvers.  The default runs it to its
+  // attainable accuracy (the tau system's near-null modes, high radial orders in the
+  // compactified domain, are barely seen by a residual test).  With a forcing factor c
+  // the target follows ||F||: c min(1, ||F||) relative, which keeps Newton quadratic
+  // while early iterations need only a few inner steps.
+  const double eta = S->forcing > 0.0 ? std::max(1e-13, S->forcing * std::min(1.0, err)) : 1e-13;
+  // A refinement result is a usable Newton direction below this relative residual.
+  constexpr double kAcceptResid = 1e-6;
+  if (solver == 5) {
+    // fp64 factors of an earlier J precondition flexible GMRES on the current J while
+    // the previous refinement took at most `limit` inner steps; otherwise the current J
+    // is factored in place and solved directly, and its factors are kept (the next fill
+    // goes to the other buffer)
+    S->lo_valid = false;
+    const int limit = S->reuse_inner > 0 ? S->reuse_inner : 30;
+    bool solved = false;
+    if (S->fac_valid && S->prev_inner <= limit) {
+      rc = kad_ir_solve_scaled(cx, n, S->J, n, KAD_PREC_F64, S->Jfac, n, S->perm, nullptr, nullptr, S->F, S->X,
+                               eta, 8, 1, 0, S->last_iters, &S->last_resid);
+      if (rc < 0 && rc != KAD_ERR_NOCONV) return rc;
+      S->prev_inner = S->last_iters[1];
+      solved = rc == KAD_OK || S->last_resid <= std::max(eta, kAcceptResid);
+      S->last_factored = 0;
+    }
+    if (!solved) {
+      KAD_HIP_CHECK(hipStreamSynchronize(st));
+      const double tg0 = now_s();
+      if ((rc = kad_lu_factor(cx, KAD_PREC_F64, n, S->J, n, S->ipiv, S->perm, 0)) < 0) return rc;
+      tf1 += now_s() - tg0;
+      info = rc;
+      if (info == 0) {
+        KAD_HIP_CHECK(hipMemcpyAsync(S->X, S->F, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        if ((rc = kad_lu_solve(cx, KAD_PREC_F64, n, S->J, n, S->perm, S->X, 0))) return rc;
+      }
+      std::swap(S->J, S->Jfac);
+      S->fac_valid = (info == 0);
+      S->prev_inner = 0;
+      S->last_factored = 1;
+    }
+  } else if (solver == 0 || solver == 4) {
     S->lo_valid = false;  // the fp64 factorisation takes over ipiv / perm
+    S->fac_valid = false;
     S->last_factored = 1;
     double* M = S->J;
     if (solver == 4) {
@@ -1247,6 +1291,7 @@ static int newton_direction(kad_system_impl* S, int solver, double err, double* 
       if ((rc = kad_lu_solve(cx, KAD_PREC_F64, n, M, n, S->perm, S->X, 0))) return rc;
     }
   } else {
+    S->fac_valid = false;  // the low-precision factorisation takes over ipiv / perm
     // low-precision factors of the equilibrated R J C (exact powers of two); the
     // refinement keeps the unscaled fp64 J, so its stopping test is on ||F - J X||.
     if (!S->Rs) {
@@ -1257,12 +1302,6 @@ static int newton_direction(kad_system_impl* S, int solver, double err, double* 
     // The factors of an earlier J precondition the refinement on the current one as
     // long as it keeps converging quickly; J itself (the residuals) is always current.
     bool reuse = S->reuse_inner > 0 && S->lo_valid && S->lo_prec == prec && S->prev_inner <= S->reuse_inner;
-    // Target of the refinement.  The default runs it to its attainable accuracy (the tau
-    // system's near-null modes, high radial orders in the compactified domain, are
-    // barely seen by a residual test).  With a forcing factor c the target follows
-    // ||F||: c min(1, ||F||) relative, which keeps Newton quadratic while early
-    // iterations need only a few inner steps.
-    const double eta = S->forcing > 0.0 ? std::max(1e-13, S->forcing * std::min(1.0, err)) : 1e-13;
     for (int attempt = 0; attempt < 2; ++attempt) {
       if (!reuse) {
         if ((rc = kad_cast_equilibrated(cx, n, S->J, n, S->Jlo, n, S->Rs, S->Cs))) return rc;
@@ -1293,7 +1332,6 @@ static int newton_direction(kad_system_impl* S, int solver, double err, double* 
       // that made no progress leaves X = 0): the step is then solved with an fp64
       // factorisation of the same (unscaled, untouched) J, as the reference's pdgesv_
       // would, and the fallback is reported through kad_xcts_last_solve.
-      constexpr double kAcceptResid = 1e-6;
       if (rc == KAD_ERR_NOCONV && !(S->last_resid <= std::max(eta, kAcceptResid))) {
         S->last_fallback = 1;
         S->lo_valid = false;  // the fp64 factorisation below takes over ipiv / perm
@@ -1309,7 +1347,7 @@ static int newton_direction(kad_system_impl* S, int solver, double err, double* 
       }
     }
   }
-  if (times) times[4] = (solver == 0 || solver == 4) ? tf1 - tf0 : tf1;
+  if (times) times[4] = (solver == 0 || solver == 4) ? tf1 - tf0 : tf1;  // solvers 1-3, 5 accumulate in tf1
   if (info != 0) {
     fprintf(stderr, "Jacobian is singular! Stopping.\n");
     return KAD_ERR_SINGULAR;
@@ -1328,12 +1366,14 @@ static void reset_solve_record(kad_system_impl* S) {
 
 // One Newton-Raphson iteration (System_of_eqs::do_newton, src_par/do_newton.cpp:65-430).
 // solver: 0 fp64 LU; 1 fp32 LU + fp64 IR; 2 fp32 LU + fp64 GMRES-IR; 3 bf16-update LU +
-// fp64 GMRES-IR; 4 Gauss-Newton (fp64 J^T J X = J^T F, the reference's GAUSS_NEWTON).
+// fp64 GMRES-IR; 4 Gauss-Newton (fp64 J^T J X = J^T F, the reference's GAUSS_NEWTON);
+// 5 fp64 LU of an earlier J (refactored when it stops preconditioning well) + fp64
+// flexible GMRES on the current J.
 // Returns 1 when ||F||_inf < precision at entry (converged; no step taken), 0 after a step,
 // < 0 on error, KAD_ERR_SINGULAR (-5) when the factorisation meets an exact zero pivot.
 // times[5] (may be NULL): residual, Jacobian fill, factor+solve, update, factor alone (seconds).
 int kad_xcts_do_newton(kad_system* s, double precision, double* error, int solver, double* times) {
-  if (!s || !error || solver < 0 || solver > 4) return KAD_ERR_ARG;
+  if (!s || !error || solver < 0 || solver > 5) return KAD_ERR_ARG;
   auto* S = reinterpret_cast<kad_system_impl*>(s);
   hipStream_t st = S->ctx->stream;
   int rc;
@@ -1367,7 +1407,7 @@ int kad_xcts_do_newton(kad_system* s, double precision, double* error, int solve
 // NULL): the lambda applied and the number of residual evaluations of the search.
 int kad_system_do_newton_linesearch(kad_system* s, double precision, double* error, int ntrymax, double stepmax,
                                     int solver, double* lambda_out, int* ntry_out) {
-  if (!s || !error || ntrymax < 1 || !(stepmax > 0.0) || solver < 0 || solver > 4) return KAD_ERR_ARG;
+  if (!s || !error || ntrymax < 1 || !(stepmax > 0.0) || solver < 0 || solver > 5) return KAD_ERR_ARG;
   auto* S = reinterpret_cast<kad_system_impl*>(s);
   SystemBase& sy = *S->sys;
   hipStream_t st = S->ctx->stream;

@@ -11,7 +11,7 @@ import torch
 from ._lib import lib, check, c_vp
 
 FIELDS = ("P", "N", "bx", "by", "bz")
-SOLVERS = {"f64": 0, "f32-ir": 1, "f32-gmres": 2, "bf16-gmres": 3, "gauss-newton": 4}
+SOLVERS = {"f64": 0, "f32-ir": 1, "f32-gmres": 2, "bf16-gmres": 3, "gauss-newton": 4, "f64-reuse": 5}
 
 
 def _dp(a):

@@ -105,3 +105,35 @@ def test_gauss_newton_reaches_the_newton_solution(ctx):
     assert worst <= 1e-10
     gn.close()
     nr.close()
+
+
+def test_reused_f64_factors_reach_the_newton_solution(ctx):
+    """Solver 5: the fp64 factors of an earlier Jacobian precondition flexible GMRES on the
+    current one (refactored when they stop converging quickly).  From the same guess it
+    must reach ||F||_inf < 1e-10 and the same solution as the factor-every-step Newton
+    (1e-10), with at least one step that reuses factors (no factorisation)."""
+    from oracle import xcts as OX
+    sp = OX.Space(RES, BOUNDS)
+    sy = OX.XCTSSystem(sp, 1, 3, n0=0.5, a=1.0, ome=0.0)
+    sy.set_fields(OX.initial_fields(sy, 1.0, OX.initial_perturbation(sy)))
+    a, b = (programs.kerr(ctx, RES, BOUNDS, n0=0.5, a=1.0, ome=0.0).build(order=1, batch=256) for _ in range(2))
+    _load(a, sy.fields)
+    _load(b, sy.fields)
+    reused = 0
+    for it in range(15):
+        conv, err, _, ir = a.do_newton(1e-10, "f64-reuse")
+        print(f"f64-reuse it {it}: |F| {err:.3e} inner {ir[1]} factored {ir[4]}", flush=True)
+        if conv:
+            break
+        reused += ir[4] == 0
+    assert conv and reused >= 1
+    b.solve(1e-10, "f64")
+    worst = 0.0
+    for name, f in NAMES.items():
+        for d in range(1, len(BOUNDS) + 1):
+            x, y = a.coef(f, d), b.coef(f, d)
+            worst = max(worst, np.abs(x - y).max() / max(1.0, np.abs(y).max()))
+    print(f"f64-reuse vs f64 solution: {worst:.2e}", flush=True)
+    assert worst <= 1e-10
+    a.close()
+    b.close()
