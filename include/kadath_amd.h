@@ -74,6 +74,47 @@ int kad_gemm(kad_ctx* ctx, int prec, int op, int m, int n, int k, const void* A,
  * factors are then not valid). */
 int kad_lu_factor(kad_ctx* ctx, int prec, int n, void* A, int64_t lda, int* ipiv, int* perm, int nb);
 
+/* nbat LUs of order n (a multiple of 256, nbat <= 16) factored together: matrix z
+ * at A + z strideA (same lda), interchanges at ipiv + z n (0-based, local to the
+ * matrix), composed row orders at perm + z n (may be NULL); info: HOST, nbat words
+ * (first exact zero pivot, 1-based, or 0).  Every launch covers all matrices, so the
+ * panel chain is one matrix's: the diagonal domain blocks of the domain-decomposed
+ * solve below, which replaces the single pdgesv_ factorisation
+ * (src_par/do_newton.cpp:317-320) by independent per-domain ones.  The matrices are
+ * taken as dense; each pivot search is LAPACK's within its matrix. */
+int kad_lu_factor_batched(kad_ctx* ctx, int prec, int n, int nbat, void* A, int64_t lda, int64_t strideA, int* ipiv,
+                          int* perm, int* info);
+
+/* B_z <- A_z^{-1} B_z (nrhs columns each, B_z at B + z strideB) with the FP64
+ * factors of kad_lu_factor_batched (n a multiple of 32). */
+int kad_lu_solve_batched(kad_ctx* ctx, int n, int nbat, const double* LU, int64_t lda, int64_t strideA, const int* perm,
+                         double* B, int64_t ldb, int64_t strideB, int nrhs);
+
+/* Domain-decomposed solve of J x = b, replacing the single pdgesv_ of
+ * src_par/do_newton.cpp:317-320 for a system whose unknowns and rows split into
+ * blocks (its domains) coupled only through a few rows (the interface matching rows,
+ * take_into_account, src/System_of_eqs/solver.cpp:181,203).  Block b holds the J rows
+ * rows[o_b .. o_b + blk_size[b]) and the unknowns cols[o_b ..) (o_b = sum of the
+ * earlier sizes; HOST arrays, every row and column of J once, each block square).
+ * kad_dd_factor factors the diagonal blocks together (kad_lu_factor_batched), then
+ * the Woodbury coupling: D^{-1} of the coupling rows' unit vectors and the
+ * capacitance matrix I + C D^{-1} P (C: the coupling rows outside their own block),
+ * with its own LU.  Returns 0, < 0 an error, > 0 a singular block or capacitance
+ * matrix.  kad_dd_solve runs flexible GMRES on the fp64 J (device, ld >= n),
+ * preconditioned by that solve, to ||b - J x||_inf <= tol ||b||_inf (KAD_ERR_NOCONV
+ * otherwise; iters as kad_ir_solve).  kad_dd_apply applies the preconditioner alone
+ * (v: rows in, unknowns out).  kad_dd_info: sizes [padded block order, coupling rows,
+ * the most per block, blocks], times (s) of the last factorisation [coupling scan and
+ * block gather, block LU, D^{-1} P, capacitance product, capacitance LU]. */
+typedef struct kad_dd_s kad_dd;
+int kad_dd_create(kad_ctx* ctx, int n, int nblk, const int* blk_size, const int* rows, const int* cols, kad_dd** out);
+int kad_dd_factor(kad_dd* dd, const double* J, int64_t ldj);
+int kad_dd_solve(kad_dd* dd, const double* J, int64_t ldj, const double* b, double* x, double tol, int max_outer,
+                 int* iters, double* resid);
+int kad_dd_apply(kad_dd* dd, double* v);
+int kad_dd_info(const kad_dd* dd, int* sizes, double* times);
+int kad_dd_destroy(kad_dd* dd);
+
 /* Solve op(A) x = b in place with the factors (b fp64).  trans = 1: A^T x = b
  * (adjoint solve; the reference's Matrice::solve uses trans 'T' on row-major
  * storage, matrice.cpp:221-250). */
@@ -208,7 +249,12 @@ int kad_xcts_sensitivity(kad_system* s, double* out, double* dQdu);
  * beside J, 3 n^2 doubles), 5 fp64 LU factors of an earlier J as the preconditioner of
  * fp64 flexible GMRES on the current J, refactored (and solved directly) when the
  * previous refinement took more than the reuse limit (kad_system_set_option 1, default
- * 30) inner steps or did not converge; it holds two n^2 fp64 buffers.  Returns 1 if ||F||_inf < precision at
+ * 30) inner steps or did not converge; it holds two n^2 fp64 buffers.  6: the
+ * domain-decomposed fp64 factors (kad_dd_factor over the blocks of
+ * kad_system_dd_blocks: the domains, each block factored with partial pivoting inside
+ * it, the blocks together) precondition fp64 flexible GMRES on the same J; 7: the same,
+ * the factors of an earlier J kept as for 5.  6 and 7 need square domain blocks (no
+ * integral equations or double unknowns).  Returns 1 if ||F||_inf < precision at
  * entry (no step), 0 after a step, KAD_ERR_SINGULAR on an exactly singular J.
  * times[5]: residual, Jacobian fill, factor + solve (incl. refinement), update,
  * factorisation alone (seconds). */
@@ -227,6 +273,9 @@ int kad_xcts_last_solve(kad_system* s, int* iters, double* resid);
  * value instr, tangent instr, value slots, tangent programs; returns #layout entries */
 int kad_xcts_host_info(int nr, int nt, int np, const double* bounds, int nbounds, int dmin, int dmax, double n0,
                        double a, double ome, int order, int* info, int* layout, int cap);
+/* kad_system_dd_blocks of the same system, traced on the host only (no device) */
+int kad_xcts_host_dd_blocks(int nr, int nt, int np, const double* bounds, int nbounds, int dmin, int dmax, int order,
+                            int* sizes, int cap, int* rows, int* cols);
 
 /* As kad_ir_solve when LUlo holds the factors of diag(Rs) A diag(Cs) (see
  * kad_cast_equilibrated): the preconditioner is v -> Cs .* LU^{-1} (Rs .* v);
@@ -332,7 +381,7 @@ int kad_system_sensitivity(kad_system* s, int which, double* out, double* dQdu);
 /* number of double unknowns; the current value of double unknown k (HOST out) */
 int kad_system_ndouble(kad_system* s);
 int kad_system_get_double(kad_system* s, int k, double* value);
-/* Options of the iterative solvers (1-3, 5) of do_newton: which 0 = forcing factor
+/* Options of the iterative solvers (1-3, 5-7) of do_newton: which 0 = forcing factor
  * c >= 0, the refinement's target becomes max(1e-13, c min(1, ||F||_inf)) relative
  * (0, the default: 1e-13, i.e. its attainable accuracy); which 1 = reuse the
  * low-precision factors of an earlier step while the previous refinement took at most
@@ -344,7 +393,7 @@ int kad_system_do_newton(kad_system* s, double precision, double* error, int sol
 /* Newton iteration with backtracking line search:
  * System_of_eqs::do_newton_with_linesearch(precision, error, ntrymax, stepmax)
  * (src_par/do_newton.cpp:454-517; reference defaults ntrymax 10, stepmax 1.0).  The
- * step p solves J p = F with `solver` (0-4 as kad_system_do_newton) and is scaled by
+ * step p solves J p = F with `solver` (as kad_system_do_newton) and is scaled by
  * stepmax / max(p) when its signed maximum exceeds stepmax; u = u_old - lambda p with
  * lambda from the quadratic, then cubic, model until sum F^2 <= f_old + 1e-4 lambda
  * slope (slope = -2 f_old), sum F^2 <= 5e-13, lambda <= 1e-10, or ntrymax tries.
@@ -355,6 +404,15 @@ int kad_system_do_newton(kad_system* s, double precision, double* error, int sol
 int kad_system_do_newton_linesearch(kad_system* s, double precision, double* error, int ntrymax, double stepmax,
                                     int solver, double* lambda, int* ntry);
 int kad_system_last_solve(kad_system* s, int* iters, double* resid);
+/* The domain blocks of solvers 6-7 (no reference counterpart: the reference factors J
+ * whole): per domain, its unknowns and the rows of the equations living there, an
+ * interface's value-matching rows in the inner domain and its derivative-matching
+ * rows in the outer one.  sizes (HOST, cap), rows and cols (HOST, n each, may be
+ * NULL) as kad_dd_create takes them.  Returns the number of blocks, or KAD_ERR_ARG
+ * when they are not square. */
+int kad_system_dd_blocks(kad_system* s, int* sizes, int cap, int* rows, int* cols);
+/* kad_dd_info of the system's last domain-decomposed factorisation */
+int kad_system_dd_info(kad_system* s, int* sizes, double* times);
 
 /* F(u) (device, n_rows): System_of_eqs::sec_member (solver.cpp:49-85) */
 int kad_residual(kad_system* s, double* F);

@@ -49,7 +49,9 @@ enum SOLVER {
   NEWTON_F32_GMRES = 2,
   NEWTON_BF16_GMRES = 3,
   GAUSS_NEWTON = 4,
-  NEWTON_F64_REUSE = 5  // fp64 factors of an earlier Jacobian + fp64 flexible GMRES
+  NEWTON_F64_REUSE = 5,    // fp64 factors of an earlier Jacobian + fp64 flexible GMRES
+  NEWTON_F64_DD = 6,       // domain-decomposed fp64 factors + fp64 flexible GMRES
+  NEWTON_F64_DD_REUSE = 7  // the same, the factors of an earlier Jacobian kept while they precondition well
 };
 
 namespace detail {

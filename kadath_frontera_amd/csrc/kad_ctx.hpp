@@ -17,13 +17,27 @@ namespace kad {
 //  list mode (cnt != NULL): only the pairs rlist[0..cnt[0]) x clist[0..cnt[1]) are
 //    computed, compacted over the grid so the active tiles spread over every XCD
 //    (C -= AB only).
+//  batch: nbat products of one shape, launched together (grid z); product z reads
+//    A + z sa and B + z sb and writes C + z sc (masks and lists: nbat = 1 only).
 struct TileSel {
   const int* rmask = nullptr;
   const int* cmask = nullptr;
   const int* rlist = nullptr;
   const int* clist = nullptr;
   const int* cnt = nullptr;
+  int nbat = 1;
+  int64_t sa = 0, sb = 0, sc = 0;
 };
+
+// A batch of equally sized matrices factored together: matrix z at A + z sA (same
+// leading dimension), its interchanges at ipiv + z sP, indices local to the matrix.
+struct LuBatch {
+  int nbat = 1;
+  int64_t sA = 0;
+  int sP = 0;
+  int* info = nullptr;  // nbat words (first zero pivot of each matrix); NULL: LuWork::info
+};
+constexpr int LU_MAX_BATCH = 16;
 
 // Scratch of the LU factorisation.
 struct LuWork {
@@ -32,6 +46,7 @@ struct LuWork {
   int* red_idx = nullptr;
   unsigned* counter = nullptr;  // last-block counter (reset by the last block)
   int* info = nullptr;          // first exactly-zero pivot (1-based), 0 if none
+  int* binfo = nullptr;         // the same per matrix of a batched factorisation (LU_MAX_BATCH)
   int red_cap = 0;
   void* tinv = nullptr;         // unit-lower inverse, nb x nb
   void* tmp = nullptr;          // GEMM scratch for L11^{-1} A12
@@ -128,6 +143,8 @@ struct Ctx {
   size_t cap_vwork = 0;
   double* krylov = nullptr;
   size_t cap_krylov = 0;
+  double* bsolve = nullptr;     // permuted right-hand sides of the batched solve
+  size_t cap_bsolve = 0;
   double* trs_tab = nullptr;    // triangular-solve tables of the current factors
   size_t cap_trs_tab = 0;
   int* trs_flag = nullptr;      // nonzero flags of the factor tiles (set-up scratch)

@@ -1,0 +1,498 @@
+// Domain-decomposed direct solve of the Newton system, single MI355X.
+//
+// The reference factors the whole Jacobian with one pdgesv_ (src_par/do_newton.cpp:
+// 317-320), whose panel chain runs over all n columns.  In a multi-domain spectral
+// space the unknowns and the equations split by domain, and only the matching rows
+// of an interface couple two domains (take_into_account, src/System_of_eqs/
+// solver.cpp:181,203).  With each interface's value-matching rows counted in the
+// inner domain and its derivative-matching rows in the outer one, every domain's
+// diagonal block D_b is square: the domain's own operator with a Dirichlet-type outer
+// and a Neumann-type inner interface condition.  Then
+//   J = D + P C,   D = diag(D_b),   P = unit columns of the r coupling rows,
+//   C = the entries of those rows outside their own block,
+// and (Woodbury)  J^{-1} v = z - W (I + C W)^{-1} C z,  z = D^{-1} v,  W = D^{-1} P.
+//
+// Set-up (kad_dd_factor), all on the device:
+//  * the coupling rows: a scan of J for entries outside each row's own block;
+//  * the diagonal blocks, padded to one order nd (a multiple of 256; identity on the
+//    padding), factored together by the batched LU: one panel chain of nd columns
+//    instead of n, every launch covering all blocks;
+//  * W: the unit right-hand sides placed directly in pivoted row order, then the
+//    batched unit-lower and upper triangular solves (MFMA GEMM updates);
+//  * C as a dense r x (blocks x nd) copy (zero in each row's own block), the
+//    capacitance matrix I + C W by one GEMM per block, and its LU (order r).
+// Solve (kad_dd_solve): flexible GMRES on the fp64 J, preconditioned by the Woodbury
+// solve.  That solve is direct up to rounding, so one or two inner steps reach the
+// fp64 floor; the products with J keep the residual exact whatever the blocks'
+// conditioning.
+#include "kad_common.hpp"
+#include "kad_ctx.hpp"
+#include "../../include/kadath_amd.h"
+#include <algorithm>
+#include <chrono>
+#include <functional>
+#include <vector>
+
+namespace kad {
+
+int gemm_launch(int prec, int op, int m, int n, int k, const void* A, int64_t lda, const void* B, int64_t ldb, void* C,
+                int64_t ldc, hipStream_t s);
+int gemm_launch_batched(int prec, int op, int m, int n, int k, const void* A, int64_t lda, int64_t sa, const void* B,
+                        int64_t ldb, int64_t sb, void* C, int64_t ldc, int64_t sc, int nbat, hipStream_t s);
+int lu_solve_permuted_batched(const double* LU, int64_t lda, int n, int nbat, int64_t sA, double* B, int64_t ldb,
+                              int64_t sB, int nrhs, LuWork& w);
+int ir_loop(Ctx* c, int n, const double* A64, int64_t lda, const std::function<int(double*)>& precond,
+            const double* b, double* x, double tol, int max_outer, int mode, int trans, int* iters, double* resid);
+size_t trsv_table_doubles(int n);
+size_t trsv_flag_ints(int n);
+template <typename T>
+int trsv_prepare(const T* LU, int64_t lda, int n, int trans, double* W, int* flag, TrsvExtents& ext, hipStream_t s);
+template <typename T>
+int getrs_fast(const T* LU, int64_t lda, int n, const int* perm, const double* W, const TrsvExtents& ext, double* b,
+               int trans, double* work, hipStream_t s);
+
+constexpr int DD_COLS = 512;  // columns per workgroup of the coupling-row scan
+constexpr int DD_CHUNK = 1024;  // columns per partial sum of the C products
+
+// flag[i] = 1 when row i of J has an entry outside its own block (columns
+// [by DD_COLS, +DD_COLS) of this workgroup).
+__global__ __launch_bounds__(256) void dd_flag_kernel(const double* __restrict__ J, int64_t ld, int n,
+                                                      const int* __restrict__ rblk, const int* __restrict__ cblk,
+                                                      int* __restrict__ flag) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int c0 = blockIdx.y * DD_COLS, c1 = min(n, c0 + DD_COLS);
+  const int b = rblk[i];
+  int hit = 0;
+  for (int c = c0; c < c1; ++c) hit |= (cblk[c] != b) & (J[i + (int64_t)c * ld] != 0.0);
+  if (hit) flag[i] = 1;
+}
+
+// D_z(i, j) = J(rowmap_z[i], colmap_z[j]); identity where either is padding.
+__global__ __launch_bounds__(256) void dd_blocks_kernel(const double* __restrict__ J, int64_t ld,
+                                                        const int* __restrict__ rowmap, const int* __restrict__ colmap,
+                                                        int nd, double* __restrict__ D) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int j = blockIdx.y, z = blockIdx.z;
+  if (i >= nd) return;
+  const int row = rowmap[(int64_t)z * nd + i], col = colmap[(int64_t)z * nd + j];
+  D[(int64_t)z * nd * nd + i + (int64_t)j * nd] = (row >= 0 && col >= 0) ? J[row + (int64_t)col * ld]
+                                                                         : (i == j ? 1.0 : 0.0);
+}
+
+// W_z(k, j) = 1 at each unit right-hand side's pivoted position (W zeroed before).
+__global__ void dd_units_kernel(const int64_t* __restrict__ pos, int r, double* __restrict__ W) {
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  if (q < r) W[pos[q]] = 1.0;
+}
+
+// Jc(q, z nd + j) = J(crow[q], colmap_z[j]) outside row q's own block, else 0.
+__global__ __launch_bounds__(256) void dd_coupling_kernel(const double* __restrict__ J, int64_t ld,
+                                                          const int* __restrict__ crow, const int* __restrict__ cz,
+                                                          const int* __restrict__ colmap, int nd, int r,
+                                                          double* __restrict__ Jc) {
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  const int j = blockIdx.y, z = blockIdx.z;
+  if (q >= r) return;
+  const int col = colmap[(int64_t)z * nd + j];
+  Jc[q + ((int64_t)z * nd + j) * r] = (col >= 0 && cz[q] != z) ? J[crow[q] + (int64_t)col * ld] : 0.0;
+}
+
+__global__ void dd_add_identity_kernel(double* __restrict__ A, int64_t ld, int r) {
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  if (q < r) A[q + (int64_t)q * ld] += 1.0;
+}
+
+// vb_z[i] = v[rowmap_z[perm_z[i]]]: the right-hand side of block z in pivoted order.
+__global__ void dd_gather_kernel(const double* __restrict__ v, const int* __restrict__ rowmap,
+                                 const int* __restrict__ perm, int nd, double* __restrict__ vb) {
+  const int i = blockIdx.x * 256 + threadIdx.x, z = blockIdx.y;
+  if (i >= nd) return;
+  const int row = rowmap[(int64_t)z * nd + perm[(int64_t)z * nd + i]];
+  vb[(int64_t)z * nd + i] = row >= 0 ? v[row] : 0.0;
+}
+
+// x[colmap_z[j]] = vb_z[j].
+__global__ void dd_scatter_kernel(const double* __restrict__ vb, const int* __restrict__ colmap, int nd,
+                                  double* __restrict__ x) {
+  const int j = blockIdx.x * 256 + threadIdx.x, z = blockIdx.y;
+  if (j >= nd) return;
+  const int col = colmap[(int64_t)z * nd + j];
+  if (col >= 0) x[col] = vb[(int64_t)z * nd + j];
+}
+
+// part[chunk r + q] = sum over the chunk's columns c of Jc(q, c) vb[c].
+__global__ __launch_bounds__(256) void dd_cprod_kernel(const double* __restrict__ Jc, int r, int64_t ncol,
+                                                       const double* __restrict__ vb, double* __restrict__ part) {
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= r) return;
+  const int64_t c0 = (int64_t)blockIdx.y * DD_CHUNK, c1 = min(ncol, c0 + DD_CHUNK);
+  double s0 = 0.0, s1 = 0.0;
+  int64_t c = c0;
+  for (; c + 2 <= c1; c += 2) {
+    s0 += Jc[q + c * r] * vb[c];
+    s1 += Jc[q + (c + 1) * r] * vb[c + 1];
+  }
+  if (c < c1) s0 += Jc[q + c * r] * vb[c];
+  part[(int64_t)blockIdx.y * r + q] = s0 + s1;
+}
+
+// t[q] = sum of the partial sums, in a fixed order.
+__global__ void dd_creduce_kernel(const double* __restrict__ part, int nchunk, int r, double* __restrict__ t) {
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= r) return;
+  double s = 0.0;
+  for (int k = 0; k < nchunk; ++k) s += part[(int64_t)k * r + q];
+  t[q] = s;
+}
+
+// yb[z kmax + j] = t[q] for coupling row q = q0_z + j of block z.
+__global__ void dd_spread_kernel(const double* __restrict__ t, int r, const int* __restrict__ cz,
+                                 const int* __restrict__ q0, int kmax, double* __restrict__ yb) {
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= r) return;
+  const int z = cz[q];
+  yb[(int64_t)z * kmax + (q - q0[z])] = t[q];
+}
+
+}  // namespace kad
+
+using kad::Ctx;
+
+struct kad_dd_s {
+  Ctx* c = nullptr;
+  int n = 0, nblk = 0, nd = 0;
+  std::vector<int> bsize;
+  std::vector<int> h_rowmap, h_colmap;  // nblk x nd, -1 on padding
+  std::vector<int> h_rblk;              // block of each row of J
+  std::vector<int> h_rloc;              // local index of each row of J in its block
+  int* rowmap = nullptr;
+  int* colmap = nullptr;
+  int* rblk = nullptr;
+  int* cblk = nullptr;
+  int* flag = nullptr;
+  double* D = nullptr;
+  int* ipiv = nullptr;
+  int* perm = nullptr;
+  // coupling (sized at the last factorisation)
+  int r = 0, kmax = 0;
+  std::vector<int> kb, q0;
+  size_t cap_r = 0, cap_w = 0;  // capacities: coupling rows, W entries
+  int* crow = nullptr;          // r rows of J, ordered by (block, local row)
+  int* cz = nullptr;            // block of each coupling row
+  int* dq0 = nullptr;           // first coupling index of each block
+  int64_t* upos = nullptr;      // W offset of each unit right-hand side
+  double* W = nullptr;
+  double* Jc = nullptr;
+  double* cap = nullptr;
+  int* cap_ipiv = nullptr;
+  int* cap_perm = nullptr;
+  double* cap_tab = nullptr;
+  int* cap_flag = nullptr;
+  kad::TrsvExtents cap_ext;
+  double* vb = nullptr;         // nblk x nd
+  double* t = nullptr;          // r, then 3 r of capacitance-solve work
+  double* yb = nullptr;         // nblk x kmax
+  double* part = nullptr;       // partial sums of C vb
+  bool factored = false;
+  double times[5] = {0, 0, 0, 0, 0};
+
+  int dmalloc(void** p, size_t bytes) {
+    KAD_HIP_CHECK(hipMalloc(p, bytes < 16 ? 16 : bytes));
+    return KAD_OK;
+  }
+  void release_coupling() {
+    void* bufs[] = {crow, cz, dq0, upos, W, Jc, cap, cap_ipiv, cap_perm, cap_tab, cap_flag, t, yb, part};
+    for (void* b : bufs)
+      if (b) (void)hipFree(b);
+    crow = cz = dq0 = nullptr;
+    upos = nullptr;
+    W = Jc = cap = cap_tab = t = yb = part = nullptr;
+    cap_ipiv = cap_perm = cap_flag = nullptr;
+    cap_r = cap_w = 0;
+  }
+  ~kad_dd_s() {
+    release_coupling();
+    void* bufs[] = {rowmap, colmap, rblk, cblk, flag, D, ipiv, perm, vb};
+    for (void* b : bufs)
+      if (b) (void)hipFree(b);
+  }
+};
+
+namespace {
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Coupling buffers for r rows (kmax per block) of a system with nblk blocks of order nd.
+int ensure_coupling(kad_dd* dd, int r, int kmax) {
+  const size_t w = (size_t)dd->nblk * dd->nd * kmax;
+  if ((size_t)r <= dd->cap_r && w <= dd->cap_w) return KAD_OK;
+  dd->release_coupling();
+  const size_t R = std::max(r, 1), ncol = (size_t)dd->nblk * dd->nd;
+  const size_t nchunk = kad::cdiv((int64_t)ncol, kad::DD_CHUNK);
+  int rc;
+  if ((rc = dd->dmalloc((void**)&dd->crow, R * sizeof(int)))) return rc;
+  if ((rc = dd->dmalloc((void**)&dd->cz, R * sizeof(int)))) return rc;
+  if ((rc = dd->dmalloc((void**)&dd->dq0, (size_t)dd->nblk * sizeof(int)))) return rc;
+  if ((rc = dd->dmalloc((void**)&dd->upos, R * sizeof(int64_t)))) return rc;
+  if ((rc = dd->dmalloc((void**)&dd->W, std::max<size_t>(w, 1) * sizeof(double)))) return rc;
+  if ((rc = dd->dmalloc((void**)&dd->Jc, R * ncol * sizeof(double)))) return rc;
+  if ((rc = dd->dmalloc((void**)&dd->cap, R * R * sizeof(double)))) return rc;
+  if ((rc = dd->dmalloc((void**)&dd->cap_ipiv, R * sizeof(int)))) return rc;
+  if ((rc = dd->dmalloc((void**)&dd->cap_perm, R * sizeof(int)))) return rc;
+  if ((rc = dd->dmalloc((void**)&dd->cap_tab, kad::trsv_table_doubles((int)R) * sizeof(double)))) return rc;
+  if ((rc = dd->dmalloc((void**)&dd->cap_flag, kad::trsv_flag_ints((int)R) * sizeof(int)))) return rc;
+  if ((rc = dd->dmalloc((void**)&dd->t, 4 * R * sizeof(double)))) return rc;
+  if ((rc = dd->dmalloc((void**)&dd->yb, (size_t)dd->nblk * std::max(kmax, 1) * sizeof(double)))) return rc;
+  if ((rc = dd->dmalloc((void**)&dd->part, nchunk * R * sizeof(double)))) return rc;
+  dd->cap_r = R;
+  dd->cap_w = std::max<size_t>(w, 1);
+  return KAD_OK;
+}
+
+// v <- J^{-1} v through the factored blocks and the capacitance matrix (v: rows of J
+// in, unknowns out).
+int dd_apply(kad_dd* dd, double* v) {
+  Ctx* c = dd->c;
+  hipStream_t s = c->stream;
+  const int nd = dd->nd, nb = dd->nblk;
+  const dim3 gv((unsigned)kad::cdiv(nd, 256), (unsigned)nb);
+  kad::dd_gather_kernel<<<gv, 256, 0, s>>>(v, dd->rowmap, dd->perm, nd, dd->vb);
+  KAD_HIP_CHECK(hipGetLastError());
+  int rc = kad::lu_solve_permuted_batched(dd->D, nd, nd, nb, (int64_t)nd * nd, dd->vb, nd, nd, 1, c->lu);
+  if (rc) return rc;
+  if (dd->r > 0) {
+    const int r = dd->r;
+    const int64_t ncol = (int64_t)nb * nd;
+    const unsigned nchunk = (unsigned)kad::cdiv(ncol, kad::DD_CHUNK);
+    const unsigned gr = (unsigned)kad::cdiv(r, 256);
+    kad::dd_cprod_kernel<<<dim3(gr, nchunk), 256, 0, s>>>(dd->Jc, r, ncol, dd->vb, dd->part);
+    kad::dd_creduce_kernel<<<gr, 256, 0, s>>>(dd->part, (int)nchunk, r, dd->t);
+    KAD_HIP_CHECK(hipGetLastError());
+    if ((rc = kad::getrs_fast<double>(dd->cap, r, r, dd->cap_perm, dd->cap_tab, dd->cap_ext, dd->t, 0, dd->t + r, s)))
+      return rc;
+    KAD_HIP_CHECK(hipMemsetAsync(dd->yb, 0, (size_t)nb * dd->kmax * sizeof(double), s));
+    kad::dd_spread_kernel<<<gr, 256, 0, s>>>(dd->t, r, dd->cz, dd->dq0, dd->kmax, dd->yb);
+    KAD_HIP_CHECK(hipGetLastError());
+    // z_b -= W_b y_b
+    if ((rc = kad::gemm_launch_batched(kad::F64, 0, nd, 1, dd->kmax, dd->W, nd, (int64_t)nd * dd->kmax, dd->yb,
+                                       dd->kmax, dd->kmax, dd->vb, nd, nd, nb, s)))
+      return rc;
+  }
+  kad::dd_scatter_kernel<<<gv, 256, 0, s>>>(dd->vb, dd->colmap, nd, v);
+  KAD_HIP_CHECK(hipGetLastError());
+  return KAD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kad_dd_create(kad_ctx* ctx, int n, int nblk, const int* blk_size, const int* rows, const int* cols, kad_dd** out) {
+  if (!ctx || !out || n <= 0 || nblk < 1 || nblk > kad::LU_MAX_BATCH || !blk_size || !rows || !cols)
+    return KAD_ERR_ARG;
+  *out = nullptr;
+  int64_t tot = 0, mx = 0;
+  for (int b = 0; b < nblk; ++b) {
+    if (blk_size[b] < 1) return KAD_ERR_ARG;
+    tot += blk_size[b];
+    mx = std::max<int64_t>(mx, blk_size[b]);
+  }
+  if (tot != n) return KAD_ERR_ARG;
+  // every row and every column of J in exactly one block
+  std::vector<int> seen_r(n, 0), seen_c(n, 0);
+  for (int i = 0; i < n; ++i) {
+    if (rows[i] < 0 || rows[i] >= n || cols[i] < 0 || cols[i] >= n) return KAD_ERR_ARG;
+    if (seen_r[rows[i]]++ || seen_c[cols[i]]++) return KAD_ERR_ARG;
+  }
+  auto dd = new kad_dd_s();
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  dd->c = c;
+  dd->n = n;
+  dd->nblk = nblk;
+  dd->nd = (int)kad::cdiv(mx, 256) * 256;
+  dd->bsize.assign(blk_size, blk_size + nblk);
+  const int nd = dd->nd;
+  dd->h_rowmap.assign((size_t)nblk * nd, -1);
+  dd->h_colmap.assign((size_t)nblk * nd, -1);
+  dd->h_rblk.assign(n, -1);
+  dd->h_rloc.assign(n, -1);
+  std::vector<int> hcb(n, -1);
+  int64_t o = 0;
+  for (int b = 0; b < nblk; ++b)
+    for (int i = 0; i < blk_size[b]; ++i, ++o) {
+      dd->h_rowmap[(size_t)b * nd + i] = rows[o];
+      dd->h_colmap[(size_t)b * nd + i] = cols[o];
+      dd->h_rblk[rows[o]] = b;
+      dd->h_rloc[rows[o]] = i;
+      hcb[cols[o]] = b;
+    }
+  int rc = KAD_OK;
+  auto fail = [&](int e) {
+    delete dd;
+    return e;
+  };
+  KAD_HIP_CHECK(hipSetDevice(c->device));
+  if ((rc = dd->dmalloc((void**)&dd->rowmap, dd->h_rowmap.size() * sizeof(int)))) return fail(rc);
+  if ((rc = dd->dmalloc((void**)&dd->colmap, dd->h_colmap.size() * sizeof(int)))) return fail(rc);
+  if ((rc = dd->dmalloc((void**)&dd->rblk, (size_t)n * sizeof(int)))) return fail(rc);
+  if ((rc = dd->dmalloc((void**)&dd->cblk, (size_t)n * sizeof(int)))) return fail(rc);
+  if ((rc = dd->dmalloc((void**)&dd->flag, (size_t)n * sizeof(int)))) return fail(rc);
+  if ((rc = dd->dmalloc((void**)&dd->D, (size_t)nblk * nd * nd * sizeof(double)))) return fail(rc);
+  if ((rc = dd->dmalloc((void**)&dd->ipiv, (size_t)nblk * nd * sizeof(int)))) return fail(rc);
+  if ((rc = dd->dmalloc((void**)&dd->perm, (size_t)nblk * nd * sizeof(int)))) return fail(rc);
+  if ((rc = dd->dmalloc((void**)&dd->vb, (size_t)nblk * nd * sizeof(double)))) return fail(rc);
+  if (hipMemcpy(dd->rowmap, dd->h_rowmap.data(), dd->h_rowmap.size() * sizeof(int), hipMemcpyHostToDevice) ||
+      hipMemcpy(dd->colmap, dd->h_colmap.data(), dd->h_colmap.size() * sizeof(int), hipMemcpyHostToDevice) ||
+      hipMemcpy(dd->rblk, dd->h_rblk.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice) ||
+      hipMemcpy(dd->cblk, hcb.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice))
+    return fail(KAD_ERR_HIP);
+  *out = dd;
+  return KAD_OK;
+}
+
+int kad_dd_destroy(kad_dd* dd) {
+  delete dd;
+  return KAD_OK;
+}
+
+int kad_dd_factor(kad_dd* dd, const double* J, int64_t ldj) {
+  if (!dd || !J || ldj < dd->n) return KAD_ERR_ARG;
+  Ctx* c = dd->c;
+  hipStream_t s = c->stream;
+  const int n = dd->n, nd = dd->nd, nb = dd->nblk;
+  dd->factored = false;
+  int rc;
+  KAD_HIP_CHECK(hipStreamSynchronize(s));
+  double t0 = now_s();
+  // coupling rows
+  KAD_HIP_CHECK(hipMemsetAsync(dd->flag, 0, (size_t)n * sizeof(int), s));
+  kad::dd_flag_kernel<<<dim3((unsigned)kad::cdiv(n, 256), (unsigned)kad::cdiv(n, kad::DD_COLS)), 256, 0, s>>>(
+      J, ldj, n, dd->rblk, dd->cblk, dd->flag);
+  KAD_HIP_CHECK(hipGetLastError());
+  kad::dd_blocks_kernel<<<dim3((unsigned)kad::cdiv(nd, 256), (unsigned)nd, (unsigned)nb), 256, 0, s>>>(
+      J, ldj, dd->rowmap, dd->colmap, nd, dd->D);
+  KAD_HIP_CHECK(hipGetLastError());
+  std::vector<int> hflag(n);
+  KAD_HIP_CHECK(hipMemcpyAsync(hflag.data(), dd->flag, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, s));
+  KAD_HIP_CHECK(hipStreamSynchronize(s));
+  double t1 = now_s();
+  // the diagonal blocks, factored together
+  std::vector<int> info(nb, 0);
+  rc = kad_lu_factor_batched(reinterpret_cast<kad_ctx*>(c), KAD_PREC_F64, nd, nb, dd->D, nd, (int64_t)nd * nd,
+                             dd->ipiv, dd->perm, info.data());
+  if (rc) return rc;
+  for (int b = 0; b < nb; ++b)
+    if (info[b] != 0) return info[b];  // a singular domain block (1-based column inside it)
+  double t2 = now_s();
+  // coupling rows by (block, local row); their unit right-hand sides in pivoted order
+  std::vector<std::vector<int>> byblk(nb);
+  for (int i = 0; i < n; ++i)
+    if (hflag[i]) byblk[dd->h_rblk[i]].push_back(i);
+  dd->kb.assign(nb, 0);
+  dd->q0.assign(nb, 0);
+  int r = 0, kmax = 0;
+  for (int b = 0; b < nb; ++b) {
+    std::sort(byblk[b].begin(), byblk[b].end(),
+              [&](int x, int y) { return dd->h_rloc[x] < dd->h_rloc[y]; });
+    dd->q0[b] = r;
+    dd->kb[b] = (int)byblk[b].size();
+    r += dd->kb[b];
+    kmax = std::max(kmax, dd->kb[b]);
+  }
+  dd->r = r;
+  dd->kmax = kmax;
+  if (r > 0) {
+    if ((rc = ensure_coupling(dd, r, kmax))) return rc;
+    std::vector<int> hperm((size_t)nb * nd), hcrow(r), hcz(r);
+    std::vector<int64_t> hpos(r);
+    KAD_HIP_CHECK(hipMemcpyAsync(hperm.data(), dd->perm, hperm.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+    KAD_HIP_CHECK(hipStreamSynchronize(s));
+    std::vector<int> inv(nd);
+    for (int b = 0; b < nb; ++b) {
+      for (int k = 0; k < nd; ++k) inv[hperm[(size_t)b * nd + k]] = k;
+      for (int j = 0; j < dd->kb[b]; ++j) {
+        const int q = dd->q0[b] + j, row = byblk[b][j];
+        hcrow[q] = row;
+        hcz[q] = b;
+        // e_{local row} in pivoted order sits at inv[local row]; column j of W_b
+        hpos[q] = (int64_t)b * nd * kmax + (int64_t)j * nd + inv[dd->h_rloc[row]];
+      }
+    }
+    KAD_HIP_CHECK(hipMemcpyAsync(dd->crow, hcrow.data(), r * sizeof(int), hipMemcpyHostToDevice, s));
+    KAD_HIP_CHECK(hipMemcpyAsync(dd->cz, hcz.data(), r * sizeof(int), hipMemcpyHostToDevice, s));
+    KAD_HIP_CHECK(hipMemcpyAsync(dd->dq0, dd->q0.data(), nb * sizeof(int), hipMemcpyHostToDevice, s));
+    KAD_HIP_CHECK(hipMemcpyAsync(dd->upos, hpos.data(), r * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    // W = D^{-1} P
+    KAD_HIP_CHECK(hipMemsetAsync(dd->W, 0, (size_t)nb * nd * kmax * sizeof(double), s));
+    kad::dd_units_kernel<<<(unsigned)kad::cdiv(r, 256), 256, 0, s>>>(dd->upos, r, dd->W);
+    KAD_HIP_CHECK(hipGetLastError());
+    if ((rc = kad::lu_solve_permuted_batched(dd->D, nd, nd, nb, (int64_t)nd * nd, dd->W, nd, (int64_t)nd * kmax, kmax,
+                                             c->lu)))
+      return rc;
+    KAD_HIP_CHECK(hipStreamSynchronize(s));
+    double t3 = now_s();
+    // C, then I + C W
+    kad::dd_coupling_kernel<<<dim3((unsigned)kad::cdiv(r, 256), (unsigned)nd, (unsigned)nb), 256, 0, s>>>(
+        J, ldj, dd->crow, dd->cz, dd->colmap, nd, r, dd->Jc);
+    KAD_HIP_CHECK(hipGetLastError());
+    for (int b = 0; b < nb; ++b) {
+      if (dd->kb[b] == 0) continue;
+      if ((rc = kad::gemm_launch(kad::F64, 1, r, dd->kb[b], nd, dd->Jc + (int64_t)b * nd * r, r,
+                                 dd->W + (int64_t)b * nd * kmax, nd, dd->cap + (int64_t)dd->q0[b] * r, r, s)))
+        return rc;
+    }
+    kad::dd_add_identity_kernel<<<(unsigned)kad::cdiv(r, 256), 256, 0, s>>>(dd->cap, r, r);
+    KAD_HIP_CHECK(hipGetLastError());
+    KAD_HIP_CHECK(hipStreamSynchronize(s));
+    double t4 = now_s();
+    rc = kad_lu_factor(reinterpret_cast<kad_ctx*>(c), KAD_PREC_F64, r, dd->cap, r, dd->cap_ipiv, dd->cap_perm, 0);
+    if (rc < 0) return rc;
+    if (rc > 0) return n + rc;  // singular capacitance matrix
+    if ((rc = kad::trsv_prepare<double>(dd->cap, r, r, 0, dd->cap_tab, dd->cap_flag, dd->cap_ext, s))) return rc;
+    KAD_HIP_CHECK(hipStreamSynchronize(s));
+    double t5 = now_s();
+    dd->times[2] = t3 - t2;
+    dd->times[3] = t4 - t3;
+    dd->times[4] = t5 - t4;
+  } else {
+    dd->times[2] = dd->times[3] = dd->times[4] = 0.0;
+  }
+  dd->times[0] = t1 - t0;
+  dd->times[1] = t2 - t1;
+  dd->factored = true;
+  return KAD_OK;
+}
+
+int kad_dd_solve(kad_dd* dd, const double* J, int64_t ldj, const double* b, double* x, double tol, int max_outer,
+                 int* iters, double* resid) {
+  if (!dd || !J || !b || !x || ldj < dd->n || max_outer < 1) return KAD_ERR_ARG;
+  if (!dd->factored) return KAD_ERR_ARG;
+  auto pre = [dd](double* v) { return dd_apply(dd, v); };
+  return kad::ir_loop(dd->c, dd->n, J, ldj, pre, b, x, tol, max_outer, 1, 0, iters, resid);
+}
+
+int kad_dd_apply(kad_dd* dd, double* v) {
+  if (!dd || !v || !dd->factored) return KAD_ERR_ARG;
+  const int rc = dd_apply(dd, v);
+  if (rc) return rc;
+  KAD_HIP_CHECK(hipStreamSynchronize(dd->c->stream));
+  return KAD_OK;
+}
+
+int kad_dd_info(const kad_dd* dd, int* sizes, double* times) {
+  if (!dd) return KAD_ERR_ARG;
+  if (sizes) {
+    sizes[0] = dd->nd;
+    sizes[1] = dd->r;
+    sizes[2] = dd->kmax;
+    sizes[3] = dd->nblk;
+  }
+  if (times)
+    for (int q = 0; q < 5; ++q) times[q] = dd->times[q];
+  return KAD_OK;
+}
+
+}  // extern "C"

@@ -97,6 +97,9 @@ __global__ __launch_bounds__(GNT) void gemm_k4_kernel(int m, int n, int k,
   constexpr int NB = BM * BK / GNT;
   constexpr int WT = BM / 2, TI = WT / 16;  // wave tile and its 16x16 MFMA tiles per side
   using acc_t = typename std::conditional<sizeof(T) == 8, d4v, f4v>::type;
+  A += (int64_t)blockIdx.z * sel.sa;
+  B += (int64_t)blockIdx.z * sel.sb;
+  C += (int64_t)blockIdx.z * sel.sc;
 
   __shared__ T As[2][BK * LDA_S];
   __shared__ T Bs[2][BK * LDB_S];
@@ -217,6 +220,9 @@ __global__ __launch_bounds__(GNT) void gemm_bf16_kernel(int m, int n, int k, con
   constexpr int BK = GemmCfg<BF16>::BK;
   constexpr int LDK = BK + 8;  // bf16 elements per LDS row (pad 16 bytes)
   constexpr int NA = GBM * BK / GNT, NB = GBN * BK / GNT;
+  A += (int64_t)blockIdx.z * sel.sa;
+  B += (int64_t)blockIdx.z * sel.sb;
+  C += (int64_t)blockIdx.z * sel.sc;
   __shared__ __bf16 As[2][GBM * LDK];
   __shared__ __bf16 Bs[2][GBN * LDK];
 
@@ -346,6 +352,16 @@ int gemm_launch(int prec, int op, int m, int n, int k, const void* A, int64_t ld
   return gemm_launch_sel(prec, op, m, n, k, A, lda, B, ldb, C, ldc, TileSel(), s);
 }
 
+int gemm_launch_batched(int prec, int op, int m, int n, int k, const void* A, int64_t lda, int64_t sa, const void* B,
+                        int64_t ldb, int64_t sb, void* C, int64_t ldc, int64_t sc, int nbat, hipStream_t s) {
+  TileSel sel;
+  sel.nbat = nbat;
+  sel.sa = sa;
+  sel.sb = sb;
+  sel.sc = sc;
+  return gemm_launch_sel(prec, op, m, n, k, A, lda, B, ldb, C, ldc, sel, s);
+}
+
 int gemm_launch_masked(int prec, int op, int m, int n, int k, const void* A, int64_t lda, const void* B, int64_t ldb,
                        void* C, int64_t ldc, const int* rmask, const int* cmask, hipStream_t s) {
   TileSel sel;
@@ -359,18 +375,23 @@ int gemm_launch_masked(int prec, int op, int m, int n, int k, const void* A, int
 int gemm_launch_sel(int prec, int op, int m, int n, int k, const void* A, int64_t lda, const void* B, int64_t ldb,
                     void* C, int64_t ldc, const TileSel& sel, hipStream_t s) {
   if (sel.cnt && op != OP_SUB) return KAD_ERR_ARG;
+  if (sel.nbat < 1 || sel.nbat > 65535 || (sel.nbat > 1 && (sel.cnt || sel.rmask || sel.cmask))) {
+    fprintf(stderr, "[kadath_amd] gemm: batch of %d with masks or lists\n", sel.nbat);
+    return KAD_ERR_ARG;
+  }
   if (m <= 0 || n <= 0) return KAD_OK;
   if (k <= 0) {
     if (op == OP_SET) {
       const size_t es = (prec == F64) ? 8 : 4;
-      KAD_HIP_CHECK(hipMemset2DAsync(C, ldc * es, 0, (size_t)m * es, n, s));
+      for (int z = 0; z < sel.nbat; ++z)
+        KAD_HIP_CHECK(hipMemset2DAsync((char*)C + (size_t)z * sel.sc * es, ldc * es, 0, (size_t)m * es, n, s));
     }
     return KAD_OK;
   }
   const int nblocks = (int)(cdiv(m, GBM) * cdiv(n, GBN));
   // fewer 128-tiles than two per CU (and not a compacted tile list): 64-tiles
-  const bool small = !sel.cnt && prec != BF16 && nblocks < 512;
-  dim3 grid(small ? (unsigned)(cdiv(m, 64) * cdiv(n, 64)) : (unsigned)nblocks), block(GNT);
+  const bool small = !sel.cnt && prec != BF16 && (int64_t)nblocks * sel.nbat < 512;
+  dim3 grid(small ? (unsigned)(cdiv(m, 64) * cdiv(n, 64)) : (unsigned)nblocks, 1, (unsigned)sel.nbat), block(GNT);
   GemmTimer* tm = g_gemm_timer;
   bool timed = false;
   if (tm) {
@@ -380,7 +401,7 @@ int gemm_launch_sel(int prec, int op, int m, int n, int k, const void* A, int64_
     double* slot = tm->scale_slot();
     const bool masked = sel.cnt || sel.rmask || sel.cmask;
     if (slot && masked) tile_fraction_kernel<<<1, 256, 0, s>>>(sel, nbm, nbn, slot);
-    const int rc = tm->begin(s, 2.0 * m * n * (double)k, masked);
+    const int rc = tm->begin(s, 2.0 * m * n * (double)k * sel.nbat, masked);
     if (rc < 0) return rc;
     timed = rc == 0;
   }

@@ -32,9 +32,17 @@ int gemm_launch_masked(int prec, int op, int m, int n, int k, const void* A, int
                        void* C, int64_t ldc, const int* rmask, const int* cmask, hipStream_t s);
 int gemm_launch_sel(int prec, int op, int m, int n, int k, const void* A, int64_t lda, const void* B, int64_t ldb,
                     void* C, int64_t ldc, const TileSel& sel, hipStream_t s);
+int gemm_launch_batched(int prec, int op, int m, int n, int k, const void* A, int64_t lda, int64_t sa, const void* B,
+                        int64_t ldb, int64_t sb, void* C, int64_t ldc, int64_t sc, int nbat, hipStream_t s);
 constexpr int GOP_SUB = 0, GOP_SET = 1;
 
 constexpr int PANEL_BASE = 32;  // columns factored by the per-column kernel
+
+// An argument error of the factorisation or its solves, with its reason on stderr.
+inline int arg_error(const char* why) {
+  std::fprintf(stderr, "[kadath_amd] %s\n", why);
+  return KAD_ERR_ARG;
+}
 
 // Block-wide (max |a|, first index) reduction.
 template <typename T>
@@ -62,7 +70,10 @@ __device__ void block_argmax(double v, int idx, double* sv, int* si, double& bv,
 
 // Row interchanges of pivots [k1, k2) applied to columns [c0, c1): one thread per column.
 template <typename T>
-__global__ void laswp_kernel(T* A, int64_t lda, int c0, int c1, const int* __restrict__ ipiv, int k1, int k2) {
+__global__ void laswp_kernel(T* A, int64_t lda, int c0, int c1, const int* __restrict__ ipiv, int k1, int k2,
+                             int64_t sA, int sP) {
+  A += (int64_t)blockIdx.y * sA;  // matrix blockIdx.y of a batch
+  ipiv += (int64_t)blockIdx.y * sP;
   const int c = c0 + blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= c1) return;
   T* col = A + (int64_t)c * lda;
@@ -245,8 +256,11 @@ constexpr int TRSM_FUSED = 32;
 
 template <typename T>
 __global__ __launch_bounds__(64, 4) void trsm_leaf_kernel(const T* __restrict__ L, int64_t lda, T* __restrict__ B,
-                                                       int64_t ldb, int ncols, const int* __restrict__ cmask) {
+                                                       int64_t ldb, int ncols, const int* __restrict__ cmask,
+                                                       int64_t sL, int64_t sB) {
   constexpr int R = TRSM_FUSED;
+  L += (int64_t)blockIdx.y * sL;  // system blockIdx.y of a batch
+  B += (int64_t)blockIdx.y * sB;
   __builtin_amdgcn_s_setprio(2);  // a serial chain on the critical path, beside the bulk GEMM
   __shared__ T Ls[R * R + R];  // column-major, Ls[i + R k] = L(i, k); R entries of padding read past the end
   const int c0 = blockIdx.x * 64;
@@ -270,24 +284,34 @@ __global__ __launch_bounds__(64, 4) void trsm_leaf_kernel(const T* __restrict__ 
   }
 }
 
+// nbat > 1: nbat systems at L + z sL, A12 + z sB (no masks; r a multiple of 32, so
+// every leaf is a 32-row kernel).
 template <typename T>
 int trsm_left_unit_lower(const T* L, int64_t lda, int r, T* A12, int64_t ld12, int ncols, LuWork& w, int gemm_prec,
-                         const int* cmask = nullptr) {
+                         const int* cmask = nullptr, int nbat = 1, int64_t sL = 0, int64_t sB = 0) {
   if (ncols <= 0 || r <= 0) return KAD_OK;
+  if (nbat > 1 && cmask) return arg_error("trsm_left_unit_lower: masks with a batch");
   if (r > TRSM_LEAF || r == 2 * TRSM_FUSED) {
     const int h = std::max(32, (r / 2) / 32 * 32);
-    int rc = trsm_left_unit_lower<T>(L, lda, h, A12, ld12, ncols, w, gemm_prec, cmask);
+    int rc = trsm_left_unit_lower<T>(L, lda, h, A12, ld12, ncols, w, gemm_prec, cmask, nbat, sL, sB);
     if (rc) return rc;
-    rc = gemm_launch_masked(gemm_prec == BF16 ? F32 : gemm_prec, GOP_SUB, r - h, ncols, h, L + h, lda, A12, ld12,
-                            A12 + h, ld12, nullptr, cmask, w.stream);
+    const int gp = gemm_prec == BF16 ? F32 : gemm_prec;
+    rc = nbat > 1 ? gemm_launch_batched(gp, GOP_SUB, r - h, ncols, h, L + h, lda, sL, A12, ld12, sB, A12 + h, ld12, sB,
+                                        nbat, w.stream)
+                  : gemm_launch_masked(gp, GOP_SUB, r - h, ncols, h, L + h, lda, A12, ld12, A12 + h, ld12, nullptr,
+                                       cmask, w.stream);
     if (rc) return rc;
-    return trsm_left_unit_lower<T>(L + h + (int64_t)h * lda, lda, r - h, A12 + h, ld12, ncols, w, gemm_prec, cmask);
+    return trsm_left_unit_lower<T>(L + h + (int64_t)h * lda, lda, r - h, A12 + h, ld12, ncols, w, gemm_prec, cmask,
+                                   nbat, sL, sB);
   }
   if (r == TRSM_FUSED) {
-    trsm_leaf_kernel<T><<<(unsigned)cdiv(ncols, 64), 64, 0, w.stream>>>(L, lda, A12, ld12, ncols, cmask);
+    trsm_leaf_kernel<T><<<dim3((unsigned)cdiv(ncols, 64), nbat), 64, 0, w.stream>>>(L, lda, A12, ld12, ncols, cmask,
+                                                                                  sL, sB);
     KAD_HIP_CHECK(hipGetLastError());
     return KAD_OK;
   }
+  // leaves of other orders use an explicit inverse (one system)
+  if (nbat > 1) return arg_error("trsm_left_unit_lower: a batch needs orders that are multiples of 32");
   T* linv = (T*)w.tinv;
   // Column chunks so that the scratch holds r x chunk; with more than one chunk they
   // are multiples of 128 (the mask granularity), else the masks are not used.
@@ -311,6 +335,76 @@ int trsm_left_unit_lower(const T* L, int64_t lda, int r, T* A12, int64_t ld12, i
   return KAD_OK;
 }
 
+// U X = B in place for a non-unit upper U of order 32 and B of 32 rows: the leaf of
+// trsm_left_upper.  The rows are taken in reverse (U'(i, k) = U(31 - i, 31 - k) is
+// lower), so the lane's register window works as in trsm_leaf_kernel; each step
+// divides by the diagonal, then updates, as LAPACK's xTRSM ('L', 'U', 'N', 'N').
+template <typename T>
+__global__ __launch_bounds__(64, 4) void trsm_leaf_upper_kernel(const T* __restrict__ U, int64_t ldu,
+                                                             T* __restrict__ B, int64_t ldb, int ncols, int64_t sU,
+                                                             int64_t sB) {
+  constexpr int R = TRSM_FUSED;
+  __builtin_amdgcn_s_setprio(2);
+  U += (int64_t)blockIdx.y * sU;  // system blockIdx.y of a batch
+  B += (int64_t)blockIdx.y * sB;
+  __shared__ T Us[R * R + R];  // Us[i + R k] = U(R-1-i, R-1-k); R entries of padding read past the end
+  for (int e = threadIdx.x; e < R * R; e += 64) {
+    const int i = e % R, k = e / R;
+    Us[e] = U[(int64_t)(R - 1 - i) + (int64_t)(R - 1 - k) * ldu];
+  }
+  if (threadIdx.x < R) Us[R * R + threadIdx.x] = T(0);
+  const int c0 = blockIdx.x * 64;
+  const int c = c0 + threadIdx.x;
+  const bool act = c < ncols;
+  T* b = B + (int64_t)(act ? c : c0) * ldb;
+  T y[R];
+#pragma unroll
+  for (int i = 0; i < R; ++i) y[i] = act ? b[R - 1 - i] : T(0);
+  __syncthreads();
+#pragma unroll 1
+  for (int k = 0; k < R; ++k) {
+    const T* uk = Us + k + R * k;  // uk[i] = U'(k + i, k)
+    const T xk = y[0] / uk[0];
+    if (act) b[R - 1 - k] = xk;
+#pragma unroll
+    for (int i = 1; i < R; ++i) y[i - 1] = y[i] - uk[i] * xk;
+  }
+}
+
+// B <- U^{-1} B with U non-unit upper (r x r, r a multiple of 32): X2 = U22^{-1} B2,
+// B1 -= U12 X2, X1 = U11^{-1} B1, recursively down to 32-row leaves.  nbat systems at
+// U + z sU, B + z sB.
+template <typename T>
+int trsm_left_upper(const T* U, int64_t ldu, int r, T* B, int64_t ldb, int ncols, LuWork& w, int gemm_prec, int nbat,
+                    int64_t sU, int64_t sB) {
+  if (ncols <= 0 || r <= 0) return KAD_OK;
+  if (r % TRSM_FUSED != 0) return arg_error("trsm_left_upper: order not a multiple of 32");
+  if (r == TRSM_FUSED) {
+    trsm_leaf_upper_kernel<T><<<dim3((unsigned)cdiv(ncols, 64), nbat), 64, 0, w.stream>>>(U, ldu, B, ldb, ncols, sU,
+                                                                                        sB);
+    KAD_HIP_CHECK(hipGetLastError());
+    return KAD_OK;
+  }
+  const int h = std::max(32, (r / 2) / 32 * 32);
+  int rc = trsm_left_upper<T>(U + h + (int64_t)h * ldu, ldu, r - h, B + h, ldb, ncols, w, gemm_prec, nbat, sU, sB);
+  if (rc) return rc;
+  rc = gemm_launch_batched(gemm_prec, GOP_SUB, h, ncols, r - h, U + (int64_t)h * ldu, ldu, sU, B + h, ldb, sB, B, ldb,
+                           sB, nbat, w.stream);
+  if (rc) return rc;
+  return trsm_left_upper<T>(U, ldu, h, B, ldb, ncols, w, gemm_prec, nbat, sU, sB);
+}
+
+// B <- A^{-1} B for nbat factored matrices of order n (n a multiple of 32) whose
+// right-hand sides are already in pivoted row order (B_z rows permuted by perm_z):
+// the unit-lower, then the upper triangular solve, in FP64.
+int lu_solve_permuted_batched(const double* LU, int64_t lda, int n, int nbat, int64_t sA, double* B, int64_t ldb,
+                              int64_t sB, int nrhs, LuWork& w) {
+  if (n % TRSM_FUSED != 0 || nbat < 1) return arg_error("lu_solve_permuted_batched: order or batch");
+  int rc = trsm_left_unit_lower<double>(LU, lda, n, B, ldb, nrhs, w, F64, nullptr, nbat, sA, sB);
+  if (rc) return rc;
+  return trsm_left_upper<double>(LU, lda, n, B, ldb, nrhs, w, F64, nbat, sA, sB);
+}
+
 // Row interchanges of pivots [k1, k2) (at most 512) applied to COLS columns per
 // workgroup through LDS.  Slots 0 .. nsw-1 hold rows k1 .. k2-1; slot nsw + j holds the
 // pivot row of step j when that row lies below k2 and step j is its first occurrence.
@@ -319,7 +413,10 @@ int trsm_left_unit_lower(const T* L, int64_t lda, int r, T* A12, int64_t ld12, i
 // result equals the sequential interchanges.
 template <typename T, int NSW, int COLS>
 __global__ __launch_bounds__(256) void laswp_lds_kernel(T* __restrict__ A, int64_t lda, int c0, int c1,
-                                                        const int* __restrict__ ipiv, int k1, int k2) {
+                                                        const int* __restrict__ ipiv, int k1, int k2, int64_t sA,
+                                                        int sP) {
+  A += (int64_t)blockIdx.y * sA;  // matrix blockIdx.y of a batch
+  ipiv += (int64_t)blockIdx.y * sP;
   __shared__ double vals[2 * NSW * COLS];  // [slot][column]
   __shared__ int slot_of[NSW];
   __shared__ int prow[NSW];                // row of slot nsw + j, or -1
@@ -380,17 +477,22 @@ __global__ __launch_bounds__(256) void laswp_lds_kernel(T* __restrict__ A, int64
 }
 
 template <typename T>
-int laswp(T* A, int64_t lda, int c0, int c1, const int* ipiv, int k1, int k2, LuWork& w) {
+int laswp(T* A, int64_t lda, int c0, int c1, const int* ipiv, int k1, int k2, LuWork& w,
+          const LuBatch& bt = LuBatch()) {
   if (c1 <= c0 || k2 <= k1) return KAD_OK;
   const int nsw = k2 - k1;
+  const unsigned nz = (unsigned)bt.nbat;
   // 8 (4) columns per workgroup: ~33 KB of LDS, so the workgroups fit beside the
   // concurrent update's GEMM workgroups instead of waiting for a whole CU
   if (nsw <= 256) {
-    laswp_lds_kernel<T, 256, 8><<<(unsigned)cdiv(c1 - c0, 8), 256, 0, w.stream>>>(A, lda, c0, c1, ipiv, k1, k2);
+    laswp_lds_kernel<T, 256, 8><<<dim3((unsigned)cdiv(c1 - c0, 8), nz), 256, 0, w.stream>>>(A, lda, c0, c1, ipiv, k1,
+                                                                                           k2, bt.sA, bt.sP);
   } else if (nsw <= 512) {
-    laswp_lds_kernel<T, 512, 4><<<(unsigned)cdiv(c1 - c0, 4), 256, 0, w.stream>>>(A, lda, c0, c1, ipiv, k1, k2);
+    laswp_lds_kernel<T, 512, 4><<<dim3((unsigned)cdiv(c1 - c0, 4), nz), 256, 0, w.stream>>>(A, lda, c0, c1, ipiv, k1,
+                                                                                           k2, bt.sA, bt.sP);
   } else {
-    laswp_kernel<T><<<(unsigned)cdiv(c1 - c0, 256), 256, 0, w.stream>>>(A, lda, c0, c1, ipiv, k1, k2);
+    laswp_kernel<T><<<dim3((unsigned)cdiv(c1 - c0, 256), nz), 256, 0, w.stream>>>(A, lda, c0, c1, ipiv, k1, k2, bt.sA,
+                                                                                bt.sP);
   }
   KAD_HIP_CHECK(hipGetLastError());
   return KAD_OK;
@@ -665,7 +767,8 @@ __global__ __launch_bounds__(FFP_THREADS) void ffp_kernel(T* __restrict__ A, int
                                                           int k0, int k1, int rpw,
                                                           unsigned long long* __restrict__ xbuf, unsigned flag0,
                                                           int* __restrict__ ipiv, int* __restrict__ info,
-                                                          int* __restrict__ sync, int spin_limit) {
+                                                          int* __restrict__ sync, int spin_limit, int64_t sA, int sP,
+                                                          int64_t sX) {
   static_assert(FFP_MAX_GRID <= 64, "one candidate per lane of a wave");
   static_assert(FFP_THREADS == 256, "256-slot strips");
   static_assert((W & (W - 1)) == 0, "rotation mask");
@@ -683,6 +786,12 @@ __global__ __launch_bounds__(FFP_THREADS) void ffp_kernel(T* __restrict__ A, int
   __shared__ int s_src[2 * W];
   __shared__ int s_nrows;
   __shared__ __align__(16) T stage[2][W];  // the candidate row and row j, for their waves
+  // matrix blockIdx.y of a batch: its own rows, interchanges, exchange slots and info
+  // word; the workgroups of one matrix (blockIdx.x) exchange only among themselves
+  A += (int64_t)blockIdx.y * sA;
+  ipiv += (int64_t)blockIdx.y * sP;
+  xbuf += (int64_t)blockIdx.y * sX;
+  info += blockIdx.y;
   const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x;
   const T* const jrow = reinterpret_cast<const T*>(xim + G * S::REC);
   const int r0 = c0 + g * rpw;
@@ -928,8 +1037,11 @@ __global__ __launch_bounds__(FFP_THREADS) void ffp_kernel(T* __restrict__ A, int
   }
 }
 
-// Bytes of the exchange slots of the persistent base panel (W columns, FP64 records).
-size_t ffp_bytes() { return (size_t)PANEL_BASE * FfpSlots<double, PANEL_BASE>::COL * sizeof(unsigned long long); }
+// Exchange slots of the persistent base panel for one matrix (W columns, FP64
+// records), in 64-bit words, and the bytes of the whole buffer (one region per
+// matrix of a batch, LU_MAX_BATCH regions).
+constexpr int64_t ffp_matrix_words() { return (int64_t)PANEL_BASE * FfpSlots<double, PANEL_BASE>::COL; }
+size_t ffp_bytes() { return (size_t)LU_MAX_BATCH * ffp_matrix_words() * sizeof(unsigned long long); }
 
 // Launch shape of the persistent base panel for m rows of element type T: grid and
 // rows per workgroup, or false when it does not fit (the per-column kernels run).
@@ -946,11 +1058,16 @@ bool ffp_shape(int m, int ncu, int& grid, int& rpw) {
 
 // One persistent base panel: columns [c0, c1) of rows [c0, n), interchanges applied
 // across the panel's columns [k0, k1).
+// A batch launches the workgroups of every matrix together (grid y); all of them must
+// be resident at once, so a batch holds at most one workgroup per CU in total.
 template <typename T>
-int ffp_launch(T* A, int64_t lda, int n, int c0, int c1, int k0, int k1, int* ipiv, LuWork& w) {
+int ffp_launch(T* A, int64_t lda, int n, int c0, int c1, int k0, int k1, int* ipiv, LuWork& w,
+               const LuBatch& bt = LuBatch()) {
   int grid, rpw;
   if (!ffp_shape<T>(n - c0, w.ncu, grid, rpw) || rpw > FFP_THREADS || c1 - c0 > PANEL_BASE || c1 > n)
-    return KAD_ERR_ARG;
+    return arg_error("ffp_launch: panel shape");
+  if (bt.nbat < 1 || bt.nbat > LU_MAX_BATCH || grid * bt.nbat > w.ncu || (bt.nbat > 1 && !bt.info))
+    return arg_error("ffp_launch: batch does not fit the CUs");
   // flags: PANEL_BASE per launch, never one the slots still hold (the buffer starts
   // zeroed and the sequence at 1; before the 32-bit flags wrap it is cleared again)
   if (++w.ffp_seq >= (1u << 26)) {
@@ -958,8 +1075,9 @@ int ffp_launch(T* A, int64_t lda, int n, int c0, int c1, int k0, int k1, int* ip
     w.ffp_seq = 1;
   }
   const unsigned flag0 = w.ffp_seq * (unsigned)PANEL_BASE;
-  ffp_kernel<T, PANEL_BASE><<<grid, FFP_THREADS, 0, w.stream>>>(
-      A, lda, n, c0, c1, k0, k1, rpw, (unsigned long long*)w.ffp, flag0, ipiv, w.info, w.ffp_sync, w.ffp_spin);
+  ffp_kernel<T, PANEL_BASE><<<dim3(grid, bt.nbat), FFP_THREADS, 0, w.stream>>>(
+      A, lda, n, c0, c1, k0, k1, rpw, (unsigned long long*)w.ffp, flag0, ipiv, bt.info ? bt.info : w.info, w.ffp_sync,
+      w.ffp_spin, bt.sA, bt.sP, ffp_matrix_words());
   KAD_HIP_CHECK(hipGetLastError());
   return KAD_OK;
 }
@@ -997,24 +1115,28 @@ int panel_base(T* A, int64_t lda, int n, int c0, int c1, int* ipiv, LuWork& w) {
 // Recursive panel factorisation of columns [c0, c1), rows [c0, n), inside the panel
 // [pk0, pk1).  fused: the base panels are the persistent kernel, which applies its
 // interchanges across [pk0, pk1) itself, so the recursion swaps no rows.
+// A batch (bt.nbat > 1) runs the persistent kernel only.
 template <typename T>
-int panel_rec(T* A, int64_t lda, int n, int c0, int c1, int* ipiv, LuWork& w, int pk0, int pk1, bool fused) {
+int panel_rec(T* A, int64_t lda, int n, int c0, int c1, int* ipiv, LuWork& w, int pk0, int pk1, bool fused,
+              const LuBatch& bt = LuBatch()) {
   const int wdt = c1 - c0;
+  if (bt.nbat > 1 && !fused) return arg_error("panel_rec: a batch needs the persistent panel");
   if (wdt <= PANEL_BASE)
-    return fused ? ffp_launch<T>(A, lda, n, c0, c1, pk0, pk1, ipiv, w) : panel_base<T>(A, lda, n, c0, c1, ipiv, w);
+    return fused ? ffp_launch<T>(A, lda, n, c0, c1, pk0, pk1, ipiv, w, bt) : panel_base<T>(A, lda, n, c0, c1, ipiv, w);
   int h = wdt / 2;
   h = (h + 15) / 16 * 16;
   const int cm = c0 + h;
-  int rc = panel_rec<T>(A, lda, n, c0, cm, ipiv, w, pk0, pk1, fused);
+  int rc = panel_rec<T>(A, lda, n, c0, cm, ipiv, w, pk0, pk1, fused, bt);
   if (rc) return rc;
   if (!fused && (rc = laswp<T>(A, lda, cm, c1, ipiv, c0, cm, w))) return rc;
   rc = trsm_left_unit_lower<T>(A + c0 + (int64_t)c0 * lda, lda, h, A + c0 + (int64_t)cm * lda, lda, c1 - cm, w,
-                               prec_of<T>());
+                               prec_of<T>(), nullptr, bt.nbat, bt.sA, bt.sA);
   if (rc) return rc;
-  rc = gemm_launch(prec_of<T>(), GOP_SUB, n - cm, c1 - cm, h, A + cm + (int64_t)c0 * lda, lda,
-                   A + c0 + (int64_t)cm * lda, lda, A + cm + (int64_t)cm * lda, lda, w.stream);
+  rc = gemm_launch_batched(prec_of<T>(), GOP_SUB, n - cm, c1 - cm, h, A + cm + (int64_t)c0 * lda, lda, bt.sA,
+                           A + c0 + (int64_t)cm * lda, lda, bt.sA, A + cm + (int64_t)cm * lda, lda, bt.sA, bt.nbat,
+                           w.stream);
   if (rc) return rc;
-  rc = panel_rec<T>(A, lda, n, cm, c1, ipiv, w, pk0, pk1, fused);
+  rc = panel_rec<T>(A, lda, n, cm, c1, ipiv, w, pk0, pk1, fused, bt);
   if (rc) return rc;
   return fused ? KAD_OK : laswp<T>(A, lda, c0, cm, ipiv, cm, c1, w);
 }
@@ -1102,8 +1224,12 @@ int apply_left_swaps(T* A, int64_t lda, int n, const int* ipiv, int nb, LuWork& 
 // its own triangular-inverse / GEMM scratch.  The L21 row-tile masks rotate over
 // three buffers: part B of step p still reads panel p's mask while the panel stream
 // writes panel p+2's.
+//
+// A batch (bt.nbat > 1) factors bt.nbat matrices of order n together: every launch
+// covers all of them, so the panel chain is that of one matrix.  The matrices are
+// taken as dense (no structural-zero masks or row extents).
 template <typename T>
-int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w) {
+int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w, const LuBatch& bt) {
   // both streams' scratch must exist before anything is launched
   if (!w.stream2 || !w.tinv || !w.tinv_m || !w.tmp || !w.tmp_m || w.tmp_m_bytes < w.tmp_bytes || !w.tmp2 ||
       !w.mask_r || !w.mask_r2 || !w.mask_r3 || !w.ext)
@@ -1127,14 +1253,19 @@ int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w)
   // A panel's rows at or below R are zero in its columns and stay zero: the
   // interchanges and the update of an earlier panel q touch rows above R_q only,
   // so R_p = max(initial extent of p's columns, R_q for q < p) bounds them.
+  const bool bat = bt.nbat > 1;
+  if (bt.nbat < 1 || bt.nbat > LU_MAX_BATCH) return arg_error("getrf: batch size");
   const int ntc = (int)cdiv(n, 128), nrc = (int)cdiv(n, 2048);
-  col_extent_kernel<T><<<dim3(ntc, nrc), 256, 0, w.stream>>>(A, lda, n, w.ext);
-  KAD_HIP_CHECK(hipGetLastError());
-  std::vector<int> hext((size_t)ntc * nrc), ext(ntc, -1);
-  KAD_HIP_CHECK(hipMemcpyAsync(hext.data(), w.ext, hext.size() * sizeof(int), hipMemcpyDeviceToHost, w.stream));
-  KAD_HIP_CHECK(hipStreamSynchronize(w.stream));
-  for (int r = 0; r < nrc; ++r)
-    for (int t = 0; t < ntc; ++t) ext[t] = std::max(ext[t], hext[(size_t)r * ntc + t]);
+  std::vector<int> ext(ntc, bat ? n - 1 : -1);
+  if (!bat) {
+    col_extent_kernel<T><<<dim3(ntc, nrc), 256, 0, w.stream>>>(A, lda, n, w.ext);
+    KAD_HIP_CHECK(hipGetLastError());
+    std::vector<int> hext((size_t)ntc * nrc);
+    KAD_HIP_CHECK(hipMemcpyAsync(hext.data(), w.ext, hext.size() * sizeof(int), hipMemcpyDeviceToHost, w.stream));
+    KAD_HIP_CHECK(hipStreamSynchronize(w.stream));
+    for (int r = 0; r < nrc; ++r)
+      for (int t = 0; t < ntc; ++t) ext[t] = std::max(ext[t], hext[(size_t)r * ntc + t]);
+  }
   int r_prev = 0;
   // factor panel [k0, k1) on the panel stream (rows [k0, R)), then its L21 mask
   auto panel = [&](int k0, int k1, int* mask) -> int {
@@ -1144,9 +1275,9 @@ int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w)
     r_prev = R;
     int fg, frpw;
     const bool fused = wp.ffp && !wp.ffp_off && ffp_shape<T>(R - k0, wp.ncu, fg, frpw);
-    int rc = panel_rec<T>(A, lda, R, k0, k1, ipiv, wp, k0, k1, fused);
+    int rc = panel_rec<T>(A, lda, R, k0, k1, ipiv, wp, k0, k1, fused, bt);
     if (rc) return rc;
-    if (k1 < n) {
+    if (k1 < n && mask) {
       // the panel's rows at or below R are zero in its columns
       row_tile_mask_kernel<T><<<(unsigned)cdiv(n - k1, 128), 256, 0, wp.stream>>>(A + k1 + (int64_t)k0 * lda, lda,
                                                                                  n - k1, k1 - k0, mask, R - k1);
@@ -1157,7 +1288,7 @@ int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w)
   // the caller's work on the main stream precedes the first panel
   KAD_HIP_CHECK(hipEventRecord(w.ev_start, w.stream));
   KAD_HIP_CHECK(hipStreamWaitEvent(wp.stream, w.ev_start, 0));
-  int rc = panel(0, std::min(nb, n), mask_rows[0]);
+  int rc = panel(0, std::min(nb, n), bat ? nullptr : mask_rows[0]);
   if (rc) return rc;
   KAD_HIP_CHECK(hipEventRecord(w.ev_panel[0], wp.stream));
   for (int p = 0, k0 = 0; k0 < n; ++p, k0 += nb) {
@@ -1165,13 +1296,23 @@ int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w)
     const int k2 = std::min(k1 + nb, n);
     const int k3 = std::min(k2 + nb, n);
     const int par = p & 1;
-    const int* mrows = mask_rows[p % 3];
+    const int* mrows = bat ? nullptr : mask_rows[p % 3];
     const T* L21 = A + k1 + (int64_t)k0 * lda;
     const T* L11 = A + k0 + (int64_t)k0 * lda;
     // ---- main stream, part A: columns [k2, k3)
     KAD_HIP_CHECK(hipStreamWaitEvent(w.stream, w.ev_panel[par], 0));
     if (k1 >= n) break;
-    if (k2 < n) {
+    if (k2 < n && bat) {
+      rc = laswp<T>(A, lda, k2, k3, ipiv, k0, k1, wm, bt);
+      if (rc) return rc;
+      rc = trsm_left_unit_lower<T>(L11, lda, k1 - k0, A + k0 + (int64_t)k2 * lda, lda, k3 - k2, wm, gemm_prec,
+                                   nullptr, bt.nbat, bt.sA, bt.sA);
+      if (rc) return rc;
+      rc = gemm_launch_batched(gemm_prec, GOP_SUB, n - k1, k3 - k2, k1 - k0, L21, lda, bt.sA,
+                               A + k0 + (int64_t)k2 * lda, lda, bt.sA, A + k1 + (int64_t)k2 * lda, lda, bt.sA,
+                               bt.nbat, w.stream);
+      if (rc) return rc;
+    } else if (k2 < n) {
       rc = laswp<T>(A, lda, k2, k3, ipiv, k0, k1, wm);
       if (rc) return rc;
       // structural zeros of A12 (column tiles; U12 = L11^{-1} A12 keeps them)
@@ -1187,7 +1328,17 @@ int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w)
     }
     KAD_HIP_CHECK(hipEventRecord(w.ev_next[par], w.stream));
     // ---- main stream, part B: columns [k3, n), beside the next panel
-    if (k3 < n) {
+    if (k3 < n && bat) {
+      rc = laswp<T>(A, lda, k3, n, ipiv, k0, k1, wm, bt);
+      if (rc) return rc;
+      rc = trsm_left_unit_lower<T>(L11, lda, k1 - k0, A + k0 + (int64_t)k3 * lda, lda, n - k3, wm, gemm_prec,
+                                   nullptr, bt.nbat, bt.sA, bt.sA);
+      if (rc) return rc;
+      rc = gemm_launch_batched(gemm_prec, GOP_SUB, n - k1, n - k3, k1 - k0, L21, lda, bt.sA,
+                               A + k0 + (int64_t)k3 * lda, lda, bt.sA, A + k1 + (int64_t)k3 * lda, lda, bt.sA,
+                               bt.nbat, w.stream);
+      if (rc) return rc;
+    } else if (k3 < n) {
       rc = laswp<T>(A, lda, k3, n, ipiv, k0, k1, wm);
       if (rc) return rc;
       col_tile_mask_kernel<T><<<(unsigned)cdiv(n - k3, 128), 256, 0, w.stream>>>(A + k0 + (int64_t)k3 * lda, lda,
@@ -1210,22 +1361,28 @@ int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w)
     }
     // ---- panel stream: the next panel's columns [k1, k2), then the next panel
     if (p > 0) KAD_HIP_CHECK(hipStreamWaitEvent(wp.stream, w.ev_next[par ^ 1], 0));
-    rc = laswp<T>(A, lda, k1, k2, ipiv, k0, k1, wp);
+    rc = laswp<T>(A, lda, k1, k2, ipiv, k0, k1, wp, bt);
     if (rc) return rc;
-    rc = trsm_left_unit_lower<T>(L11, lda, k1 - k0, A + k0 + (int64_t)k1 * lda, lda, k2 - k1, wp, gemm_prec);
+    rc = trsm_left_unit_lower<T>(L11, lda, k1 - k0, A + k0 + (int64_t)k1 * lda, lda, k2 - k1, wp, gemm_prec, nullptr,
+                                 bt.nbat, bt.sA, bt.sA);
     if (rc) return rc;
-    rc = gemm_launch_masked(gemm_prec, GOP_SUB, n - k1, k2 - k1, k1 - k0, L21, lda, A + k0 + (int64_t)k1 * lda, lda,
-                            A + k1 + (int64_t)k1 * lda, lda, mrows, nullptr, wp.stream);
+    rc = bat ? gemm_launch_batched(gemm_prec, GOP_SUB, n - k1, k2 - k1, k1 - k0, L21, lda, bt.sA,
+                                   A + k0 + (int64_t)k1 * lda, lda, bt.sA, A + k1 + (int64_t)k1 * lda, lda, bt.sA,
+                                   bt.nbat, wp.stream)
+             : gemm_launch_masked(gemm_prec, GOP_SUB, n - k1, k2 - k1, k1 - k0, L21, lda, A + k0 + (int64_t)k1 * lda,
+                                  lda, A + k1 + (int64_t)k1 * lda, lda, mrows, nullptr, wp.stream);
     if (rc) return rc;
-    rc = panel(k1, k2, mask_rows[(p + 1) % 3]);
+    rc = panel(k1, k2, bat ? nullptr : mask_rows[(p + 1) % 3]);
     if (rc) return rc;
     KAD_HIP_CHECK(hipEventRecord(w.ev_panel[par ^ 1], wp.stream));
   }
-  return apply_left_swaps<T>(A, lda, n, ipiv, nb, w);
+  for (int z = 0; z < bt.nbat; ++z)
+    if ((rc = apply_left_swaps<T>(A + z * bt.sA, lda, n, ipiv + (int64_t)z * bt.sP, nb, w))) return rc;
+  return KAD_OK;
 }
 
-template int getrf<double>(double*, int64_t, int, int*, int, int, LuWork&);
-template int getrf<float>(float*, int64_t, int, int*, int, int, LuWork&);
+template int getrf<double>(double*, int64_t, int, int*, int, int, LuWork&, const LuBatch&);
+template int getrf<float>(float*, int64_t, int, int*, int, int, LuWork&, const LuBatch&);
 
 // perm from LAPACK-style ipiv (host side, O(n)).
 // Returns false (perm unspecified) if some ipiv[i] is outside [i, n): interchanges read

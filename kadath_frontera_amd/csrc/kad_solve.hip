@@ -13,11 +13,15 @@
 #include <algorithm>
 #include <cstring>
 #include <cstdlib>
+#include <functional>
 
 namespace kad {
 int gemm_launch(int prec, int op, int m, int n, int k, const void* A, int64_t lda, const void* B, int64_t ldb, void* C,
                 int64_t ldc, hipStream_t s);
-template <typename T> int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w);
+template <typename T>
+int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w, const LuBatch& bt = LuBatch());
+int lu_solve_permuted_batched(const double* LU, int64_t lda, int n, int nbat, int64_t sA, double* B, int64_t ldb,
+                              int64_t sB, int nrhs, LuWork& w);
 size_t trsv_table_doubles(int n);
 size_t trsv_flag_ints(int n);
 size_t ffp_bytes();
@@ -109,9 +113,11 @@ int Ctx::lu_work(int n, int nb) {
   }
   lu.red_cap = red;
   if (!lu.counter) {
-    KAD_HIP_CHECK(hipMalloc((void**)&lu.counter, 64));
-    KAD_HIP_CHECK(hipMemsetAsync(lu.counter, 0, 64, stream));
+    // counter, info (+16 B), the persistent panel's words (+32 B), batch info (+64 B)
+    KAD_HIP_CHECK(hipMalloc((void**)&lu.counter, 64 + LU_MAX_BATCH * sizeof(int)));
+    KAD_HIP_CHECK(hipMemsetAsync(lu.counter, 0, 64 + LU_MAX_BATCH * sizeof(int), stream));
     lu.info = (int*)((char*)lu.counter + 16);
+    lu.binfo = (int*)((char*)lu.counter + 64);
   }
   lu.ffp_sync = (int*)((char*)lu.counter + 32);  // [2]: abort flag (zeroed above)
   if (!lu.ffp) {
@@ -195,7 +201,7 @@ Ctx::~Ctx() {
   for (auto& kv : plans) delete kv.second;
   for (void* p : owned) (void)hipFree(p);
   void* bufs[] = {lu.red_val, lu.red_idx, lu.counter, lu.tinv,    lu.tmp,    lu.tmp2,  lu.map,  lu.mask_r,
-                  lu.swap_map, lu.ext,     lu.ffp,     gemv_part, vwork,     krylov,   trs_tab, trs_flag};
+                  lu.swap_map, lu.ext,     lu.ffp,     gemv_part, vwork,     krylov,   trs_tab, trs_flag, bsolve};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (int q = 0; q < 2; ++q) {
@@ -208,6 +214,9 @@ Ctx::~Ctx() {
   if (lu.stream2) (void)hipStreamDestroy(lu.stream2);
   if (own_stream && stream) (void)hipStreamDestroy(stream);
 }
+
+int ir_loop(Ctx* c, int n, const double* A64, int64_t lda, const std::function<int(double*)>& precond,
+            const double* b, double* x, double tol, int max_outer, int mode, int trans, int* iters, double* resid);
 
 }  // namespace kad
 
@@ -369,6 +378,95 @@ int kad_lu_solve(kad_ctx* ctx, int prec, int n, const void* LU, int64_t lda, con
   return c->solve(prec, n, LU, lda, perm, b, trans);
 }
 
+// nbat matrices of order n (a multiple of 256) at A + z strideA, factored together.
+int kad_lu_factor_batched(kad_ctx* ctx, int prec, int n, int nbat, void* A, int64_t lda, int64_t strideA, int* ipiv,
+                          int* perm, int* info) {
+  if (!ctx || !A || !ipiv || !info || n <= 0 || n % 256 != 0 || nbat < 1 || nbat > kad::LU_MAX_BATCH || lda < n ||
+      prec < 0 || prec > 2 || (nbat > 1 && strideA < lda * (int64_t)n)) {
+    std::fprintf(stderr, "kad_lu_factor_batched: bad arguments (n %d, nbat %d, lda %lld, strideA %lld)\n", n, nbat,
+                 (long long)lda, (long long)strideA);
+    return KAD_ERR_ARG;
+  }
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  const int nb = 256;
+  int rc = c->lu_work(n, nb);
+  if (rc) return rc;
+  // every matrix's workgroups of the persistent panel are resident together
+  if ((int)kad::cdiv(n, 256) * nbat > c->lu.ncu) {
+    std::fprintf(stderr, "kad_lu_factor_batched: %d matrices of order %d exceed the %d CUs\n", nbat, n, c->lu.ncu);
+    return KAD_ERR_ARG;
+  }
+  KAD_HIP_CHECK(hipMemsetAsync(c->lu.binfo, 0, nbat * sizeof(int), c->stream));
+  kad::LuBatch bt;
+  bt.nbat = nbat;
+  bt.sA = strideA;
+  bt.sP = n;
+  bt.info = c->lu.binfo;
+  if (prec == kad::F64)
+    rc = kad::getrf<double>((double*)A, lda, n, ipiv, nb, kad::F64, c->lu, bt);
+  else
+    rc = kad::getrf<float>((float*)A, lda, n, ipiv, nb, prec, c->lu, bt);
+  KAD_HIP_CHECK(hipStreamSynchronize(c->lu.stream2));
+  int words[4] = {0, 0, 0, 0};
+  KAD_HIP_CHECK(hipMemcpyAsync(words, c->lu.ffp_sync, sizeof(words), hipMemcpyDeviceToHost, c->stream));
+  KAD_HIP_CHECK(hipMemcpyAsync(info, c->lu.binfo, nbat * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  KAD_HIP_CHECK(hipStreamSynchronize(c->stream));
+  if (words[2] != 0) {
+    std::fprintf(stderr, "kad_lu_factor_batched: persistent panel wait timed out\n");
+    KAD_HIP_CHECK(hipMemsetAsync(c->lu.ffp_sync, 0, 4 * sizeof(int), c->stream));
+    KAD_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return KAD_ERR_HIP;
+  }
+  if (rc) return rc;
+  if (perm) {
+    std::vector<int> hp((size_t)n * nbat), hperm((size_t)n * nbat);
+    KAD_HIP_CHECK(hipMemcpyAsync(hp.data(), ipiv, hp.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    KAD_HIP_CHECK(hipStreamSynchronize(c->stream));
+    for (int z = 0; z < nbat; ++z)
+      if (!kad::ipiv_to_perm(hp.data() + (size_t)z * n, n, hperm.data() + (size_t)z * n)) return KAD_ERR_HIP;
+    KAD_HIP_CHECK(hipMemcpyAsync(perm, hperm.data(), hperm.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    KAD_HIP_CHECK(hipStreamSynchronize(c->stream));
+  }
+  return KAD_OK;
+}
+
+namespace {
+__global__ void gather_rows_batched_kernel(const double* __restrict__ B, int64_t ldb, int64_t sB,
+                                           const int* __restrict__ perm, int n, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int col = blockIdx.y, z = blockIdx.z;
+  if (i < n) out[i + (int64_t)n * (col + (int64_t)gridDim.y * z)] = B[perm[(int64_t)z * n + i] + col * ldb + z * sB];
+}
+__global__ void put_back_batched_kernel(const double* __restrict__ in, int n, double* __restrict__ B, int64_t ldb,
+                                        int64_t sB) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int col = blockIdx.y, z = blockIdx.z;
+  if (i < n) B[i + col * ldb + z * sB] = in[i + (int64_t)n * (col + (int64_t)gridDim.y * z)];
+}
+}  // namespace
+
+// B_z <- A_z^{-1} B_z (nrhs columns each) with the factors of kad_lu_factor_batched (FP64).
+int kad_lu_solve_batched(kad_ctx* ctx, int n, int nbat, const double* LU, int64_t lda, int64_t strideA, const int* perm,
+                         double* B, int64_t ldb, int64_t strideB, int nrhs) {
+  if (!ctx || !LU || !perm || !B || n <= 0 || n % 32 != 0 || nbat < 1 || nbat > 65535 || lda < n || ldb < n ||
+      nrhs < 1 || nrhs > 65535 || (nbat > 1 && strideB < ldb * (int64_t)nrhs))
+    return KAD_ERR_ARG;
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  int rc = c->lu_work(n, 256);
+  if (rc) return rc;
+  const size_t need = (size_t)n * nrhs * nbat * sizeof(double);
+  if ((rc = c->ensure((void**)&c->bsolve, &c->cap_bsolve, need))) return rc;
+  dim3 g((unsigned)kad::cdiv(n, 256), (unsigned)nrhs, (unsigned)nbat);
+  gather_rows_batched_kernel<<<g, 256, 0, c->stream>>>(B, ldb, strideB, perm, n, c->bsolve);
+  KAD_HIP_CHECK(hipGetLastError());
+  if ((rc = kad::lu_solve_permuted_batched(LU, lda, n, nbat, strideA, c->bsolve, n, (int64_t)n * nrhs, nrhs, c->lu)))
+    return rc;
+  put_back_batched_kernel<<<g, 256, 0, c->stream>>>(c->bsolve, n, B, ldb, strideB);
+  KAD_HIP_CHECK(hipGetLastError());
+  KAD_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return KAD_OK;
+}
+
 int kad_gemv64(kad_ctx* ctx, int n, const double* A, int64_t lda, const double* x, const double* b, double* y,
                int trans) {
   if (!ctx || n <= 0 || lda < n) return KAD_ERR_ARG;
@@ -461,11 +559,7 @@ int kad_ir_solve_scaled(kad_ctx* ctx, int n, const double* A64, int64_t lda, int
   if (rc) return rc;
   hipStream_t s = c->stream;
   const unsigned g = (unsigned)kad::cdiv(n, 256);
-  double* r = c->vwork + (size_t)2 * n;
-  double* d = c->vwork + (size_t)3 * n;
   if ((rc = c->solve_prepare(prec_lo, n, LUlo, ldlu, trans))) return rc;
-  if ((rc = c->jmap.build(A64, lda, n, s))) return rc;
-  const kad::GemvMap& Am = c->jmap;
   const double* pre = trans ? Cs : Rs;   // applied before the factor solve
   const double* post = trans ? Rs : Cs;  // applied after
   auto solve_lo = [&](double* v) {
@@ -474,6 +568,27 @@ int kad_ir_solve_scaled(kad_ctx* ctx, int n, const double* A64, int64_t lda, int
     if (!r && post) kad::vec_scale_kernel<<<g, 256, 0, s>>>(n, post, v);
     return r;
   };
+  return kad::ir_loop(c, n, A64, lda, solve_lo, b, x, tol, max_outer, mode, trans, iters, resid);
+}
+
+}  // extern "C"
+
+namespace kad {
+
+// The refinement loop of kad_ir_solve_scaled for any preconditioner: precond(v)
+// replaces v by M^{-1} v (op(M) for trans = 1).  Residuals and products use the
+// fp64 A; mode 0 is classical refinement, mode 1 flexible GMRES.
+int ir_loop(Ctx* c, int n, const double* A64, int64_t lda, const std::function<int(double*)>& precond,
+            const double* b, double* x, double tol, int max_outer, int mode, int trans, int* iters, double* resid) {
+  int rc = c->vec_work(n);
+  if (rc) return rc;
+  hipStream_t s = c->stream;
+  const unsigned g = (unsigned)kad::cdiv(n, 256);
+  double* r = c->vwork + (size_t)2 * n;
+  double* d = c->vwork + (size_t)3 * n;
+  if ((rc = c->jmap.build(A64, lda, n, s))) return rc;
+  const kad::GemvMap& Am = c->jmap;
+  auto& solve_lo = precond;
   const double bnorm = c->absmax(b, n);
   c->hist_inner.clear();
   c->hist_outer.assign(1, bnorm);
@@ -590,6 +705,10 @@ int kad_ir_solve_scaled(kad_ctx* ctx, int n, const double* A64, int64_t lda, int
   KAD_HIP_CHECK(hipStreamSynchronize(s));
   return (rel <= tol) ? KAD_OK : KAD_ERR_NOCONV;
 }
+
+}  // namespace kad
+
+extern "C" {
 
 // GMRES-IR Krylov dimension per restart of the context (default 100).
 int kad_ir_set_restart(kad_ctx* ctx, int m) {

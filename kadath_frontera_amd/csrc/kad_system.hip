@@ -20,7 +20,8 @@
 #include <cstring>
 
 namespace kad {
-template <typename T> int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w);
+template <typename T>
+int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w, const LuBatch& bt = LuBatch());
 bool ipiv_to_perm(const int* ipiv, int n, int* perm);
 }  // namespace kad
 
@@ -407,8 +408,11 @@ struct kad_system_impl {
   int prev_inner = 0;     // inner steps of the previous refinement
   double* Jfac = nullptr;  // solver 5: fp64 factors of an earlier Jacobian (J and Jfac alternate)
   bool fac_valid = false;  // Jfac / perm hold usable fp64 factors
+  kad_dd* dd = nullptr;    // solvers 6-7: the domain-decomposed factors (their own copy of the blocks)
+  bool dd_valid = false;
 
   ~kad_system_impl() {
+    if (dd) kad_dd_destroy(dd);
     for (void* p : allocs) (void)hipFree(p);
     delete sys;
   }
@@ -1013,6 +1017,46 @@ static int check_square(const SystemBase& sy) {
   return KAD_ERR_ARG;
 }
 
+// An interface condition that matches normal derivatives ("dn..." of the hand-traced
+// system, or a string holding dn( or dr( ) rather than values.
+static bool matches_derivative(const std::string& which) {
+  return which.rfind("dn", 0) == 0 || which.find("dn(") != std::string::npos ||
+         which.find("dr(") != std::string::npos;
+}
+
+// Blocks of the domain-decomposed solve: one per domain, with the domain's unknowns
+// and the rows of the equations that live there.  An interface's matching condition
+// lives in the inner domain when it matches values and in the outer one when it
+// matches normal derivatives, which makes every block square (the domain's operator
+// with a Dirichlet-type outer and a Neumann-type inner interface condition).  Integral
+// equations and double unknowns couple every domain: such systems have no blocks.
+static int dd_blocks(const SystemBase& sy, std::vector<int>& size, std::vector<int>& rows, std::vector<int>& cols) {
+  if (sy.ndouble > 0 || sy.n_rows != sy.n_unknowns) return KAD_ERR_ARG;
+  const int nb = sy.dmax - sy.dmin + 1;
+  std::vector<std::vector<int>> R(nb), C(nb);
+  for (auto& e : sy.eqs) {
+    if (e.kind == 4) return KAD_ERR_ARG;
+    const int d = (e.kind == 1 && matches_derivative(e.which)) ? e.d + 1 : e.d;
+    if (d < sy.dmin || d > sy.dmax) return KAD_ERR_ARG;
+    int len = 0;
+    for (int c : e.nrows) len += c;
+    for (int q = 0; q < len; ++q) R[d - sy.dmin].push_back(e.row0 + q);
+  }
+  for (auto& L : sy.layout)
+    for (int q = 0; q < L[3]; ++q) C[L[1] - sy.dmin].push_back(L[2] + q);
+  size.clear();
+  rows.clear();
+  cols.clear();
+  for (int b = 0; b < nb; ++b) {
+    if (R[b].size() != C[b].size() || R[b].empty()) return KAD_ERR_ARG;
+    std::sort(R[b].begin(), R[b].end());
+    size.push_back((int)R[b].size());
+    rows.insert(rows.end(), R[b].begin(), R[b].end());
+    cols.insert(cols.end(), C[b].begin(), C[b].end());
+  }
+  return KAD_OK;
+}
+
 // Set a parameter of the system: which indexes its parameters (XCTS: 0 = Ome, the
 // horizon rotation of kerr.cpp:91-95).  The constant field is rewritten in every
 // domain that holds it.
@@ -1234,7 +1278,57 @@ static int newton_direction(kad_system_impl* S, int solver, double err, double* 
   const double eta = S->forcing > 0.0 ? std::max(1e-13, S->forcing * std::min(1.0, err)) : 1e-13;
   // A refinement result is a usable Newton direction below this relative residual.
   constexpr double kAcceptResid = 1e-6;
-  if (solver == 5) {
+  if (solver == 6 || solver == 7) {
+    // domain-decomposed factors (the blocks factored together, then the Woodbury
+    // coupling) precondition flexible GMRES on the current J; solver 7 keeps them for
+    // later Jacobians while the refinement takes at most `limit` inner steps
+    S->lo_valid = false;
+    S->fac_valid = false;
+    if (!S->dd) {
+      std::vector<int> bs, rows, cols;
+      if (dd_blocks(sy, bs, rows, cols)) {
+        fprintf(stderr, "domain-decomposed solve: the system's domain blocks are not square\n");
+        return KAD_ERR_ARG;
+      }
+      if ((rc = kad_dd_create(cx, n, (int)bs.size(), bs.data(), rows.data(), cols.data(), &S->dd))) return rc;
+    }
+    const int limit = S->reuse_inner > 0 ? S->reuse_inner : 30;
+    bool solved = false;
+    if (solver == 7 && S->dd_valid && S->prev_inner <= limit) {
+      rc = kad_dd_solve(S->dd, S->J, n, S->F, S->X, eta, 8, S->last_iters, &S->last_resid);
+      if (rc < 0 && rc != KAD_ERR_NOCONV) return rc;
+      S->prev_inner = S->last_iters[1];
+      solved = rc == KAD_OK || S->last_resid <= std::max(eta, kAcceptResid);
+      S->last_factored = 0;
+    }
+    if (!solved) {
+      KAD_HIP_CHECK(hipStreamSynchronize(st));
+      const double tg0 = now_s();
+      if ((rc = kad_dd_factor(S->dd, S->J, n)) < 0) return rc;
+      tf1 += now_s() - tg0;
+      info = rc;
+      S->dd_valid = (info == 0);
+      S->last_factored = 1;
+      if (info == 0) {
+        rc = kad_dd_solve(S->dd, S->J, n, S->F, S->X, eta, 8, S->last_iters, &S->last_resid);
+        if (rc < 0 && rc != KAD_ERR_NOCONV) return rc;
+        S->prev_inner = S->last_iters[1];
+        if (rc == KAD_ERR_NOCONV && !(S->last_resid <= std::max(eta, kAcceptResid))) {
+          // not a usable Newton direction: the fp64 LU of the same, untouched J
+          S->last_fallback = 1;
+          const double tg1 = now_s();
+          if ((rc = kad_lu_factor(cx, KAD_PREC_F64, n, S->J, n, S->ipiv, S->perm, 0)) < 0) return rc;
+          tf1 += now_s() - tg1;
+          info = rc;
+          S->dd_valid = false;
+          if (info == 0) {
+            KAD_HIP_CHECK(hipMemcpyAsync(S->X, S->F, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, st));
+            if ((rc = kad_lu_solve(cx, KAD_PREC_F64, n, S->J, n, S->perm, S->X, 0))) return rc;
+          }
+        }
+      }
+    }
+  } else if (solver == 5) {
     // fp64 factors of an earlier J precondition flexible GMRES on the current J while
     // the previous refinement took at most `limit` inner steps; otherwise the current J
     // is factored in place and solved directly, and its factors are kept (the next fill
@@ -1268,6 +1362,7 @@ static int newton_direction(kad_system_impl* S, int solver, double err, double* 
   } else if (solver == 0 || solver == 4) {
     S->lo_valid = false;  // the fp64 factorisation takes over ipiv / perm
     S->fac_valid = false;
+    S->dd_valid = false;
     S->last_factored = 1;
     double* M = S->J;
     if (solver == 4) {
@@ -1347,7 +1442,7 @@ static int newton_direction(kad_system_impl* S, int solver, double err, double* 
       }
     }
   }
-  if (times) times[4] = (solver == 0 || solver == 4) ? tf1 - tf0 : tf1;  // solvers 1-3, 5 accumulate in tf1
+  if (times) times[4] = (solver == 0 || solver == 4) ? tf1 - tf0 : tf1;  // the others accumulate in tf1
   if (info != 0) {
     fprintf(stderr, "Jacobian is singular! Stopping.\n");
     return KAD_ERR_SINGULAR;
@@ -1368,12 +1463,14 @@ static void reset_solve_record(kad_system_impl* S) {
 // solver: 0 fp64 LU; 1 fp32 LU + fp64 IR; 2 fp32 LU + fp64 GMRES-IR; 3 bf16-update LU +
 // fp64 GMRES-IR; 4 Gauss-Newton (fp64 J^T J X = J^T F, the reference's GAUSS_NEWTON);
 // 5 fp64 LU of an earlier J (refactored when it stops preconditioning well) + fp64
-// flexible GMRES on the current J.
+// flexible GMRES on the current J; 6 domain-decomposed fp64 factors (kad_dd_factor) +
+// fp64 flexible GMRES; 7 the same, the factors of an earlier J kept while they
+// precondition well.
 // Returns 1 when ||F||_inf < precision at entry (converged; no step taken), 0 after a step,
 // < 0 on error, KAD_ERR_SINGULAR (-5) when the factorisation meets an exact zero pivot.
 // times[5] (may be NULL): residual, Jacobian fill, factor+solve, update, factor alone (seconds).
 int kad_xcts_do_newton(kad_system* s, double precision, double* error, int solver, double* times) {
-  if (!s || !error || solver < 0 || solver > 5) return KAD_ERR_ARG;
+  if (!s || !error || solver < 0 || solver > 7) return KAD_ERR_ARG;
   auto* S = reinterpret_cast<kad_system_impl*>(s);
   hipStream_t st = S->ctx->stream;
   int rc;
@@ -1407,7 +1504,7 @@ int kad_xcts_do_newton(kad_system* s, double precision, double* error, int solve
 // NULL): the lambda applied and the number of residual evaluations of the search.
 int kad_system_do_newton_linesearch(kad_system* s, double precision, double* error, int ntrymax, double stepmax,
                                     int solver, double* lambda_out, int* ntry_out) {
-  if (!s || !error || ntrymax < 1 || !(stepmax > 0.0) || solver < 0 || solver > 5) return KAD_ERR_ARG;
+  if (!s || !error || ntrymax < 1 || !(stepmax > 0.0) || solver < 0 || solver > 7) return KAD_ERR_ARG;
   auto* S = reinterpret_cast<kad_system_impl*>(s);
   SystemBase& sy = *S->sys;
   hipStream_t st = S->ctx->stream;
@@ -1562,6 +1659,34 @@ extern "C" {
 // Host-only tracing (no device work): sizes of the system and of its programs.
 // info[0] n_unknowns, [1] n_rows, [2] value instructions, [3] tangent instructions,
 // [4] value slots, [5] tangent programs.  layout: 4 ints per (field, domain) entry.
+// kad_system_dd_blocks of the hand-traced system, traced on the host only.
+int kad_xcts_host_dd_blocks(int nr, int nt, int np, const double* bounds, int nbounds, int dmin, int dmax, int order,
+                            int* sizes, int cap, int* rows, int* cols) {
+  if (!bounds || !sizes || order < 0 || order > 1) return KAD_ERR_ARG;
+  XctsParams p;
+  p.nr = nr;
+  p.nt = nt;
+  p.np = np;
+  p.bounds.assign(bounds, bounds + nbounds);
+  p.dmin = dmin;
+  p.dmax = dmax;
+  p.order = order;
+  try {
+    XctsSystem sy(p);
+    sy.build();
+    std::vector<int> bs, rr, cc;
+    const int rc = dd_blocks(sy, bs, rr, cc);
+    if (rc) return rc;
+    for (int b = 0; b < (int)bs.size() && b < cap; ++b) sizes[b] = bs[b];
+    if (rows) std::copy(rr.begin(), rr.end(), rows);
+    if (cols) std::copy(cc.begin(), cc.end(), cols);
+    return (int)bs.size();
+  } catch (const std::exception& e) {
+    fprintf(stderr, "kad_xcts_host_dd_blocks: %s\n", e.what());
+    return KAD_ERR_ARG;
+  }
+}
+
 int kad_xcts_host_info(int nr, int nt, int np, const double* bounds, int nbounds, int dmin, int dmax, double n0,
                        double a, double ome, int order, int* info, int* layout, int cap) {
   if (!bounds || !info || order < 0 || order > 1) return KAD_ERR_ARG;
@@ -1933,6 +2058,29 @@ int kad_system_do_newton(kad_system* s, double precision, double* error, int sol
   return kad_xcts_do_newton(s, precision, error, solver, times);
 }
 int kad_system_last_solve(kad_system* s, int* iters, double* resid) { return kad_xcts_last_solve(s, iters, resid); }
+
+// The domain blocks of solvers 6-7 (HOST out): sizes (cap entries), rows and cols (n
+// each, may be NULL).  Returns the number of blocks, KAD_ERR_ARG when they are not
+// square (integral equations, double unknowns, unpaired interface conditions).
+int kad_system_dd_blocks(kad_system* s, int* sizes, int cap, int* rows, int* cols) {
+  if (!s || !sizes) return KAD_ERR_ARG;
+  auto* S = reinterpret_cast<kad_system_impl*>(s);
+  std::vector<int> bs, rr, cc;
+  int rc = dd_blocks(*S->sys, bs, rr, cc);
+  if (rc) return rc;
+  for (int b = 0; b < (int)bs.size() && b < cap; ++b) sizes[b] = bs[b];
+  if (rows) std::copy(rr.begin(), rr.end(), rows);
+  if (cols) std::copy(cc.begin(), cc.end(), cols);
+  return (int)bs.size();
+}
+
+// Sizes and set-up times of the last domain-decomposed factorisation (kad_dd_info).
+int kad_system_dd_info(kad_system* s, int* sizes, double* times) {
+  if (!s) return KAD_ERR_ARG;
+  auto* S = reinterpret_cast<kad_system_impl*>(s);
+  if (!S->dd) return KAD_ERR_ARG;
+  return kad_dd_info(S->dd, sizes, times);
+}
 
 // F(u) (device, n_rows): System_of_eqs::sec_member.
 int kad_residual(kad_system* s, double* F) { return kad_xcts_residual(s, F); }

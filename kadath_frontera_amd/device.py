@@ -89,6 +89,31 @@ class Context:
         self.sync()
         return b
 
+    def lu_factor_batched(self, prec, A):
+        """In-place LU of the column-major matrices A[z] (A of shape (nbat, n, n) with
+        A[z].T contiguous), factored together."""
+        nbat, n = A.shape[0], A.shape[1]
+        if A.stride(1) != 1:
+            raise ValueError("lu_factor_batched: A[z] must be column-major")
+        ipiv = torch.empty(nbat, n, dtype=torch.int32, device=A.device)
+        perm = torch.empty(nbat, n, dtype=torch.int32, device=A.device)
+        info = (ctypes.c_int * nbat)()
+        self._enter()
+        check(lib.kad_lu_factor_batched(self.h, PREC[prec], n, nbat, _ptr(A), A.stride(2), A.stride(0), _ptr(ipiv),
+                                        _ptr(perm), info), "kad_lu_factor_batched")
+        return ipiv, perm, list(info)
+
+    def lu_solve_batched(self, LU, perm, B):
+        """B[z] <- A[z]^{-1} B[z] with fp64 batched factors (B: (nbat, n, nrhs), B[z].T contiguous)."""
+        nbat, n, nrhs = B.shape
+        if B.stride(1) != 1 or LU.stride(1) != 1:
+            raise ValueError("lu_solve_batched: B[z] and LU[z] must be column-major")
+        ldb = B.stride(2) if nrhs > 1 else n
+        self._enter()
+        check(lib.kad_lu_solve_batched(self.h, n, nbat, _ptr(LU), LU.stride(2), LU.stride(0), _ptr(perm), _ptr(B),
+                                       ldb, B.stride(0), nrhs), "kad_lu_solve_batched")
+        return B
+
     def gemv64(self, A, x, b=None, trans=0):
         n = A.shape[0]
         y = torch.empty(n, dtype=torch.float64, device=A.device)
@@ -169,3 +194,57 @@ def host_op1d_matrix(op, base, n):
     ob = ctypes.c_int()
     check(lib.kad_host_op1d_matrix(op, base, n, buf, n * n, ctypes.byref(ob)), "kad_host_op1d_matrix")
     return np.frombuffer(buf, dtype=np.float64).copy().reshape(n, n), ob.value
+
+
+class DomainSolve:
+    """Domain-decomposed solve of J x = b (kad_dd_*): blocks = [(rows, cols), ...], the
+    rows and unknowns of J in each (square) block; J column-major fp64 on the device."""
+
+    def __init__(self, ctx, n, blocks):
+        self.ctx = ctx
+        sizes = _iarr([len(r) for r, _ in blocks])
+        rows = _iarr(np.concatenate([np.asarray(r) for r, _ in blocks]))
+        cols = _iarr(np.concatenate([np.asarray(c) for _, c in blocks]))
+        h = ctypes.c_void_p()
+        ctx._enter()
+        check(lib.kad_dd_create(ctx.h, n, len(blocks), sizes[1], rows[1], cols[1], ctypes.byref(h)), "kad_dd_create")
+        self.h = h
+        self.n = n
+
+    def factor(self, J):
+        self.ctx._enter()
+        rc = lib.kad_dd_factor(self.h, _ptr(J), J.stride(1))
+        check(rc, "kad_dd_factor")
+        return rc
+
+    def solve(self, J, b, tol=1e-13, max_outer=8):
+        x = torch.empty_like(b)
+        it = (ctypes.c_int * 2)()
+        res = ctypes.c_double()
+        self.ctx._enter()
+        rc = lib.kad_dd_solve(self.h, _ptr(J), J.stride(1), _ptr(b), _ptr(x), tol, max_outer, it, ctypes.byref(res))
+        if rc != -4:
+            check(rc, "kad_dd_solve")
+        return x, (it[0], it[1]), res.value
+
+    def apply(self, v):
+        self.ctx._enter()
+        check(lib.kad_dd_apply(self.h, _ptr(v)), "kad_dd_apply")
+        return v
+
+    def info(self):
+        sz = (ctypes.c_int * 4)()
+        tm = (ctypes.c_double * 5)()
+        check(lib.kad_dd_info(self.h, sz, tm), "kad_dd_info")
+        return dict(nd=sz[0], coupling_rows=sz[1], kmax=sz[2], blocks=sz[3], times=list(tm))
+
+    def close(self):
+        if self.h:
+            lib.kad_dd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

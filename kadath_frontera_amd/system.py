@@ -285,6 +285,29 @@ class DeviceSystem:
         if reuse_inner is not None:
             check(lib.kad_system_set_option(self.h, 1, float(reuse_inner)), "kad_system_set_option")
 
+    def dd_blocks(self):
+        """The (rows, unknowns) of each domain block of the domain-decomposed solvers
+        (kad_system_dd_blocks)."""
+        sizes = (ctypes.c_int * 64)()
+        rows = np.empty(self.n, dtype=np.int32)
+        cols = np.empty(self.n, dtype=np.int32)
+        nb = check(lib.kad_system_dd_blocks(self.h, sizes, 64, rows.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                            cols.ctypes.data_as(ctypes.POINTER(ctypes.c_int))),
+                   "kad_system_dd_blocks")
+        out, o = [], 0
+        for b in range(nb):
+            out.append((rows[o:o + sizes[b]].copy(), cols[o:o + sizes[b]].copy()))
+            o += sizes[b]
+        return out
+
+    def dd_info(self):
+        """Sizes and set-up times (s) of the last domain-decomposed factorisation."""
+        sz = (ctypes.c_int * 4)()
+        tm = (ctypes.c_double * 5)()
+        check(lib.kad_system_dd_info(self.h, sz, tm), "kad_system_dd_info")
+        return dict(nd=sz[0], coupling_rows=sz[1], kmax=sz[2], blocks=sz[3],
+                    times=dict(gather=tm[0], block_lu=tm[1], spikes=tm[2], capacitance=tm[3], capacitance_lu=tm[4]))
+
     def do_newton(self, prec, solver="f64"):
         err = ctypes.c_double()
         times = np.zeros(5)

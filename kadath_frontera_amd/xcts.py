@@ -11,7 +11,8 @@ import torch
 from ._lib import lib, check, c_vp
 
 FIELDS = ("P", "N", "bx", "by", "bz")
-SOLVERS = {"f64": 0, "f32-ir": 1, "f32-gmres": 2, "bf16-gmres": 3, "gauss-newton": 4, "f64-reuse": 5}
+SOLVERS = {"f64": 0, "f32-ir": 1, "f32-gmres": 2, "bf16-gmres": 3, "gauss-newton": 4, "f64-reuse": 5,
+           "f64-dd": 6, "f64-dd-reuse": 7}
 
 
 def _dp(a):
@@ -30,6 +31,26 @@ def host_info(res, bounds, n0=0.5, a=1.0, ome=0.0, order=0):
     layout = [tuple(lay[4 * q:4 * q + 4]) for q in range(nl)]
     keys = ("n_unknowns", "n_rows", "value_instr", "tangent_instr", "value_slots", "tangent_programs")
     return dict(zip(keys, list(info))), layout
+
+
+def host_dd_blocks(res, bounds, order=0):
+    """Host-only trace of the system: the (rows, unknowns) of each domain block of the
+    domain-decomposed solvers (kad_xcts_host_dd_blocks)."""
+    b = np.asarray(bounds, dtype=np.float64)
+    info, _ = host_info(res, bounds, order=order)
+    n = info["n_unknowns"]
+    sizes = (ctypes.c_int * 64)()
+    rows = np.empty(n, dtype=np.int32)
+    cols = np.empty(n, dtype=np.int32)
+    ip = ctypes.POINTER(ctypes.c_int)
+    nb = check(lib.kad_xcts_host_dd_blocks(res[0], res[1], res[2], _dp(b), len(b), 1, len(b), order, sizes, 64,
+                                           rows.ctypes.data_as(ip), cols.ctypes.data_as(ip)),
+               "kad_xcts_host_dd_blocks")
+    out, o = [], 0
+    for q in range(nb):
+        out.append((rows[o:o + sizes[q]].copy(), cols[o:o + sizes[q]].copy()))
+        o += sizes[q]
+    return out
 
 
 class XctsSystem:
@@ -145,6 +166,29 @@ class XctsSystem:
             check(lib.kad_system_set_option(self.h, 0, float(forcing)), "kad_system_set_option")
         if reuse_inner is not None:
             check(lib.kad_system_set_option(self.h, 1, float(reuse_inner)), "kad_system_set_option")
+
+    def dd_blocks(self):
+        """The (rows, unknowns) of each domain block of the domain-decomposed solvers
+        (kad_system_dd_blocks)."""
+        sizes = (ctypes.c_int * 64)()
+        rows = np.empty(self.n, dtype=np.int32)
+        cols = np.empty(self.n, dtype=np.int32)
+        nb = check(lib.kad_system_dd_blocks(self.h, sizes, 64, rows.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                            cols.ctypes.data_as(ctypes.POINTER(ctypes.c_int))),
+                   "kad_system_dd_blocks")
+        out, o = [], 0
+        for b in range(nb):
+            out.append((rows[o:o + sizes[b]].copy(), cols[o:o + sizes[b]].copy()))
+            o += sizes[b]
+        return out
+
+    def dd_info(self):
+        """Sizes and set-up times (s) of the last domain-decomposed factorisation."""
+        sz = (ctypes.c_int * 4)()
+        tm = (ctypes.c_double * 5)()
+        check(lib.kad_system_dd_info(self.h, sz, tm), "kad_system_dd_info")
+        return dict(nd=sz[0], coupling_rows=sz[1], kmax=sz[2], blocks=sz[3],
+                    times=dict(gather=tm[0], block_lu=tm[1], spikes=tm[2], capacitance=tm[3], capacitance_lu=tm[4]))
 
     def do_newton(self, prec, solver="f64"):
         err = ctypes.c_double()
