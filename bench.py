@@ -11,12 +11,17 @@ plus the seeded 1e-3 perturbation of every coefficient.
 One "step" = one System_of_eqs::do_newton iteration: residual F(u) + batched-JVP
 Jacobian fill + factorisation + solve (+ fp64 refinement) + field update, all
 device-resident.  The K timed steps are the iterations of the Newton solve from
-the guess (restarting from the guess once ||F||_inf < 1e-10).  The
-system is assembled in domain-major order (block-tridiagonal J) and the FP64 LU
-skips the structurally zero tile pairs.  The headline solver is FP64 with a factorisation every step (the reference's
-algorithm); FP32-LU and BF16-LU + FP64 GMRES-IR, and FP64 factors of an earlier
-Jacobian + FP64 GMRES ("f64-reuse"), run beside it to the same test, with time to
-solution and the converged fields compared.
+the guess (restarting from the guess once ||F||_inf < 1e-10, with no factors kept).
+The system is assembled in domain-major order.  The headline solver ("f64-dd")
+factors every step's own Jacobian in FP64: the five domain blocks with partial
+pivoting inside each, factored together (one panel chain of 12032 columns instead
+of 58905), the interface rows through a Woodbury capacitance matrix, then FP64
+flexible GMRES on J to a relative residual of 1e-13; it reaches the same iterates as
+the single FP64 LU of the whole J (the reference's pdgesv_, "f64", reported beside
+it) to 1e-14.  The other legs run to the same test from the same guess with time to
+solution and the converged fields compared: the domain factors of an earlier
+Jacobian reused ("f64-dd-reuse"), the FP64 LU of an earlier Jacobian reused
+("f64-reuse"), and FP32-LU / BF16-LU + FP64 GMRES-IR.
 
 Prints ONE JSON line (rank 0).
 """
@@ -145,8 +150,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=6, help="timed Newton iterations")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--solver", default="f64", choices=["f64", "f32-ir", "f32-gmres", "bf16-gmres", "f64-reuse"])
-    ap.add_argument("--compare", default="f32-gmres,bf16-gmres,f64-reuse",
+    ap.add_argument("--solver", default="f64-dd",
+                    choices=["f64", "f32-ir", "f32-gmres", "bf16-gmres", "f64-reuse", "f64-dd", "f64-dd-reuse"])
+    ap.add_argument("--compare", default="f64,f64-dd-reuse,f64-reuse,f32-gmres,bf16-gmres",
                     help="comma-separated solvers run to ||F|| < 1e-10 beside the headline one (or 'none')")
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--order", type=int, default=1, choices=[0, 1],
@@ -181,7 +187,8 @@ def main():
     sysd = XctsSystem(ctx, RES, BOUNDS, n0=N0, a=A, ome=OME, batch=args.batch, order=args.order)
     n = sysd.n
     X0 = sysd.perturbation(SEED)
-    prec_of = {"f64": "f64", "f32-ir": "f32", "f32-gmres": "f32", "bf16-gmres": "bf16", "f64-reuse": "f64"}
+    prec_of = {"f64": "f64", "f32-ir": "f32", "f32-gmres": "f32", "bf16-gmres": "bf16", "f64-reuse": "f64",
+               "f64-dd": "f64", "f64-dd-reuse": "f64"}
     prec = 1e-10
 
     def log(msg):
@@ -238,6 +245,7 @@ def main():
         then the fields and global quantities there, and again after one more step of the
         same solver ("polished": ||F|| at the floor of fp64 residual evaluation)."""
         set_guess()
+        sysd.set_option(drop_factors=True)  # every leg starts without kept factors
         recs, t_total, conv = [], 0.0, False
         for it in range(args.max_newton + 1):
             conv, err, dt, times, ir = newton_iter(solver)
@@ -247,10 +255,11 @@ def main():
                 break
             recs.append((err, dt, list(times), ir))
         f_stop, q_stop = fields(), sysd.observables()
+        dd = sysd.dd_info() if solver.startswith("f64-dd") else None
         _, err_p, _, _, _ = newton_iter_prec(solver, 0.0)
         f_pol, q_pol = fields(), sysd.observables()
         return dict(recs=recs, t=t_total, conv=conv, err=err, f_stop=f_stop, q_stop=q_stop, f_pol=f_pol,
-                    q_pol=q_pol)
+                    q_pol=q_pol, dd=dd)
 
     def newton_iter_prec(solver, p):
         torch.cuda.synchronize()
@@ -262,10 +271,12 @@ def main():
     # warmup: first iteration(s) from the guess (allocations, kernel loads)
     for _ in range(args.warmup):
         set_guess()
+        sysd.set_option(drop_factors=True)
         newton_iter(args.solver)
     # timed: K Newton iterations along the converging sequence (restarting from the
     # guess when it has converged); every timed call is a full iteration
     set_guess()
+    sysd.set_option(drop_factors=True)
     step_times, step_phases, step_ir = [], [], []
     torch.cuda.synchronize()
     t_start = time.perf_counter()
@@ -273,6 +284,7 @@ def main():
         conv, err, dt, times, ir = newton_iter(args.solver)
         if conv:
             set_guess()
+            sysd.set_option(drop_factors=True)  # a restart is a new solve: no factors kept
             continue
         step_times.append(dt)
         step_phases.append(list(times))
@@ -288,6 +300,7 @@ def main():
     # iteration from the guess: kind 0 MFMA GEMMs (flops), kind 1 fused 3-D spectral
     # passes of the residual / Jacobian programs (algorithmic HBM bytes)
     set_guess()
+    sysd.set_option(drop_factors=True)
     lib.kad_kernel_timing(ctx.h, 0, 1)
     lib.kad_kernel_timing(ctx.h, 1, 1)
     newton_iter(args.solver)
@@ -306,8 +319,10 @@ def main():
                  "unit": "TFLOP/s", "traffic": None, "traffic_model_bytes": GEMM_TRAFFIC_MODEL(gw, gn),
                  "traffic_note": GEMM_TRAFFIC_NOTE,
                  "work_per_launch_flop": gw / max(gn, 1),
-                 "kernel": "gemm_k4_kernel: LU trailing update + blocked-TRSM updates, executed tile pairs "
-                           "(%d launches, %.3e flop, %.1f ms per iteration)" % (gn, gw, gms)}
+                 "kernel": ("gemm_k4_kernel: batched trailing updates of the domain-block LU, blocked-TRSM "
+                            "updates and the Woodbury products" if args.solver.startswith("f64-dd") else
+                            "gemm_k4_kernel: LU trailing update + blocked-TRSM updates, executed tile pairs") +
+                           " (%d launches, %.3e flop, %.1f ms per iteration)" % (gn, gw, gms)}
     roof_gemm["frac"] = roof_gemm["achieved"] / roof_gemm["peak"]
     roof_apply = {"bound": "hbm", "achieved": aw / (ams / 1e3) / 1e9, "peak": 8000.0, "unit": "GB/s",
                   "traffic": APPLY3D_TRAFFIC_RATIO * aw / max(an, 1), "algorithmic_bytes_per_launch": aw / max(an, 1),
@@ -333,6 +348,8 @@ def main():
             info["refinement_rel_resid"] = [r[3][2] for r in recs]
             info["fp64_fallback_steps"] = int(sum(r[3][3] for r in recs))
             info["factorizations"] = int(sum(r[3][4] for r in recs))
+        if k.startswith("f64-dd"):
+            info["domain_blocks"] = v["dd"]
         if k != ref_key:
             info["vs_" + ref_key] = {
                 "fields_max_rel_at_stop": float(np.abs(v["f_stop"] - ref["f_stop"]).max() / sc),
@@ -369,7 +386,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"f64": "f64", "f32-ir": "f32-lu+f64-ir", "f32-gmres": "f32-lu+f64-gmres-ir",
-                  "bf16-gmres": "bf16-lu+f64-gmres-ir", "f64-reuse": "f64"}[args.solver],
+                  "bf16-gmres": "bf16-lu+f64-gmres-ir", "f64-reuse": "f64", "f64-dd": "f64",
+                  "f64-dd-reuse": "f64"}[args.solver],
         "data": "synthetic (the converged Ome = 0.10 hole plus a seeded 1e-3 perturbation)",
         "config": {"workload": "spinning-BH XCTS Newton iteration, 5 spherical domains 17x13x12, n=%d" % n,
                    "n_unknowns": n, "solver": args.solver, "jvp_batch": args.batch,
@@ -377,7 +395,8 @@ def main():
                    "horizon_rotation": OME, "chi_at_solution": chi},
         "lu_tflops": lu_tflops,
         "lu_tflops_note": "GEMM flops executed in one FP64 factorisation (structurally zero tile pairs skipped, "
-                          "not counted) / the factorisation's wall time",
+                          "not counted; for f64-dd the domain blocks, D^-1 P and the capacitance matrix with its LU) "
+                          "/ the factorisation's wall time",
         "phases_s": {"residual": phases[0], "jacobian_fill": phases[1], "factor_solve": phases[2],
                      "update": phases[3], "factor": phases[4]},
         "convergence": conv_info,

@@ -386,7 +386,9 @@ int kad_system_get_double(kad_system* s, int k, double* value);
  * (0, the default: 1e-13, i.e. its attainable accuracy); which 1 = reuse the
  * low-precision factors of an earlier step while the previous refinement took at most
  * `value` inner steps (0, the default: factor every step).  Reused factors that stop
- * converging are replaced by a factorisation of the current J within the step. */
+ * converging are replaced by a factorisation of the current J within the step.
+ * which 2 = forget the factorisations kept for reuse (solvers 5 and 7 too): the next
+ * step factors its own J. */
 int kad_system_set_option(kad_system* s, int which, double value);
 /* as kad_xcts_do_newton / kad_xcts_last_solve */
 int kad_system_do_newton(kad_system* s, double precision, double* error, int solver, double* times);

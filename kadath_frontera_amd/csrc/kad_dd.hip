@@ -30,6 +30,7 @@
 #include "../../include/kadath_amd.h"
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <functional>
 #include <vector>
 
@@ -45,6 +46,9 @@ int ir_loop(Ctx* c, int n, const double* A64, int64_t lda, const std::function<i
             const double* b, double* x, double tol, int max_outer, int mode, int trans, int* iters, double* resid);
 size_t trsv_table_doubles(int n);
 size_t trsv_flag_ints(int n);
+int trsv_prepare_batched(const double* LU, int64_t lda, int n, int nbat, int64_t sA, double* W, hipStream_t s);
+int getrs_permuted_batched(const double* LU, int64_t lda, int n, int nbat, int64_t sA, const double* W, double* x,
+                           double* y, hipStream_t s);
 template <typename T>
 int trsv_prepare(const T* LU, int64_t lda, int n, int trans, double* W, int* flag, TrsvExtents& ext, hipStream_t s);
 template <typename T>
@@ -191,6 +195,8 @@ struct kad_dd_s {
   int* cap_flag = nullptr;
   kad::TrsvExtents cap_ext;
   double* vb = nullptr;         // nblk x nd
+  double* vy = nullptr;         // nblk x nd (work of the block sweeps)
+  double* trs = nullptr;        // the block factors' sweep tables, nblk x trsv_table_doubles(nd)
   double* t = nullptr;          // r, then 3 r of capacitance-solve work
   double* yb = nullptr;         // nblk x kmax
   double* part = nullptr;       // partial sums of C vb
@@ -213,7 +219,7 @@ struct kad_dd_s {
   }
   ~kad_dd_s() {
     release_coupling();
-    void* bufs[] = {rowmap, colmap, rblk, cblk, flag, D, ipiv, perm, vb};
+    void* bufs[] = {rowmap, colmap, rblk, cblk, flag, D, ipiv, perm, vb, vy, trs};
     for (void* b : bufs)
       if (b) (void)hipFree(b);
   }
@@ -261,7 +267,7 @@ int dd_apply(kad_dd* dd, double* v) {
   const dim3 gv((unsigned)kad::cdiv(nd, 256), (unsigned)nb);
   kad::dd_gather_kernel<<<gv, 256, 0, s>>>(v, dd->rowmap, dd->perm, nd, dd->vb);
   KAD_HIP_CHECK(hipGetLastError());
-  int rc = kad::lu_solve_permuted_batched(dd->D, nd, nd, nb, (int64_t)nd * nd, dd->vb, nd, nd, 1, c->lu);
+  int rc = kad::getrs_permuted_batched(dd->D, nd, nd, nb, (int64_t)nd * nd, dd->trs, dd->vb, dd->vy, s);
   if (rc) return rc;
   if (dd->r > 0) {
     const int r = dd->r;
@@ -312,7 +318,12 @@ int kad_dd_create(kad_ctx* ctx, int n, int nblk, const int* blk_size, const int*
   dd->c = c;
   dd->n = n;
   dd->nblk = nblk;
-  dd->nd = (int)kad::cdiv(mx, 256) * 256;
+  // padded block order: a multiple of the batched factorisation's panel width
+  // (KAD_LU_NB when it is a multiple of 256, else 256)
+  int q = 256;
+  const char* e = std::getenv("KAD_LU_NB");
+  if (e && std::atoi(e) > 0 && std::atoi(e) % 256 == 0) q = std::atoi(e);
+  dd->nd = (int)(kad::cdiv(mx, q) * q);
   dd->bsize.assign(blk_size, blk_size + nblk);
   const int nd = dd->nd;
   dd->h_rowmap.assign((size_t)nblk * nd, -1);
@@ -344,6 +355,9 @@ int kad_dd_create(kad_ctx* ctx, int n, int nblk, const int* blk_size, const int*
   if ((rc = dd->dmalloc((void**)&dd->ipiv, (size_t)nblk * nd * sizeof(int)))) return fail(rc);
   if ((rc = dd->dmalloc((void**)&dd->perm, (size_t)nblk * nd * sizeof(int)))) return fail(rc);
   if ((rc = dd->dmalloc((void**)&dd->vb, (size_t)nblk * nd * sizeof(double)))) return fail(rc);
+  if ((rc = dd->dmalloc((void**)&dd->vy, (size_t)nblk * nd * sizeof(double)))) return fail(rc);
+  if ((rc = dd->dmalloc((void**)&dd->trs, (size_t)nblk * kad::trsv_table_doubles(nd) * sizeof(double))))
+    return fail(rc);
   if (hipMemcpy(dd->rowmap, dd->h_rowmap.data(), dd->h_rowmap.size() * sizeof(int), hipMemcpyHostToDevice) ||
       hipMemcpy(dd->colmap, dd->h_colmap.data(), dd->h_colmap.size() * sizeof(int), hipMemcpyHostToDevice) ||
       hipMemcpy(dd->rblk, dd->h_rblk.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice) ||
@@ -386,6 +400,8 @@ int kad_dd_factor(kad_dd* dd, const double* J, int64_t ldj) {
   if (rc) return rc;
   for (int b = 0; b < nb; ++b)
     if (info[b] != 0) return info[b];  // a singular domain block (1-based column inside it)
+  if ((rc = kad::trsv_prepare_batched(dd->D, nd, nd, nb, (int64_t)nd * nd, dd->trs, s))) return rc;
+  KAD_HIP_CHECK(hipStreamSynchronize(s));
   double t2 = now_s();
   // coupling rows by (block, local row); their unit right-hand sides in pivoted order
   std::vector<std::vector<int>> byblk(nb);

@@ -388,7 +388,10 @@ int kad_lu_factor_batched(kad_ctx* ctx, int prec, int n, int nbat, void* A, int6
     return KAD_ERR_ARG;
   }
   Ctx* c = reinterpret_cast<Ctx*>(ctx);
-  const int nb = 256;
+  // block size: KAD_LU_NB when it is a multiple of 256 that divides n, else 256
+  int nb = 256;
+  const char* e = std::getenv("KAD_LU_NB");
+  if (e && std::atoi(e) > 0 && std::atoi(e) % 256 == 0 && n % std::atoi(e) == 0) nb = std::atoi(e);
   int rc = c->lu_work(n, nb);
   if (rc) return rc;
   // every matrix's workgroups of the persistent panel are resident together

@@ -2186,9 +2186,17 @@ extern "C" {
 int kad_system_set_option(kad_system* s, int which, double value) {
   if (!s || !(value >= 0.0)) return KAD_ERR_ARG;
   auto* S = reinterpret_cast<kad_system_impl*>(s);
-  if (which == 0) S->forcing = value;
-  else if (which == 1) S->reuse_inner = (int)value;
-  else return KAD_ERR_ARG;
+  if (which == 0) {
+    S->forcing = value;
+  } else if (which == 1) {
+    S->reuse_inner = (int)value;
+  } else if (which == 2) {
+    // forget every factorisation kept for reuse: the next step factors its own J
+    S->lo_valid = S->fac_valid = S->dd_valid = false;
+    S->prev_inner = 0;
+  } else {
+    return KAD_ERR_ARG;
+  }
   return KAD_OK;
 }
 }  // extern "C"

@@ -36,7 +36,10 @@ __device__ __forceinline__ double tri_at(const T* F, int64_t lda, int r, int c) 
 // tr_out (the inverse of the transposed block).  Out: first TBS columns of W_k.
 template <typename T>
 __global__ __launch_bounds__(256) void diag_inverse_kernel(const T* __restrict__ F, int64_t lda, int n, int uplo,
-                                                           int tr_out, double* __restrict__ W) {
+                                                           int tr_out, double* __restrict__ W, int64_t sF = 0,
+                                                           int64_t sW = 0) {
+  F += (int64_t)blockIdx.y * sF;  // factors blockIdx.y of a batch
+  W += (int64_t)blockIdx.y * sW;
   __shared__ double M[TBS][TBS + 1];
   __shared__ double col[TBS];
   const int blk = blockIdx.x;
@@ -93,7 +96,9 @@ __global__ __launch_bounds__(256) void diag_inverse_kernel(const T* __restrict__
 // p = k - 1 (forward) or k + 1 (backward).  One workgroup per k with a p.
 template <typename T, bool TR, bool FWD>
 __global__ __launch_bounds__(256) void couple_kernel(const T* __restrict__ F, int64_t lda, int n, int nblk,
-                                                     double* __restrict__ W) {
+                                                     double* __restrict__ W, int64_t sF = 0, int64_t sW = 0) {
+  F += (int64_t)blockIdx.y * sF;  // factors blockIdx.y of a batch
+  W += (int64_t)blockIdx.y * sW;
   __shared__ double B[TBS][TBS + 1];
   const int k = FWD ? blockIdx.x + 1 : blockIdx.x;
   const int p = FWD ? k - 1 : k + 1;
@@ -130,7 +135,13 @@ __global__ __launch_bounds__(256) void couple_kernel(const T* __restrict__ F, in
 template <typename T, bool TR, bool FWD>
 __global__ __launch_bounds__(256) void sweep_step_kernel(const T* __restrict__ F, int64_t lda, int n, int nblk,
                                                          const double* __restrict__ W, int cur, int rlo, int rhi,
-                                                         double* __restrict__ rhs, double* __restrict__ sol) {
+                                                         double* __restrict__ rhs, double* __restrict__ sol,
+                                                         int64_t sF, int64_t sW, int64_t sV) {
+  // system blockIdx.y of a batch: its factors, tables and vectors
+  F += (int64_t)blockIdx.y * sF;
+  W += (int64_t)blockIdx.y * sW;
+  rhs += (int64_t)blockIdx.y * sV;
+  sol += (int64_t)blockIdx.y * sV;
   __shared__ double vin[WCOLS];
   __shared__ double part[4][PANEL_ROWS];
   __shared__ double cpart[8][CHAIN_ROWS];
@@ -308,9 +319,10 @@ int trsv_prepare(const T* LU, int64_t lda, int n, int trans, double* W, int* fla
   return KAD_OK;
 }
 
+// nbat > 1: nbat systems at LU + z sF, W + z sW, rhs / sol + z sV.
 template <typename T, bool TR, bool FWD>
 static void sweep(const T* LU, int64_t lda, int n, int nblk, const double* W, const TrsvExtents& ext, double* rhs,
-                  double* sol, hipStream_t s) {
+                  double* sol, hipStream_t s, int nbat = 1, int64_t sF = 0, int64_t sW = 0, int64_t sV = 0) {
   if (FWD) {
     for (int cur = -1; cur < nblk - 1; ++cur) {
       // panel rows: beyond block cur + 1, up to the column block's last nonzero row tile
@@ -318,7 +330,8 @@ static void sweep(const T* LU, int64_t lda, int n, int nblk, const double* W, co
       const int rhi = cur < 0 ? 0 : ext.fhi.empty() ? n : std::min(n, ext.fhi[cur]);
       const int rows = rhi - rlo;
       const unsigned grid = CHAIN_WG + ((cur >= 0 && rows > 0) ? (unsigned)cdiv(rows, PANEL_ROWS) : 0u);
-      sweep_step_kernel<T, TR, true><<<grid, 256, 0, s>>>(LU, lda, n, nblk, W, cur, rlo, rhi, rhs, sol);
+      sweep_step_kernel<T, TR, true><<<dim3(grid, nbat), 256, 0, s>>>(LU, lda, n, nblk, W, cur, rlo, rhi, rhs, sol,
+                                                                     sF, sW, sV);
     }
   } else {
     for (int cur = nblk; cur >= 1; --cur) {
@@ -327,7 +340,8 @@ static void sweep(const T* LU, int64_t lda, int n, int nblk, const double* W, co
       const int rlo = cur >= nblk ? rhi : ext.blo.empty() ? 0 : ext.blo[cur];
       const int rows = rhi - rlo;
       const unsigned grid = CHAIN_WG + ((cur < nblk && rows > 0) ? (unsigned)cdiv(rows, PANEL_ROWS) : 0u);
-      sweep_step_kernel<T, TR, false><<<grid, 256, 0, s>>>(LU, lda, n, nblk, W, cur, rlo, rhi, rhs, sol);
+      sweep_step_kernel<T, TR, false><<<dim3(grid, nbat), 256, 0, s>>>(LU, lda, n, nblk, W, cur, rlo, rhi, rhs, sol,
+                                                                      sF, sW, sV);
     }
   }
 }
@@ -353,6 +367,38 @@ int getrs_fast(const T* LU, int64_t lda, int n, const int* perm, const double* W
     sweep<T, true, false>(LU, lda, n, nblk, Wb, ext, y, z, s);  // L^T w = z
     scatter_perm_kernel<<<g, 256, 0, s>>>(z, perm, b, n);  // x = P^T w
   }
+  KAD_HIP_CHECK(hipGetLastError());
+  return KAD_OK;
+}
+
+// Batched: the W tables of nbat fp64 factorisations (LU + z sA, order n, trans 0, no
+// row extents) at W + z trsv_table_doubles(n).
+int trsv_prepare_batched(const double* LU, int64_t lda, int n, int nbat, int64_t sA, double* W, hipStream_t s) {
+  const int nblk = (int)cdiv(n, TBS);
+  const int64_t sW = (int64_t)trsv_table_doubles(n);
+  double* Wf = W;
+  double* Wb = W + (size_t)nblk * TBS * WCOLS;
+  diag_inverse_kernel<double><<<dim3(nblk, nbat), 256, 0, s>>>(LU, lda, n, 0, 0, Wf, sA, sW);
+  diag_inverse_kernel<double><<<dim3(nblk, nbat), 256, 0, s>>>(LU, lda, n, 1, 0, Wb, sA, sW);
+  if (nblk > 1) {
+    couple_kernel<double, false, true><<<dim3(nblk - 1, nbat), 256, 0, s>>>(LU, lda, n, nblk, Wf, sA, sW);
+    couple_kernel<double, false, false><<<dim3(nblk - 1, nbat), 256, 0, s>>>(LU, lda, n, nblk, Wb, sA, sW);
+  }
+  KAD_HIP_CHECK(hipGetLastError());
+  return KAD_OK;
+}
+
+// x_z <- U_z^{-1} L_z^{-1} x_z for nbat systems whose right-hand sides x (nbat x n, at
+// x + z n) are already in pivoted order; y: nbat x n doubles of work.
+int getrs_permuted_batched(const double* LU, int64_t lda, int n, int nbat, int64_t sA, const double* W, double* x,
+                           double* y, hipStream_t s) {
+  const int nblk = (int)cdiv(n, TBS);
+  const int64_t sW = (int64_t)trsv_table_doubles(n);
+  const double* Wf = W;
+  const double* Wb = W + (size_t)nblk * TBS * WCOLS;
+  const TrsvExtents none;
+  sweep<double, false, true>(LU, lda, n, nblk, Wf, none, x, y, s, nbat, sA, sW, n);   // L y = x
+  sweep<double, false, false>(LU, lda, n, nblk, Wb, none, y, x, s, nbat, sA, sW, n);  // U x = y
   KAD_HIP_CHECK(hipGetLastError());
   return KAD_OK;
 }

@@ -159,9 +159,12 @@ class XctsSystem:
         check(lib.kad_xcts_get_coef(self.h, f, d, _dp(out)), "kad_xcts_get_coef")
         return out.reshape(npt + 2, nt, nr)
 
-    def set_option(self, forcing=None, reuse_inner=None):
-        """Low-precision solver options: forcing factor of the refinement target and the
-        inner-step threshold for reusing earlier low-precision factors."""
+    def set_option(self, forcing=None, reuse_inner=None, drop_factors=False):
+        """Iterative solver options: forcing factor of the refinement target, the
+        inner-step threshold for reusing earlier factors, and dropping the factors kept
+        for reuse (the next step factors its own Jacobian)."""
+        if drop_factors:
+            check(lib.kad_system_set_option(self.h, 2, 0.0), "kad_system_set_option")
         if forcing is not None:
             check(lib.kad_system_set_option(self.h, 0, float(forcing)), "kad_system_set_option")
         if reuse_inner is not None:
