@@ -194,12 +194,14 @@ def main():
     def log(msg):
         print("[bench] " + msg, file=sys.stderr, flush=True)
 
-    # the guess's hole: Newton continuation in Ome from the flat guess (untimed)
+    # the guess's hole: Newton continuation in Ome from the flat guess (untimed), with
+    # the headline's fp64 direct solver when it is one (a profile then sees its kernels only)
+    cont_solver = args.solver if args.solver in ("f64", "f64-dd") else "f64"
     sysd.set_guess(None)
     for om in OME_PATH:
         sysd.set_ome(om)
         for it in range(30):
-            conv, err, _, _ = sysd.do_newton(prec, "f64")
+            conv, err, _, _ = sysd.do_newton(prec, cont_solver)
             if conv:
                 break
         if not conv:

@@ -26,6 +26,13 @@ CFLAGS = [
 ]
 
 
+# Per-source flags.  The MFMA GEMM keeps its accumulators in ordinary VGPRs (the
+# instructions' VGPR form): with the accumulation registers the compiler copied every
+# accumulator out of them and back at each k step, draining the matrix pipeline, and
+# the kernel needed 368 registers (one wave per SIMD) instead of 256 (two).
+EXTRA = {"kad_gemm.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+
+
 def sources():
     out = []
     for root, _, files in os.walk(CSRC):
@@ -38,7 +45,7 @@ def sources():
 def _compile(src):
     rel = os.path.relpath(src, CSRC).replace(os.sep, "_")
     obj = os.path.join(BUILD, rel + ".o")
-    cmd = [HIPCC] + CFLAGS + ["-c", src, "-o", obj]
+    cmd = [HIPCC] + CFLAGS + EXTRA.get(os.path.basename(src), []) + ["-c", src, "-o", obj]
     if src.endswith(".cpp"):
         cmd = [HIPCC] + CFLAGS + ["-x", "hip", "-c", src, "-o", obj]
     p = subprocess.run(cmd, capture_output=True, text=True)

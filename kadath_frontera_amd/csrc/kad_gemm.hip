@@ -14,6 +14,7 @@
 // of C: the column-major epilogue is coalesced.
 #include "kad_common.hpp"
 #include "kad_ctx.hpp"
+#include <type_traits>
 
 namespace kad {
 
@@ -122,69 +123,96 @@ __global__ __launch_bounds__(GNT) void gemm_k4_kernel(int m, int n, int k,
 #pragma unroll
     for (int j = 0; j < TI; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
-  T ra[NA], rb[NB];
-  auto gload = [&](int k0) {
+  // The k loop, with bounds checks on the loads only for tiles at the matrix's edge
+  // (EDGE) or a k that is not a whole number of steps.  Inside the matrix the loop is
+  // one basic block, so the accumulators stay in the matrix-core registers across it
+  // (guarded loads split the loop, and the compiler then copied every accumulator out
+  // and back at each step, draining the MFMA pipeline).
+  auto k_loop = [&](auto edge_tag) {
+    constexpr bool EDGE = decltype(edge_tag)::value;
+    T ra[NA], rb[NB];
+    auto gload = [&](int k0) {
 #pragma unroll
-    for (int e = 0; e < NA; ++e) {
-      const int idx = tid + e * GNT;
-      const int r = idx % BM, kk = idx / BM;
-      const int gr = row0 + r, gk = k0 + kk;
-      ra[e] = (gr < m && gk < k) ? A[gr + (int64_t)gk * lda] : T(0);
-    }
-#pragma unroll
-    for (int e = 0; e < NB; ++e) {
-      const int idx = tid + e * GNT;
-      const int kk = idx % BK, c = idx / BK;
-      const int gk = k0 + kk, gc = col0 + c;
-      rb[e] = (gk < k && gc < n) ? B[gk + (int64_t)gc * ldb] : T(0);
-    }
-  };
-  auto sstore = [&](int buf) {
-#pragma unroll
-    for (int e = 0; e < NA; ++e) {
-      const int idx = tid + e * GNT;
-      As[buf][(idx / BM) * LDA_S + (idx % BM)] = ra[e];
-    }
-#pragma unroll
-    for (int e = 0; e < NB; ++e) {
-      const int idx = tid + e * GNT;
-      Bs[buf][(idx % BK) * LDB_S + (idx / BK)] = rb[e];
-    }
-  };
-
-  const int nk = (int)cdiv(k, BK);
-  gload(0);
-  sstore(0);
-  __syncthreads();
-  for (int kt = 0; kt < nk; ++kt) {
-    const int cur = kt & 1;
-    if (kt + 1 < nk) gload((kt + 1) * BK);
-    const T* as = As[cur];
-    const T* bs = Bs[cur];
-#pragma unroll
-    for (int kk = 0; kk < BK; kk += 4) {
-      const int kr = kk + (lane >> 4);
-      T af[TI], bf[TI];
-#pragma unroll
-      for (int t = 0; t < TI; ++t) {
-        af[t] = as[kr * LDA_S + wm + t * 16 + (lane & 15)];
-        bf[t] = bs[kr * LDB_S + wn + t * 16 + (lane & 15)];
+      for (int e = 0; e < NA; ++e) {
+        const int idx = tid + e * GNT;
+        const int r = idx % BM, kk = idx / BM;
+        const int gr = row0 + r, gk = k0 + kk;
+        if constexpr (EDGE) {
+          ra[e] = (gr < m && gk < k) ? A[gr + (int64_t)gk * lda] : T(0);
+        } else {
+          // a wave-uniform column (idx / BM is the same for the wave's lanes) on a
+          // scalar base, the row as the lane's 32-bit offset: one address register
+          const int kc = min(k0 + __builtin_amdgcn_readfirstlane(kk), k - 1);  // clamped: the last step's prefetch
+          const T* col = A + row0 + (int64_t)kc * lda;
+          ra[e] = col[r];
+        }
       }
 #pragma unroll
-      for (int i = 0; i < TI; ++i)
-#pragma unroll
-        for (int j = 0; j < TI; ++j) {
-          if constexpr (sizeof(T) == 8)
-            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j], af[i], acc[i][j], 0, 0, 0);
-          else
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(bf[j], af[i], acc[i][j], 0, 0, 0);
+      for (int e = 0; e < NB; ++e) {
+        const int idx = tid + e * GNT;
+        const int kk = idx % BK, c = idx / BK;
+        const int gk = k0 + kk, gc = col0 + c;
+        if constexpr (EDGE) {
+          rb[e] = (gk < k && gc < n) ? B[gk + (int64_t)gc * ldb] : T(0);
+        } else {
+          // scalar base for the element's column group, lane offset kk + (c - c_e) ldb
+          const int k0c = min(k0, k - BK);  // the last step's prefetch re-reads the last step
+          const int ce = col0 + e * (GNT / BK);
+          const T* base = B + k0c + (int64_t)ce * ldb;
+          rb[e] = base[kk + (c - e * (GNT / BK)) * (int)ldb];
         }
-    }
-    if (kt + 1 < nk) {
-      sstore(cur ^ 1);
-    }
+      }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+      for (int e = 0; e < NA; ++e) {
+        const int idx = tid + e * GNT;
+        As[buf][(idx / BM) * LDA_S + (idx % BM)] = ra[e];
+      }
+#pragma unroll
+      for (int e = 0; e < NB; ++e) {
+        const int idx = tid + e * GNT;
+        Bs[buf][(idx % BK) * LDB_S + (idx / BK)] = rb[e];
+      }
+    };
+    const int nk = (int)cdiv(k, BK);
+    gload(0);
+    sstore(0);
     __syncthreads();
-  }
+    for (int kt = 0; kt < nk; ++kt) {
+      const int cur = kt & 1;
+      // inside the matrix the last step prefetches (clamped) and stores a tile that is
+      // never read: no branch in the loop
+      if (!EDGE || kt + 1 < nk) gload((kt + 1) * BK);
+      const T* as = As[cur];
+      const T* bs = Bs[cur];
+#pragma unroll
+      for (int kk = 0; kk < BK; kk += 4) {
+        const int kr = kk + (lane >> 4);
+        T af[TI], bf[TI];
+#pragma unroll
+        for (int t = 0; t < TI; ++t) {
+          af[t] = as[kr * LDA_S + wm + t * 16 + (lane & 15)];
+          bf[t] = bs[kr * LDB_S + wn + t * 16 + (lane & 15)];
+        }
+#pragma unroll
+        for (int i = 0; i < TI; ++i)
+#pragma unroll
+          for (int j = 0; j < TI; ++j) {
+            if constexpr (sizeof(T) == 8)
+              acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j], af[i], acc[i][j], 0, 0, 0);
+            else
+              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(bf[j], af[i], acc[i][j], 0, 0, 0);
+          }
+      }
+      if (!EDGE || kt + 1 < nk) sstore(cur ^ 1);
+      __syncthreads();
+    }
+  };
+  if (row0 + BM <= m && col0 + BM <= n && k % BK == 0)
+    k_loop(std::false_type{});
+  else
+    k_loop(std::true_type{});
 
   // Epilogue: lane holds row (lane&15) of tile i and 4 columns of tile j:
   // f32 16x16x4 accumulators are D'[4*(lane>>4)+r][lane&15], f64 16x16x4 ones
@@ -280,7 +308,8 @@ __global__ __launch_bounds__(GNT) void gemm_bf16_kernel(int m, int n, int k, con
   __syncthreads();
   for (int kt = 0; kt < nk; ++kt) {
     const int cur = kt & 1;
-    if (kt + 1 < nk) gload((kt + 1) * BK);
+    // the last step loads (clamped, in bounds) and stores a tile nobody reads
+    gload((kt + 1) * BK);
     const __bf16* as = As[cur];
     const __bf16* bs = Bs[cur];
 #pragma unroll
