@@ -34,6 +34,47 @@ struct DevRow {
   double gw;
 };
 
+__device__ __forceinline__ double ew_op(int kind, double x, double y, double s) {
+  switch (kind) {
+    case EW_ADD: return x + y;
+    case EW_SUB: return x - y;
+    case EW_MUL: return x * y;
+    case EW_DIV: return x / y;
+    case EW_SCAL: return x * s;
+    case EW_SDIV: return x / s;
+    case EW_ADDS: return x + s;
+    default: return x;
+  }
+}
+
+// The same over 16-byte-aligned rows of even length: every thread takes EW_U pairs of
+// elements, their loads issued together (one element per thread left each wave
+// waiting out one load's latency at a time).
+constexpr int EW_U = 2;
+__global__ __launch_bounds__(256) void ew2_kernel(int kind, double* __restrict__ dst, int64_t dstride,
+                                                  const double* __restrict__ a, int64_t astride,
+                                                  const double* __restrict__ bsrc, int64_t bstride, double s,
+                                                  int size2, int nbatch) {
+  for (int bb = blockIdx.y; bb < nbatch; bb += gridDim.y) {
+    const double2* ar = reinterpret_cast<const double2*>(a + (int64_t)bb * astride);
+    const double2* br = bsrc ? reinterpret_cast<const double2*>(bsrc + (int64_t)bb * bstride) : nullptr;
+    double2* dr = reinterpret_cast<double2*>(dst + (int64_t)bb * dstride);
+    const int e0 = (blockIdx.x * 256 * EW_U) + threadIdx.x;
+    double2 x[EW_U], y[EW_U];
+#pragma unroll
+    for (int u = 0; u < EW_U; ++u) {
+      const int e = e0 + u * 256;
+      x[u] = e < size2 ? ar[e] : double2{0.0, 0.0};
+      y[u] = (br && e < size2) ? br[e] : double2{0.0, 0.0};
+    }
+#pragma unroll
+    for (int u = 0; u < EW_U; ++u) {
+      const int e = e0 + u * 256;
+      if (e < size2) dr[e] = double2{ew_op(kind, x[u].x, y[u].x, s), ew_op(kind, x[u].y, y[u].y, s)};
+    }
+  }
+}
+
 // dst[b][e] = op(a[b or 0][e], b[b or 0][e], s)
 // Element-wise instruction over a batch: blockIdx.y walks the batch rows (a stride of
 // 0 broadcasts one array to every row), blockIdx.x / threadIdx.x the elements.
@@ -638,10 +679,20 @@ int kad_system_impl::run(const std::vector<Instr>& code, int nb, int u0, int see
       }
       case I_EW: {
         const int batch = (in.dst.space == SP_VAL) ? 1 : nb;
-        const dim3 grid((unsigned)kad::cdiv(in.size, 256), (unsigned)std::min(batch, 65535));
-        ew_kernel<<<grid, 256, 0, s>>>(
-            in.ew, slot_ptr(in.dst), slot_stride(in.dst), slot_ptr(in.a), slot_stride(in.a),
-            in.b.slot >= 0 ? slot_ptr(in.b) : nullptr, in.b.slot >= 0 ? slot_stride(in.b) : 0, in.s, in.size, batch);
+        double* dp = slot_ptr(in.dst);
+        const double* ap = slot_ptr(in.a);
+        const double* bp = in.b.slot >= 0 ? slot_ptr(in.b) : nullptr;
+        const int64_t ds = slot_stride(in.dst), as = slot_stride(in.a), bs = in.b.slot >= 0 ? slot_stride(in.b) : 0;
+        auto a16 = [](const void* p) { return ((uintptr_t)p & 15) == 0; };
+        if (in.size % 2 == 0 && ds % 2 == 0 && as % 2 == 0 && bs % 2 == 0 && a16(dp) && a16(ap) &&
+            (!bp || a16(bp))) {
+          const int size2 = in.size / 2;
+          const dim3 grid((unsigned)kad::cdiv(size2, 256 * EW_U), (unsigned)std::min(batch, 65535));
+          ew2_kernel<<<grid, 256, 0, s>>>(in.ew, dp, ds, ap, as, bp, bs, in.s, size2, batch);
+        } else {
+          const dim3 grid((unsigned)kad::cdiv(in.size, 256), (unsigned)std::min(batch, 65535));
+          ew_kernel<<<grid, 256, 0, s>>>(in.ew, dp, ds, ap, as, bp, bs, in.s, in.size, batch);
+        }
         break;
       }
       case I_EXPORT_IN:
