@@ -3,8 +3,9 @@ Space_spheric / Scalar front end over the C-ABI) runs tutorials/tuto5.cpp's step
 
 CPU: it compiles with the host compiler, links libkadath_amd.so, declares the system
 and stops cleanly at the first device call (exit status 3).  GPU: it reaches the
-reference's known answer 1 + a/r within 1e-8 at every collocation point and a
-declaration with an operator that has no device form is rejected (exit status 0).
+reference's known answer 1 + a/r within 1e-8 at every collocation point, with the
+single LU and with the domain-decomposed solver (NEWTON_F64_DD), and a declaration with
+an operator that has no device form is rejected (exit status 0).
 """
 import os
 import shutil
@@ -51,3 +52,4 @@ def test_cpp_caller_reaches_tuto5_known_answer(tmp_path):
     print(p.stdout, p.stderr, flush=True)
     assert p.returncode == 0, p.stdout + p.stderr
     assert "rejected:" in p.stdout
+    assert "domain-decomposed: max |numerical - analytic|" in p.stdout

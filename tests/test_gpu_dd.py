@@ -15,6 +15,8 @@
   as the factor-every-step fp64 Newton reaches ||F||_inf < 1e-10 and the same
   solution (1e-10), on the hand-traced system (domain-major and reference order) and
   on the system parsed from kerr.cpp's strings.
+- A system whose unknowns or equations couple every domain (the spin-fixed hole) is
+  refused with an argument error.
 """
 import numpy as np
 import pytest
@@ -180,3 +182,32 @@ def test_newton_with_domain_solve_reaches_the_fp64_solution(ctx, make, solver):
     assert worst <= 1e-10
     a.close()
     b.close()
+
+
+def test_domain_solver_refuses_systems_coupling_every_domain(ctx):
+    """The spin-fixed hole (a double unknown Ome and the spin integral, which couple every
+    domain) has no square domain blocks: kad_system_dd_blocks and solvers 6-7 refuse it
+    with an argument error, and the system stays usable."""
+    from kadath_frontera_amd._lib import KadError
+    from oracle import xcts as OX
+    res, bounds = (5, 5, 4), [1.0, 2.0, 4.0]
+    sy = OX.XCTSSystem(OX.Space(res, bounds), 1, 3, n0=0.5, a=1.0, ome=0.05)
+    sy.set_fields(OX.initial_fields(sy, 1.0, OX.initial_perturbation(sy)))
+    s = programs.kerr(ctx, res, bounds, n0=0.5, a=1.0, ome=0.05, spin=0.1).build(order=1, batch=64)
+    for name, f in NAMES.items():
+        for d in range(1, 4):
+            v = sy.fields[name][d]
+            v.coef()
+            s.set_coef(f, d, v.cf)
+    F0 = s.residual().cpu().numpy()
+    with pytest.raises(KadError):
+        s.dd_blocks()
+    for solver in ("f64-dd", "f64-dd-reuse"):
+        with pytest.raises(KadError):
+            s.do_newton(1e-10, solver)
+    # the refusals left the fields untouched: the same residual, and an fp64 step runs
+    assert np.array_equal(s.residual().cpu().numpy(), F0)
+    conv, err, _, _ = s.do_newton(1e-10, "f64")
+    print(f"after the refusals, an fp64 step from |F| {err:.3e}", flush=True)
+    assert np.isfinite(err) and err == np.abs(F0).max()
+    s.close()
