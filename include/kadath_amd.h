@@ -254,7 +254,9 @@ int kad_xcts_sensitivity(kad_system* s, double* out, double* dQdu);
  * kad_system_dd_blocks: the domains, each block factored with partial pivoting inside
  * it, the blocks together) precondition fp64 flexible GMRES on the same J; 7: the same,
  * the factors of an earlier J kept as for 5.  6 and 7 need square domain blocks (no
- * integral equations or double unknowns).  Returns 1 if ||F||_inf < precision at
+ * integral equations or double unknowns); a singular domain block or capacitance
+ * matrix, or a refinement that does not converge, falls back to the fp64 LU of the
+ * same J (reported by kad_xcts_last_solve iters[2]).  Returns 1 if ||F||_inf < precision at
  * entry (no step), 0 after a step, KAD_ERR_SINGULAR on an exactly singular J.
  * times[5]: residual, Jacobian fill, factor + solve (incl. refinement), update,
  * factorisation alone (seconds). */

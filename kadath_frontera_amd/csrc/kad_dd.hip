@@ -345,7 +345,7 @@ int kad_dd_create(kad_ctx* ctx, int n, int nblk, const int* blk_size, const int*
     delete dd;
     return e;
   };
-  KAD_HIP_CHECK(hipSetDevice(c->device));
+  if (hipSetDevice(c->device) != hipSuccess) return fail(KAD_ERR_HIP);
   if ((rc = dd->dmalloc((void**)&dd->rowmap, dd->h_rowmap.size() * sizeof(int)))) return fail(rc);
   if ((rc = dd->dmalloc((void**)&dd->colmap, dd->h_colmap.size() * sizeof(int)))) return fail(rc);
   if ((rc = dd->dmalloc((void**)&dd->rblk, (size_t)n * sizeof(int)))) return fail(rc);

@@ -1357,25 +1357,28 @@ static int newton_direction(kad_system_impl* S, int solver, double err, double* 
       const double tg0 = now_s();
       if ((rc = kad_dd_factor(S->dd, S->J, n)) < 0) return rc;
       tf1 += now_s() - tg0;
-      info = rc;
-      S->dd_valid = (info == 0);
+      const int dd_info = rc;  // > 0: a singular domain block or capacitance matrix
+      S->dd_valid = (dd_info == 0);
       S->last_factored = 1;
-      if (info == 0) {
+      bool fallback = dd_info != 0;
+      if (dd_info == 0) {
         rc = kad_dd_solve(S->dd, S->J, n, S->F, S->X, eta, 8, S->last_iters, &S->last_resid);
         if (rc < 0 && rc != KAD_ERR_NOCONV) return rc;
         S->prev_inner = S->last_iters[1];
-        if (rc == KAD_ERR_NOCONV && !(S->last_resid <= std::max(eta, kAcceptResid))) {
-          // not a usable Newton direction: the fp64 LU of the same, untouched J
-          S->last_fallback = 1;
-          const double tg1 = now_s();
-          if ((rc = kad_lu_factor(cx, KAD_PREC_F64, n, S->J, n, S->ipiv, S->perm, 0)) < 0) return rc;
-          tf1 += now_s() - tg1;
-          info = rc;
-          S->dd_valid = false;
-          if (info == 0) {
-            KAD_HIP_CHECK(hipMemcpyAsync(S->X, S->F, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, st));
-            if ((rc = kad_lu_solve(cx, KAD_PREC_F64, n, S->J, n, S->perm, S->X, 0))) return rc;
-          }
+        fallback = rc == KAD_ERR_NOCONV && !(S->last_resid <= std::max(eta, kAcceptResid));
+      }
+      if (fallback) {
+        // a singular block (J itself need not be singular) or not a usable Newton
+        // direction: the fp64 LU of the same, untouched J decides
+        S->last_fallback = 1;
+        const double tg1 = now_s();
+        if ((rc = kad_lu_factor(cx, KAD_PREC_F64, n, S->J, n, S->ipiv, S->perm, 0)) < 0) return rc;
+        tf1 += now_s() - tg1;
+        info = rc;
+        S->dd_valid = false;
+        if (info == 0) {
+          KAD_HIP_CHECK(hipMemcpyAsync(S->X, S->F, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, st));
+          if ((rc = kad_lu_solve(cx, KAD_PREC_F64, n, S->J, n, S->perm, S->X, 0))) return rc;
         }
       }
     }

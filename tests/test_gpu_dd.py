@@ -211,3 +211,48 @@ def test_domain_solver_refuses_systems_coupling_every_domain(ctx):
     print(f"after the refusals, an fp64 step from |F| {err:.3e}", flush=True)
     assert np.isfinite(err) and err == np.abs(F0).max()
     s.close()
+
+
+def test_domain_solve_of_a_general_matrix(ctx):
+    """kad_dd_* on a matrix given as blocks of scattered rows and unknowns (two blocks of
+    250 and 350, padded to 512; 20 rows of each block couple into the other): the solve
+    equals the dense solution; a matrix whose diagonal blocks are zero (J nonsingular)
+    reports a singular block instead of factoring."""
+    rng = np.random.default_rng(3)
+    n, sizes = 600, [250, 350]
+    rows = rng.permutation(n)
+    cols = rng.permutation(n)
+    rb = [rows[:250], rows[250:]]
+    cb = [cols[:250], cols[250:]]
+    J = np.zeros((n, n))
+    for b in range(2):
+        J[np.ix_(rb[b], cb[b])] = rng.standard_normal((sizes[b], sizes[b])) + 30 * np.eye(sizes[b])
+    for b in range(2):
+        cpl = rng.choice(rb[b], 20, replace=False)
+        J[np.ix_(cpl, cb[1 - b])] = rng.standard_normal((20, sizes[1 - b]))
+    F = rng.standard_normal(n)
+    x_ref = np.linalg.solve(J, F)
+    Jd = torch.tensor(J.T.copy(), dtype=torch.float64, device="cuda").T  # column-major
+    Fd = torch.tensor(F, dtype=torch.float64, device="cuda")
+    dd = DomainSolve(ctx, n, [(rb[0], cb[0]), (rb[1], cb[1])])
+    assert dd.factor(Jd) == 0
+    info = dd.info()
+    print("general matrix:", info, flush=True)
+    assert info["nd"] == 512 and info["coupling_rows"] == 40
+    x, iters, resid = dd.solve(Jd, Fd)
+    err = np.abs(x.cpu().numpy() - x_ref).max() / np.abs(x_ref).max()
+    print(f"solve {err:.2e} ({iters}, resid {resid:.1e})", flush=True)
+    assert err <= 1e-12 and resid <= 1e-13
+    dd.close()
+    # zero diagonal blocks, identity couplings: J is a permutation, the blocks are singular
+    P = np.zeros((n, n))
+    P[np.ix_(rb[0][:250], cb[1][:250])] = np.eye(250)
+    P[np.ix_(rb[1][:250], cb[0][:250])] = np.eye(250)
+    P[np.ix_(rb[1][250:], cb[1][250:])] = np.eye(100)
+    assert abs(np.linalg.det(P)) == 1.0
+    Pd = torch.tensor(P.T.copy(), dtype=torch.float64, device="cuda").T
+    dd2 = DomainSolve(ctx, n, [(rb[0], cb[0]), (rb[1], cb[1])])
+    rc = dd2.factor(Pd)
+    print(f"zero diagonal blocks: kad_dd_factor -> {rc}", flush=True)
+    assert rc > 0
+    dd2.close()
