@@ -48,18 +48,24 @@ PEAK = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0}  # MI355X dense matrix TFLOP/
 #   apply3d: over the residual and Jacobian fill of this workload (--fill-only, 6566
 #     launches, 3.405e11 algorithmic bytes), traffic / algorithmic bytes = 1.0040; the
 #     bench reports that ratio times its live algorithmic bytes per launch.
-#   gemm_k4: mean over all launches of three FP64 Newton iterations of this workload,
-#     measured on an earlier build of the LU (same GEMM kernel; 7580 launches per
-#     iteration against 8256 now).
 APPLY3D_TRAFFIC_RATIO = (2.0 * 83618276.7 + 166635597.5) * 1024.0 / 340514582928.0
-# gemm_k4: the counter pass over the current LU ended in a segfault of the profiler's host
-# counter path and was not repeated on that build, so the bench reports no measured traffic
-# for it; traffic_model_bytes is the kernel's operand traffic per launch
-# (each executed 128 x 128 C tile reads its 128 x k A and k x 128 B panels, reads and writes
-# C), an upper bound on HBM bytes when nothing is reused through L2.
-GEMM_TRAFFIC_NOTE = ("traffic is not measured for gemm_k4 on this build (the --pmc pass crashed in the "
-                     "profiler's host code); traffic_model_bytes = operand bytes per launch of the executed "
-                     "tiles (A and B panels + C read and write, fp64), with no L2 reuse")
+# gemm_k4: the launches of one iteration span many shapes (64- and 128-tiles, the domain
+# blocks' updates, the triangular solves' and the Woodbury products), and counter passes
+# over the whole LU are not taken (the one tried ended in a segfault of the profiler's host
+# counter path, with the persistent panel kernel spinning beside the GEMM), so the line's
+# traffic stays null. The kernel is measured on its own at one domain block's trailing
+# update instead (--gemm-only: C 12032 x 12032 -= A 12032 x 256 B, same calibration):
+# 2 x 980019.3 KiB fetched + 1131008 KiB written = 3.165 GB per launch, 1.338 x the
+# algorithmic 2.366 GB (C read and written once, the A and B panels re-fetched about 17
+# times over eight L2s), at 23.4 flop/B, above the FP64 ridge: compute-bound.
+# traffic_model_bytes is the operand traffic per launch of the executed tiles (each 128 x
+# 128 C tile reads its 128 x k A and k x 128 B panels, reads and writes C), an upper bound
+# on HBM bytes when nothing is reused through L2.
+GEMM_TRAFFIC_NOTE = ("traffic is not measured over the LU's mixed GEMM launches; measured alone at a domain "
+                     "block's trailing update (12032 x 12032 x 256, --gemm-only) gemm_k4 moves 3.165 GB per "
+                     "launch = 1.338 x its algorithmic bytes (C once, A/B panels re-fetched), compute-bound at "
+                     "23.4 flop/B; traffic_model_bytes = operand bytes per launch of the executed tiles with no "
+                     "L2 reuse")
 
 
 def GEMM_TRAFFIC_MODEL(flops, launches):
@@ -164,6 +170,10 @@ def main():
     ap.add_argument("--fill-only", action="store_true",
                     help="only the residual and the Jacobian fill once (the run behind the HBM counter passes of "
                          "apply3d): prints the launches and algorithmic bytes of apply3d and exits")
+    ap.add_argument("--gemm-only", action="store_true",
+                    help="only four FP64 GEMM launches (one untimed) of a domain block's trailing-update shape, C (12032 x 12032) "
+                         "-= A (12032 x 256) B (256 x 12032) (the run behind the HBM counter passes of gemm_k4): "
+                         "prints their algorithmic bytes and exits")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only, "
@@ -184,6 +194,23 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     ctx = Context(0)
+    if args.gemm_only:
+        mg, kg = 12032, 256
+        Ag = torch.randn(kg, mg, dtype=torch.float64, device="cuda").T  # column-major views
+        Bg = torch.randn(mg, kg, dtype=torch.float64, device="cuda").T
+        Cg = torch.randn(mg, mg, dtype=torch.float64, device="cuda").T
+        torch.cuda.synchronize()
+        ctx.gemm("f64", 0, Ag, Bg, Cg)  # untimed: loads the code object
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            ctx.gemm("f64", 0, Ag, Bg, Cg)
+        ctx.sync()
+        dt = (time.perf_counter() - t0) / 3
+        print(json.dumps({"gemm_only": True, "m": mg, "n": mg, "k": kg, "launches": 4, "timed": 3,
+                          "ms_per_launch": 1e3 * dt, "tflops": 2.0 * mg * mg * kg / dt / 1e12,
+                          "algorithmic_bytes_per_launch": 8.0 * (2 * mg * mg + 2 * mg * kg)}), flush=True)
+        return
     sysd = XctsSystem(ctx, RES, BOUNDS, n0=N0, a=A, ome=OME, batch=args.batch, order=args.order)
     n = sysd.n
     X0 = sysd.perturbation(SEED)
