@@ -390,7 +390,9 @@ int kad_system_get_double(kad_system* s, int k, double* value);
  * `value` inner steps (0, the default: factor every step).  Reused factors that stop
  * converging are replaced by a factorisation of the current J within the step.
  * which 2 = forget the factorisations kept for reuse (solvers 5 and 7 too): the next
- * step factors its own J. */
+ * step factors its own J.  which 3 = streams of the Jacobian fill (1-4, default 3): the
+ * tangent programs are dealt to that many streams, each with its own tangent arena
+ * (nphys x batch x slot doubles more per extra stream); J is the same bit for bit. */
 int kad_system_set_option(kad_system* s, int which, double value);
 /* as kad_xcts_do_newton / kad_xcts_last_solve */
 int kad_system_do_newton(kad_system* s, double precision, double* error, int solver, double* times);

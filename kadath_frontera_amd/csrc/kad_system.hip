@@ -451,9 +451,22 @@ struct kad_system_impl {
   bool fac_valid = false;  // Jfac / perm hold usable fp64 factors
   kad_dd* dd = nullptr;    // solvers 6-7: the domain-decomposed factors (their own copy of the blocks)
   bool dd_valid = false;
+  // Jacobian fill on several streams (kad_system_set_option 3): the tangent programs are
+  // dealt to the streams by estimated cost, each stream with its own tangent arena;
+  // every program writes only its own columns of J, so the streams never touch the
+  // same memory.  Stream 0 is the context's stream with tan_arena.
+  static constexpr int kFillMax = 4;
+  int nfill = 3;  // 17x13x12 x 5 domains, batch 4096: 0.241 s on one stream, 0.207 / 0.201 / 0.210 on 2 / 3 / 4
+  hipStream_t fstream[kFillMax] = {};
+  double* ftan[kFillMax] = {};
+  hipEvent_t fev[kFillMax] = {};
 
   ~kad_system_impl() {
     if (dd) kad_dd_destroy(dd);
+    for (int i = 0; i < kFillMax; ++i) {
+      if (fstream[i]) (void)hipStreamDestroy(fstream[i]);
+      if (fev[i]) (void)hipEventDestroy(fev[i]);
+    }
     for (void* p : allocs) (void)hipFree(p);
     delete sys;
   }
@@ -469,16 +482,20 @@ struct kad_system_impl {
     if (!h.empty()) KAD_HIP_CHECK(hipMemcpy(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
     return KAD_OK;
   }
-  double* slot_ptr(const Opnd& o) const {
-    return o.space == SP_VAL ? val_arena + (int64_t)o.slot * stride : tan_arena + (int64_t)o.slot * B * stride;
+  double* slot_ptr(const Opnd& o, double* tan) const {
+    return o.space == SP_VAL ? val_arena + (int64_t)o.slot * stride : tan + (int64_t)o.slot * B * stride;
   }
   int64_t slot_stride(const Opnd& o) const { return o.space == SP_VAL ? 0 : stride; }
   int setup();
   // out: fp64 (dtype 0) or fp32 (dtype 1) column-major rows x columns with leading
   // dimension ldo, columns from col0; with w != nullptr the exports are transposed
-  // instead (out: fp64 per column, += w . row)
+  // instead (out: fp64 per column, += w . row).  st / tan: the stream and tangent arena
+  // (default: the context's stream and tan_arena).
   int run(const std::vector<Instr>& code, int nb, int u0, int seed_tab, void* out, int64_t col0, int64_t ldo,
-          int dtype = 0, const double* w = nullptr);
+          int dtype = 0, const double* w = nullptr, hipStream_t st = nullptr, double* tan = nullptr);
+  // every tangent program's columns of J (fp64, ld ldj), after the value pass and the
+  // double unknowns' columns, on nfill streams; the context's stream waits for all
+  int fill_columns(double* Jd, int64_t ldj);
   bool vals_current = false;  // the value arena holds the value pass of the current fields
   int ensure_values() { return vals_current ? KAD_OK : value_pass(); }
   int value_pass();
@@ -664,8 +681,10 @@ int kad_system_impl::setup() {
 }
 
 int kad_system_impl::run(const std::vector<Instr>& code, int nb, int u0, int seed_tab, void* out, int64_t col0,
-                         int64_t ldo, int dtype, const double* w) {
-  hipStream_t s = ctx->stream;
+                         int64_t ldo, int dtype, const double* w, hipStream_t st, double* tan) {
+  hipStream_t s = st ? st : ctx->stream;
+  if (!tan) tan = tan_arena;
+  auto slot_ptr = [&](const Opnd& o) { return this->slot_ptr(o, tan); };
   for (const Instr& in : code) {
     const Domain& dm = sys->doms[in.dom];
     switch (in.kind) {
@@ -775,12 +794,48 @@ int kad_system_impl::jacobian(double* Jd, int64_t ldj) {
   if (rc) return rc;
   KAD_HIP_CHECK(hipMemsetAsync(Jd, 0, (size_t)ldj * sys->n_unknowns * sizeof(double), ctx->stream));
   if ((rc = double_columns(Jd, ldj, 0, sys->ndouble, 0))) return rc;
-  for (auto& p : tprogs) {
+  return fill_columns(Jd, ldj);
+}
+
+int kad_system_impl::fill_columns(double* Jd, int64_t ldj) {
+  hipStream_t s0 = ctx->stream;
+  const int ns = std::max(1, std::min(nfill, (int)tprogs.size()));
+  for (int i = 1; i < ns; ++i) {
+    if (!fstream[i]) KAD_HIP_CHECK(hipStreamCreateWithFlags(&fstream[i], hipStreamNonBlocking));
+    if (!ftan[i]) {
+      const int rc = dmalloc((void**)&ftan[i], (size_t)std::max(1, nphys_max) * B * stride * sizeof(double));
+      if (rc) return rc;
+    }
+  }
+  for (int i = 0; i < ns && ns > 1; ++i)
+    if (!fev[i]) KAD_HIP_CHECK(hipEventCreateWithFlags(&fev[i], hipEventDisableTiming));
+  if (ns > 1) {
+    KAD_HIP_CHECK(hipEventRecord(fev[0], s0));  // J cleared, value pass done
+    for (int i = 1; i < ns; ++i) KAD_HIP_CHECK(hipStreamWaitEvent(fstream[i], fev[0], 0));
+  }
+  // longest programs first, each to the stream with the least work so far
+  std::vector<int> order(tprogs.size());
+  for (size_t q = 0; q < order.size(); ++q) order[q] = (int)q;
+  auto cost = [&](int q) { return (double)tprogs[q].cnt * (double)tprogs[q].code.size(); };
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
+  double load[kFillMax] = {0, 0, 0, 0};
+  for (int q : order) {
+    int i = 0;
+    for (int k = 1; k < ns; ++k)
+      if (load[k] < load[i]) i = k;
+    load[i] += cost(q);
+    const DevProgram& p = tprogs[q];
     for (int u0 = 0; u0 < p.cnt; u0 += B) {
       const int nb = std::min(B, p.cnt - u0);
       // exports write columns off+u0 .. off+u0+nb-1
-      if ((rc = run(p.code, nb, u0, p.seed_tab, Jd, (int64_t)p.off + u0, ldj))) return rc;
+      const int rc = run(p.code, nb, u0, p.seed_tab, Jd, (int64_t)p.off + u0, ldj, 0, nullptr, i ? fstream[i] : s0,
+                         i ? ftan[i] : tan_arena);
+      if (rc) return rc;
     }
+  }
+  for (int i = 1; i < ns; ++i) {
+    KAD_HIP_CHECK(hipEventRecord(fev[i], fstream[i]));
+    KAD_HIP_CHECK(hipStreamWaitEvent(s0, fev[i], 0));
   }
   return KAD_OK;
 }
@@ -1311,11 +1366,7 @@ static int newton_direction(kad_system_impl* S, int solver, double err, double* 
     if ((rc = S->value_pass())) return rc;
   KAD_HIP_CHECK(hipMemsetAsync(S->J, 0, (size_t)n * n * sizeof(double), st));
   if ((rc = S->double_columns(S->J, n, 0, sy.ndouble, 0))) return rc;
-  for (auto& p : S->tprogs)
-    for (int u0 = 0; u0 < p.cnt; u0 += S->B) {
-      const int nb = std::min(S->B, p.cnt - u0);
-      if ((rc = S->run(p.code, nb, u0, p.seed_tab, S->J, (int64_t)p.off + u0, n))) return rc;
-    }
+  if ((rc = S->fill_columns(S->J, n))) return rc;
   KAD_HIP_CHECK(hipStreamSynchronize(st));
   double t2 = now_s();
   if (times) times[1] = t2 - t1;
@@ -2248,6 +2299,10 @@ int kad_system_set_option(kad_system* s, int which, double value) {
     // forget every factorisation kept for reuse: the next step factors its own J
     S->lo_valid = S->fac_valid = S->dd_valid = false;
     S->prev_inner = 0;
+  } else if (which == 3) {
+    // streams of the Jacobian fill
+    if (value < 1.0 || value > (double)kad_system_impl::kFillMax || value != (double)(int)value) return KAD_ERR_ARG;
+    S->nfill = (int)value;
   } else {
     return KAD_ERR_ARG;
   }

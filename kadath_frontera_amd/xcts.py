@@ -159,10 +159,13 @@ class XctsSystem:
         check(lib.kad_xcts_get_coef(self.h, f, d, _dp(out)), "kad_xcts_get_coef")
         return out.reshape(npt + 2, nt, nr)
 
-    def set_option(self, forcing=None, reuse_inner=None, drop_factors=False):
+    def set_option(self, forcing=None, reuse_inner=None, drop_factors=False, fill_streams=None):
         """Iterative solver options: forcing factor of the refinement target, the
         inner-step threshold for reusing earlier factors, and dropping the factors kept
-        for reuse (the next step factors its own Jacobian)."""
+        for reuse (the next step factors its own Jacobian); and the number of streams
+        (1-4) of the Jacobian fill."""
+        if fill_streams is not None:
+            check(lib.kad_system_set_option(self.h, 3, float(fill_streams)), "kad_system_set_option")
         if drop_factors:
             check(lib.kad_system_set_option(self.h, 2, 0.0), "kad_system_set_option")
         if forcing is not None:

@@ -62,6 +62,23 @@ def test_jacobian_matches_oracle(oracle_sys, gpu_sys):
     assert (np.abs(J)[Jo == 0] / np.broadcast_to(colscale, J.shape)[Jo == 0]).max() < 1e-12
 
 
+def test_jacobian_fill_streams_give_the_same_J(oracle_sys, gpu_sys):
+    # the tangent programs dealt to 1-4 streams (batch 128: several column tiles per
+    # program, run one after another on their stream's arena) fill J bit for bit alike
+    _, X0 = oracle_sys
+    gpu_sys.set_guess(X0)
+    try:
+        gpu_sys.set_option(fill_streams=1)
+        J1 = gpu_sys.jacobian().clone()
+        for k in (2, 3, 4):
+            gpu_sys.set_option(fill_streams=k)
+            assert torch.equal(gpu_sys.jacobian(), J1), k
+        with pytest.raises(RuntimeError):
+            gpu_sys.set_option(fill_streams=5)
+    finally:
+        gpu_sys.set_option(fill_streams=3)
+
+
 @pytest.mark.parametrize("solver", ["f64", "f32-gmres"])
 def test_newton_converges_to_oracle_solution(oracle_sys, gpu_sys, solver):
     sy, X0 = oracle_sys
