@@ -217,6 +217,39 @@ __global__ __launch_bounds__(GNT) void gemm_k4_kernel(int m, int n, int k,
   // Epilogue: lane holds row (lane&15) of tile i and 4 columns of tile j:
   // f32 16x16x4 accumulators are D'[4*(lane>>4)+r][lane&15], f64 16x16x4 ones
   // D'[4*r+(lane>>4)][lane&15] (measured lane layouts of the two instructions).
+  // Inside the matrix there are no guards: C -= AB then reads half of the lane's C
+  // entries in one batch (a register array) before it writes them back.  With a guard
+  // around each entry every read of C was its own basic block, waited for alone: 64
+  // memory round trips per tile, as long as the k = 256 loop itself.
+  if (row0 + BM <= m && col0 + BM <= n) {
+#pragma unroll
+    for (int h = 0; h < TI; h += TI / 2) {
+      T cv[TI / 2][TI][4];
+      if constexpr (OP == OP_SUB) {
+#pragma unroll
+        for (int i = 0; i < TI / 2; ++i)
+#pragma unroll
+          for (int j = 0; j < TI; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int gr = row0 + wm + (h + i) * 16 + (lane & 15);
+              const int gc = col0 + wn + j * 16 + (sizeof(T) == 8 ? 4 * r + (lane >> 4) : 4 * (lane >> 4) + r);
+              cv[i][j][r] = C[gr + (int64_t)gc * ldc];
+            }
+      }
+#pragma unroll
+      for (int i = 0; i < TI / 2; ++i)
+#pragma unroll
+        for (int j = 0; j < TI; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int gr = row0 + wm + (h + i) * 16 + (lane & 15);
+            const int gc = col0 + wn + j * 16 + (sizeof(T) == 8 ? 4 * r + (lane >> 4) : 4 * (lane >> 4) + r);
+            C[gr + (int64_t)gc * ldc] = OP == OP_SUB ? cv[i][j][r] - acc[h + i][j][r] : acc[h + i][j][r];
+          }
+    }
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < TI; ++i) {
     const int gr = row0 + wm + i * 16 + (lane & 15);

@@ -47,6 +47,37 @@ def test_gemm(ctx, prec, tol, m, n, k):
         assert err < tol, (prec, op, err)
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("m", [64, 600, 4096])
+def test_gemm_update_rounds_once(ctx, prec, m):
+    # C -= A B with A's first column all ones and its other columns zero: every
+    # product is exact, so each entry must be C - B[0, j] rounded once -- the same bits
+    # as numpy's subtraction, denormal operands and results included (the update is a
+    # no-return atomic add in L2; 600 and 4096 give the 64- and 128-row tiles, edges too)
+    print(f"gemm update rounding {prec} m={m}", flush=True)
+    npt = np.float64 if prec == "f64" else np.float32
+    dt = torch.float64 if prec == "f64" else torch.float32
+    tiny = np.finfo(npt).tiny
+    rng = np.random.default_rng(m)
+    k = 4
+    A = np.zeros((m, k), npt)
+    A[:, 0] = 1.0
+    B = np.zeros((k, m), npt)
+    B[0] = rng.standard_normal(m).astype(npt)
+    B[0, ::7] = (tiny * rng.uniform(0.5, 3.0, B[0, ::7].size)).astype(npt)
+    C = rng.standard_normal((m, m)).astype(npt)
+    C[::5] = (tiny * rng.uniform(0.25, 4.0, C[::5].shape)).astype(npt)  # denormal and just-normal rows
+    C[1::11, ::7] = 0.0
+    dA, dB, dC = colmajor(A, dt), colmajor(B, dt), colmajor(C, dt)
+    ctx.gemm(prec, 0, dA, dB, dC)
+    ctx.sync()
+    got = dC.cpu().numpy()
+    want = C - B[0][None, :]
+    assert np.array_equal(got.view(np.uint64 if prec == "f64" else np.uint32),
+                          want.view(np.uint64 if prec == "f64" else np.uint32))
+    assert (np.abs(want[want != 0]) < tiny).any()  # the case includes denormal results
+
+
 def _spd_like(n, seed):
     rng = np.random.default_rng(seed)
     return rng.standard_normal((n, n)) + 0.5 * np.sqrt(n) * np.eye(n)
