@@ -85,6 +85,18 @@ int kad_lu_factor(kad_ctx* ctx, int prec, int n, void* A, int64_t lda, int* ipiv
 int kad_lu_factor_batched(kad_ctx* ctx, int prec, int n, int nbat, void* A, int64_t lda, int64_t strideA, int* ipiv,
                           int* perm, int* info);
 
+/* As kad_lu_factor_batched without row interchanges (static pivoting; no reference
+ * counterpart, ScaLAPACK pivots every column): the caller has already put the rows of
+ * every matrix in the order of an earlier factorisation's pivots, so row j is the
+ * pivot of column j and no pivot search is made -- each 32-column panel is one launch
+ * with no exchange between workgroups.  One matrix may have any order; a batch,
+ * orders that are multiples of 256.  ipiv receives the identity; info (HOST, nbat)
+ * the first exactly-zero pivot; growth (HOST, nbat) the largest |multiplier| of each
+ * matrix (partial pivoting keeps it at 1; a large one means the order no longer suits
+ * the matrix and the factors should not be used). */
+int kad_lu_factor_batched_static(kad_ctx* ctx, int prec, int n, int nbat, void* A, int64_t lda, int64_t strideA,
+                                 int* ipiv, int* info, double* growth);
+
 /* B_z <- A_z^{-1} B_z (nrhs columns each, B_z at B + z strideB) with the FP64
  * factors of kad_lu_factor_batched (n a multiple of 32). */
 int kad_lu_solve_batched(kad_ctx* ctx, int n, int nbat, const double* LU, int64_t lda, int64_t strideA, const int* perm,
@@ -109,10 +121,21 @@ int kad_lu_solve_batched(kad_ctx* ctx, int n, int nbat, const double* LU, int64_
 typedef struct kad_dd_s kad_dd;
 int kad_dd_create(kad_ctx* ctx, int n, int nblk, const int* blk_size, const int* rows, const int* cols, kad_dd** out);
 int kad_dd_factor(kad_dd* dd, const double* J, int64_t ldj);
+/* kad_dd_factor with static pivoting (kad_lu_factor_batched_static): the rows of every
+ * block and of the capacitance matrix are taken in the pivot orders of the last
+ * kad_dd_factor, for a J close to the one it factored (the next Newton step's).
+ * Returns 0 (factored), < 0 an error, or > 0 when the orders do not apply: 1 no
+ * earlier kad_dd_factor, 2 a zero pivot or a multiplier above max_growth (>= 1) in a
+ * block, 3 other coupling rows, 4 the same in the capacitance matrix; the factors are
+ * then not usable and kad_dd_factor should follow. */
+int kad_dd_factor_static(kad_dd* dd, const double* J, int64_t ldj, double max_growth);
 int kad_dd_solve(kad_dd* dd, const double* J, int64_t ldj, const double* b, double* x, double tol, int max_outer,
                  int* iters, double* resid);
 int kad_dd_apply(kad_dd* dd, double* v);
 int kad_dd_info(const kad_dd* dd, int* sizes, double* times);
+/* whether the last factorisation took the kept pivot orders (kad_dd_factor_static)
+ * and its largest |multiplier| (0 after kad_dd_factor); HOST out, either may be NULL */
+int kad_dd_pivoting(const kad_dd* dd, int* static_orders, double* growth);
 int kad_dd_destroy(kad_dd* dd);
 
 /* Solve op(A) x = b in place with the factors (b fp64).  trans = 1: A^T x = b
@@ -252,7 +275,9 @@ int kad_xcts_sensitivity(kad_system* s, double* out, double* dQdu);
  * 30) inner steps or did not converge; it holds two n^2 fp64 buffers.  6: the
  * domain-decomposed fp64 factors (kad_dd_factor over the blocks of
  * kad_system_dd_blocks: the domains, each block factored with partial pivoting inside
- * it, the blocks together) precondition fp64 flexible GMRES on the same J; 7: the same,
+ * it, the blocks together; once a step of the solve has done that, later steps factor
+ * with its pivot orders, kad_dd_factor_static, kad_system_set_option 4) precondition
+ * fp64 flexible GMRES on the same J; 7: the same,
  * the factors of an earlier J kept as for 5.  6 and 7 need square domain blocks (no
  * integral equations or double unknowns); a singular domain block or capacitance
  * matrix, or a refinement that does not converge, falls back to the fp64 LU of the
@@ -392,7 +417,13 @@ int kad_system_get_double(kad_system* s, int k, double* value);
  * which 2 = forget the factorisations kept for reuse (solvers 5 and 7 too): the next
  * step factors its own J.  which 3 = streams of the Jacobian fill (1-4, default 3): the
  * tangent programs are dealt to that many streams, each with its own tangent arena
- * (nphys x batch x slot doubles more per extra stream); J is the same bit for bit. */
+ * (nphys x batch x slot doubles more per extra stream); J is the same bit for bit.
+ * which 4 = static pivoting in solvers 6-7 (1, the default; 0: partial pivoting at
+ * every factorisation): after a factorisation with partial pivoting, later steps of
+ * the solve factor their J with its pivot orders (kad_dd_factor_static), falling
+ * back to partial pivoting when a multiplier exceeds 1e8 or the refinement needs more
+ * than 6 inner steps; after a step whose multipliers exceed 1e3 the next step factors
+ * with partial pivoting again (the orders drift with the Jacobian). */
 int kad_system_set_option(kad_system* s, int which, double value);
 /* as kad_xcts_do_newton / kad_xcts_last_solve */
 int kad_system_do_newton(kad_system* s, double precision, double* error, int solver, double* times);
@@ -419,6 +450,8 @@ int kad_system_last_solve(kad_system* s, int* iters, double* resid);
 int kad_system_dd_blocks(kad_system* s, int* sizes, int cap, int* rows, int* cols);
 /* kad_dd_info of the system's last domain-decomposed factorisation */
 int kad_system_dd_info(kad_system* s, int* sizes, double* times);
+/* kad_dd_pivoting of the system's last domain-decomposed factorisation */
+int kad_system_dd_pivoting(kad_system* s, int* static_orders, double* growth);
 
 /* F(u) (device, n_rows): System_of_eqs::sec_member (solver.cpp:49-85) */
 int kad_residual(kad_system* s, double* F);

@@ -120,15 +120,19 @@ SIGNATURES = {
     "kad_check_equations": (c_int, [c_vp, c_dp, c_int]),
     "kad_jvp_fill": (c_int, [c_vp, c_int, c_int, c_vp, c_i64, c_int]),
     "kad_lu_factor_batched": (c_int, [c_vp, c_int, c_int, c_int, c_vp, c_i64, c_i64, c_vp, c_vp, c_ip]),
+    "kad_lu_factor_batched_static": (c_int, [c_vp, c_int, c_int, c_int, c_vp, c_i64, c_i64, c_vp, c_ip, c_dp]),
     "kad_lu_solve_batched": (c_int, [c_vp, c_int, c_int, c_vp, c_i64, c_i64, c_vp, c_vp, c_i64, c_i64, c_int]),
     "kad_dd_create": (c_int, [c_vp, c_int, c_int, c_ip, c_ip, c_ip, ctypes.POINTER(c_vp)]),
     "kad_dd_factor": (c_int, [c_vp, c_vp, c_i64]),
+    "kad_dd_factor_static": (c_int, [c_vp, c_vp, c_i64, c_dbl]),
+    "kad_dd_pivoting": (c_int, [c_vp, c_ip, c_dp]),
     "kad_dd_solve": (c_int, [c_vp, c_vp, c_i64, c_vp, c_vp, c_dbl, c_int, c_ip, c_dp]),
     "kad_dd_apply": (c_int, [c_vp, c_vp]),
     "kad_dd_info": (c_int, [c_vp, c_ip, c_dp]),
     "kad_dd_destroy": (c_int, [c_vp]),
     "kad_system_dd_blocks": (c_int, [c_vp, c_ip, c_int, c_ip, c_ip]),
     "kad_system_dd_info": (c_int, [c_vp, c_ip, c_dp]),
+    "kad_system_dd_pivoting": (c_int, [c_vp, c_ip, c_dp]),
     "kad_vjp": (c_int, [c_vp, c_vp, c_vp]),
     "kad_param_tangent": (c_int, [c_vp, c_int, c_vp]),
 }

@@ -36,6 +36,11 @@ struct LuBatch {
   int64_t sA = 0;
   int sP = 0;
   int* info = nullptr;  // nbat words (first zero pivot of each matrix); NULL: LuWork::info
+  // static pivoting: no row interchanges (the rows were put in the order of an earlier
+  // factorisation's pivots); growth[z] receives the largest |multiplier| of matrix z
+  // (bits of a non-negative double, zeroed by getrf)
+  bool nopiv = false;
+  unsigned long long* growth = nullptr;
 };
 constexpr int LU_MAX_BATCH = 16;
 
@@ -47,6 +52,8 @@ struct LuWork {
   unsigned* counter = nullptr;  // last-block counter (reset by the last block)
   int* info = nullptr;          // first exactly-zero pivot (1-based), 0 if none
   int* binfo = nullptr;         // the same per matrix of a batched factorisation (LU_MAX_BATCH)
+  unsigned* bcnt = nullptr;     // arrival counters of the static-pivot base panel (LU_MAX_BATCH, kept at 0)
+  unsigned long long* bgrowth = nullptr;  // largest |multiplier| per matrix of a static-pivot factorisation
   int red_cap = 0;
   void* tinv = nullptr;         // unit-lower inverse, nb x nb
   void* tmp = nullptr;          // GEMM scratch for L11^{-1} A12

@@ -30,6 +30,7 @@
 #include "../../include/kadath_amd.h"
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdlib>
 #include <functional>
 #include <vector>
@@ -72,16 +73,26 @@ __global__ __launch_bounds__(256) void dd_flag_kernel(const double* __restrict__
   if (hit) flag[i] = 1;
 }
 
-// D_z(i, j) = J(rowmap_z[i], colmap_z[j]); identity where either is padding.
+// D_z(i, j) = J(rowmap_z[li], colmap_z[j]), li = rperm_z[i] (rows in an earlier
+// factorisation's pivot order) or i (rperm NULL); identity where either is padding.
 __global__ __launch_bounds__(256) void dd_blocks_kernel(const double* __restrict__ J, int64_t ld,
                                                         const int* __restrict__ rowmap, const int* __restrict__ colmap,
-                                                        int nd, double* __restrict__ D) {
+                                                        const int* __restrict__ rperm, int nd, double* __restrict__ D) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   const int j = blockIdx.y, z = blockIdx.z;
   if (i >= nd) return;
-  const int row = rowmap[(int64_t)z * nd + i], col = colmap[(int64_t)z * nd + j];
+  const int li = rperm ? rperm[(int64_t)z * nd + i] : i;
+  const int row = rowmap[(int64_t)z * nd + li], col = colmap[(int64_t)z * nd + j];
   D[(int64_t)z * nd * nd + i + (int64_t)j * nd] = (row >= 0 && col >= 0) ? J[row + (int64_t)col * ld]
-                                                                         : (i == j ? 1.0 : 0.0);
+                                                                         : (li == j ? 1.0 : 0.0);
+}
+
+// B(i, :) = A(perm[i], :), r x r column-major (ld r).
+__global__ __launch_bounds__(256) void dd_rows_kernel(const double* __restrict__ A, const int* __restrict__ perm,
+                                                      int r, double* __restrict__ B) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i < r) B[i + (int64_t)j * r] = A[perm[i] + (int64_t)j * r];
 }
 
 // W_z(k, j) = 1 at each unit right-hand side's pivoted position (W zeroed before).
@@ -200,22 +211,32 @@ struct kad_dd_s {
   double* t = nullptr;          // r, then 3 r of capacitance-solve work
   double* yb = nullptr;         // nblk x kmax
   double* part = nullptr;       // partial sums of C vb
+  double* cap2 = nullptr;       // r x r: the capacitance rows in the kept pivot order
+  size_t cap2_r = 0;
   bool factored = false;
   double times[5] = {0, 0, 0, 0, 0};
+  // the pivot orders of the last factorisation with partial pivoting (perm, cap_perm),
+  // kept for factorisations with static pivoting while the coupling rows stay the same
+  bool piv_valid = false;
+  std::vector<int> h_perm;      // nblk x nd, that factorisation's block orders
+  std::vector<int> h_crow;      // its coupling rows
+  bool last_static = false;     // the last factorisation used the kept orders
+  double last_growth = 0.0;     // its largest |multiplier| (0: partial pivoting)
 
   int dmalloc(void** p, size_t bytes) {
     KAD_HIP_CHECK(hipMalloc(p, bytes < 16 ? 16 : bytes));
     return KAD_OK;
   }
   void release_coupling() {
-    void* bufs[] = {crow, cz, dq0, upos, W, Jc, cap, cap_ipiv, cap_perm, cap_tab, cap_flag, t, yb, part};
+    void* bufs[] = {crow, cz, dq0, upos, W, Jc, cap, cap_ipiv, cap_perm, cap_tab, cap_flag, t, yb, part, cap2};
     for (void* b : bufs)
       if (b) (void)hipFree(b);
     crow = cz = dq0 = nullptr;
     upos = nullptr;
-    W = Jc = cap = cap_tab = t = yb = part = nullptr;
+    W = Jc = cap = cap_tab = t = yb = part = cap2 = nullptr;
     cap_ipiv = cap_perm = cap_flag = nullptr;
-    cap_r = cap_w = 0;
+    cap_r = cap_w = cap2_r = 0;
+    piv_valid = false;  // cap_perm is gone
   }
   ~kad_dd_s() {
     release_coupling();
@@ -372,12 +393,28 @@ int kad_dd_destroy(kad_dd* dd) {
   return KAD_OK;
 }
 
-int kad_dd_factor(kad_dd* dd, const double* J, int64_t ldj) {
+}  // extern "C"
+
+namespace {
+
+// The factorisation of kad_dd_factor (stat false: partial pivoting) and of
+// kad_dd_factor_static (stat true: the rows of every block and of the capacitance
+// matrix in the pivot orders of the last factorisation with partial pivoting, factored
+// without interchanges).  Static returns > 0 when the orders do not apply:
+// kStaticNone no orders kept, kStaticBlocks a zero pivot or a multiplier above
+// max_growth in a block, kStaticCoupling other coupling rows, kStaticCapacitance the
+// same in the capacitance matrix.
+constexpr int kStaticNone = 1, kStaticBlocks = 2, kStaticCoupling = 3, kStaticCapacitance = 4;
+int dd_factor_impl(kad_dd* dd, const double* J, int64_t ldj, bool stat, double max_growth) {
   if (!dd || !J || ldj < dd->n) return KAD_ERR_ARG;
   Ctx* c = dd->c;
   hipStream_t s = c->stream;
   const int n = dd->n, nd = dd->nd, nb = dd->nblk;
   dd->factored = false;
+  dd->last_static = stat;
+  dd->last_growth = 0.0;
+  if (stat && !dd->piv_valid) return kStaticNone;
+  if (!stat) dd->piv_valid = false;
   int rc;
   KAD_HIP_CHECK(hipStreamSynchronize(s));
   double t0 = now_s();
@@ -387,7 +424,7 @@ int kad_dd_factor(kad_dd* dd, const double* J, int64_t ldj) {
       J, ldj, n, dd->rblk, dd->cblk, dd->flag);
   KAD_HIP_CHECK(hipGetLastError());
   kad::dd_blocks_kernel<<<dim3((unsigned)kad::cdiv(nd, 256), (unsigned)nd, (unsigned)nb), 256, 0, s>>>(
-      J, ldj, dd->rowmap, dd->colmap, nd, dd->D);
+      J, ldj, dd->rowmap, dd->colmap, stat ? dd->perm : nullptr, nd, dd->D);
   KAD_HIP_CHECK(hipGetLastError());
   std::vector<int> hflag(n);
   KAD_HIP_CHECK(hipMemcpyAsync(hflag.data(), dd->flag, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, s));
@@ -395,11 +432,22 @@ int kad_dd_factor(kad_dd* dd, const double* J, int64_t ldj) {
   double t1 = now_s();
   // the diagonal blocks, factored together
   std::vector<int> info(nb, 0);
-  rc = kad_lu_factor_batched(reinterpret_cast<kad_ctx*>(c), KAD_PREC_F64, nd, nb, dd->D, nd, (int64_t)nd * nd,
-                             dd->ipiv, dd->perm, info.data());
-  if (rc) return rc;
-  for (int b = 0; b < nb; ++b)
-    if (info[b] != 0) return info[b];  // a singular domain block (1-based column inside it)
+  if (!stat) {
+    rc = kad_lu_factor_batched(reinterpret_cast<kad_ctx*>(c), KAD_PREC_F64, nd, nb, dd->D, nd, (int64_t)nd * nd,
+                               dd->ipiv, dd->perm, info.data());
+    if (rc) return rc;
+    for (int b = 0; b < nb; ++b)
+      if (info[b] != 0) return info[b];  // a singular domain block (1-based column inside it)
+  } else {
+    std::vector<double> gr(nb, 0.0);
+    rc = kad_lu_factor_batched_static(reinterpret_cast<kad_ctx*>(c), KAD_PREC_F64, nd, nb, dd->D, nd,
+                                      (int64_t)nd * nd, dd->ipiv, info.data(), gr.data());
+    if (rc) return rc;
+    for (int b = 0; b < nb; ++b) {
+      dd->last_growth = std::max(dd->last_growth, std::isnan(gr[b]) ? INFINITY : gr[b]);
+      if (info[b] != 0 || !(gr[b] <= max_growth)) return kStaticBlocks;
+    }
+  }
   if ((rc = kad::trsv_prepare_batched(dd->D, nd, nd, nb, (int64_t)nd * nd, dd->trs, s))) return rc;
   KAD_HIP_CHECK(hipStreamSynchronize(s));
   double t2 = now_s();
@@ -407,32 +455,42 @@ int kad_dd_factor(kad_dd* dd, const double* J, int64_t ldj) {
   std::vector<std::vector<int>> byblk(nb);
   for (int i = 0; i < n; ++i)
     if (hflag[i]) byblk[dd->h_rblk[i]].push_back(i);
-  dd->kb.assign(nb, 0);
-  dd->q0.assign(nb, 0);
+  std::vector<int> kb(nb, 0), q0(nb, 0);
   int r = 0, kmax = 0;
   for (int b = 0; b < nb; ++b) {
     std::sort(byblk[b].begin(), byblk[b].end(),
               [&](int x, int y) { return dd->h_rloc[x] < dd->h_rloc[y]; });
-    dd->q0[b] = r;
-    dd->kb[b] = (int)byblk[b].size();
-    r += dd->kb[b];
-    kmax = std::max(kmax, dd->kb[b]);
+    q0[b] = r;
+    kb[b] = (int)byblk[b].size();
+    r += kb[b];
+    kmax = std::max(kmax, kb[b]);
   }
+  std::vector<int> hcrow(r), hcz(r);
+  for (int b = 0; b < nb; ++b)
+    for (int j = 0; j < kb[b]; ++j) {
+      hcrow[q0[b] + j] = byblk[b][j];
+      hcz[q0[b] + j] = b;
+    }
+  // the kept capacitance order belongs to the same coupling rows
+  if (stat && hcrow != dd->h_crow) return kStaticCoupling;
+  dd->kb = kb;
+  dd->q0 = q0;
   dd->r = r;
   dd->kmax = kmax;
   if (r > 0) {
     if ((rc = ensure_coupling(dd, r, kmax))) return rc;
-    std::vector<int> hperm((size_t)nb * nd), hcrow(r), hcz(r);
+    if (!stat) {
+      dd->h_perm.resize((size_t)nb * nd);
+      KAD_HIP_CHECK(hipMemcpyAsync(dd->h_perm.data(), dd->perm, dd->h_perm.size() * sizeof(int),
+                                   hipMemcpyDeviceToHost, s));
+      KAD_HIP_CHECK(hipStreamSynchronize(s));
+    }
     std::vector<int64_t> hpos(r);
-    KAD_HIP_CHECK(hipMemcpyAsync(hperm.data(), dd->perm, hperm.size() * sizeof(int), hipMemcpyDeviceToHost, s));
-    KAD_HIP_CHECK(hipStreamSynchronize(s));
     std::vector<int> inv(nd);
     for (int b = 0; b < nb; ++b) {
-      for (int k = 0; k < nd; ++k) inv[hperm[(size_t)b * nd + k]] = k;
-      for (int j = 0; j < dd->kb[b]; ++j) {
-        const int q = dd->q0[b] + j, row = byblk[b][j];
-        hcrow[q] = row;
-        hcz[q] = b;
+      for (int k = 0; k < nd; ++k) inv[dd->h_perm[(size_t)b * nd + k]] = k;
+      for (int j = 0; j < kb[b]; ++j) {
+        const int q = q0[b] + j, row = byblk[b][j];
         // e_{local row} in pivoted order sits at inv[local row]; column j of W_b
         hpos[q] = (int64_t)b * nd * kmax + (int64_t)j * nd + inv[dd->h_rloc[row]];
       }
@@ -464,9 +522,31 @@ int kad_dd_factor(kad_dd* dd, const double* J, int64_t ldj) {
     KAD_HIP_CHECK(hipGetLastError());
     KAD_HIP_CHECK(hipStreamSynchronize(s));
     double t4 = now_s();
-    rc = kad_lu_factor(reinterpret_cast<kad_ctx*>(c), KAD_PREC_F64, r, dd->cap, r, dd->cap_ipiv, dd->cap_perm, 0);
-    if (rc < 0) return rc;
-    if (rc > 0) return n + rc;  // singular capacitance matrix
+    if (!stat) {
+      rc = kad_lu_factor(reinterpret_cast<kad_ctx*>(c), KAD_PREC_F64, r, dd->cap, r, dd->cap_ipiv, dd->cap_perm, 0);
+      if (rc < 0) return rc;
+      if (rc > 0) return n + rc;  // singular capacitance matrix
+    } else {
+      // its rows in the kept order (cap_perm), into the second buffer, factored there
+      if (dd->cap2_r < dd->cap_r) {  // the capacity of cap: the two buffers trade places
+        if (dd->cap2) (void)hipFree(dd->cap2);
+        dd->cap2 = nullptr;
+        dd->cap2_r = 0;
+        if ((rc = dd->dmalloc((void**)&dd->cap2, dd->cap_r * dd->cap_r * sizeof(double)))) return rc;
+        dd->cap2_r = dd->cap_r;
+      }
+      kad::dd_rows_kernel<<<dim3((unsigned)kad::cdiv(r, 256), (unsigned)r), 256, 0, s>>>(dd->cap, dd->cap_perm, r,
+                                                                                         dd->cap2);
+      KAD_HIP_CHECK(hipGetLastError());
+      int cinfo = 0;
+      double cg = 0.0;
+      rc = kad_lu_factor_batched_static(reinterpret_cast<kad_ctx*>(c), KAD_PREC_F64, r, 1, dd->cap2, r, 0,
+                                        dd->cap_ipiv, &cinfo, &cg);
+      if (rc) return rc;
+      dd->last_growth = std::max(dd->last_growth, std::isnan(cg) ? INFINITY : cg);
+      if (cinfo != 0 || !(cg <= max_growth)) return kStaticCapacitance;
+      std::swap(dd->cap, dd->cap2);  // the factors of the kept-order rows, for the solves
+    }
     if ((rc = kad::trsv_prepare<double>(dd->cap, r, r, 0, dd->cap_tab, dd->cap_flag, dd->cap_ext, s))) return rc;
     KAD_HIP_CHECK(hipStreamSynchronize(s));
     double t5 = now_s();
@@ -478,8 +558,23 @@ int kad_dd_factor(kad_dd* dd, const double* J, int64_t ldj) {
   }
   dd->times[0] = t1 - t0;
   dd->times[1] = t2 - t1;
+  if (!stat) {
+    dd->h_crow = hcrow;
+    dd->piv_valid = true;
+  }
   dd->factored = true;
   return KAD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kad_dd_factor(kad_dd* dd, const double* J, int64_t ldj) { return dd_factor_impl(dd, J, ldj, false, 0.0); }
+
+int kad_dd_factor_static(kad_dd* dd, const double* J, int64_t ldj, double max_growth) {
+  if (!(max_growth >= 1.0)) return KAD_ERR_ARG;
+  return dd_factor_impl(dd, J, ldj, true, max_growth);
 }
 
 int kad_dd_solve(kad_dd* dd, const double* J, int64_t ldj, const double* b, double* x, double tol, int max_outer,
@@ -495,6 +590,13 @@ int kad_dd_apply(kad_dd* dd, double* v) {
   const int rc = dd_apply(dd, v);
   if (rc) return rc;
   KAD_HIP_CHECK(hipStreamSynchronize(dd->c->stream));
+  return KAD_OK;
+}
+
+int kad_dd_pivoting(const kad_dd* dd, int* static_orders, double* growth) {
+  if (!dd) return KAD_ERR_ARG;
+  if (static_orders) *static_orders = dd->last_static ? 1 : 0;
+  if (growth) *growth = dd->last_growth;
   return KAD_OK;
 }
 

@@ -1082,6 +1082,131 @@ int ffp_launch(T* A, int64_t lda, int n, int c0, int c1, int k0, int k1, int* ip
   return KAD_OK;
 }
 
+// Base panel without row interchanges (static pivoting): the rows were put in the
+// order of an earlier factorisation's pivots beforehand, so row j is the pivot of
+// column j.  Columns [c0, c1) (at most W), rows [c0, n); matrix blockIdx.y of a batch.
+// - Every workgroup factors the w x w diagonal block itself, row by row (threads
+//   t < w hold its rows in registers; row j, final, goes to LDS before the rows below
+//   it use it).  The operations and their order are the same in every workgroup, so
+//   all hold the same factored block.
+// - Its rows below the block then solve l U11 = a, one row per thread in registers,
+//   with the same row operations.
+// - The last workgroup to arrive (a counter, returned to zero by that workgroup) writes
+//   the factored diagonal block: every other one has read the block by then, so none
+//   reads a block that was already overwritten.  It also writes the identity
+//   interchanges and the first zero pivot.
+// - growth: the largest |multiplier| of the matrix, as the bits of a non-negative
+//   double (an unsigned maximum orders them; inf and NaN above every finite value).
+template <typename T, int W>
+__global__ __launch_bounds__(256) void nopiv_panel_kernel(T* __restrict__ A, int64_t lda, int n, int c0, int c1,
+                                                          int* __restrict__ ipiv, int* __restrict__ info,
+                                                          unsigned* __restrict__ cnt,
+                                                          unsigned long long* __restrict__ growth, int64_t sA,
+                                                          int sP) {
+  A += (int64_t)blockIdx.y * sA;
+  ipiv += (int64_t)blockIdx.y * sP;
+  info += blockIdx.y;
+  cnt += blockIdx.y;
+  growth += blockIdx.y;
+  __shared__ T U[W][W + 1];  // the factored diagonal block, row j once it is final
+  __shared__ unsigned long long s_g[256 / 64];
+  __shared__ int s_last;
+  const int w = c1 - c0, tid = threadIdx.x;
+  // this thread's row below the block (loaded first: its latency overlaps the block)
+  const int64_t i = (int64_t)c0 + (int64_t)blockIdx.x * 256 + tid;
+  const bool mine = i >= c0 + w && i < n;
+  T row[W];
+#pragma unroll
+  for (int c = 0; c < W; ++c) row[c] = mine && c < w ? A[i + (int64_t)(c0 + c) * lda] : T(0);
+  // the diagonal block, row t per thread t < w
+  T drow[W];
+#pragma unroll
+  for (int c = 0; c < W; ++c) drow[c] = tid < w && c < w ? A[(int64_t)(c0 + tid) + (int64_t)(c0 + c) * lda] : T(0);
+#pragma unroll
+  for (int j = 0; j < W; ++j) {
+    if (j < w) {
+      if (tid == j) {
+#pragma unroll
+        for (int c = 0; c < W; ++c) U[j][c] = drow[c];
+      }
+      __syncthreads();
+      if (tid > j && tid < w) {
+        const T l = drow[j] / U[j][j];
+        drow[j] = l;
+#pragma unroll
+        for (int c = j + 1; c < W; ++c) drow[c] -= l * U[j][c];
+      }
+    }
+  }
+  __syncthreads();
+  // every thread has read its part of the diagonal block (its values are in LDS, behind
+  // the barrier): count this workgroup in
+  if (tid == 0)
+    s_last = __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1;
+  unsigned long long g = 0;
+  auto take = [&g](T l) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(fabs((double)l));
+    g = b > g ? b : g;
+  };
+  if (mine) {
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      if (j < w) {
+        const T l = row[j] / U[j][j];
+        row[j] = l;
+        take(l);
+#pragma unroll
+        for (int c = j + 1; c < W; ++c) row[c] -= l * U[j][c];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < W; ++c)
+      if (c < w) A[i + (int64_t)(c0 + c) * lda] = row[c];
+  }
+  __syncthreads();  // s_last
+  if (s_last) {
+    if (tid < w) {
+#pragma unroll
+      for (int c = 0; c < W; ++c) {
+        if (c < w) A[(int64_t)(c0 + tid) + (int64_t)(c0 + c) * lda] = U[tid][c];
+        if (c < tid && c < w) take(U[tid][c]);
+      }
+      ipiv[c0 + tid] = c0 + tid;
+    }
+    if (tid == 0) {
+      __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      for (int j = 0; j < w; ++j)
+        if (U[j][j] == T(0)) {
+          if (*info == 0) *info = c0 + j + 1;
+          break;
+        }
+    }
+  }
+  // the workgroup's largest multiplier, then the matrix's
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    const unsigned long long h = __shfl_xor(g, o);
+    g = h > g ? h : g;
+  }
+  if ((tid & 63) == 0) s_g[tid >> 6] = g;
+  __syncthreads();
+  if (tid == 0) {
+    for (int q = 1; q < 256 / 64; ++q) g = s_g[q] > g ? s_g[q] : g;
+    __hip_atomic_fetch_max(growth, g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+template <typename T>
+int nopiv_launch(T* A, int64_t lda, int n, int c0, int c1, int* ipiv, LuWork& w, const LuBatch& bt) {
+  if (c1 - c0 > PANEL_BASE || c1 <= c0 || c1 > n) return arg_error("nopiv_launch: panel shape");
+  if (bt.nbat < 1 || bt.nbat > LU_MAX_BATCH || !bt.growth || !w.bcnt) return arg_error("nopiv_launch: batch");
+  const dim3 grid((unsigned)cdiv(n - c0, 256), (unsigned)bt.nbat);
+  nopiv_panel_kernel<T, PANEL_BASE><<<grid, 256, 0, w.stream>>>(A, lda, n, c0, c1, ipiv, bt.info ? bt.info : w.info,
+                                                                 w.bcnt, bt.growth, bt.sA, bt.sP);
+  KAD_HIP_CHECK(hipGetLastError());
+  return KAD_OK;
+}
+
 // Base panel: columns [c0, c1) (c1 - c0 <= PANEL_BASE), rows [c0, n).
 template <typename T>
 int panel_base(T* A, int64_t lda, int n, int c0, int c1, int* ipiv, LuWork& w) {
@@ -1120,9 +1245,11 @@ template <typename T>
 int panel_rec(T* A, int64_t lda, int n, int c0, int c1, int* ipiv, LuWork& w, int pk0, int pk1, bool fused,
               const LuBatch& bt = LuBatch()) {
   const int wdt = c1 - c0;
-  if (bt.nbat > 1 && !fused) return arg_error("panel_rec: a batch needs the persistent panel");
-  if (wdt <= PANEL_BASE)
+  if (bt.nbat > 1 && !fused && !bt.nopiv) return arg_error("panel_rec: a batch needs the persistent panel");
+  if (wdt <= PANEL_BASE) {
+    if (bt.nopiv) return nopiv_launch<T>(A, lda, n, c0, c1, ipiv, w, bt);
     return fused ? ffp_launch<T>(A, lda, n, c0, c1, pk0, pk1, ipiv, w, bt) : panel_base<T>(A, lda, n, c0, c1, ipiv, w);
+  }
   int h = wdt / 2;
   h = (h + 15) / 16 * 16;
   const int cm = c0 + h;
@@ -1253,8 +1380,15 @@ int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w,
   // A panel's rows at or below R are zero in its columns and stay zero: the
   // interchanges and the update of an earlier panel q touch rows above R_q only,
   // so R_p = max(initial extent of p's columns, R_q for q < p) bounds them.
-  const bool bat = bt.nbat > 1;
+  // a batch, or static pivoting: no masks, no row extents, the persistent panel or the
+  // static-pivot panel (no interchanges at all)
+  const bool bat = bt.nbat > 1 || bt.nopiv;
+  const bool piv = !bt.nopiv;
   if (bt.nbat < 1 || bt.nbat > LU_MAX_BATCH) return arg_error("getrf: batch size");
+  if (bt.nopiv) {
+    if (!bt.growth) return arg_error("getrf: static pivoting needs growth words");
+    KAD_HIP_CHECK(hipMemsetAsync(bt.growth, 0, bt.nbat * sizeof(unsigned long long), w.stream));
+  }
   const int ntc = (int)cdiv(n, 128), nrc = (int)cdiv(n, 2048);
   std::vector<int> ext(ntc, bat ? n - 1 : -1);
   if (!bat) {
@@ -1274,7 +1408,7 @@ int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w,
     R = std::min(R, n);
     r_prev = R;
     int fg, frpw;
-    const bool fused = wp.ffp && !wp.ffp_off && ffp_shape<T>(R - k0, wp.ncu, fg, frpw);
+    const bool fused = bt.nopiv || (wp.ffp && !wp.ffp_off && ffp_shape<T>(R - k0, wp.ncu, fg, frpw));
     int rc = panel_rec<T>(A, lda, R, k0, k1, ipiv, wp, k0, k1, fused, bt);
     if (rc) return rc;
     if (k1 < n && mask) {
@@ -1303,8 +1437,7 @@ int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w,
     KAD_HIP_CHECK(hipStreamWaitEvent(w.stream, w.ev_panel[par], 0));
     if (k1 >= n) break;
     if (k2 < n && bat) {
-      rc = laswp<T>(A, lda, k2, k3, ipiv, k0, k1, wm, bt);
-      if (rc) return rc;
+      if (piv && (rc = laswp<T>(A, lda, k2, k3, ipiv, k0, k1, wm, bt))) return rc;
       rc = trsm_left_unit_lower<T>(L11, lda, k1 - k0, A + k0 + (int64_t)k2 * lda, lda, k3 - k2, wm, gemm_prec,
                                    nullptr, bt.nbat, bt.sA, bt.sA);
       if (rc) return rc;
@@ -1329,8 +1462,7 @@ int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w,
     KAD_HIP_CHECK(hipEventRecord(w.ev_next[par], w.stream));
     // ---- main stream, part B: columns [k3, n), beside the next panel
     if (k3 < n && bat) {
-      rc = laswp<T>(A, lda, k3, n, ipiv, k0, k1, wm, bt);
-      if (rc) return rc;
+      if (piv && (rc = laswp<T>(A, lda, k3, n, ipiv, k0, k1, wm, bt))) return rc;
       rc = trsm_left_unit_lower<T>(L11, lda, k1 - k0, A + k0 + (int64_t)k3 * lda, lda, n - k3, wm, gemm_prec,
                                    nullptr, bt.nbat, bt.sA, bt.sA);
       if (rc) return rc;
@@ -1361,8 +1493,7 @@ int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w,
     }
     // ---- panel stream: the next panel's columns [k1, k2), then the next panel
     if (p > 0) KAD_HIP_CHECK(hipStreamWaitEvent(wp.stream, w.ev_next[par ^ 1], 0));
-    rc = laswp<T>(A, lda, k1, k2, ipiv, k0, k1, wp, bt);
-    if (rc) return rc;
+    if (piv && (rc = laswp<T>(A, lda, k1, k2, ipiv, k0, k1, wp, bt))) return rc;
     rc = trsm_left_unit_lower<T>(L11, lda, k1 - k0, A + k0 + (int64_t)k1 * lda, lda, k2 - k1, wp, gemm_prec, nullptr,
                                  bt.nbat, bt.sA, bt.sA);
     if (rc) return rc;
@@ -1376,7 +1507,7 @@ int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w,
     if (rc) return rc;
     KAD_HIP_CHECK(hipEventRecord(w.ev_panel[par ^ 1], wp.stream));
   }
-  for (int z = 0; z < bt.nbat; ++z)
+  for (int z = 0; z < bt.nbat && piv; ++z)
     if ((rc = apply_left_swaps<T>(A + z * bt.sA, lda, n, ipiv + (int64_t)z * bt.sP, nb, w))) return rc;
   return KAD_OK;
 }

@@ -113,11 +113,15 @@ int Ctx::lu_work(int n, int nb) {
   }
   lu.red_cap = red;
   if (!lu.counter) {
-    // counter, info (+16 B), the persistent panel's words (+32 B), batch info (+64 B)
-    KAD_HIP_CHECK(hipMalloc((void**)&lu.counter, 64 + LU_MAX_BATCH * sizeof(int)));
-    KAD_HIP_CHECK(hipMemsetAsync(lu.counter, 0, 64 + LU_MAX_BATCH * sizeof(int), stream));
+    // counter, info (+16 B), the persistent panel's words (+32 B), batch info (+64 B),
+    // the static-pivot panel's arrival counters, then its growth words (8-byte aligned)
+    constexpr size_t words = 64 + 2 * LU_MAX_BATCH * sizeof(int) + LU_MAX_BATCH * sizeof(unsigned long long);
+    KAD_HIP_CHECK(hipMalloc((void**)&lu.counter, words));
+    KAD_HIP_CHECK(hipMemsetAsync(lu.counter, 0, words, stream));
     lu.info = (int*)((char*)lu.counter + 16);
     lu.binfo = (int*)((char*)lu.counter + 64);
+    lu.bcnt = (unsigned*)((char*)lu.counter + 64 + LU_MAX_BATCH * sizeof(int));
+    lu.bgrowth = (unsigned long long*)((char*)lu.counter + 64 + 2 * LU_MAX_BATCH * sizeof(int));
   }
   lu.ffp_sync = (int*)((char*)lu.counter + 32);  // [2]: abort flag (zeroed above)
   if (!lu.ffp) {
@@ -429,6 +433,49 @@ int kad_lu_factor_batched(kad_ctx* ctx, int prec, int n, int nbat, void* A, int6
       if (!kad::ipiv_to_perm(hp.data() + (size_t)z * n, n, hperm.data() + (size_t)z * n)) return KAD_ERR_HIP;
     KAD_HIP_CHECK(hipMemcpyAsync(perm, hperm.data(), hperm.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
     KAD_HIP_CHECK(hipStreamSynchronize(c->stream));
+  }
+  return KAD_OK;
+}
+
+// The same without row interchanges (static pivoting): the caller has put the rows of
+// every matrix in the order of an earlier factorisation's pivots (one matrix may have
+// any order; a batch, orders that are multiples of 256).  ipiv (nbat x n) gets
+// the identity; info[z] the first exactly-zero pivot; growth[z] the largest |multiplier|
+// of matrix z (partial pivoting keeps it at most 1; a large one means the order no longer
+// suits the matrix).  HOST info and growth.
+int kad_lu_factor_batched_static(kad_ctx* ctx, int prec, int n, int nbat, void* A, int64_t lda, int64_t strideA,
+                                 int* ipiv, int* info, double* growth) {
+  if (!ctx || !A || !ipiv || !info || !growth || n <= 0 || (nbat > 1 && n % 256 != 0) || nbat < 1 ||
+      nbat > kad::LU_MAX_BATCH || lda < n || prec < 0 || prec > 2 || (nbat > 1 && strideA < lda * (int64_t)n)) {
+    std::fprintf(stderr, "kad_lu_factor_batched_static: bad arguments (n %d, nbat %d, lda %lld, strideA %lld)\n", n,
+                 nbat, (long long)lda, (long long)strideA);
+    return KAD_ERR_ARG;
+  }
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  int rc = c->lu_work(n, 256);
+  if (rc) return rc;
+  KAD_HIP_CHECK(hipMemsetAsync(c->lu.binfo, 0, nbat * sizeof(int), c->stream));
+  kad::LuBatch bt;
+  bt.nbat = nbat;
+  bt.sA = strideA;
+  bt.sP = n;
+  bt.info = c->lu.binfo;
+  bt.nopiv = true;
+  bt.growth = c->lu.bgrowth;
+  if (prec == kad::F64)
+    rc = kad::getrf<double>((double*)A, lda, n, ipiv, 256, kad::F64, c->lu, bt);
+  else
+    rc = kad::getrf<float>((float*)A, lda, n, ipiv, 256, prec, c->lu, bt);
+  KAD_HIP_CHECK(hipStreamSynchronize(c->lu.stream2));
+  unsigned long long g[kad::LU_MAX_BATCH];
+  KAD_HIP_CHECK(hipMemcpyAsync(info, c->lu.binfo, nbat * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  KAD_HIP_CHECK(hipMemcpyAsync(g, c->lu.bgrowth, nbat * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+  KAD_HIP_CHECK(hipStreamSynchronize(c->stream));
+  if (rc) return rc;
+  for (int z = 0; z < nbat; ++z) {
+    double v;
+    std::memcpy(&v, &g[z], sizeof v);
+    growth[z] = v;
   }
   return KAD_OK;
 }

@@ -451,6 +451,10 @@ struct kad_system_impl {
   bool fac_valid = false;  // Jfac / perm hold usable fp64 factors
   kad_dd* dd = nullptr;    // solvers 6-7: the domain-decomposed factors (their own copy of the blocks)
   bool dd_valid = false;
+  // solvers 6-7: factor with the pivot orders of the solve's last factorisation with
+  // partial pivoting (static pivoting, kad_dd_factor_static) once there is one
+  bool dd_static = true;
+  bool dd_orders = false;  // this solve has such a factorisation (option 2 forgets it)
   // Jacobian fill on several streams (kad_system_set_option 3): the tangent programs are
   // dealt to the streams by estimated cost, each stream with its own tangent arena;
   // every program writes only its own columns of J, so the streams never touch the
@@ -1380,6 +1384,21 @@ static int newton_direction(kad_system_impl* S, int solver, double err, double* 
   const double eta = S->forcing > 0.0 ? std::max(1e-13, S->forcing * std::min(1.0, err)) : 1e-13;
   // A refinement result is a usable Newton direction below this relative residual.
   constexpr double kAcceptResid = 1e-6;
+  // Static pivoting (solvers 6-7): the kept orders are rejected above this multiplier
+  // (partial pivoting keeps every one at most 1), and their factors are replaced when
+  // the refinement needs more inner steps than this (2 with the exact Woodbury solve).
+  // The refinement on the fp64 J is the real check: on the 17x13x12 spinning hole the
+  // orders of one Newton step give the next step's blocks multipliers up to 4e5 (in
+  // columns of rounding-level entries) and the preconditioned refinement still reaches
+  // 2e-15 in two inner steps, as with partial pivoting; the bound only turns away
+  // factors past any use (and the infinities of an exactly zero pivot).
+  constexpr double kDdMaxGrowth = 1e8;
+  constexpr int kDdStaticInner = 6;
+  // Orders kept over several steps drift from the Jacobian (the same hole: multipliers
+  // 4e5, 6e9, 4e14 one, two, three steps on, and then 700 inner steps), so a step whose
+  // static factors show multipliers above this takes them, and the next step factors
+  // with partial pivoting again for fresh orders.
+  constexpr double kDdRefresh = 1e3;
   if (solver == 6 || solver == 7) {
     // domain-decomposed factors (the blocks factored together, then the Woodbury
     // coupling) precondition flexible GMRES on the current J; solver 7 keeps them for
@@ -1406,17 +1425,49 @@ static int newton_direction(kad_system_impl* S, int solver, double err, double* 
     if (!solved) {
       KAD_HIP_CHECK(hipStreamSynchronize(st));
       const double tg0 = now_s();
-      if ((rc = kad_dd_factor(S->dd, S->J, n)) < 0) return rc;
+      // static pivoting first when the solve has pivot orders; partial pivoting when
+      // they do not apply (a multiplier above kDdMaxGrowth, other coupling rows)
+      int dd_info = 1;
+      bool stat = false;
+      if (S->dd_static && S->dd_orders) {
+        if ((rc = kad_dd_factor_static(S->dd, S->J, n, kDdMaxGrowth)) < 0) return rc;
+        dd_info = rc;
+        stat = rc == 0;
+        double growth = 0.0;
+        if (stat && kad_dd_pivoting(S->dd, nullptr, &growth) == KAD_OK && growth > kDdRefresh)
+          S->dd_orders = false;  // used once; the next step refreshes them
+      }
+      auto factor_pivoting = [&]() -> int {
+        const int frc = kad_dd_factor(S->dd, S->J, n);
+        S->dd_orders = frc == 0;
+        return frc;
+      };
+      if (!stat && (dd_info = factor_pivoting()) < 0) return dd_info;
       tf1 += now_s() - tg0;
-      const int dd_info = rc;  // > 0: a singular domain block or capacitance matrix
+      // dd_info > 0: a singular domain block or capacitance matrix
       S->dd_valid = (dd_info == 0);
       S->last_factored = 1;
       bool fallback = dd_info != 0;
       if (dd_info == 0) {
         rc = kad_dd_solve(S->dd, S->J, n, S->F, S->X, eta, 8, S->last_iters, &S->last_resid);
         if (rc < 0 && rc != KAD_ERR_NOCONV) return rc;
+        bool poor = rc == KAD_ERR_NOCONV && !(S->last_resid <= std::max(eta, kAcceptResid));
+        if (stat && (poor || S->last_iters[1] > kDdStaticInner)) {
+          // the kept orders made a poor preconditioner: partial pivoting, solved again
+          KAD_HIP_CHECK(hipStreamSynchronize(st));
+          const double tg2 = now_s();
+          if ((dd_info = factor_pivoting()) < 0) return dd_info;
+          tf1 += now_s() - tg2;
+          S->dd_valid = (dd_info == 0);
+          poor = dd_info != 0;
+          if (dd_info == 0) {
+            rc = kad_dd_solve(S->dd, S->J, n, S->F, S->X, eta, 8, S->last_iters, &S->last_resid);
+            if (rc < 0 && rc != KAD_ERR_NOCONV) return rc;
+            poor = rc == KAD_ERR_NOCONV && !(S->last_resid <= std::max(eta, kAcceptResid));
+          }
+        }
         S->prev_inner = S->last_iters[1];
-        fallback = rc == KAD_ERR_NOCONV && !(S->last_resid <= std::max(eta, kAcceptResid));
+        fallback = poor;
       }
       if (fallback) {
         // a singular block (J itself need not be singular) or not a usable Newton
@@ -2187,6 +2238,14 @@ int kad_system_dd_info(kad_system* s, int* sizes, double* times) {
   return kad_dd_info(S->dd, sizes, times);
 }
 
+// Pivoting of the last domain-decomposed factorisation (kad_dd_pivoting).
+int kad_system_dd_pivoting(kad_system* s, int* static_orders, double* growth) {
+  if (!s) return KAD_ERR_ARG;
+  auto* S = reinterpret_cast<kad_system_impl*>(s);
+  if (!S->dd) return KAD_ERR_ARG;
+  return kad_dd_pivoting(S->dd, static_orders, growth);
+}
+
 // F(u) (device, n_rows): System_of_eqs::sec_member.
 int kad_residual(kad_system* s, double* F) { return kad_xcts_residual(s, F); }
 
@@ -2296,9 +2355,15 @@ int kad_system_set_option(kad_system* s, int which, double value) {
   } else if (which == 1) {
     S->reuse_inner = (int)value;
   } else if (which == 2) {
-    // forget every factorisation kept for reuse: the next step factors its own J
-    S->lo_valid = S->fac_valid = S->dd_valid = false;
+    // forget every factorisation kept for reuse, and the kept pivot orders: the next
+    // step factors its own J with partial pivoting
+    S->lo_valid = S->fac_valid = S->dd_valid = S->dd_orders = false;
     S->prev_inner = 0;
+  } else if (which == 4) {
+    // static pivoting in solvers 6-7 (1, the default) or partial pivoting at every
+    // factorisation (0)
+    if (value != 0.0 && value != 1.0) return KAD_ERR_ARG;
+    S->dd_static = value != 0.0;
   } else if (which == 3) {
     // streams of the Jacobian fill
     if (value < 1.0 || value > (double)kad_system_impl::kFillMax || value != (double)(int)value) return KAD_ERR_ARG;

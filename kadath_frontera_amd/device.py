@@ -103,6 +103,20 @@ class Context:
                                         _ptr(perm), info), "kad_lu_factor_batched")
         return ipiv, perm, list(info)
 
+    def lu_factor_batched_static(self, prec, A):
+        """In-place LU of the column-major matrices A[z] without row interchanges (their rows
+        already in an earlier factorisation's pivot order); returns (info, growth) per matrix."""
+        nbat, n = A.shape[0], A.shape[1]
+        if A.stride(1) != 1:
+            raise ValueError("lu_factor_batched_static: A[z] must be column-major")
+        ipiv = torch.empty(nbat, n, dtype=torch.int32, device=A.device)
+        info = (ctypes.c_int * nbat)()
+        growth = (ctypes.c_double * nbat)()
+        self._enter()
+        check(lib.kad_lu_factor_batched_static(self.h, PREC[prec], n, nbat, _ptr(A), A.stride(2), A.stride(0),
+                                               _ptr(ipiv), info, growth), "kad_lu_factor_batched_static")
+        return list(info), list(growth)
+
     def lu_solve_batched(self, LU, perm, B):
         """B[z] <- A[z]^{-1} B[z] with fp64 batched factors (B: (nbat, n, nrhs), B[z].T contiguous)."""
         nbat, n, nrhs = B.shape
@@ -216,6 +230,21 @@ class DomainSolve:
         rc = lib.kad_dd_factor(self.h, _ptr(J), J.stride(1))
         check(rc, "kad_dd_factor")
         return rc
+
+    def factor_static(self, J, max_growth=8.0):
+        """kad_dd_factor_static: 0 factored with the pivot orders of the last factor(),
+        > 0 when they do not apply (then call factor)."""
+        self.ctx._enter()
+        rc = lib.kad_dd_factor_static(self.h, _ptr(J), J.stride(1), float(max_growth))
+        check(rc, "kad_dd_factor_static")
+        return rc
+
+    def pivoting(self):
+        """(the last factorisation took the kept orders, its largest |multiplier|)"""
+        st = ctypes.c_int()
+        g = ctypes.c_double()
+        check(lib.kad_dd_pivoting(self.h, ctypes.byref(st), ctypes.byref(g)), "kad_dd_pivoting")
+        return bool(st.value), g.value
 
     def solve(self, J, b, tol=1e-13, max_outer=8):
         x = torch.empty_like(b)

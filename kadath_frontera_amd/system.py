@@ -277,13 +277,17 @@ class DeviceSystem:
         check(lib.kad_param_tangent(self.h, self.params[name], ctypes.c_void_p(out.data_ptr())), "kad_param_tangent")
         return out
 
-    def set_option(self, forcing=None, reuse_inner=None, drop_factors=False, fill_streams=None):
+    def set_option(self, forcing=None, reuse_inner=None, drop_factors=False, fill_streams=None,
+                   static_pivoting=None):
         """Iterative solver options: forcing factor of the refinement target, the
         inner-step threshold for reusing earlier factors, and dropping the factors kept
-        for reuse (the next step factors its own Jacobian); and the number of streams
-        (1-4) of the Jacobian fill."""
+        for reuse (the next step factors its own Jacobian, with partial pivoting); the
+        number of streams (1-4) of the Jacobian fill; static pivoting in the
+        domain-decomposed solvers (True, the default, or False)."""
         if fill_streams is not None:
             check(lib.kad_system_set_option(self.h, 3, float(fill_streams)), "kad_system_set_option")
+        if static_pivoting is not None:
+            check(lib.kad_system_set_option(self.h, 4, 1.0 if static_pivoting else 0.0), "kad_system_set_option")
         if drop_factors:
             check(lib.kad_system_set_option(self.h, 2, 0.0), "kad_system_set_option")
         if forcing is not None:
@@ -311,8 +315,12 @@ class DeviceSystem:
         sz = (ctypes.c_int * 4)()
         tm = (ctypes.c_double * 5)()
         check(lib.kad_system_dd_info(self.h, sz, tm), "kad_system_dd_info")
+        st = ctypes.c_int()
+        g = ctypes.c_double()
+        check(lib.kad_system_dd_pivoting(self.h, ctypes.byref(st), ctypes.byref(g)), "kad_system_dd_pivoting")
         return dict(nd=sz[0], coupling_rows=sz[1], kmax=sz[2], blocks=sz[3],
-                    times=dict(gather=tm[0], block_lu=tm[1], spikes=tm[2], capacitance=tm[3], capacitance_lu=tm[4]))
+                    times=dict(gather=tm[0], block_lu=tm[1], spikes=tm[2], capacitance=tm[3], capacitance_lu=tm[4]),
+                    static_pivoting=bool(st.value), growth=g.value)
 
     def do_newton(self, prec, solver="f64"):
         err = ctypes.c_double()
