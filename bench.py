@@ -55,15 +55,15 @@ APPLY3D_TRAFFIC_RATIO = (2.0 * 83618276.7 + 166635597.5) * 1024.0 / 340514582928
 # counter path, with the persistent panel kernel spinning beside the GEMM), so the line's
 # traffic stays null. The kernel is measured on its own at one domain block's trailing
 # update instead (--gemm-only: C 12032 x 12032 -= A 12032 x 256 B, same calibration):
-# 2 x 980019.3 KiB fetched + 1131008 KiB written = 3.165 GB per launch, 1.338 x the
+# 2 x 981412.0 KiB fetched + 1131008 KiB written = 3.168 GB per launch, 1.339 x the
 # algorithmic 2.366 GB (C read and written once, the A and B panels re-fetched about 17
 # times over eight L2s), at 23.4 flop/B, above the FP64 ridge: compute-bound.
 # traffic_model_bytes is the operand traffic per launch of the executed tiles (each 128 x
 # 128 C tile reads its 128 x k A and k x 128 B panels, reads and writes C), an upper bound
 # on HBM bytes when nothing is reused through L2.
 GEMM_TRAFFIC_NOTE = ("traffic is not measured over the LU's mixed GEMM launches; measured alone at a domain "
-                     "block's trailing update (12032 x 12032 x 256, --gemm-only) gemm_k4 moves 3.165 GB per "
-                     "launch = 1.338 x its algorithmic bytes (C once, A/B panels re-fetched), compute-bound at "
+                     "block's trailing update (12032 x 12032 x 256, --gemm-only) gemm_k4 moves 3.168 GB per "
+                     "launch = 1.339 x its algorithmic bytes (C once, A/B panels re-fetched), compute-bound at "
                      "23.4 flop/B; traffic_model_bytes = operand bytes per launch of the executed tiles with no "
                      "L2 reuse")
 
