@@ -13,9 +13,10 @@ Jacobian fill + factorisation + solve (+ fp64 refinement) + field update, all
 device-resident.  The K timed steps are the iterations of the Newton solve from
 the guess (restarting from the guess once ||F||_inf < 1e-10, with no factors kept).
 The system is assembled in domain-major order.  The headline solver ("f64-dd")
-factors every step's own Jacobian in FP64: the five domain blocks with partial
-pivoting inside each, factored together (one panel chain of 12032 columns instead
-of 58905), the interface rows through a Woodbury capacitance matrix, then FP64
+factors every step's own Jacobian in FP64: the five domain blocks factored together
+(one panel chain of 12032 columns instead of 58905), with partial pivoting inside each
+or with the pivot orders of the previous step (static pivoting, refreshed after large
+multipliers), the interface rows through a Woodbury capacitance matrix, then FP64
 flexible GMRES on J to a relative residual of 1e-13; it reaches the same iterates as
 the single FP64 LU of the whole J (the reference's pdgesv_, "f64", reported beside
 it) to 1e-14.  The other legs run to the same test from the same guess with time to
