@@ -423,7 +423,10 @@ int kad_system_get_double(kad_system* s, int k, double* value);
  * the solve factor their J with its pivot orders (kad_dd_factor_static), falling
  * back to partial pivoting when a multiplier exceeds 1e8 or the refinement needs more
  * than 6 inner steps; after a step whose multipliers exceed 1e3 the next step factors
- * with partial pivoting again (the orders drift with the Jacobian). */
+ * with partial pivoting again (the orders drift with the Jacobian).  which 5 = an
+ * element-wise instruction whose operand is made by the spectral pass just before it,
+ * and used nowhere else, is applied in that pass's store phase (1, the default; 0:
+ * launched on its own); J is the same bit for bit. */
 int kad_system_set_option(kad_system* s, int which, double value);
 /* as kad_xcts_do_newton / kad_xcts_last_solve */
 int kad_system_do_newton(kad_system* s, double precision, double* error, int solver, double* times);

@@ -382,6 +382,20 @@ int kad_lu_solve(kad_ctx* ctx, int prec, int n, const void* LU, int64_t lda, con
   return c->solve(prec, n, LU, lda, perm, b, trans);
 }
 
+}  // extern "C"
+
+namespace {
+// Block size of the batched factorisations: KAD_LU_NB when it is a multiple of 256 that
+// divides n, else 256.
+int batched_nb(int n) {
+  const char* e = std::getenv("KAD_LU_NB");
+  const int v = e ? std::atoi(e) : 0;
+  return (v > 0 && v % 256 == 0 && n % v == 0) ? v : 256;
+}
+}  // namespace
+
+extern "C" {
+
 // nbat matrices of order n (a multiple of 256) at A + z strideA, factored together.
 int kad_lu_factor_batched(kad_ctx* ctx, int prec, int n, int nbat, void* A, int64_t lda, int64_t strideA, int* ipiv,
                           int* perm, int* info) {
@@ -392,10 +406,7 @@ int kad_lu_factor_batched(kad_ctx* ctx, int prec, int n, int nbat, void* A, int6
     return KAD_ERR_ARG;
   }
   Ctx* c = reinterpret_cast<Ctx*>(ctx);
-  // block size: KAD_LU_NB when it is a multiple of 256 that divides n, else 256
-  int nb = 256;
-  const char* e = std::getenv("KAD_LU_NB");
-  if (e && std::atoi(e) > 0 && std::atoi(e) % 256 == 0 && n % std::atoi(e) == 0) nb = std::atoi(e);
+  const int nb = batched_nb(n);
   int rc = c->lu_work(n, nb);
   if (rc) return rc;
   // every matrix's workgroups of the persistent panel are resident together
@@ -452,7 +463,8 @@ int kad_lu_factor_batched_static(kad_ctx* ctx, int prec, int n, int nbat, void* 
     return KAD_ERR_ARG;
   }
   Ctx* c = reinterpret_cast<Ctx*>(ctx);
-  int rc = c->lu_work(n, 256);
+  const int nb = batched_nb(n);
+  int rc = c->lu_work(n, nb);
   if (rc) return rc;
   KAD_HIP_CHECK(hipMemsetAsync(c->lu.binfo, 0, nbat * sizeof(int), c->stream));
   kad::LuBatch bt;
@@ -463,9 +475,9 @@ int kad_lu_factor_batched_static(kad_ctx* ctx, int prec, int n, int nbat, void* 
   bt.nopiv = true;
   bt.growth = c->lu.bgrowth;
   if (prec == kad::F64)
-    rc = kad::getrf<double>((double*)A, lda, n, ipiv, 256, kad::F64, c->lu, bt);
+    rc = kad::getrf<double>((double*)A, lda, n, ipiv, nb, kad::F64, c->lu, bt);
   else
-    rc = kad::getrf<float>((float*)A, lda, n, ipiv, 256, prec, c->lu, bt);
+    rc = kad::getrf<float>((float*)A, lda, n, ipiv, nb, prec, c->lu, bt);
   KAD_HIP_CHECK(hipStreamSynchronize(c->lu.stream2));
   unsigned long long g[kad::LU_MAX_BATCH];
   KAD_HIP_CHECK(hipMemcpyAsync(info, c->lu.binfo, nbat * sizeof(int), hipMemcpyDeviceToHost, c->stream));

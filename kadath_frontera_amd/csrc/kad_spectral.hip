@@ -97,7 +97,8 @@ __device__ __forceinline__ void pass_lines(const double* x, double* y,
 __global__ __launch_bounds__(256) void apply3d_kernel(const double* __restrict__ in, int64_t in_stride,
                                                       double* __restrict__ out, int64_t out_stride, int d0, int d1,
                                                       int d2, Passes P, const double* __restrict__ pool,
-                                                      int accumulate, double scale, int stage_pool, int inplace) {
+                                                      int accumulate, double scale, int stage_pool, int inplace,
+                                                      PostOp post) {
   extern __shared__ double lds[];
   // the two ping-pong buffers are selected by value (not through an array of
   // pointers), so every access stays an LDS access and the matrix-pool loads can be
@@ -226,6 +227,14 @@ __global__ __launch_bounds__(256) void apply3d_kernel(const double* __restrict__
         if (e < nfin) dst[e] = v[u] + scale * fin[e];
       }
     }
+  } else if (post.kind >= 0) {
+    // the fused element-wise instruction (scale is 1 here)
+    const double* orow = post.o ? post.o + b * post.os : nullptr;
+    double* drow = post.dst + b * post.ds;
+    for (int e = threadIdx.x; e < nfin; e += blockDim.x) {
+      const double v = fin[e], o = orow ? orow[e] : 0.0;
+      drow[e] = post.lin_b ? ew_op(post.kind, o, v, post.s) : ew_op(post.kind, v, o, post.s);
+    }
   } else {
     for (int e = threadIdx.x; e < nfin; e += blockDim.x) dst[e] = scale * fin[e];
   }
@@ -257,8 +266,11 @@ __global__ __launch_bounds__(256) void apply_lines_kernel(const double* __restri
 }
 
 int apply3d_launch(const double* in, int64_t in_stride, double* out, int64_t out_stride, int nbatch, int d0, int d1,
-                   int d2, const Passes& P, const double* pool, int accumulate, double scale, hipStream_t s) {
+                   int d2, const Passes& P, const double* pool, int accumulate, double scale, hipStream_t s,
+                   const PostOp* post) {
   if (nbatch <= 0) return KAD_OK;
+  if (post && post->kind >= 0 && (accumulate || scale != 1.0 || !post->dst)) return KAD_ERR_ARG;
+  const PostOp pop = post ? *post : PostOp();
   size_t nl = 0;
   for (int p = 0; p < P.npass; ++p) nl += (size_t)P.nlines[p];
   // the matrix pool is staged in LDS only if some pass has a line length outside
@@ -286,12 +298,14 @@ int apply3d_launch(const double* in, int64_t in_stride, double* out, int64_t out
     int od[3] = {d0, d1, d2};
     for (int p = 0; p < P.npass; ++p) od[P.axis[p]] = P.nout[p];
     const double nin = (double)d0 * d1 * d2, nout = (double)od[0] * od[1] * od[2];
-    const int rc = tm->begin(s, 8.0 * nbatch * (nin + nout * (accumulate ? 2 : 1)));
+    // (a fused instruction's other operand adds its array per batch element)
+    const double extra = (pop.kind >= 0 && pop.o && pop.os != 0) ? nout : 0.0;
+    const int rc = tm->begin(s, 8.0 * nbatch * (nin + nout * (accumulate ? 2 : 1) + extra));
     if (rc < 0) return rc;
     timed = rc == 0;
   }
   apply3d_kernel<<<nbatch, 256, lds, s>>>(in, in_stride, out, out_stride, d0, d1, d2, P, pool, accumulate, scale,
-                                          stage_pool, inplace);
+                                          stage_pool, inplace, pop);
   KAD_HIP_CHECK(hipGetLastError());
   if (timed && tm->end(s)) return KAD_ERR_HIP;
   return KAD_OK;

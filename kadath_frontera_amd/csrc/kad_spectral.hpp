@@ -1,6 +1,7 @@
 // Launch interface of the batched 1-D / 3-D spectral kernels.
 #pragma once
 #include "kad_common.hpp"
+#include "kad_ew.hpp"
 
 namespace kad {
 
@@ -22,8 +23,11 @@ struct Passes {
   const int* line_mat[MAX_PASSES];
 };
 
+// post (may be NULL): an element-wise instruction applied in the store phase; out is
+// then not written (accumulate 0 and scale 1 only)
 int apply3d_launch(const double* in, int64_t in_stride, double* out, int64_t out_stride, int nbatch, int d0, int d1,
-                   int d2, const Passes& P, const double* pool, int accumulate, double scale, hipStream_t s);
+                   int d2, const Passes& P, const double* pool, int accumulate, double scale, hipStream_t s,
+                   const PostOp* post = nullptr);
 int apply_lines_launch(const double* in, double* out, int64_t L, int nin, int nout, const double* M, hipStream_t s);
 
 }  // namespace kad

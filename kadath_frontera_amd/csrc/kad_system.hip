@@ -34,18 +34,11 @@ struct DevRow {
   double gw;
 };
 
-__device__ __forceinline__ double ew_op(int kind, double x, double y, double s) {
-  switch (kind) {
-    case EW_ADD: return x + y;
-    case EW_SUB: return x - y;
-    case EW_MUL: return x * y;
-    case EW_DIV: return x / y;
-    case EW_SCAL: return x * s;
-    case EW_SDIV: return x / s;
-    case EW_ADDS: return x + s;
-    default: return x;
-  }
-}
+using kad::ew_op;
+static_assert(EW_ADD == kad::EWOP_ADD && EW_SUB == kad::EWOP_SUB && EW_MUL == kad::EWOP_MUL &&
+                  EW_DIV == kad::EWOP_DIV && EW_SCAL == kad::EWOP_SCAL && EW_ADDS == kad::EWOP_ADDS &&
+                  EW_COPY == kad::EWOP_COPY && EW_SDIV == kad::EWOP_SDIV,
+              "the tracer's element-wise kinds are the kernels'");
 
 // The same over 16-byte-aligned rows of even length: every thread takes EW_U pairs of
 // elements, their loads issued together (one element per thread left each wave
@@ -459,6 +452,7 @@ struct kad_system_impl {
   // dealt to the streams by estimated cost, each stream with its own tangent arena;
   // every program writes only its own columns of J, so the streams never touch the
   // same memory.  Stream 0 is the context's stream with tan_arena.
+  bool fuse_ew = true;  // apply marked element-wise instructions in the preceding pass (option 5)
   static constexpr int kFillMax = 4;
   int nfill = 3;  // 17x13x12 x 5 domains, batch 4096: 0.241 s on one stream, 0.207 / 0.201 / 0.210 on 2 / 3 / 4
   hipStream_t fstream[kFillMax] = {};
@@ -517,8 +511,39 @@ struct kad_system_impl {
   }
 };
 
+// Marks each spectral pass (I_LIN) whose result feeds only the element-wise instruction
+// right after it: the pass then applies that instruction in its store phase and the
+// intermediate array never goes to HBM (one launch and two array transfers fewer).
+// Conditions: tangent operands (the slots are the program's own), the result is the
+// instruction's a or b but not both, it is read nowhere else, and the instruction covers
+// the whole output array.  Called on the virtual slots, before allocate_slots.
+static void mark_fusions(std::vector<Instr>& code, int nvirt, const std::vector<DevPlanX>& plans) {
+  std::vector<int> reads(nvirt, 0);
+  for (const Instr& in : code) {
+    if (in.a.slot >= 0 && in.a.space == SP_TAN) reads[in.a.slot]++;
+    if (in.b.slot >= 0 && in.b.space == SP_TAN) reads[in.b.slot]++;
+  }
+  for (size_t t = 0; t + 1 < code.size(); ++t) {
+    Instr& l = code[t];
+    const Instr& e = code[t + 1];
+    l.fuse = 0;
+    if (l.kind != I_LIN || e.kind != I_EW || l.dst.space != SP_TAN || e.dst.space != SP_TAN || l.dst.slot < 0)
+      continue;
+    const bool in_a = e.a.space == SP_TAN && e.a.slot == l.dst.slot;
+    const bool in_b = e.b.slot >= 0 && e.b.space == SP_TAN && e.b.slot == l.dst.slot;
+    if (in_a == in_b || reads[l.dst.slot] != 1 || e.ew < 0 || e.ew > EW_SDIV) continue;
+    if (l.plan < 0 || l.plan >= (int)plans.size()) continue;
+    const kad::Passes& P = plans[l.plan].P;
+    int od[3] = {l.din[0], l.din[1], l.din[2]};
+    for (int p = 0; p < P.npass; ++p) od[P.axis[p]] = P.nout[p];
+    if (e.size != od[0] * od[1] * od[2]) continue;
+    l.fuse = in_a ? 1 : 2;
+  }
+}
+
 // Liveness-based reuse of tangent slots within one program.
-static int allocate_slots(std::vector<Instr>& code, int nvirt) {
+static int allocate_slots(std::vector<Instr>& code, int nvirt, const std::vector<DevPlanX>& plans) {
+  mark_fusions(code, nvirt, plans);
   std::vector<int> last(nvirt, -1);
   auto use = [&](const Opnd& o, int t) {
     if (o.slot >= 0 && o.space == SP_TAN) last[o.slot] = std::max(last[o.slot], t);
@@ -626,7 +651,7 @@ int kad_system_impl::setup() {
   for (size_t p = 0; p < sys->tprog.size(); ++p) {
     DevProgram dp;
     dp.code = sys->tprog[p].code;
-    dp.nphys = allocate_slots(dp.code, sys->tprog[p].nslots);
+    dp.nphys = allocate_slots(dp.code, sys->tprog[p].nslots, plans);
     dp.field = sys->layout[p][0];
     dp.dom = sys->layout[p][1];
     dp.off = sys->layout[p][2];
@@ -639,7 +664,7 @@ int kad_system_impl::setup() {
   for (size_t p = 0; p < sys->obs_tprog.size(); ++p) {
     DevProgram dp;
     dp.code = sys->obs_tprog[p].code;
-    dp.nphys = allocate_slots(dp.code, sys->obs_tprog[p].nslots);
+    dp.nphys = allocate_slots(dp.code, sys->obs_tprog[p].nslots, plans);
     dp.field = sys->obs_tprog_lay[p][0];
     dp.dom = sys->obs_tprog_lay[p][1];
     dp.off = sys->obs_tprog_lay[p][2];
@@ -651,7 +676,7 @@ int kad_system_impl::setup() {
   for (size_t p = 0; p < sys->par_prog.size(); ++p) {
     DevProgram dp;
     dp.code = sys->par_prog[p].code;
-    dp.nphys = allocate_slots(dp.code, sys->par_prog[p].nslots);
+    dp.nphys = allocate_slots(dp.code, sys->par_prog[p].nslots, plans);
     dp.seed_tab = sys->par_tab[p];
     dp.cnt = 1;
     nphys_max = std::max(nphys_max, dp.nphys);
@@ -660,7 +685,7 @@ int kad_system_impl::setup() {
   for (size_t p = 0; p < sys->glob_tprog.size(); ++p) {
     DevProgram dp;
     dp.code = sys->glob_tprog[p].code;
-    dp.nphys = allocate_slots(dp.code, sys->glob_tprog[p].nslots);
+    dp.nphys = allocate_slots(dp.code, sys->glob_tprog[p].nslots, plans);
     dp.field = sys->layout[p][0];
     dp.dom = sys->layout[p][1];
     dp.off = sys->layout[p][2];
@@ -672,7 +697,7 @@ int kad_system_impl::setup() {
   for (size_t p = 0; p < sys->glob_par_prog.size(); ++p) {
     DevProgram dp;
     dp.code = sys->glob_par_prog[p].code;
-    dp.nphys = allocate_slots(dp.code, sys->glob_par_prog[p].nslots);
+    dp.nphys = allocate_slots(dp.code, sys->glob_par_prog[p].nslots, plans);
     dp.seed_tab = sys->par_tab[p];
     dp.cnt = 1;
     nphys_max = std::max(nphys_max, dp.nphys);
@@ -689,14 +714,28 @@ int kad_system_impl::run(const std::vector<Instr>& code, int nb, int u0, int see
   hipStream_t s = st ? st : ctx->stream;
   if (!tan) tan = tan_arena;
   auto slot_ptr = [&](const Opnd& o) { return this->slot_ptr(o, tan); };
-  for (const Instr& in : code) {
+  for (size_t t = 0; t < code.size(); ++t) {
+    const Instr& in = code[t];
     const Domain& dm = sys->doms[in.dom];
     switch (in.kind) {
       case I_LIN: {
         const DevPlanX& p = plans[in.plan];
         const int batch = (in.dst.space == SP_VAL) ? 1 : nb;
+        kad::PostOp post;
+        if (in.fuse && fuse_ew && t + 1 < code.size()) {
+          // the next instruction, applied in this pass's store phase (mark_fusions)
+          const Instr& e = code[++t];
+          const Opnd& other = in.fuse == 1 ? e.b : e.a;
+          post.kind = e.ew;
+          post.lin_b = in.fuse == 2;
+          post.o = other.slot >= 0 ? slot_ptr(other) : nullptr;
+          post.os = other.slot >= 0 ? slot_stride(other) : 0;
+          post.s = e.s;
+          post.dst = slot_ptr(e.dst);
+          post.ds = slot_stride(e.dst);
+        }
         int rc = kad::apply3d_launch(slot_ptr(in.a), stride, slot_ptr(in.dst), stride, batch, in.din[0], in.din[1],
-                                     in.din[2], p.P, p.pool, 0, 1.0, s);
+                                     in.din[2], p.P, p.pool, 0, 1.0, s, &post);
         if (rc) return rc;
         break;
       }
@@ -2359,6 +2398,11 @@ int kad_system_set_option(kad_system* s, int which, double value) {
     // step factors its own J with partial pivoting
     S->lo_valid = S->fac_valid = S->dd_valid = S->dd_orders = false;
     S->prev_inner = 0;
+  } else if (which == 5) {
+    // element-wise instructions applied in the spectral pass that feeds them (1, the
+    // default) or launched on their own (0)
+    if (value != 0.0 && value != 1.0) return KAD_ERR_ARG;
+    S->fuse_ew = value != 0.0;
   } else if (which == 4) {
     // static pivoting in solvers 6-7 (1, the default) or partial pivoting at every
     // factorisation (0)

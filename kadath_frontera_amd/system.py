@@ -278,16 +278,19 @@ class DeviceSystem:
         return out
 
     def set_option(self, forcing=None, reuse_inner=None, drop_factors=False, fill_streams=None,
-                   static_pivoting=None):
+                   static_pivoting=None, fuse=None):
         """Iterative solver options: forcing factor of the refinement target, the
         inner-step threshold for reusing earlier factors, and dropping the factors kept
         for reuse (the next step factors its own Jacobian, with partial pivoting); the
         number of streams (1-4) of the Jacobian fill; static pivoting in the
-        domain-decomposed solvers (True, the default, or False)."""
+        domain-decomposed solvers (True, the default, or False); element-wise
+        instructions applied in the spectral pass that feeds them (True, the default)."""
         if fill_streams is not None:
             check(lib.kad_system_set_option(self.h, 3, float(fill_streams)), "kad_system_set_option")
         if static_pivoting is not None:
             check(lib.kad_system_set_option(self.h, 4, 1.0 if static_pivoting else 0.0), "kad_system_set_option")
+        if fuse is not None:
+            check(lib.kad_system_set_option(self.h, 5, 1.0 if fuse else 0.0), "kad_system_set_option")
         if drop_factors:
             check(lib.kad_system_set_option(self.h, 2, 0.0), "kad_system_set_option")
         if forcing is not None:

@@ -21,7 +21,8 @@ def decaying(rng, shape):
 
 @pytest.mark.parametrize("base,n", [(S.CHEB, 9), (S.CHEB, 13), (S.CHEB, 17), (S.CHEB, 25), (S.CHEB, 33),
                                     (S.COS_EVEN, 13), (S.SIN_ODD, 13), (S.COS_ODD, 9), (S.SIN_EVEN, 17),
-                                    (S.COSSIN, 4), (S.COSSIN, 12), (S.COSSIN, 32)])
+                                    (S.COSSIN, 4), (S.COSSIN, 8), (S.COSSIN, 12), (S.COSSIN, 16),
+                                    (S.COSSIN, 24), (S.COSSIN, 32)])
 def test_lines(ctx, base, n):
     print(f"lines base={base} n={n}", flush=True)
     rng = np.random.default_rng(base * 100 + n)

@@ -79,6 +79,20 @@ def test_jacobian_fill_streams_give_the_same_J(oracle_sys, gpu_sys):
         gpu_sys.set_option(fill_streams=3)
 
 
+def test_fused_element_wise_instructions_give_the_same_J(oracle_sys, gpu_sys):
+    # the element-wise instructions applied in the spectral pass that feeds them, or
+    # launched on their own: the same J bit for bit
+    _, X0 = oracle_sys
+    gpu_sys.set_guess(X0)
+    try:
+        gpu_sys.set_option(fuse=False)
+        J0 = gpu_sys.jacobian().clone()
+        gpu_sys.set_option(fuse=True)
+        assert torch.equal(gpu_sys.jacobian(), J0)
+    finally:
+        gpu_sys.set_option(fuse=True)
+
+
 @pytest.mark.parametrize("solver", ["f64", "f32-gmres"])
 def test_newton_converges_to_oracle_solution(oracle_sys, gpu_sys, solver):
     sy, X0 = oracle_sys
