@@ -70,8 +70,9 @@ struct Instr {
   int seed_tab = -1;
   // EXPORT_INT: theta basis of phi coefficient 0 (row0 = quantity index, s = factor)
   int aux = 0;
-  // LIN: 1 or 2 when the next instruction is element-wise with this result as its a or b
-  // and the only use of it (set on the device side; the pass then applies it)
+  // LIN or EW: the number of following element-wise instructions this one absorbs (each
+  // the only reader of the previous result; set on the device side, which then applies
+  // them as the result is stored)
   int fuse = 0;
   // EXPORT_GL: constant added to the row (value programs); a.slot < 0: the constant alone;
   // pt = 1: the point value a[0] of a constant field instead of an integral

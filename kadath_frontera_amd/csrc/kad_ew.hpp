@@ -22,18 +22,33 @@ __device__ __forceinline__ double ew_op(int kind, double x, double y, double s) 
   }
 }
 
-// An element-wise instruction applied in the store phase of the spectral pass that
-// produces one of its operands: out[b][e] = op(v, o[b][e]) (lin_b: op(o[b][e], v)),
-// v the pass's result, o the other operand (NULL: none; stride 0: one array for the
-// whole batch).
-struct PostOp {
-  int kind = -1;  // -1: none
-  int lin_b = 0;
+// A chain of element-wise instructions applied to a value v as it is stored, each to
+// the previous one's result: step k makes v = op(v, o[b][e]) (side 0) or op(o[b][e], v)
+// (side 1), o its other operand (NULL: none; stride 0: one array for the whole batch).
+// The spectral pass applies one to its result, and the chain kernel to the result of
+// its first instruction; the intermediate arrays are never stored.
+constexpr int kMaxChain = 3;
+struct EwStep {
+  int kind = 0;
+  int side = 0;
   const double* o = nullptr;
   int64_t os = 0;
   double s = 0.0;
+};
+struct PostOp {
+  int n = 0;  // steps (0: none)
+  EwStep op[kMaxChain];
   double* dst = nullptr;
   int64_t ds = 0;
 };
+
+__device__ __forceinline__ double post_apply(const PostOp& p, int64_t b, int e, double v) {
+  for (int k = 0; k < p.n; ++k) {
+    const EwStep& st = p.op[k];
+    const double o = st.o ? st.o[b * st.os + e] : 0.0;
+    v = st.side ? ew_op(st.kind, o, v, st.s) : ew_op(st.kind, v, o, st.s);
+  }
+  return v;
+}
 
 }  // namespace kad

@@ -227,14 +227,10 @@ __global__ __launch_bounds__(256) void apply3d_kernel(const double* __restrict__
         if (e < nfin) dst[e] = v[u] + scale * fin[e];
       }
     }
-  } else if (post.kind >= 0) {
-    // the fused element-wise instruction (scale is 1 here)
-    const double* orow = post.o ? post.o + b * post.os : nullptr;
+  } else if (post.n > 0) {
+    // the element-wise instructions fed by this pass (scale is 1 here)
     double* drow = post.dst + b * post.ds;
-    for (int e = threadIdx.x; e < nfin; e += blockDim.x) {
-      const double v = fin[e], o = orow ? orow[e] : 0.0;
-      drow[e] = post.lin_b ? ew_op(post.kind, o, v, post.s) : ew_op(post.kind, v, o, post.s);
-    }
+    for (int e = threadIdx.x; e < nfin; e += blockDim.x) drow[e] = post_apply(post, b, e, fin[e]);
   } else {
     for (int e = threadIdx.x; e < nfin; e += blockDim.x) dst[e] = scale * fin[e];
   }
@@ -269,7 +265,7 @@ int apply3d_launch(const double* in, int64_t in_stride, double* out, int64_t out
                    int d2, const Passes& P, const double* pool, int accumulate, double scale, hipStream_t s,
                    const PostOp* post) {
   if (nbatch <= 0) return KAD_OK;
-  if (post && post->kind >= 0 && (accumulate || scale != 1.0 || !post->dst)) return KAD_ERR_ARG;
+  if (post && post->n > 0 && (post->n > kMaxChain || accumulate || scale != 1.0 || !post->dst)) return KAD_ERR_ARG;
   const PostOp pop = post ? *post : PostOp();
   size_t nl = 0;
   for (int p = 0; p < P.npass; ++p) nl += (size_t)P.nlines[p];
@@ -298,8 +294,10 @@ int apply3d_launch(const double* in, int64_t in_stride, double* out, int64_t out
     int od[3] = {d0, d1, d2};
     for (int p = 0; p < P.npass; ++p) od[P.axis[p]] = P.nout[p];
     const double nin = (double)d0 * d1 * d2, nout = (double)od[0] * od[1] * od[2];
-    // (a fused instruction's other operand adds its array per batch element)
-    const double extra = (pop.kind >= 0 && pop.o && pop.os != 0) ? nout : 0.0;
+    // (each fused instruction's other operand adds its array per batch element)
+    double extra = 0.0;
+    for (int k = 0; k < pop.n; ++k)
+      if (pop.op[k].o && pop.op[k].os != 0) extra += nout;
     const int rc = tm->begin(s, 8.0 * nbatch * (nin + nout * (accumulate ? 2 : 1) + extra));
     if (rc < 0) return rc;
     timed = rc == 0;

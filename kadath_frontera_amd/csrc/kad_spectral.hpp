@@ -23,8 +23,8 @@ struct Passes {
   const int* line_mat[MAX_PASSES];
 };
 
-// post (may be NULL): an element-wise instruction applied in the store phase; out is
-// then not written (accumulate 0 and scale 1 only)
+// post (may be NULL): element-wise instructions applied in the store phase; out is then
+// not written (accumulate 0 and scale 1 only)
 int apply3d_launch(const double* in, int64_t in_stride, double* out, int64_t out_stride, int nbatch, int d0, int d1,
                    int d2, const Passes& P, const double* pool, int accumulate, double scale, hipStream_t s,
                    const PostOp* post = nullptr);

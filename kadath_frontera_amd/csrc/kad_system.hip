@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <stdexcept>
 #include <cstring>
+#include <functional>
 
 namespace kad {
 template <typename T>
@@ -64,6 +65,30 @@ __global__ __launch_bounds__(256) void ew2_kernel(int kind, double* __restrict__
     for (int u = 0; u < EW_U; ++u) {
       const int e = e0 + u * 256;
       if (e < size2) dr[e] = double2{ew_op(kind, x[u].x, y[u].x, s), ew_op(kind, x[u].y, y[u].y, s)};
+    }
+  }
+}
+
+// An element-wise instruction followed by the ones it absorbed (kad::PostOp): the
+// first instruction's result goes through the chain and only the last one is stored.
+__global__ __launch_bounds__(256) void ewc_kernel(int kind, const double* __restrict__ a, int64_t astride,
+                                                  const double* __restrict__ bsrc, int64_t bstride, double s,
+                                                  kad::PostOp post, int size, int nbatch) {
+  for (int bb = blockIdx.y; bb < nbatch; bb += gridDim.y) {
+    const double* ar = a + (int64_t)bb * astride;
+    const double* br = bsrc ? bsrc + (int64_t)bb * bstride : nullptr;
+    double* dr = post.dst + (int64_t)bb * post.ds;
+    const int e0 = blockIdx.x * 256 * EW_U + threadIdx.x;
+    double v[EW_U];
+#pragma unroll
+    for (int u = 0; u < EW_U; ++u) {
+      const int e = e0 + u * 256;
+      v[u] = e < size ? ew_op(kind, ar[e], br ? br[e] : 0.0, s) : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < EW_U; ++u) {
+      const int e = e0 + u * 256;
+      if (e < size) dr[e] = kad::post_apply(post, bb, e, v[u]);
     }
   }
 }
@@ -511,34 +536,70 @@ struct kad_system_impl {
   }
 };
 
-// Marks each spectral pass (I_LIN) whose result feeds only the element-wise instruction
-// right after it: the pass then applies that instruction in its store phase and the
-// intermediate array never goes to HBM (one launch and two array transfers fewer).
-// Conditions: tangent operands (the slots are the program's own), the result is the
-// instruction's a or b but not both, it is read nowhere else, and the instruction covers
-// the whole output array.  Called on the virtual slots, before allocate_slots.
+// Chains of element-wise instructions that run inside the instruction before them: an
+// instruction (a spectral pass or an element-wise one) whose result feeds only the
+// element-wise instruction right after it absorbs that instruction, which then applies
+// to the value as it is stored, and so on up to kad::kMaxChain of them.  The
+// intermediate arrays never go to HBM (one launch and two array transfers fewer per
+// absorbed instruction).  Conditions, on the virtual slots (before allocate_slots):
+// tangent destinations (the slots are the program's own), the result is the next
+// instruction's a or b but not both, it is read nowhere else, and the next
+// instruction covers the whole array.  code[t].fuse = the number absorbed.
 static void mark_fusions(std::vector<Instr>& code, int nvirt, const std::vector<DevPlanX>& plans) {
   std::vector<int> reads(nvirt, 0);
-  for (const Instr& in : code) {
+  for (Instr& in : code) {
+    in.fuse = 0;
     if (in.a.slot >= 0 && in.a.space == SP_TAN) reads[in.a.slot]++;
     if (in.b.slot >= 0 && in.b.space == SP_TAN) reads[in.b.slot]++;
   }
-  for (size_t t = 0; t + 1 < code.size(); ++t) {
-    Instr& l = code[t];
-    const Instr& e = code[t + 1];
-    l.fuse = 0;
-    if (l.kind != I_LIN || e.kind != I_EW || l.dst.space != SP_TAN || e.dst.space != SP_TAN || l.dst.slot < 0)
-      continue;
-    const bool in_a = e.a.space == SP_TAN && e.a.slot == l.dst.slot;
-    const bool in_b = e.b.slot >= 0 && e.b.space == SP_TAN && e.b.slot == l.dst.slot;
-    if (in_a == in_b || reads[l.dst.slot] != 1 || e.ew < 0 || e.ew > EW_SDIV) continue;
-    if (l.plan < 0 || l.plan >= (int)plans.size()) continue;
-    const kad::Passes& P = plans[l.plan].P;
-    int od[3] = {l.din[0], l.din[1], l.din[2]};
-    for (int p = 0; p < P.npass; ++p) od[P.axis[p]] = P.nout[p];
-    if (e.size != od[0] * od[1] * od[2]) continue;
-    l.fuse = in_a ? 1 : 2;
+  for (size_t t = 0; t < code.size(); ++t) {
+    Instr& h = code[t];
+    if ((h.kind != I_LIN && h.kind != I_EW) || h.dst.space != SP_TAN || h.dst.slot < 0) continue;
+    int size = h.size;
+    if (h.kind == I_LIN) {
+      if (h.plan < 0 || h.plan >= (int)plans.size()) continue;
+      const kad::Passes& P = plans[h.plan].P;
+      int od[3] = {h.din[0], h.din[1], h.din[2]};
+      for (int p = 0; p < P.npass; ++p) od[P.axis[p]] = P.nout[p];
+      size = od[0] * od[1] * od[2];
+    }
+    int m = 0;
+    while (m < kad::kMaxChain && t + m + 1 < code.size()) {
+      const Instr& prev = code[t + m];
+      const Instr& e = code[t + m + 1];
+      if (e.kind != I_EW || e.dst.space != SP_TAN || e.size != size || e.ew < 0 || e.ew > EW_SDIV) break;
+      const bool in_a = e.a.space == SP_TAN && e.a.slot == prev.dst.slot;
+      const bool in_b = e.b.slot >= 0 && e.b.space == SP_TAN && e.b.slot == prev.dst.slot;
+      if (in_a == in_b || reads[prev.dst.slot] != 1) break;
+      ++m;
+    }
+    h.fuse = m;
+    t += m;  // the absorbed instructions run inside this one
   }
+}
+
+// The absorbed instructions code[t+1 .. t+fuse] as steps on the value of code[t].
+static kad::PostOp chain_of(const std::vector<Instr>& code, size_t t, const std::function<double*(const Opnd&)>& ptr,
+                            const std::function<int64_t(const Opnd&)>& stride) {
+  kad::PostOp post;
+  const int m = code[t].fuse;
+  for (int k = 0; k < m; ++k) {
+    const Instr& prev = code[t + k];
+    const Instr& e = code[t + k + 1];
+    // the chain value is prev's result; the other operand is the instruction's other one
+    const int side = (e.a.space == SP_TAN && e.a.slot == prev.dst.slot) ? 0 : 1;
+    const Opnd& other = side == 0 ? e.b : e.a;
+    kad::EwStep& st = post.op[k];
+    st.kind = e.ew;
+    st.side = side;
+    st.o = other.slot >= 0 ? ptr(other) : nullptr;
+    st.os = other.slot >= 0 ? stride(other) : 0;
+    st.s = e.s;
+  }
+  post.n = m;
+  post.dst = ptr(code[t + m].dst);
+  post.ds = stride(code[t + m].dst);
+  return post;
 }
 
 // Liveness-based reuse of tangent slots within one program.
@@ -722,17 +783,10 @@ int kad_system_impl::run(const std::vector<Instr>& code, int nb, int u0, int see
         const DevPlanX& p = plans[in.plan];
         const int batch = (in.dst.space == SP_VAL) ? 1 : nb;
         kad::PostOp post;
-        if (in.fuse && fuse_ew && t + 1 < code.size()) {
-          // the next instruction, applied in this pass's store phase (mark_fusions)
-          const Instr& e = code[++t];
-          const Opnd& other = in.fuse == 1 ? e.b : e.a;
-          post.kind = e.ew;
-          post.lin_b = in.fuse == 2;
-          post.o = other.slot >= 0 ? slot_ptr(other) : nullptr;
-          post.os = other.slot >= 0 ? slot_stride(other) : 0;
-          post.s = e.s;
-          post.dst = slot_ptr(e.dst);
-          post.ds = slot_stride(e.dst);
+        if (in.fuse > 0 && fuse_ew) {
+          // the following element-wise instructions, applied in this pass's store phase
+          post = chain_of(code, t, slot_ptr, [&](const Opnd& o) { return slot_stride(o); });
+          t += in.fuse;
         }
         int rc = kad::apply3d_launch(slot_ptr(in.a), stride, slot_ptr(in.dst), stride, batch, in.din[0], in.din[1],
                                      in.din[2], p.P, p.pool, 0, 1.0, s, &post);
@@ -745,6 +799,14 @@ int kad_system_impl::run(const std::vector<Instr>& code, int nb, int u0, int see
         const double* ap = slot_ptr(in.a);
         const double* bp = in.b.slot >= 0 ? slot_ptr(in.b) : nullptr;
         const int64_t ds = slot_stride(in.dst), as = slot_stride(in.a), bs = in.b.slot >= 0 ? slot_stride(in.b) : 0;
+        if (in.fuse > 0 && fuse_ew) {
+          // this instruction and the ones it absorbed, in one pass over the arrays
+          const kad::PostOp post = chain_of(code, t, slot_ptr, [&](const Opnd& o) { return slot_stride(o); });
+          t += in.fuse;
+          const dim3 grid((unsigned)kad::cdiv(in.size, 256 * EW_U), (unsigned)std::min(batch, 65535));
+          ewc_kernel<<<grid, 256, 0, s>>>(in.ew, ap, as, bp, bs, in.s, post, in.size, batch);
+          break;
+        }
         auto a16 = [](const void* p) { return ((uintptr_t)p & 15) == 0; };
         if (in.size % 2 == 0 && ds % 2 == 0 && as % 2 == 0 && bs % 2 == 0 && a16(dp) && a16(ap) &&
             (!bp || a16(bp))) {
