@@ -545,7 +545,43 @@ struct kad_system_impl {
 // tangent destinations (the slots are the program's own), the result is the next
 // instruction's a or b but not both, it is read nowhere else, and the next
 // instruction covers the whole array.  code[t].fuse = the number absorbed.
+// A pass read by a single element-wise instruction a few places later moves down to just
+// before it (the slots are written once, so the instructions in between read neither
+// its result nor anything it overwrites), unless the instruction before that reader
+// already feeds it.
+static void pull_passes_to_readers(std::vector<Instr>& code, int nvirt) {
+  constexpr int kReach = 3;
+  auto reader_of = [&](int slot, size_t from) -> long {
+    long at = -1;
+    for (size_t u = from; u < code.size(); ++u) {
+      const Instr& in = code[u];
+      const bool ra = in.a.space == SP_TAN && in.a.slot == slot, rb = in.b.slot >= 0 && in.b.space == SP_TAN && in.b.slot == slot;
+      if (ra || rb) {
+        if (at >= 0) return -1;  // read twice
+        at = (long)u;
+      }
+    }
+    return at;
+  };
+  (void)nvirt;
+  for (size_t t = 0; t + 2 < code.size(); ++t) {
+    const Instr& l = code[t];
+    if (l.kind != I_LIN || l.dst.space != SP_TAN || l.dst.slot < 0) continue;
+    const long u = reader_of(l.dst.slot, t + 1);
+    if (u < (long)t + 2 || u > (long)t + 1 + kReach || code[u].kind != I_EW) continue;
+    const Instr& e = code[u];
+    const Instr& pv = code[u - 1];
+    const bool fed_by_prev = pv.dst.space == SP_TAN && pv.dst.slot >= 0 &&
+                             ((e.a.space == SP_TAN && e.a.slot == pv.dst.slot) ||
+                              (e.b.slot >= 0 && e.b.space == SP_TAN && e.b.slot == pv.dst.slot));
+    if (fed_by_prev) continue;
+    std::rotate(code.begin() + t, code.begin() + t + 1, code.begin() + u);  // the pass now at u - 1
+    --t;  // look again at what moved into position t
+  }
+}
+
 static void mark_fusions(std::vector<Instr>& code, int nvirt, const std::vector<DevPlanX>& plans) {
+  pull_passes_to_readers(code, nvirt);
   std::vector<int> reads(nvirt, 0);
   for (Instr& in : code) {
     in.fuse = 0;
