@@ -519,6 +519,14 @@ struct kad_system_impl {
   // every tangent program's columns of J (fp64, ld ldj), after the value pass and the
   // double unknowns' columns, on nfill streams; the context's stream waits for all
   int fill_columns(double* Jd, int64_t ldj);
+  // the side streams (and, with arenas, their tangent arenas) and events for ns streams
+  int ensure_streams(int ns, bool arenas);
+  // the value program split by domain over ns streams (value_pass); false: not splittable
+  bool split_values(int ns);
+  int vsplit_ns = 0;
+  bool vsplit_ok = false;
+  std::vector<std::vector<Instr>> vparts;
+  std::vector<Instr> vexports;
   bool vals_current = false;  // the value arena holds the value pass of the current fields
   int ensure_values() { return vals_current ? KAD_OK : value_pass(); }
   int value_pass();
@@ -924,8 +932,74 @@ int kad_system_impl::run(const std::vector<Instr>& code, int nb, int u0, int see
   return KAD_OK;
 }
 
+int kad_system_impl::ensure_streams(int ns, bool arenas) {
+  for (int i = 1; i < ns; ++i) {
+    if (!fstream[i]) KAD_HIP_CHECK(hipStreamCreateWithFlags(&fstream[i], hipStreamNonBlocking));
+    if (arenas && !ftan[i]) {
+      const int rc = dmalloc((void**)&ftan[i], (size_t)std::max(1, nphys_max) * B * stride * sizeof(double));
+      if (rc) return rc;
+    }
+  }
+  for (int i = 0; i < ns && ns > 1; ++i)
+    if (!fev[i]) KAD_HIP_CHECK(hipEventCreateWithFlags(&fev[i], hipEventDisableTiming));
+  return KAD_OK;
+}
+
+// The value program split over ns streams: every instruction but the row exports goes to
+// the stream of its domain, in program order; the exports follow on the context's
+// stream once all have finished.  Valid when no instruction reads a value slot that
+// an instruction of another stream writes (the domains' arrays are their own) and the
+// exports feed nothing but their rows.
+bool kad_system_impl::split_values(int ns) {
+  if (vsplit_ns == ns) return vsplit_ok;
+  vsplit_ns = ns;
+  vsplit_ok = false;
+  vparts.assign(ns, {});
+  vexports.clear();
+  std::vector<int> writer(std::max(1, sys->tr.nval_slots), -1);
+  auto is_export = [](const Instr& in) {
+    return in.kind == I_EXPORT_IN || in.kind == I_EXPORT_BD || in.kind == I_EXPORT_INT || in.kind == I_EXPORT_GL;
+  };
+  for (const Instr& in : sys->tr.val.code) {
+    if (is_export(in)) {
+      vexports.push_back(in);
+      continue;
+    }
+    const int k = ((in.dom - sys->dmin) % ns + ns) % ns;
+    for (const Opnd* o : {&in.a, &in.b})
+      if (o->slot >= 0 && o->space == SP_VAL && o->slot < (int)writer.size() && writer[o->slot] >= 0 &&
+          writer[o->slot] != k)
+        return false;
+    if (in.dst.slot >= 0) {
+      if (in.dst.space != SP_VAL || in.dst.slot >= (int)writer.size()) return false;
+      writer[in.dst.slot] = k;
+    }
+    vparts[k].push_back(in);
+  }
+  int used = 0;
+  for (auto& v : vparts) used += !v.empty();
+  vsplit_ok = used > 1;
+  return vsplit_ok;
+}
+
 int kad_system_impl::value_pass() {
-  int rc = run(sys->tr.val.code, 1, 0, -1, F, 0, 0);
+  int rc;
+  const int ns = std::max(1, std::min(nfill, kFillMax));
+  if (ns > 1 && split_values(ns)) {
+    hipStream_t s0 = ctx->stream;
+    if ((rc = ensure_streams(ns, false))) return rc;
+    KAD_HIP_CHECK(hipEventRecord(fev[0], s0));
+    for (int i = 1; i < ns; ++i) KAD_HIP_CHECK(hipStreamWaitEvent(fstream[i], fev[0], 0));
+    for (int i = 0; i < ns && rc == KAD_OK; ++i)
+      rc = run(vparts[i], 1, 0, -1, F, 0, 0, 0, nullptr, i ? fstream[i] : s0, tan_arena);
+    for (int i = 1; i < ns; ++i) {
+      KAD_HIP_CHECK(hipEventRecord(fev[i], fstream[i]));
+      KAD_HIP_CHECK(hipStreamWaitEvent(s0, fev[i], 0));
+    }
+    if (rc == KAD_OK) rc = run(vexports, 1, 0, -1, F, 0, 0);
+  } else {
+    rc = run(sys->tr.val.code, 1, 0, -1, F, 0, 0);
+  }
   vals_current = (rc == KAD_OK);
   return rc;
 }
@@ -941,15 +1015,10 @@ int kad_system_impl::jacobian(double* Jd, int64_t ldj) {
 int kad_system_impl::fill_columns(double* Jd, int64_t ldj) {
   hipStream_t s0 = ctx->stream;
   const int ns = std::max(1, std::min(nfill, (int)tprogs.size()));
-  for (int i = 1; i < ns; ++i) {
-    if (!fstream[i]) KAD_HIP_CHECK(hipStreamCreateWithFlags(&fstream[i], hipStreamNonBlocking));
-    if (!ftan[i]) {
-      const int rc = dmalloc((void**)&ftan[i], (size_t)std::max(1, nphys_max) * B * stride * sizeof(double));
-      if (rc) return rc;
-    }
+  {
+    const int rc = ensure_streams(ns, true);
+    if (rc) return rc;
   }
-  for (int i = 0; i < ns && ns > 1; ++i)
-    if (!fev[i]) KAD_HIP_CHECK(hipEventCreateWithFlags(&fev[i], hipEventDisableTiming));
   if (ns > 1) {
     KAD_HIP_CHECK(hipEventRecord(fev[0], s0));  // J cleared, value pass done
     for (int i = 1; i < ns; ++i) KAD_HIP_CHECK(hipStreamWaitEvent(fstream[i], fev[0], 0));

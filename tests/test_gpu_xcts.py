@@ -64,14 +64,17 @@ def test_jacobian_matches_oracle(oracle_sys, gpu_sys):
 
 def test_jacobian_fill_streams_give_the_same_J(oracle_sys, gpu_sys):
     # the tangent programs dealt to 1-4 streams (batch 128: several column tiles per
-    # program, run one after another on their stream's arena) fill J bit for bit alike
+    # program, run one after another on their stream's arena) fill J bit for bit alike,
+    # and the value program split by domain over the same streams gives the same F
     _, X0 = oracle_sys
     gpu_sys.set_guess(X0)
     try:
         gpu_sys.set_option(fill_streams=1)
+        F1 = gpu_sys.residual().clone()
         J1 = gpu_sys.jacobian().clone()
         for k in (2, 3, 4):
             gpu_sys.set_option(fill_streams=k)
+            assert torch.equal(gpu_sys.residual(), F1), k
             assert torch.equal(gpu_sys.jacobian(), J1), k
         with pytest.raises(RuntimeError):
             gpu_sys.set_option(fill_streams=5)
