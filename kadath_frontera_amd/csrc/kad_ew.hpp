@@ -42,13 +42,25 @@ struct PostOp {
   int64_t ds = 0;
 };
 
-__device__ __forceinline__ double post_apply(const PostOp& p, int64_t b, int e, double v) {
-  for (int k = 0; k < p.n; ++k) {
-    const EwStep& st = p.op[k];
-    const double o = st.o ? st.o[b * st.os + e] : 0.0;
-    v = st.side ? ew_op(st.kind, o, v, st.s) : ew_op(st.kind, v, o, st.s);
-  }
+// The chain's other operands at element e of batch row b, all loads issued before any
+// is used (one memory latency for the whole chain), then the steps.
+__device__ __forceinline__ void post_load(const PostOp& p, int64_t b, int e, double (&o)[kMaxChain]) {
+#pragma unroll
+  for (int k = 0; k < kMaxChain; ++k) o[k] = (k < p.n && p.op[k].o) ? p.op[k].o[b * p.op[k].os + e] : 0.0;
+}
+__device__ __forceinline__ double post_ops(const PostOp& p, double v, const double (&o)[kMaxChain]) {
+#pragma unroll
+  for (int k = 0; k < kMaxChain; ++k)
+    if (k < p.n) {
+      const EwStep& st = p.op[k];
+      v = st.side ? ew_op(st.kind, o[k], v, st.s) : ew_op(st.kind, v, o[k], st.s);
+    }
   return v;
+}
+__device__ __forceinline__ double post_apply(const PostOp& p, int64_t b, int e, double v) {
+  double o[kMaxChain];
+  post_load(p, b, e, o);
+  return post_ops(p, v, o);
 }
 
 }  // namespace kad

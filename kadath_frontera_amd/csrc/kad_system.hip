@@ -79,16 +79,20 @@ __global__ __launch_bounds__(256) void ewc_kernel(int kind, const double* __rest
     const double* br = bsrc ? bsrc + (int64_t)bb * bstride : nullptr;
     double* dr = post.dst + (int64_t)bb * post.ds;
     const int e0 = blockIdx.x * 256 * EW_U + threadIdx.x;
-    double v[EW_U];
+    // every load of the EW_U elements first (operands of the head and of every step)
+    double x[EW_U], y[EW_U], o[EW_U][kad::kMaxChain];
 #pragma unroll
     for (int u = 0; u < EW_U; ++u) {
       const int e = e0 + u * 256;
-      v[u] = e < size ? ew_op(kind, ar[e], br ? br[e] : 0.0, s) : 0.0;
+      const bool in = e < size;
+      x[u] = in ? ar[e] : 0.0;
+      y[u] = (in && br) ? br[e] : 0.0;
+      kad::post_load(post, bb, in ? e : 0, o[u]);
     }
 #pragma unroll
     for (int u = 0; u < EW_U; ++u) {
       const int e = e0 + u * 256;
-      if (e < size) dr[e] = kad::post_apply(post, bb, e, v[u]);
+      if (e < size) dr[e] = kad::post_ops(post, ew_op(kind, x[u], y[u], s), o[u]);
     }
   }
 }
