@@ -228,9 +228,22 @@ __global__ __launch_bounds__(256) void apply3d_kernel(const double* __restrict__
       }
     }
   } else if (post.n > 0) {
-    // the element-wise instructions fed by this pass (scale is 1 here)
+    // the element-wise instructions fed by this pass (scale is 1 here); the other
+    // operands of UL elements per thread are loaded together, as the inputs were
     double* drow = post.dst + b * post.ds;
-    for (int e = threadIdx.x; e < nfin; e += blockDim.x) drow[e] = post_apply(post, b, e, fin[e]);
+    for (int e0 = 0; e0 < nfin; e0 += UL * 256) {
+      double o[UL][kMaxChain];
+#pragma unroll
+      for (int u = 0; u < UL; ++u) {
+        const int e = e0 + u * 256 + (int)threadIdx.x;
+        post_load(post, b, e < nfin ? e : 0, o[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < UL; ++u) {
+        const int e = e0 + u * 256 + (int)threadIdx.x;
+        if (e < nfin) drow[e] = post_ops(post, fin[e], o[u]);
+      }
+    }
   } else {
     for (int e = threadIdx.x; e < nfin; e += blockDim.x) dst[e] = scale * fin[e];
   }
