@@ -25,6 +25,9 @@ struct TileSel {
   const int* rlist = nullptr;
   const int* clist = nullptr;
   const int* cnt = nullptr;
+  // batch only (device, nbat entries): matrix z computes only the tiles holding its
+  // first zcols[z] columns of C; the others are left as they are (OP_SUB only)
+  const int* zcols = nullptr;
   int nbat = 1;
   int64_t sa = 0, sb = 0, sc = 0;
 };

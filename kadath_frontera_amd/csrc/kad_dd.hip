@@ -42,7 +42,7 @@ int gemm_launch(int prec, int op, int m, int n, int k, const void* A, int64_t ld
 int gemm_launch_batched(int prec, int op, int m, int n, int k, const void* A, int64_t lda, int64_t sa, const void* B,
                         int64_t ldb, int64_t sb, void* C, int64_t ldc, int64_t sc, int nbat, hipStream_t s);
 int lu_solve_permuted_batched(const double* LU, int64_t lda, int n, int nbat, int64_t sA, double* B, int64_t ldb,
-                              int64_t sB, int nrhs, LuWork& w);
+                              int64_t sB, int nrhs, LuWork& w, const int* zcols);
 int ir_loop(Ctx* c, int n, const double* A64, int64_t lda, const std::function<int(double*)>& precond,
             const double* b, double* x, double tol, int max_outer, int mode, int trans, int* iters, double* resid);
 size_t trsv_table_doubles(int n);
@@ -195,7 +195,7 @@ struct kad_dd_s {
   size_t cap_r = 0, cap_w = 0;  // capacities: coupling rows, W entries
   int* crow = nullptr;          // r rows of J, ordered by (block, local row)
   int* cz = nullptr;            // block of each coupling row
-  int* dq0 = nullptr;           // first coupling index of each block
+  int* dq0 = nullptr;           // first coupling index of each block, then kb (2 nblk entries)
   int64_t* upos = nullptr;      // W offset of each unit right-hand side
   double* W = nullptr;
   double* Jc = nullptr;
@@ -262,7 +262,7 @@ int ensure_coupling(kad_dd* dd, int r, int kmax) {
   int rc;
   if ((rc = dd->dmalloc((void**)&dd->crow, R * sizeof(int)))) return rc;
   if ((rc = dd->dmalloc((void**)&dd->cz, R * sizeof(int)))) return rc;
-  if ((rc = dd->dmalloc((void**)&dd->dq0, (size_t)dd->nblk * sizeof(int)))) return rc;
+  if ((rc = dd->dmalloc((void**)&dd->dq0, 2 * (size_t)dd->nblk * sizeof(int)))) return rc;
   if ((rc = dd->dmalloc((void**)&dd->upos, R * sizeof(int64_t)))) return rc;
   if ((rc = dd->dmalloc((void**)&dd->W, std::max<size_t>(w, 1) * sizeof(double)))) return rc;
   if ((rc = dd->dmalloc((void**)&dd->Jc, R * ncol * sizeof(double)))) return rc;
@@ -498,13 +498,15 @@ int dd_factor_impl(kad_dd* dd, const double* J, int64_t ldj, bool stat, double m
     KAD_HIP_CHECK(hipMemcpyAsync(dd->crow, hcrow.data(), r * sizeof(int), hipMemcpyHostToDevice, s));
     KAD_HIP_CHECK(hipMemcpyAsync(dd->cz, hcz.data(), r * sizeof(int), hipMemcpyHostToDevice, s));
     KAD_HIP_CHECK(hipMemcpyAsync(dd->dq0, dd->q0.data(), nb * sizeof(int), hipMemcpyHostToDevice, s));
+    KAD_HIP_CHECK(hipMemcpyAsync(dd->dq0 + nb, dd->kb.data(), nb * sizeof(int), hipMemcpyHostToDevice, s));
     KAD_HIP_CHECK(hipMemcpyAsync(dd->upos, hpos.data(), r * sizeof(int64_t), hipMemcpyHostToDevice, s));
-    // W = D^{-1} P
+    // W = D^{-1} P; block b has kb[b] right-hand sides, the solve skips the zero
+    // columns past them
     KAD_HIP_CHECK(hipMemsetAsync(dd->W, 0, (size_t)nb * nd * kmax * sizeof(double), s));
     kad::dd_units_kernel<<<(unsigned)kad::cdiv(r, 256), 256, 0, s>>>(dd->upos, r, dd->W);
     KAD_HIP_CHECK(hipGetLastError());
     if ((rc = kad::lu_solve_permuted_batched(dd->D, nd, nd, nb, (int64_t)nd * nd, dd->W, nd, (int64_t)nd * kmax, kmax,
-                                             c->lu)))
+                                             c->lu, dd->dq0 + nb)))
       return rc;
     KAD_HIP_CHECK(hipStreamSynchronize(s));
     double t3 = now_s();
