@@ -18,7 +18,8 @@ namespace kad {
 //    computed, compacted over the grid so the active tiles spread over every XCD
 //    (C -= AB only).
 //  batch: nbat products of one shape, launched together (grid z); product z reads
-//    A + z sa and B + z sb and writes C + z sc (masks and lists: nbat = 1 only).
+//    A + z sa and B + z sb and writes C + z sc (the masks apply to every product;
+//    lists: nbat = 1 only).
 struct TileSel {
   const int* rmask = nullptr;
   const int* cmask = nullptr;
@@ -28,6 +29,12 @@ struct TileSel {
   // batch only (device, nbat entries): matrix z computes only the tiles holding its
   // first zcols[z] columns of C; the others are left as they are (OP_SUB only)
   const int* zcols = nullptr;
+  // batch only (device, nbat x zng): the k rows of B_z are rows zthr - k .. zthr - 1 of
+  // a taller right-hand side whose columns 64g .. 64g+63 are zero above its row
+  // zstart[z zng + g]; a tile whose column groups all start at or past zthr would only
+  // subtract zeros and is skipped (OP_SUB only)
+  const int* zstart = nullptr;
+  int zng = 0, zthr = 0;
   int nbat = 1;
   int64_t sa = 0, sb = 0, sc = 0;
 };

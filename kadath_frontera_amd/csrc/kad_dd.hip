@@ -41,8 +41,10 @@ int gemm_launch(int prec, int op, int m, int n, int k, const void* A, int64_t ld
                 int64_t ldc, hipStream_t s);
 int gemm_launch_batched(int prec, int op, int m, int n, int k, const void* A, int64_t lda, int64_t sa, const void* B,
                         int64_t ldb, int64_t sb, void* C, int64_t ldc, int64_t sc, int nbat, hipStream_t s);
+int gemm_launch_sel(int prec, int op, int m, int n, int k, const void* A, int64_t lda, const void* B, int64_t ldb,
+                    void* C, int64_t ldc, const TileSel& sel, hipStream_t s);
 int lu_solve_permuted_batched(const double* LU, int64_t lda, int n, int nbat, int64_t sA, double* B, int64_t ldb,
-                              int64_t sB, int nrhs, LuWork& w, const int* zcols);
+                              int64_t sB, int nrhs, LuWork& w, const int* zcols, const int* zstart, int zng);
 int ir_loop(Ctx* c, int n, const double* A64, int64_t lda, const std::function<int(double*)>& precond,
             const double* b, double* x, double tol, int max_outer, int mode, int trans, int* iters, double* resid);
 size_t trsv_table_doubles(int n);
@@ -57,10 +59,13 @@ int getrs_fast(const T* LU, int64_t lda, int n, const int* perm, const double* W
                int trans, double* work, hipStream_t s);
 
 constexpr int DD_COLS = 512;  // columns per workgroup of the coupling-row scan
+static_assert(LU_MAX_BATCH <= 32, "the coupling-row scan keeps one bit per block");
 constexpr int DD_CHUNK = 1024;  // columns per partial sum of the C products
+constexpr int DD_KSPLIT = 8;    // k-chunks of each block's C_b W_b (when 8 divides the order)
 
-// flag[i] = 1 when row i of J has an entry outside its own block (columns
-// [by DD_COLS, +DD_COLS) of this workgroup).
+// Bit z of flag[i] is set when row i of J has an entry in block z's columns, z other
+// than the row's own block (columns [by DD_COLS, +DD_COLS) of this workgroup; at most
+// 32 blocks).  flag[i] != 0: row i is a coupling row.
 __global__ __launch_bounds__(256) void dd_flag_kernel(const double* __restrict__ J, int64_t ld, int n,
                                                       const int* __restrict__ rblk, const int* __restrict__ cblk,
                                                       int* __restrict__ flag) {
@@ -68,9 +73,12 @@ __global__ __launch_bounds__(256) void dd_flag_kernel(const double* __restrict__
   if (i >= n) return;
   const int c0 = blockIdx.y * DD_COLS, c1 = min(n, c0 + DD_COLS);
   const int b = rblk[i];
-  int hit = 0;
-  for (int c = c0; c < c1; ++c) hit |= (cblk[c] != b) & (J[i + (int64_t)c * ld] != 0.0);
-  if (hit) flag[i] = 1;
+  unsigned hit = 0;
+  for (int c = c0; c < c1; ++c) {
+    const int cb = cblk[c];
+    hit |= (cb != b && J[i + (int64_t)c * ld] != 0.0) ? 1u << cb : 0u;
+  }
+  if (hit) atomicOr(reinterpret_cast<unsigned*>(flag) + i, hit);
 }
 
 // D_z(i, j) = J(rowmap_z[li], colmap_z[j]), li = rperm_z[i] (rows in an earlier
@@ -111,6 +119,19 @@ __global__ __launch_bounds__(256) void dd_coupling_kernel(const double* __restri
   if (q >= r) return;
   const int col = colmap[(int64_t)z * nd + j];
   Jc[q + ((int64_t)z * nd + j) * r] = (col >= 0 && cz[q] != z) ? J[crow[q] + (int64_t)col * ld] : 0.0;
+}
+
+// C(i, j) = the sum of P_z(i, j) over z = 0 .. nz-1 in that order (P_z at P + z sp,
+// r x kb, ld r): the k-chunks of a split product.
+__global__ __launch_bounds__(256) void dd_ksum_kernel(const double* __restrict__ P, int r, int nz, int64_t sp,
+                                                      double* __restrict__ C) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= r) return;
+  const int64_t e = i + (int64_t)j * r;
+  double acc = P[e];
+  for (int z = 1; z < nz; ++z) acc += P[e + z * sp];
+  C[e] = acc;
 }
 
 __global__ void dd_add_identity_kernel(double* __restrict__ A, int64_t ld, int r) {
@@ -196,6 +217,9 @@ struct kad_dd_s {
   int* crow = nullptr;          // r rows of J, ordered by (block, local row)
   int* cz = nullptr;            // block of each coupling row
   int* dq0 = nullptr;           // first coupling index of each block, then kb (2 nblk entries)
+  int* zst = nullptr;           // per block and 64-column group of W: first pivoted row of its units
+  int* cmk = nullptr;           // per block and 128 coupling rows: any of them couples to the block
+  double* kpart = nullptr;      // DD_KSPLIT partial products C_b W_b, r x kmax each
   int64_t* upos = nullptr;      // W offset of each unit right-hand side
   double* W = nullptr;
   double* Jc = nullptr;
@@ -228,12 +252,12 @@ struct kad_dd_s {
     return KAD_OK;
   }
   void release_coupling() {
-    void* bufs[] = {crow, cz, dq0, upos, W, Jc, cap, cap_ipiv, cap_perm, cap_tab, cap_flag, t, yb, part, cap2};
+    void* bufs[] = {crow, cz, dq0, zst, cmk, kpart, upos, W, Jc, cap, cap_ipiv, cap_perm, cap_tab, cap_flag, t, yb, part, cap2};
     for (void* b : bufs)
       if (b) (void)hipFree(b);
-    crow = cz = dq0 = nullptr;
+    crow = cz = dq0 = zst = cmk = nullptr;
     upos = nullptr;
-    W = Jc = cap = cap_tab = t = yb = part = cap2 = nullptr;
+    W = Jc = cap = cap_tab = t = yb = part = cap2 = kpart = nullptr;
     cap_ipiv = cap_perm = cap_flag = nullptr;
     cap_r = cap_w = cap2_r = 0;
     piv_valid = false;  // cap_perm is gone
@@ -263,6 +287,10 @@ int ensure_coupling(kad_dd* dd, int r, int kmax) {
   if ((rc = dd->dmalloc((void**)&dd->crow, R * sizeof(int)))) return rc;
   if ((rc = dd->dmalloc((void**)&dd->cz, R * sizeof(int)))) return rc;
   if ((rc = dd->dmalloc((void**)&dd->dq0, 2 * (size_t)dd->nblk * sizeof(int)))) return rc;
+  if ((rc = dd->dmalloc((void**)&dd->zst, (size_t)dd->nblk * kad::cdiv(kmax, 64) * sizeof(int)))) return rc;
+  if ((rc = dd->dmalloc((void**)&dd->cmk, (size_t)dd->nblk * kad::cdiv((int64_t)R, 128) * sizeof(int)))) return rc;
+  if ((rc = dd->dmalloc((void**)&dd->kpart, (size_t)kad::DD_KSPLIT * R * std::max(kmax, 1) * sizeof(double))))
+    return rc;
   if ((rc = dd->dmalloc((void**)&dd->upos, R * sizeof(int64_t)))) return rc;
   if ((rc = dd->dmalloc((void**)&dd->W, std::max<size_t>(w, 1) * sizeof(double)))) return rc;
   if ((rc = dd->dmalloc((void**)&dd->Jc, R * ncol * sizeof(double)))) return rc;
@@ -487,26 +515,43 @@ int dd_factor_impl(kad_dd* dd, const double* J, int64_t ldj, bool stat, double m
     }
     std::vector<int64_t> hpos(r);
     std::vector<int> inv(nd);
+    const int ng = (int)kad::cdiv(kmax, 64);
+    std::vector<int> hst((size_t)nb * ng, nd);  // groups without units: nothing to solve
     for (int b = 0; b < nb; ++b) {
       for (int k = 0; k < nd; ++k) inv[dd->h_perm[(size_t)b * nd + k]] = k;
       for (int j = 0; j < kb[b]; ++j) {
         const int q = q0[b] + j, row = byblk[b][j];
         // e_{local row} in pivoted order sits at inv[local row]; column j of W_b
-        hpos[q] = (int64_t)b * nd * kmax + (int64_t)j * nd + inv[dd->h_rloc[row]];
+        const int p = inv[dd->h_rloc[row]];
+        hpos[q] = (int64_t)b * nd * kmax + (int64_t)j * nd + p;
+        int& st = hst[(size_t)b * ng + j / 64];
+        st = std::min(st, p);
       }
     }
+    KAD_HIP_CHECK(hipMemcpyAsync(dd->zst, hst.data(), hst.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    // the coupling rows with entries in block b's columns (those of the neighbouring
+    // domains' interfaces), per 128-row tile of C_b: the other tiles of C_b W_b are zero
+    const int nt = (int)kad::cdiv(r, 128);
+    std::vector<int> hmk((size_t)nb * nt, 0);
+    for (int q = 0; q < r; ++q) {
+      const unsigned bits = (unsigned)hflag[hcrow[q]];
+      for (int b = 0; b < nb; ++b)
+        if ((bits >> b) & 1u) hmk[(size_t)b * nt + q / 128] = 1;
+    }
+    KAD_HIP_CHECK(hipMemcpyAsync(dd->cmk, hmk.data(), hmk.size() * sizeof(int), hipMemcpyHostToDevice, s));
     KAD_HIP_CHECK(hipMemcpyAsync(dd->crow, hcrow.data(), r * sizeof(int), hipMemcpyHostToDevice, s));
     KAD_HIP_CHECK(hipMemcpyAsync(dd->cz, hcz.data(), r * sizeof(int), hipMemcpyHostToDevice, s));
     KAD_HIP_CHECK(hipMemcpyAsync(dd->dq0, dd->q0.data(), nb * sizeof(int), hipMemcpyHostToDevice, s));
     KAD_HIP_CHECK(hipMemcpyAsync(dd->dq0 + nb, dd->kb.data(), nb * sizeof(int), hipMemcpyHostToDevice, s));
     KAD_HIP_CHECK(hipMemcpyAsync(dd->upos, hpos.data(), r * sizeof(int64_t), hipMemcpyHostToDevice, s));
     // W = D^{-1} P; block b has kb[b] right-hand sides, the solve skips the zero
-    // columns past them
+    // columns past them and, in the unit-lower solve, the zero rows above each group's
+    // first unit (forward substitution keeps them zero)
     KAD_HIP_CHECK(hipMemsetAsync(dd->W, 0, (size_t)nb * nd * kmax * sizeof(double), s));
     kad::dd_units_kernel<<<(unsigned)kad::cdiv(r, 256), 256, 0, s>>>(dd->upos, r, dd->W);
     KAD_HIP_CHECK(hipGetLastError());
     if ((rc = kad::lu_solve_permuted_batched(dd->D, nd, nd, nb, (int64_t)nd * nd, dd->W, nd, (int64_t)nd * kmax, kmax,
-                                             c->lu, dd->dq0 + nb)))
+                                             c->lu, dd->dq0 + nb, dd->zst, ng)))
       return rc;
     KAD_HIP_CHECK(hipStreamSynchronize(s));
     double t3 = now_s();
@@ -514,11 +559,25 @@ int dd_factor_impl(kad_dd* dd, const double* J, int64_t ldj, bool stat, double m
     kad::dd_coupling_kernel<<<dim3((unsigned)kad::cdiv(r, 256), (unsigned)nd, (unsigned)nb), 256, 0, s>>>(
         J, ldj, dd->crow, dd->cz, dd->colmap, nd, r, dd->Jc);
     KAD_HIP_CHECK(hipGetLastError());
+    // C_b W_b in DD_KSPLIT k-chunks, a batch of partial products summed in chunk order:
+    // the rows that couple to one block are too few tiles to fill the chip over the
+    // full order
+    const int ks = nd % kad::DD_KSPLIT == 0 ? kad::DD_KSPLIT : 1, kc = nd / ks;
     for (int b = 0; b < nb; ++b) {
-      if (dd->kb[b] == 0) continue;
-      if ((rc = kad::gemm_launch(kad::F64, 1, r, dd->kb[b], nd, dd->Jc + (int64_t)b * nd * r, r,
-                                 dd->W + (int64_t)b * nd * kmax, nd, dd->cap + (int64_t)dd->q0[b] * r, r, s)))
+      const int kbb = dd->kb[b];
+      if (kbb == 0) continue;
+      kad::TileSel sel;
+      sel.rmask = dd->cmk + (int64_t)b * nt;
+      sel.nbat = ks;
+      sel.sa = (int64_t)kc * r;
+      sel.sb = kc;
+      sel.sc = (int64_t)r * kbb;
+      if ((rc = kad::gemm_launch_sel(kad::F64, 1, r, kbb, kc, dd->Jc + (int64_t)b * nd * r, r,
+                                     dd->W + (int64_t)b * nd * kmax, nd, dd->kpart, r, sel, s)))
         return rc;
+      kad::dd_ksum_kernel<<<dim3((unsigned)kad::cdiv(r, 256), (unsigned)kbb), 256, 0, s>>>(
+          dd->kpart, r, ks, (int64_t)r * kbb, dd->cap + (int64_t)dd->q0[b] * r);
+      KAD_HIP_CHECK(hipGetLastError());
     }
     kad::dd_add_identity_kernel<<<(unsigned)kad::cdiv(r, 256), 256, 0, s>>>(dd->cap, r, r);
     KAD_HIP_CHECK(hipGetLastError());

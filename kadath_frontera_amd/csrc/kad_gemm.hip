@@ -62,6 +62,13 @@ __device__ inline bool select_tile(const TileSel& sel, int nbm, int nbn, int& tm
   }
   tile_coords(blockIdx.x, nbm, nbn, tm, tn);
   if (sel.zcols && tn * BM >= sel.zcols[blockIdx.z]) return false;
+  if (sel.zstart) {  // the tile's 64-column groups (tn * BM < n <= 64 zng)
+    const int* st = sel.zstart + (int64_t)blockIdx.z * sel.zng;
+    const int g0 = tn * BM / 64;
+    int m0 = st[g0];
+    if (BM > 64 && g0 + 1 < sel.zng) m0 = min(m0, st[g0 + 1]);
+    if (m0 >= sel.zthr) return false;
+  }
   if ((sel.rmask && sel.rmask[tm * BM / 128] == 0) || (sel.cmask && sel.cmask[tn * BM / 128] == 0)) {
     zero_fill = true;
     return false;
@@ -392,10 +399,20 @@ __global__ void tile_fraction_kernel(TileSel sel, int nbm, int nbn, double* out)
     if (threadIdx.x == 0) *out = ((double)sel.cnt[0] / nbm) * ((double)sel.cnt[1] / nbn);
     return;
   }
-  if (sel.zcols) {  // no masks with a batch: the column tiles of each matrix
+  if (sel.zcols || sel.zstart) {  // no masks with a batch: the column tiles of each matrix
     if (threadIdx.x == 0) {
       double t = 0.0;
-      for (int z = 0; z < sel.nbat; ++z) t += min((sel.zcols[z] + GBN - 1) / GBN, nbn);
+      for (int z = 0; z < sel.nbat; ++z)
+        for (int q = 0; q < nbn; ++q) {
+          bool on = !sel.zcols || q * GBN < sel.zcols[z];
+          if (on && sel.zstart) {
+            const int* st = sel.zstart + (int64_t)z * sel.zng;
+            int m0 = st[2 * q];
+            if (2 * q + 1 < sel.zng) m0 = min(m0, st[2 * q + 1]);
+            on = m0 < sel.zthr;
+          }
+          t += on;
+        }
       *out = t / ((double)nbn * sel.nbat);
     }
     return;
@@ -445,9 +462,10 @@ int gemm_launch_masked(int prec, int op, int m, int n, int k, const void* A, int
 // masks and lists are device arrays over 128-row / 128-column tiles of C).
 int gemm_launch_sel(int prec, int op, int m, int n, int k, const void* A, int64_t lda, const void* B, int64_t ldb,
                     void* C, int64_t ldc, const TileSel& sel, hipStream_t s) {
-  if ((sel.cnt || sel.zcols) && op != OP_SUB) return KAD_ERR_ARG;
-  if (sel.nbat < 1 || sel.nbat > 65535 || (sel.nbat > 1 && (sel.cnt || sel.rmask || sel.cmask))) {
-    fprintf(stderr, "[kadath_amd] gemm: batch of %d with masks or lists\n", sel.nbat);
+  if ((sel.cnt || sel.zcols || sel.zstart) && op != OP_SUB) return KAD_ERR_ARG;
+  if (sel.zstart && (int64_t)sel.zng * 64 < n) return KAD_ERR_ARG;
+  if (sel.nbat < 1 || sel.nbat > 65535 || (sel.nbat > 1 && sel.cnt)) {
+    fprintf(stderr, "[kadath_amd] gemm: batch of %d with tile lists\n", sel.nbat);
     return KAD_ERR_ARG;
   }
   if (m <= 0 || n <= 0) return KAD_OK;
@@ -470,7 +488,7 @@ int gemm_launch_sel(int prec, int op, int m, int n, int k, const void* A, int64_
     // the launch on the same stream (outside the timed bracket, no host synchronisation)
     const int nbm = (int)cdiv(m, GBM), nbn = (int)cdiv(n, GBN);
     double* slot = tm->scale_slot();
-    const bool masked = sel.cnt || sel.rmask || sel.cmask || sel.zcols;
+    const bool masked = sel.cnt || sel.rmask || sel.cmask || sel.zcols || sel.zstart;
     if (slot && masked) tile_fraction_kernel<<<1, 256, 0, s>>>(sel, nbm, nbn, slot);
     const int rc = tm->begin(s, 2.0 * m * n * (double)k * sel.nbat, masked);
     if (rc < 0) return rc;

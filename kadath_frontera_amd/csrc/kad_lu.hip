@@ -251,14 +251,24 @@ constexpr int TRSM_LEAF = 64;
 // never used.  L is staged in LDS once and read with broadcast loads.  The order of
 // operations on every B(i) is the column-oriented substitution of LAPACK's xTRSM
 // ('L', 'L', 'N', 'U').  cmask (optional): 128-column tiles of B that are
-// structurally zero stay untouched; zcols (optional, a batch): system z solves only
-// the 64-column groups holding its first zcols[z] columns.
+// structurally zero stay untouched; a batch's ZeroRhs skips the 64-column groups whose
+// rows here are all zero.
 constexpr int TRSM_FUSED = 32;
+
+// Where a batch's right-hand sides are known to be zero (a batch of sparse right-hand
+// sides such as unit vectors); the solves skip that work, whose results would be zero.
+struct ZeroRhs {
+  const int* cols = nullptr;   // device, nbat entries: system z is zero past column cols[z]
+  const int* start = nullptr;  // device, nbat x ng: group g (columns 64g .. 64g+63) of system z
+                               // is zero above row start[z ng + g] (lower solves only)
+  int ng = 0;
+  int row0 = 0;  // the sub-system's first row in the full system (rows of start)
+};
 
 template <typename T>
 __global__ __launch_bounds__(64, 4) void trsm_leaf_kernel(const T* __restrict__ L, int64_t lda, T* __restrict__ B,
                                                        int64_t ldb, int ncols, const int* __restrict__ cmask,
-                                                       int64_t sL, int64_t sB, const int* __restrict__ zcols) {
+                                                       int64_t sL, int64_t sB, ZeroRhs zr) {
   constexpr int R = TRSM_FUSED;
   L += (int64_t)blockIdx.y * sL;  // system blockIdx.y of a batch
   B += (int64_t)blockIdx.y * sB;
@@ -266,7 +276,8 @@ __global__ __launch_bounds__(64, 4) void trsm_leaf_kernel(const T* __restrict__ 
   __shared__ T Ls[R * R + R];  // column-major, Ls[i + R k] = L(i, k); R entries of padding read past the end
   const int c0 = blockIdx.x * 64;
   if (cmask && !cmask[c0 / 128]) return;  // uniform: the block lies in one 128-column tile
-  if (zcols && c0 >= zcols[blockIdx.y]) return;
+  if (zr.cols && c0 >= zr.cols[blockIdx.y]) return;
+  if (zr.start && zr.start[(int)blockIdx.y * zr.ng + (int)blockIdx.x] >= zr.row0 + R) return;
   for (int e = threadIdx.x; e < R * R; e += 64) Ls[e] = L[(int64_t)(e % R) + (int64_t)(e / R) * lda];
   if (threadIdx.x < R) Ls[R * R + threadIdx.x] = T(0);
   const int c = c0 + threadIdx.x;
@@ -286,47 +297,58 @@ __global__ __launch_bounds__(64, 4) void trsm_leaf_kernel(const T* __restrict__ 
   }
 }
 
-// C_z -= A_z B_z over a batch at strides (sa, sb, sc); zcols (device, may be NULL):
-// matrix z updates only the tiles holding its first zcols[z] columns.
+// C_z -= A_z B_z over a batch at strides (sa, sb, sc).  zr (may be NULL): B_z is zero
+// past column zr->cols[z], and (with start) the rows of B, which are rows zr->row0 ..
+// zr->row0 + k - 1 of the full system, are zero in the 64-column groups that start
+// below them: the tiles of C that would only subtract zeros are skipped.
 static int batched_sub(int prec, int m, int n, int k, const void* A, int64_t lda, int64_t sa, const void* B,
-                       int64_t ldb, int64_t sb, void* C, int64_t ldc, int64_t sc, int nbat, const int* zcols,
+                       int64_t ldb, int64_t sb, void* C, int64_t ldc, int64_t sc, int nbat, const ZeroRhs* zr,
                        hipStream_t s) {
-  if (!zcols) return gemm_launch_batched(prec, GOP_SUB, m, n, k, A, lda, sa, B, ldb, sb, C, ldc, sc, nbat, s);
+  if (!zr) return gemm_launch_batched(prec, GOP_SUB, m, n, k, A, lda, sa, B, ldb, sb, C, ldc, sc, nbat, s);
   TileSel sel;
   sel.nbat = nbat;
   sel.sa = sa;
   sel.sb = sb;
   sel.sc = sc;
-  sel.zcols = zcols;
+  sel.zcols = zr->cols;
+  sel.zstart = zr->start;
+  sel.zng = zr->ng;
+  sel.zthr = zr->row0 + k;
   return gemm_launch_sel(prec, GOP_SUB, m, n, k, A, lda, B, ldb, C, ldc, sel, s);
 }
 
 // nbat > 1: nbat systems at L + z sL, A12 + z sB (no masks; r a multiple of 32, so
-// every leaf is a 32-row kernel).  zcols (device, nbat entries, may be NULL): system z
-// has right-hand sides only in its first zcols[z] columns, the rest of its ncols are
-// zero and stay zero; the work on them is skipped in 64-column groups.
+// every leaf is a 32-row kernel).  zr (may be NULL): where the batch's right-hand sides
+// are zero; the work on those parts, which stay zero, is skipped.
 template <typename T>
 int trsm_left_unit_lower(const T* L, int64_t lda, int r, T* A12, int64_t ld12, int ncols, LuWork& w, int gemm_prec,
                          const int* cmask = nullptr, int nbat = 1, int64_t sL = 0, int64_t sB = 0,
-                         const int* zcols = nullptr) {
+                         const ZeroRhs* zr = nullptr) {
   if (ncols <= 0 || r <= 0) return KAD_OK;
   if (nbat > 1 && cmask) return arg_error("trsm_left_unit_lower: masks with a batch");
+  if (zr && (nbat < 2 || (zr->start && zr->ng < cdiv(ncols, 64))))
+    return arg_error("trsm_left_unit_lower: zero right-hand-side parts need a batch and a group per 64 columns");
   if (r > TRSM_LEAF || r == 2 * TRSM_FUSED) {
     const int h = std::max(32, (r / 2) / 32 * 32);
-    int rc = trsm_left_unit_lower<T>(L, lda, h, A12, ld12, ncols, w, gemm_prec, cmask, nbat, sL, sB, zcols);
+    int rc = trsm_left_unit_lower<T>(L, lda, h, A12, ld12, ncols, w, gemm_prec, cmask, nbat, sL, sB, zr);
     if (rc) return rc;
     const int gp = gemm_prec == BF16 ? F32 : gemm_prec;
-    rc = nbat > 1 ? batched_sub(gp, r - h, ncols, h, L + h, lda, sL, A12, ld12, sB, A12 + h, ld12, sB, nbat, zcols,
+    rc = nbat > 1 ? batched_sub(gp, r - h, ncols, h, L + h, lda, sL, A12, ld12, sB, A12 + h, ld12, sB, nbat, zr,
                                 w.stream)
                   : gemm_launch_masked(gp, GOP_SUB, r - h, ncols, h, L + h, lda, A12, ld12, A12 + h, ld12, nullptr,
                                        cmask, w.stream);
     if (rc) return rc;
+    ZeroRhs z2;
+    if (zr) {
+      z2 = *zr;
+      z2.row0 += h;
+    }
     return trsm_left_unit_lower<T>(L + h + (int64_t)h * lda, lda, r - h, A12 + h, ld12, ncols, w, gemm_prec, cmask,
-                                   nbat, sL, sB, zcols);
+                                   nbat, sL, sB, zr ? &z2 : nullptr);
   }
   if (r == TRSM_FUSED) {
     trsm_leaf_kernel<T><<<dim3((unsigned)cdiv(ncols, 64), nbat), 64, 0, w.stream>>>(L, lda, A12, ld12, ncols, cmask,
-                                                                                  sL, sB, zcols);
+                                                                                  sL, sB, zr ? *zr : ZeroRhs());
     KAD_HIP_CHECK(hipGetLastError());
     return KAD_OK;
   }
@@ -394,7 +416,8 @@ __global__ __launch_bounds__(64, 4) void trsm_leaf_upper_kernel(const T* __restr
 
 // B <- U^{-1} B with U non-unit upper (r x r, r a multiple of 32): X2 = U22^{-1} B2,
 // B1 -= U12 X2, X1 = U11^{-1} B1, recursively down to 32-row leaves.  nbat systems at
-// U + z sU, B + z sB; zcols as in trsm_left_unit_lower.
+// U + z sU, B + z sB; zcols (device, nbat entries, may be NULL): B_z is zero past
+// column zcols[z], and those columns are skipped.
 template <typename T>
 int trsm_left_upper(const T* U, int64_t ldu, int r, T* B, int64_t ldb, int ncols, LuWork& w, int gemm_prec, int nbat,
                     int64_t sU, int64_t sB, const int* zcols = nullptr) {
@@ -410,8 +433,10 @@ int trsm_left_upper(const T* U, int64_t ldu, int r, T* B, int64_t ldb, int ncols
   int rc =
       trsm_left_upper<T>(U + h + (int64_t)h * ldu, ldu, r - h, B + h, ldb, ncols, w, gemm_prec, nbat, sU, sB, zcols);
   if (rc) return rc;
-  rc = batched_sub(gemm_prec, h, ncols, r - h, U + (int64_t)h * ldu, ldu, sU, B + h, ldb, sB, B, ldb, sB, nbat, zcols,
-                   w.stream);
+  ZeroRhs zc;
+  zc.cols = zcols;
+  rc = batched_sub(gemm_prec, h, ncols, r - h, U + (int64_t)h * ldu, ldu, sU, B + h, ldb, sB, B, ldb, sB, nbat,
+                   zcols ? &zc : nullptr, w.stream);
   if (rc) return rc;
   return trsm_left_upper<T>(U, ldu, h, B, ldb, ncols, w, gemm_prec, nbat, sU, sB, zcols);
 }
@@ -419,11 +444,19 @@ int trsm_left_upper(const T* U, int64_t ldu, int r, T* B, int64_t ldb, int ncols
 // B <- A^{-1} B for nbat factored matrices of order n (n a multiple of 32) whose
 // right-hand sides are already in pivoted row order (B_z rows permuted by perm_z):
 // the unit-lower, then the upper triangular solve, in FP64.  zcols (device, nbat
-// entries, may be NULL): B_z is zero past its first zcols[z] columns, which are skipped.
+// entries, may be NULL): B_z is zero past its first zcols[z] columns, which are skipped;
+// zstart (device, nbat x zng, may be NULL, needs zcols): columns 64g .. 64g+63 of B_z are
+// zero above row zstart[z zng + g], which the unit-lower solve skips.
 int lu_solve_permuted_batched(const double* LU, int64_t lda, int n, int nbat, int64_t sA, double* B, int64_t ldb,
-                              int64_t sB, int nrhs, LuWork& w, const int* zcols) {
+                              int64_t sB, int nrhs, LuWork& w, const int* zcols, const int* zstart, int zng) {
   if (n % TRSM_FUSED != 0 || nbat < 1) return arg_error("lu_solve_permuted_batched: order or batch");
-  int rc = trsm_left_unit_lower<double>(LU, lda, n, B, ldb, nrhs, w, F64, nullptr, nbat, sA, sB, zcols);
+  if (zstart && !zcols) return arg_error("lu_solve_permuted_batched: row starts without column counts");
+  ZeroRhs zr;
+  zr.cols = zcols;
+  zr.start = zstart;
+  zr.ng = zng;
+  int rc = trsm_left_unit_lower<double>(LU, lda, n, B, ldb, nrhs, w, F64, nullptr, nbat, sA, sB,
+                                        zcols && nbat > 1 ? &zr : nullptr);
   if (rc) return rc;
   return trsm_left_upper<double>(LU, lda, n, B, ldb, nrhs, w, F64, nbat, sA, sB, zcols);
 }

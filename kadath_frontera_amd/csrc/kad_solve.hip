@@ -21,7 +21,7 @@ int gemm_launch(int prec, int op, int m, int n, int k, const void* A, int64_t ld
 template <typename T>
 int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w, const LuBatch& bt = LuBatch());
 int lu_solve_permuted_batched(const double* LU, int64_t lda, int n, int nbat, int64_t sA, double* B, int64_t ldb,
-                              int64_t sB, int nrhs, LuWork& w, const int* zcols);
+                              int64_t sB, int nrhs, LuWork& w, const int* zcols, const int* zstart, int zng);
 size_t trsv_table_doubles(int n);
 size_t trsv_flag_ints(int n);
 size_t ffp_bytes();
@@ -521,7 +521,7 @@ int kad_lu_solve_batched(kad_ctx* ctx, int n, int nbat, const double* LU, int64_
   dim3 g((unsigned)kad::cdiv(n, 256), (unsigned)nrhs, (unsigned)nbat);
   gather_rows_batched_kernel<<<g, 256, 0, c->stream>>>(B, ldb, strideB, perm, n, c->bsolve);
   KAD_HIP_CHECK(hipGetLastError());
-  if ((rc = kad::lu_solve_permuted_batched(LU, lda, n, nbat, strideA, c->bsolve, n, (int64_t)n * nrhs, nrhs, c->lu, nullptr)))
+  if ((rc = kad::lu_solve_permuted_batched(LU, lda, n, nbat, strideA, c->bsolve, n, (int64_t)n * nrhs, nrhs, c->lu, nullptr, nullptr, 0)))
     return rc;
   put_back_batched_kernel<<<g, 256, 0, c->stream>>>(c->bsolve, n, B, ldb, strideB);
   KAD_HIP_CHECK(hipGetLastError());
