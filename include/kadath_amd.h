@@ -129,12 +129,23 @@ int kad_dd_factor(kad_dd* dd, const double* J, int64_t ldj);
  * block, 3 other coupling rows, 4 the same in the capacitance matrix; the factors are
  * then not usable and kad_dd_factor should follow. */
 int kad_dd_factor_static(kad_dd* dd, const double* J, int64_t ldj, double max_growth);
+/* kad_dd_factor (partial pivoting in every block) that keeps the capacitance matrix's
+ * pivot order: I + C D^{-1} P does not depend on the blocks' pivot orders, so when the
+ * coupling rows are those of the last factorisation its rows are taken in that order
+ * and factored without interchanges (kad_lu_factor_batched_static), kept while every
+ * multiplier stays within max_growth (>= 1); otherwise it is factored with partial
+ * pivoting as in kad_dd_factor.  Returns as kad_dd_factor; kad_dd_pivoting then gives
+ * static_orders 0 and the kept order's largest |multiplier| (0 when it was refactored).
+ * The Newton solve of do_newton.cpp:256-330 factors every step; this is the same
+ * solve with less work on the coupling. */
+int kad_dd_factor_keep(kad_dd* dd, const double* J, int64_t ldj, double max_growth);
 int kad_dd_solve(kad_dd* dd, const double* J, int64_t ldj, const double* b, double* x, double tol, int max_outer,
                  int* iters, double* resid);
 int kad_dd_apply(kad_dd* dd, double* v);
 int kad_dd_info(const kad_dd* dd, int* sizes, double* times);
 /* whether the last factorisation took the kept pivot orders (kad_dd_factor_static)
- * and its largest |multiplier| (0 after kad_dd_factor); HOST out, either may be NULL */
+ * and its largest |multiplier| (0 after kad_dd_factor; the capacitance matrix's after
+ * kad_dd_factor_keep kept its order); HOST out, either may be NULL */
 int kad_dd_pivoting(const kad_dd* dd, int* static_orders, double* growth);
 int kad_dd_destroy(kad_dd* dd);
 
@@ -423,7 +434,10 @@ int kad_system_get_double(kad_system* s, int k, double* value);
  * the solve factor their J with its pivot orders (kad_dd_factor_static), falling
  * back to partial pivoting when a multiplier exceeds 1e8 or the refinement needs more
  * than 6 inner steps; after a step whose multipliers exceed 1e3 the next step factors
- * with partial pivoting again (the orders drift with the Jacobian).  which 5 = an
+ * with partial pivoting again (the orders drift with the Jacobian); those
+ * factorisations keep the capacitance matrix's order (kad_dd_factor_keep) while its
+ * multipliers stay within 1e8, and factor it with partial pivoting too when the
+ * refinement then needs more than 6 inner steps.  which 5 = an
  * element-wise instruction whose operand is made by the spectral pass just before it,
  * and used nowhere else, is applied in that pass's store phase (1, the default; 0:
  * launched on its own); J is the same bit for bit. */

@@ -125,6 +125,7 @@ SIGNATURES = {
     "kad_dd_create": (c_int, [c_vp, c_int, c_int, c_ip, c_ip, c_ip, ctypes.POINTER(c_vp)]),
     "kad_dd_factor": (c_int, [c_vp, c_vp, c_i64]),
     "kad_dd_factor_static": (c_int, [c_vp, c_vp, c_i64, c_dbl]),
+    "kad_dd_factor_keep": (c_int, [c_vp, c_vp, c_i64, c_dbl]),
     "kad_dd_pivoting": (c_int, [c_vp, c_ip, c_dp]),
     "kad_dd_solve": (c_int, [c_vp, c_vp, c_i64, c_vp, c_vp, c_dbl, c_int, c_ip, c_dp]),
     "kad_dd_apply": (c_int, [c_vp, c_vp]),

@@ -433,7 +433,12 @@ namespace {
 // max_growth in a block, kStaticCoupling other coupling rows, kStaticCapacitance the
 // same in the capacitance matrix.
 constexpr int kStaticNone = 1, kStaticBlocks = 2, kStaticCoupling = 3, kStaticCapacitance = 4;
-int dd_factor_impl(kad_dd* dd, const double* J, int64_t ldj, bool stat, double max_growth) {
+// stat: every row order kept (multipliers within max_growth, else kStatic*); otherwise
+// partial pivoting in the blocks and, with cap_keep >= 1, the capacitance matrix in its
+// kept order when the coupling rows are the same and its multipliers stay within
+// cap_keep (I + C D^{-1} P does not depend on the blocks' pivot orders), else with
+// partial pivoting too.
+int dd_factor_impl(kad_dd* dd, const double* J, int64_t ldj, bool stat, double max_growth, double cap_keep = 0.0) {
   if (!dd || !J || ldj < dd->n) return KAD_ERR_ARG;
   Ctx* c = dd->c;
   hipStream_t s = c->stream;
@@ -442,6 +447,7 @@ int dd_factor_impl(kad_dd* dd, const double* J, int64_t ldj, bool stat, double m
   dd->last_static = stat;
   dd->last_growth = 0.0;
   if (stat && !dd->piv_valid) return kStaticNone;
+  const bool had_orders = dd->piv_valid;
   if (!stat) dd->piv_valid = false;
   int rc;
   KAD_HIP_CHECK(hipStreamSynchronize(s));
@@ -583,17 +589,16 @@ int dd_factor_impl(kad_dd* dd, const double* J, int64_t ldj, bool stat, double m
     KAD_HIP_CHECK(hipGetLastError());
     KAD_HIP_CHECK(hipStreamSynchronize(s));
     double t4 = now_s();
-    if (!stat) {
-      rc = kad_lu_factor(reinterpret_cast<kad_ctx*>(c), KAD_PREC_F64, r, dd->cap, r, dd->cap_ipiv, dd->cap_perm, 0);
-      if (rc < 0) return rc;
-      if (rc > 0) return n + rc;  // singular capacitance matrix
-    } else {
-      // its rows in the kept order (cap_perm), into the second buffer, factored there
+    // the rows in the kept order (cap_perm), into the second buffer, factored there
+    // without interchanges; cap keeps I + C W.  1: accepted (the buffers trade places),
+    // 0: a zero pivot or a multiplier above `bound`, < 0 an error
+    auto factor_kept = [&](double bound) -> int {
       if (dd->cap2_r < dd->cap_r) {  // the capacity of cap: the two buffers trade places
         if (dd->cap2) (void)hipFree(dd->cap2);
         dd->cap2 = nullptr;
         dd->cap2_r = 0;
-        if ((rc = dd->dmalloc((void**)&dd->cap2, dd->cap_r * dd->cap_r * sizeof(double)))) return rc;
+        int e;
+        if ((e = dd->dmalloc((void**)&dd->cap2, dd->cap_r * dd->cap_r * sizeof(double)))) return e;
         dd->cap2_r = dd->cap_r;
       }
       kad::dd_rows_kernel<<<dim3((unsigned)kad::cdiv(r, 256), (unsigned)r), 256, 0, s>>>(dd->cap, dd->cap_perm, r,
@@ -601,12 +606,31 @@ int dd_factor_impl(kad_dd* dd, const double* J, int64_t ldj, bool stat, double m
       KAD_HIP_CHECK(hipGetLastError());
       int cinfo = 0;
       double cg = 0.0;
-      rc = kad_lu_factor_batched_static(reinterpret_cast<kad_ctx*>(c), KAD_PREC_F64, r, 1, dd->cap2, r, 0,
-                                        dd->cap_ipiv, &cinfo, &cg);
-      if (rc) return rc;
-      dd->last_growth = std::max(dd->last_growth, std::isnan(cg) ? INFINITY : cg);
-      if (cinfo != 0 || !(cg <= max_growth)) return kStaticCapacitance;
+      const int e = kad_lu_factor_batched_static(reinterpret_cast<kad_ctx*>(c), KAD_PREC_F64, r, 1, dd->cap2, r, 0,
+                                                 dd->cap_ipiv, &cinfo, &cg);
+      if (e) return e < 0 ? e : KAD_ERR_ARG;
+      const double g = std::isnan(cg) ? INFINITY : cg;
+      if (cinfo != 0 || !(g <= bound)) {
+        if (stat) dd->last_growth = std::max(dd->last_growth, g);
+        return 0;
+      }
+      dd->last_growth = std::max(dd->last_growth, g);
       std::swap(dd->cap, dd->cap2);  // the factors of the kept-order rows, for the solves
+      return 1;
+    };
+    if (!stat) {
+      int kept = 0;
+      if (cap_keep >= 1.0 && had_orders && hcrow == dd->h_crow && (kept = factor_kept(cap_keep)) < 0) return kept;
+      if (!kept) {
+        rc = kad_lu_factor(reinterpret_cast<kad_ctx*>(c), KAD_PREC_F64, r, dd->cap, r, dd->cap_ipiv, dd->cap_perm,
+                           0);
+        if (rc < 0) return rc;
+        if (rc > 0) return n + rc;  // singular capacitance matrix
+      }
+    } else {
+      const int kept = factor_kept(max_growth);
+      if (kept < 0) return kept;
+      if (!kept) return kStaticCapacitance;
     }
     if ((rc = kad::trsv_prepare<double>(dd->cap, r, r, 0, dd->cap_tab, dd->cap_flag, dd->cap_ext, s))) return rc;
     KAD_HIP_CHECK(hipStreamSynchronize(s));
@@ -636,6 +660,11 @@ int kad_dd_factor(kad_dd* dd, const double* J, int64_t ldj) { return dd_factor_i
 int kad_dd_factor_static(kad_dd* dd, const double* J, int64_t ldj, double max_growth) {
   if (!(max_growth >= 1.0)) return KAD_ERR_ARG;
   return dd_factor_impl(dd, J, ldj, true, max_growth);
+}
+
+int kad_dd_factor_keep(kad_dd* dd, const double* J, int64_t ldj, double max_growth) {
+  if (!(max_growth >= 1.0)) return KAD_ERR_ARG;
+  return dd_factor_impl(dd, J, ldj, false, 0.0, max_growth);
 }
 
 int kad_dd_solve(kad_dd* dd, const double* J, int64_t ldj, const double* b, double* x, double tol, int max_outer,

@@ -1647,12 +1647,19 @@ static int newton_direction(kad_system_impl* S, int solver, double err, double* 
         if (stat && kad_dd_pivoting(S->dd, nullptr, &growth) == KAD_OK && growth > kDdRefresh)
           S->dd_orders = false;  // used once; the next step refreshes them
       }
-      auto factor_pivoting = [&]() -> int {
-        const int frc = kad_dd_factor(S->dd, S->J, n);
+      // keep: with static pivoting on, the capacitance matrix keeps its order through
+      // the blocks' partial pivoting while its multipliers stay within kDdMaxGrowth
+      auto factor_pivoting = [&](bool keep) -> int {
+        const int frc = keep ? kad_dd_factor_keep(S->dd, S->J, n, kDdMaxGrowth) : kad_dd_factor(S->dd, S->J, n);
         S->dd_orders = frc == 0;
         return frc;
       };
-      if (!stat && (dd_info = factor_pivoting()) < 0) return dd_info;
+      bool kept_cap = false;
+      if (!stat) {
+        if ((dd_info = factor_pivoting(S->dd_static)) < 0) return dd_info;
+        double cg = 0.0;
+        kept_cap = dd_info == 0 && S->dd_static && kad_dd_pivoting(S->dd, nullptr, &cg) == KAD_OK && cg > 0.0;
+      }
       tf1 += now_s() - tg0;
       // dd_info > 0: a singular domain block or capacitance matrix
       S->dd_valid = (dd_info == 0);
@@ -1662,11 +1669,12 @@ static int newton_direction(kad_system_impl* S, int solver, double err, double* 
         rc = kad_dd_solve(S->dd, S->J, n, S->F, S->X, eta, 8, S->last_iters, &S->last_resid);
         if (rc < 0 && rc != KAD_ERR_NOCONV) return rc;
         bool poor = rc == KAD_ERR_NOCONV && !(S->last_resid <= std::max(eta, kAcceptResid));
-        if (stat && (poor || S->last_iters[1] > kDdStaticInner)) {
-          // the kept orders made a poor preconditioner: partial pivoting, solved again
+        if ((stat || kept_cap) && (poor || S->last_iters[1] > kDdStaticInner)) {
+          // the kept orders made a poor preconditioner: partial pivoting throughout,
+          // solved again
           KAD_HIP_CHECK(hipStreamSynchronize(st));
           const double tg2 = now_s();
-          if ((dd_info = factor_pivoting()) < 0) return dd_info;
+          if ((dd_info = factor_pivoting(false)) < 0) return dd_info;
           tf1 += now_s() - tg2;
           S->dd_valid = (dd_info == 0);
           poor = dd_info != 0;

@@ -239,6 +239,14 @@ class DomainSolve:
         check(rc, "kad_dd_factor_static")
         return rc
 
+    def factor_keep(self, J, max_growth=1e3):
+        """kad_dd_factor_keep: partial pivoting in the blocks, the capacitance matrix in
+        the last factorisation's order while its multipliers stay within max_growth."""
+        self.ctx._enter()
+        rc = lib.kad_dd_factor_keep(self.h, _ptr(J), J.stride(1), float(max_growth))
+        check(rc, "kad_dd_factor_keep")
+        return rc
+
     def pivoting(self):
         """(the last factorisation took the kept orders, its largest |multiplier|)"""
         st = ctypes.c_int()

@@ -349,6 +349,56 @@ def test_domain_solve_with_static_pivoting(ctx):
     dd.close()
 
 
+def test_domain_solve_keeping_the_capacitance_order(ctx):
+    """kad_dd_factor_keep on a nearby matrix: partial pivoting in the blocks, the
+    capacitance matrix in the order of the last factorisation (its multipliers reported);
+    the solve equals the dense one.  With other coupling rows it factors the capacitance
+    matrix with partial pivoting (growth 0) and still solves."""
+    rng = np.random.default_rng(7)
+    n, sizes = 600, [250, 350]
+    rows = rng.permutation(n)
+    cols = rng.permutation(n)
+    rb = [rows[:250], rows[250:]]
+    cb = [cols[:250], cols[250:]]
+    J = np.zeros((n, n))
+    for b in range(2):
+        J[np.ix_(rb[b], cb[b])] = rng.standard_normal((sizes[b], sizes[b]))
+    for b in range(2):
+        cpl = rng.choice(rb[b], 20, replace=False)
+        J[np.ix_(cpl, cb[1 - b])] = rng.standard_normal((20, sizes[1 - b]))
+    F = rng.standard_normal(n)
+
+    def dev(M):
+        return torch.tensor(M.T.copy(), dtype=torch.float64, device="cuda").T
+
+    Fd = torch.tensor(F, dtype=torch.float64, device="cuda")
+    dd = DomainSolve(ctx, n, [(rb[0], cb[0]), (rb[1], cb[1])])
+    assert dd.factor_keep(dev(J)) == 0  # no earlier order: partial pivoting throughout
+    assert dd.pivoting() == (False, 0.0)
+    J2 = J + 1e-6 * rng.standard_normal((n, n)) * (J != 0)
+    J2d = dev(J2)
+    assert dd.factor_keep(J2d) == 0
+    st, g = dd.pivoting()
+    x, iters, resid = dd.solve(J2d, Fd)
+    x_ref = np.linalg.solve(J2, F)
+    err = np.abs(x.cpu().numpy() - x_ref).max() / np.abs(x_ref).max()
+    print(f"kept capacitance order: growth {g:.3f}, solve {err:.2e} ({iters}, resid {resid:.1e})", flush=True)
+    assert not st and 0.0 < g <= 8.0
+    assert err <= 1e-10 and resid <= 1e-13 and iters[1] <= 3
+    # block 0's rows reversed: other coupling rows, the capacitance matrix is refactored
+    J3 = J2.copy()
+    J3[rb[0]] = J2[rb[0][::-1]]
+    J3d = dev(J3)
+    assert dd.factor_keep(J3d) == 0
+    assert dd.pivoting() == (False, 0.0)
+    x3, _, resid3 = dd.solve(J3d, Fd)
+    x3_ref = np.linalg.solve(J3, F)
+    assert np.abs(x3.cpu().numpy() - x3_ref).max() / np.abs(x3_ref).max() <= 1e-10 and resid3 <= 1e-13
+    with pytest.raises(RuntimeError):
+        dd.factor_keep(J3d, max_growth=0.5)
+    dd.close()
+
+
 def test_newton_with_static_pivoting_reaches_the_same_solution(ctx):
     """do_newton with solver 6: with static pivoting (the default) the steps after the
     first try the pivot orders of the last factorisation with partial pivoting (a step
