@@ -116,8 +116,9 @@ int kad_lu_solve_batched(kad_ctx* ctx, int n, int nbat, const double* LU, int64_
  * preconditioned by that solve, to ||b - J x||_inf <= tol ||b||_inf (KAD_ERR_NOCONV
  * otherwise; iters as kad_ir_solve).  kad_dd_apply applies the preconditioner alone
  * (v: rows in, unknowns out).  kad_dd_info: sizes [padded block order, coupling rows,
- * the most per block, blocks], times (s) of the last factorisation [coupling scan and
- * block gather, block LU, D^{-1} P, capacitance product, capacitance LU]. */
+ * the most per block, blocks], times (s) of the last factorisation [block gather,
+ * block LU (the scan of J for coupling rows runs beside it on a second stream),
+ * D^{-1} P, capacitance product, capacitance LU]. */
 typedef struct kad_dd_s kad_dd;
 int kad_dd_create(kad_ctx* ctx, int n, int nblk, const int* blk_size, const int* rows, const int* cols, kad_dd** out);
 int kad_dd_factor(kad_dd* dd, const double* J, int64_t ldj);

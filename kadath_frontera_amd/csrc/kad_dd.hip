@@ -197,6 +197,8 @@ using kad::Ctx;
 
 struct kad_dd_s {
   Ctx* c = nullptr;
+  // the coupling-row scan (memory-bound) runs here beside the block LU (compute-bound)
+  hipStream_t scan_stream = nullptr;
   int n = 0, nblk = 0, nd = 0;
   std::vector<int> bsize;
   std::vector<int> h_rowmap, h_colmap;  // nblk x nd, -1 on padding
@@ -267,6 +269,7 @@ struct kad_dd_s {
     void* bufs[] = {rowmap, colmap, rblk, cblk, flag, D, ipiv, perm, vb, vy, trs};
     for (void* b : bufs)
       if (b) (void)hipFree(b);
+    if (scan_stream) (void)hipStreamDestroy(scan_stream);
   }
 };
 
@@ -452,18 +455,24 @@ int dd_factor_impl(kad_dd* dd, const double* J, int64_t ldj, bool stat, double m
   int rc;
   KAD_HIP_CHECK(hipStreamSynchronize(s));
   double t0 = now_s();
-  // coupling rows
-  KAD_HIP_CHECK(hipMemsetAsync(dd->flag, 0, (size_t)n * sizeof(int), s));
-  kad::dd_flag_kernel<<<dim3((unsigned)kad::cdiv(n, 256), (unsigned)kad::cdiv(n, kad::DD_COLS)), 256, 0, s>>>(
-      J, ldj, n, dd->rblk, dd->cblk, dd->flag);
-  KAD_HIP_CHECK(hipGetLastError());
   kad::dd_blocks_kernel<<<dim3((unsigned)kad::cdiv(nd, 256), (unsigned)nd, (unsigned)nb), 256, 0, s>>>(
       J, ldj, dd->rowmap, dd->colmap, stat ? dd->perm : nullptr, nd, dd->D);
   KAD_HIP_CHECK(hipGetLastError());
-  std::vector<int> hflag(n);
-  KAD_HIP_CHECK(hipMemcpyAsync(hflag.data(), dd->flag, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, s));
   KAD_HIP_CHECK(hipStreamSynchronize(s));
   double t1 = now_s();
+  // coupling rows, scanned on the second stream while the blocks are factored (a
+  // memory-bound scan beside compute-bound updates; it only reads J, which is complete:
+  // s was synchronised before the gather)
+  if (!dd->scan_stream) KAD_HIP_CHECK(hipStreamCreateWithFlags(&dd->scan_stream, hipStreamNonBlocking));
+  hipStream_t s2 = dd->scan_stream;
+  struct ScanJoin {  // every return waits for the scan: the caller may reuse J at once
+    hipStream_t st;
+    ~ScanJoin() { (void)hipStreamSynchronize(st); }
+  } scan_join{s2};
+  KAD_HIP_CHECK(hipMemsetAsync(dd->flag, 0, (size_t)n * sizeof(int), s2));
+  kad::dd_flag_kernel<<<dim3((unsigned)kad::cdiv(n, 256), (unsigned)kad::cdiv(n, kad::DD_COLS)), 256, 0, s2>>>(
+      J, ldj, n, dd->rblk, dd->cblk, dd->flag);
+  KAD_HIP_CHECK(hipGetLastError());
   // the diagonal blocks, factored together
   std::vector<int> info(nb, 0);
   if (!stat) {
@@ -485,6 +494,9 @@ int dd_factor_impl(kad_dd* dd, const double* J, int64_t ldj, bool stat, double m
   if ((rc = kad::trsv_prepare_batched(dd->D, nd, nd, nb, (int64_t)nd * nd, dd->trs, s))) return rc;
   KAD_HIP_CHECK(hipStreamSynchronize(s));
   double t2 = now_s();
+  std::vector<int> hflag(n);
+  KAD_HIP_CHECK(hipMemcpyAsync(hflag.data(), dd->flag, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, s2));
+  KAD_HIP_CHECK(hipStreamSynchronize(s2));
   // coupling rows by (block, local row); their unit right-hand sides in pivoted order
   std::vector<std::vector<int>> byblk(nb);
   for (int i = 0; i < n; ++i)
