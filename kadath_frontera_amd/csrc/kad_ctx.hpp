@@ -120,6 +120,9 @@ extern GemmTimer* g_apply_timer;  // apply3d launches (work = HBM bytes: arrays 
 
 // Nonzero 256 x 1024 tiles of an n x n fp64 matrix, for products that skip the
 // structural zeros (kad_struct.hip).  Valid for the contents A had at build().
+constexpr int GM_ROWS = 256;   // rows per GEMV tile
+constexpr int GM_COLS = 1024;  // columns per GEMV tile (= partial-sum chunk)
+
 struct GemvMap {
   const double* A = nullptr;
   int64_t lda = 0;
@@ -128,6 +131,9 @@ struct GemvMap {
   int* list = nullptr;  // (row tile, chunk) pairs of the nonzero tiles
   size_t cap = 0;
   int build(const double* A, int64_t lda, int n, hipStream_t s);
+  // as build, with the tile flags already made by another scan of the same A (device,
+  // nchunk x nrb, 256-row by 1024-column tiles)
+  int adopt(const double* A, int64_t lda, int n, const int* dmap, hipStream_t s);
   int apply(const double* x, const double* b, double* y, int trans, double* part, hipStream_t s) const;
   ~GemvMap();
 };

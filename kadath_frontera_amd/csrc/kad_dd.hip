@@ -46,7 +46,8 @@ int gemm_launch_sel(int prec, int op, int m, int n, int k, const void* A, int64_
 int lu_solve_permuted_batched(const double* LU, int64_t lda, int n, int nbat, int64_t sA, double* B, int64_t ldb,
                               int64_t sB, int nrhs, LuWork& w, const int* zcols, const int* zstart, int zng);
 int ir_loop(Ctx* c, int n, const double* A64, int64_t lda, const std::function<int(double*)>& precond,
-            const double* b, double* x, double tol, int max_outer, int mode, int trans, int* iters, double* resid);
+            const double* b, double* x, double tol, int max_outer, int mode, int trans, int* iters, double* resid,
+            bool map_ready = false);
 size_t trsv_table_doubles(int n);
 size_t trsv_flag_ints(int n);
 int trsv_prepare_batched(const double* LU, int64_t lda, int n, int nbat, int64_t sA, double* W, hipStream_t s);
@@ -63,22 +64,34 @@ static_assert(LU_MAX_BATCH <= 32, "the coupling-row scan keeps one bit per block
 constexpr int DD_CHUNK = 1024;  // columns per partial sum of the C products
 constexpr int DD_KSPLIT = 8;    // k-chunks of each block's C_b W_b (when 8 divides the order)
 
+static_assert(GM_ROWS == 256 && GM_COLS == 2 * DD_COLS, "a scan workgroup covers half a GEMV tile");
+
 // Bit z of flag[i] is set when row i of J has an entry in block z's columns, z other
 // than the row's own block (columns [by DD_COLS, +DD_COLS) of this workgroup; at most
-// 32 blocks).  flag[i] != 0: row i is a coupling row.
+// 32 blocks).  flag[i] != 0: row i is a coupling row.  gmap (may be NULL, zeroed
+// before): the GEMV tile flags of GemvMap for the same J, gmap[chunk nrb + rb] = 1 when
+// its 256-row, 1024-column tile has a nonzero (two workgroups per tile).
 __global__ __launch_bounds__(256) void dd_flag_kernel(const double* __restrict__ J, int64_t ld, int n,
                                                       const int* __restrict__ rblk, const int* __restrict__ cblk,
-                                                      int* __restrict__ flag) {
+                                                      int* __restrict__ flag, int* __restrict__ gmap) {
   const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
   const int c0 = blockIdx.y * DD_COLS, c1 = min(n, c0 + DD_COLS);
-  const int b = rblk[i];
-  unsigned hit = 0;
-  for (int c = c0; c < c1; ++c) {
-    const int cb = cblk[c];
-    hit |= (cb != b && J[i + (int64_t)c * ld] != 0.0) ? 1u << cb : 0u;
+  int nz = 0;
+  if (i < n) {  // no early return: the whole workgroup reaches the barrier below
+    const int b = rblk[i];
+    unsigned hit = 0;
+    for (int c = c0; c < c1; ++c) {
+      const double v = J[i + (int64_t)c * ld];
+      const int cb = cblk[c];
+      nz |= v != 0.0;
+      hit |= (cb != b && v != 0.0) ? 1u << cb : 0u;
+    }
+    if (hit) atomicOr(reinterpret_cast<unsigned*>(flag) + i, hit);
   }
-  if (hit) atomicOr(reinterpret_cast<unsigned*>(flag) + i, hit);
+  if (gmap) {  // the same for every thread of the launch
+    nz = __syncthreads_or(nz);
+    if (threadIdx.x == 0 && nz) atomicOr(gmap + (int64_t)(blockIdx.y / 2) * gridDim.x + blockIdx.x, 1);
+  }
 }
 
 // D_z(i, j) = J(rowmap_z[li], colmap_z[j]), li = rperm_z[i] (rows in an earlier
@@ -199,6 +212,11 @@ struct kad_dd_s {
   Ctx* c = nullptr;
   // the coupling-row scan (memory-bound) runs here beside the block LU (compute-bound)
   hipStream_t scan_stream = nullptr;
+  // GEMV tile flags of the last factored J (GemvMap layout), made by the coupling scan;
+  // gmap_J: that J while the flags are unused (dd_solve_after_factor takes them once)
+  int* gmap = nullptr;
+  const double* gmap_J = nullptr;
+  int64_t gmap_ld = 0;
   int n = 0, nblk = 0, nd = 0;
   std::vector<int> bsize;
   std::vector<int> h_rowmap, h_colmap;  // nblk x nd, -1 on padding
@@ -266,7 +284,7 @@ struct kad_dd_s {
   }
   ~kad_dd_s() {
     release_coupling();
-    void* bufs[] = {rowmap, colmap, rblk, cblk, flag, D, ipiv, perm, vb, vy, trs};
+    void* bufs[] = {rowmap, colmap, rblk, cblk, flag, gmap, D, ipiv, perm, vb, vy, trs};
     for (void* b : bufs)
       if (b) (void)hipFree(b);
     if (scan_stream) (void)hipStreamDestroy(scan_stream);
@@ -403,6 +421,9 @@ int kad_dd_create(kad_ctx* ctx, int n, int nblk, const int* blk_size, const int*
   if ((rc = dd->dmalloc((void**)&dd->rblk, (size_t)n * sizeof(int)))) return fail(rc);
   if ((rc = dd->dmalloc((void**)&dd->cblk, (size_t)n * sizeof(int)))) return fail(rc);
   if ((rc = dd->dmalloc((void**)&dd->flag, (size_t)n * sizeof(int)))) return fail(rc);
+  if ((rc = dd->dmalloc((void**)&dd->gmap,
+                        (size_t)kad::cdiv(n, kad::GM_ROWS) * kad::cdiv(n, kad::GM_COLS) * sizeof(int))))
+    return fail(rc);
   if ((rc = dd->dmalloc((void**)&dd->D, (size_t)nblk * nd * nd * sizeof(double)))) return fail(rc);
   if ((rc = dd->dmalloc((void**)&dd->ipiv, (size_t)nblk * nd * sizeof(int)))) return fail(rc);
   if ((rc = dd->dmalloc((void**)&dd->perm, (size_t)nblk * nd * sizeof(int)))) return fail(rc);
@@ -447,6 +468,7 @@ int dd_factor_impl(kad_dd* dd, const double* J, int64_t ldj, bool stat, double m
   hipStream_t s = c->stream;
   const int n = dd->n, nd = dd->nd, nb = dd->nblk;
   dd->factored = false;
+  dd->gmap_J = nullptr;
   dd->last_static = stat;
   dd->last_growth = 0.0;
   if (stat && !dd->piv_valid) return kStaticNone;
@@ -469,9 +491,11 @@ int dd_factor_impl(kad_dd* dd, const double* J, int64_t ldj, bool stat, double m
     hipStream_t st;
     ~ScanJoin() { (void)hipStreamSynchronize(st); }
   } scan_join{s2};
+  const size_t gcells = (size_t)kad::cdiv(n, kad::GM_ROWS) * kad::cdiv(n, kad::GM_COLS);
   KAD_HIP_CHECK(hipMemsetAsync(dd->flag, 0, (size_t)n * sizeof(int), s2));
+  KAD_HIP_CHECK(hipMemsetAsync(dd->gmap, 0, gcells * sizeof(int), s2));
   kad::dd_flag_kernel<<<dim3((unsigned)kad::cdiv(n, 256), (unsigned)kad::cdiv(n, kad::DD_COLS)), 256, 0, s2>>>(
-      J, ldj, n, dd->rblk, dd->cblk, dd->flag);
+      J, ldj, n, dd->rblk, dd->cblk, dd->flag, dd->gmap);
   KAD_HIP_CHECK(hipGetLastError());
   // the diagonal blocks, factored together
   std::vector<int> info(nb, 0);
@@ -659,6 +683,8 @@ int dd_factor_impl(kad_dd* dd, const double* J, int64_t ldj, bool stat, double m
     dd->h_crow = hcrow;
     dd->piv_valid = true;
   }
+  dd->gmap_J = J;
+  dd->gmap_ld = ldj;
   dd->factored = true;
   return KAD_OK;
 }
@@ -716,3 +742,25 @@ int kad_dd_info(const kad_dd* dd, int* sizes, double* times) {
 }
 
 }  // extern "C"
+
+namespace kad {
+
+// kad_dd_solve for a J whose entries have not changed since the factorisation of the
+// same J that just succeeded: the refinement's GEMV takes the nonzero tiles that the
+// factorisation's coupling-row scan recorded instead of reading J once more.  A caller
+// that cannot promise that (J refilled in place between the two) calls kad_dd_solve.
+int dd_solve_after_factor(kad_dd* dd, const double* J, int64_t ldj, const double* b, double* x, double tol,
+                          int max_outer, int* iters, double* resid) {
+  if (!dd || !J || !b || !x || ldj < dd->n || max_outer < 1) return KAD_ERR_ARG;
+  if (!dd->factored) return KAD_ERR_ARG;
+  bool ready = false;
+  if (dd->gmap_J == J && dd->gmap_ld == ldj) {
+    if (const int rc = dd->c->jmap.adopt(J, ldj, dd->n, dd->gmap, dd->c->stream)) return rc;
+    ready = true;
+  }
+  dd->gmap_J = nullptr;  // the flags serve one solve
+  auto pre = [dd](double* v) { return dd_apply(dd, v); };
+  return ir_loop(dd->c, dd->n, J, ldj, pre, b, x, tol, max_outer, 1, 0, iters, resid, ready);
+}
+
+}  // namespace kad

@@ -220,7 +220,8 @@ Ctx::~Ctx() {
 }
 
 int ir_loop(Ctx* c, int n, const double* A64, int64_t lda, const std::function<int(double*)>& precond,
-            const double* b, double* x, double tol, int max_outer, int mode, int trans, int* iters, double* resid);
+            const double* b, double* x, double tol, int max_outer, int mode, int trans, int* iters, double* resid,
+            bool map_ready = false);
 
 }  // namespace kad
 
@@ -639,16 +640,19 @@ namespace kad {
 
 // The refinement loop of kad_ir_solve_scaled for any preconditioner: precond(v)
 // replaces v by M^{-1} v (op(M) for trans = 1).  Residuals and products use the
-// fp64 A; mode 0 is classical refinement, mode 1 flexible GMRES.
+// fp64 A; mode 0 is classical refinement, mode 1 flexible GMRES.  map_ready: c->jmap
+// already holds A's nonzero tiles (adopted from a scan of this same A), no new scan.
 int ir_loop(Ctx* c, int n, const double* A64, int64_t lda, const std::function<int(double*)>& precond,
-            const double* b, double* x, double tol, int max_outer, int mode, int trans, int* iters, double* resid) {
+            const double* b, double* x, double tol, int max_outer, int mode, int trans, int* iters, double* resid,
+            bool map_ready) {
   int rc = c->vec_work(n);
   if (rc) return rc;
   hipStream_t s = c->stream;
   const unsigned g = (unsigned)kad::cdiv(n, 256);
   double* r = c->vwork + (size_t)2 * n;
   double* d = c->vwork + (size_t)3 * n;
-  if ((rc = c->jmap.build(A64, lda, n, s))) return rc;
+  if (!map_ready || c->jmap.A != A64 || c->jmap.lda != lda || c->jmap.n != n)
+    if ((rc = c->jmap.build(A64, lda, n, s))) return rc;
   const kad::GemvMap& Am = c->jmap;
   auto& solve_lo = precond;
   const double bnorm = c->absmax(b, n);

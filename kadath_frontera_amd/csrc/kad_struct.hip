@@ -18,8 +18,6 @@
 
 namespace kad {
 
-constexpr int GM_ROWS = 256;    // rows per GEMV tile
-constexpr int GM_COLS = 1024;   // columns per GEMV tile (= partial-sum chunk)
 
 // map[chunk * nrb + rb] = 1 if A(rb tile, chunk tile) has a nonzero.
 __global__ __launch_bounds__(256) void gemv_map_kernel(const double* __restrict__ A, int64_t lda, int n,
@@ -104,7 +102,9 @@ __global__ __launch_bounds__(256) void gemv_t_tile_kernel(const double* __restri
 }
 
 // Scan A for its nonzero GEMV tiles (device map + compact host-built list).
-int GemvMap::build(const double* A_, int64_t lda_, int n_, hipStream_t s) {
+int GemvMap::build(const double* A_, int64_t lda_, int n_, hipStream_t s) { return adopt(A_, lda_, n_, nullptr, s); }
+
+int GemvMap::adopt(const double* A_, int64_t lda_, int n_, const int* dmap, hipStream_t s) {
   A = A_;
   lda = lda_;
   n = n_;
@@ -120,8 +120,12 @@ int GemvMap::build(const double* A_, int64_t lda_, int n_, hipStream_t s) {
     KAD_HIP_CHECK(hipMalloc((void**)&list, 2 * cells * sizeof(int)));
     cap = cells;
   }
-  gemv_map_kernel<<<dim3((unsigned)nrb, (unsigned)nchunk), 256, 0, s>>>(A, lda, n, map);
-  KAD_HIP_CHECK(hipGetLastError());
+  if (dmap) {
+    KAD_HIP_CHECK(hipMemcpyAsync(map, dmap, cells * sizeof(int), hipMemcpyDeviceToDevice, s));
+  } else {
+    gemv_map_kernel<<<dim3((unsigned)nrb, (unsigned)nchunk), 256, 0, s>>>(A, lda, n, map);
+    KAD_HIP_CHECK(hipGetLastError());
+  }
   std::vector<int> h(cells), l;
   KAD_HIP_CHECK(hipMemcpyAsync(h.data(), map, cells * sizeof(int), hipMemcpyDeviceToHost, s));
   KAD_HIP_CHECK(hipStreamSynchronize(s));

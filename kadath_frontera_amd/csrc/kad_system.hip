@@ -24,6 +24,9 @@ namespace kad {
 template <typename T>
 int getrf(T* A, int64_t lda, int n, int* ipiv, int nb, int gemm_prec, LuWork& w, const LuBatch& bt = LuBatch());
 bool ipiv_to_perm(const int* ipiv, int n, int* perm);
+// kad_dd_solve right after a factorisation of the same, unchanged J (kad_dd.hip)
+int dd_solve_after_factor(kad_dd* dd, const double* J, int64_t ldj, const double* b, double* x, double tol,
+                          int max_outer, int* iters, double* resid);
 }  // namespace kad
 
 using namespace kadhost;
@@ -1666,7 +1669,7 @@ static int newton_direction(kad_system_impl* S, int solver, double err, double* 
       S->last_factored = 1;
       bool fallback = dd_info != 0;
       if (dd_info == 0) {
-        rc = kad_dd_solve(S->dd, S->J, n, S->F, S->X, eta, 8, S->last_iters, &S->last_resid);
+        rc = kad::dd_solve_after_factor(S->dd, S->J, n, S->F, S->X, eta, 8, S->last_iters, &S->last_resid);
         if (rc < 0 && rc != KAD_ERR_NOCONV) return rc;
         bool poor = rc == KAD_ERR_NOCONV && !(S->last_resid <= std::max(eta, kAcceptResid));
         if ((stat || kept_cap) && (poor || S->last_iters[1] > kDdStaticInner)) {
@@ -1679,7 +1682,7 @@ static int newton_direction(kad_system_impl* S, int solver, double err, double* 
           S->dd_valid = (dd_info == 0);
           poor = dd_info != 0;
           if (dd_info == 0) {
-            rc = kad_dd_solve(S->dd, S->J, n, S->F, S->X, eta, 8, S->last_iters, &S->last_resid);
+            rc = kad::dd_solve_after_factor(S->dd, S->J, n, S->F, S->X, eta, 8, S->last_iters, &S->last_resid);
             if (rc < 0 && rc != KAD_ERR_NOCONV) return rc;
             poor = rc == KAD_ERR_NOCONV && !(S->last_resid <= std::max(eta, kAcceptResid));
           }
